@@ -1,0 +1,158 @@
+"""Benchmark: NS3D WENO5-RK4 FP64 cell-updates/s on one MI355X.
+
+Workload (the production-size configuration the metric is quoted on):
+viscous NavierStokes3D on a periodic 256^3 grid, characteristic WENO5-JS
+with Roe upwinding, 4th-order viscous terms (Re 1600, Pr 0.72, Mach 0.1),
+classical RK4 at CFL 0.5, seeded Taylor-Green + random-phase field.
+
+A step is one RK4 step of the whole grid with the state resident in HBM.
+Prints one JSON line.  Only one GPU exists on this system: --gpus above 1
+exits with a message.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "NS3D WENO5-RK4 FP64 cell-updates/s at 256³; adjoint/forward time ratio"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+# algorithmic HBM bytes of one interface-flux launch per cell: read the 5
+# conserved variables once, write the 5 interface-flux components once
+IFACE_BYTES_PER_CELL = 80
+# algorithmic HBM bytes per cell-update of a fused 3N RK4 step (5 FP64
+# variables; stage 1 reads u, writes acc and U2; stages 2-3 read U_s, u, acc,
+# write acc, U_{s+1}; stage 4 reads U4, acc, writes u)
+STEP_BYTES_PER_CELL_UPDATE = 640
+
+
+def cpu_baseline(n=64, steps=2):
+    """Time the reference CPU path on a bounded sample of the same workload:
+    the same scheme on an n^3 periodic grid for `steps` RK4 steps.  Uses the
+    reference binary built from its sources (oracle/_ref/HyPar), single
+    thread: its OpenMP build shares the characteristic-weight scratch between
+    threads.  Falls back to the CPU oracle (a restatement) when the binary is
+    absent."""
+    from hypar_amd import decks
+    w = decks.config1(n, steps)
+    ref = os.path.join(ROOT, "oracle", "_ref", "HyPar")
+    if os.path.exists(ref):
+        with tempfile.TemporaryDirectory() as d:
+            decks.write_deck(d, w, screen_op_iter=1)
+            env = dict(os.environ, OMP_NUM_THREADS="1")
+            r = subprocess.run([ref], cwd=d, capture_output=True, text=True, env=env)
+            m = re.search(r"total wctime:\s*([0-9.eE+-]+)", r.stdout)
+            if r.returncode == 0 and m:
+                t = float(m.group(1))
+                return {"value": n ** 3 * steps / t, "unit": "cell-updates/s", "cores": 1,
+                        "kind": "reference",
+                        "sample": f"reference HyPar (gcc -O3, 1 thread), same scheme on a periodic "
+                                  f"{n}^3 grid, {steps} RK4 steps, {t:.2f} s of time integration"}
+    from oracle.oracle import Oracle
+    u0, dt = decks.prepare(w)
+    o = Oracle.for_workload(w)
+    t0 = time.time()
+    o.rk4(u0, dt, steps)
+    t = time.time() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": n ** 3 * steps / t, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+            "sample": f"CPU oracle, same scheme on a periodic {n}^3 grid, {steps} RK4 steps, {t:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this system has one GPU; --gpus {args.gpus} is not supported "
+              f"(the path would shard as described in DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from hypar_amd import decks
+    from hypar_amd.solver import NS3D
+
+    torch.cuda.set_device(0)
+    w = decks.config1(args.n, args.steps)
+    u0, dt = decks.prepare(w)
+    s = NS3D.from_workload(w)
+    u = s.to_device(u0)
+    del u0
+    ncell = s.ncell
+
+    s.rk4(u, dt, args.warmup)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    s.rk4(u, dt, args.steps)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    ms_per_step = ms / args.steps
+    value = ncell * args.steps / (ms / 1e3)
+    cfl = s.cfl(u, dt)  # also a NaN/Inf check of the final state
+
+    # per-kernel durations (HIP events on the solver's stream)
+    kt = s.profile_kernels(u, reps=5)
+    per_step = {k: 4 * v for k, v in kt.items()}  # each kernel runs once per stage
+    dom = max(per_step, key=per_step.get)
+    if dom.startswith("iface"):
+        bytes_launch = IFACE_BYTES_PER_CELL * ncell
+    elif dom == "visc_flux":
+        bytes_launch = (40 + 96) * ncell
+    else:
+        bytes_launch = (15 * 8 + 3 * 40) * ncell
+    achieved = bytes_launch / (kt[dom] / 1e3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "avg_launch_ms": kt[dom], "algorithmic_bytes_per_launch": bytes_launch}
+    step_bw = value * STEP_BYTES_PER_CELL_UPDATE / 1e9
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Taylor-Green + random-phase modes |k|<=8)",
+        "config": {"workload": f"viscous NS3D periodic {args.n}^3, characteristic WENO5-JS + Roe, "
+                               f"4th-order viscous (Re 1600, Pr 0.72, Mach 0.1), RK4 CFL 0.5",
+                   "grid": [args.n] * 3, "dt": dt, "final_cfl": cfl},
+        "roofline": roofline,
+        "step_roofline": {"bytes_per_cell_update": STEP_BYTES_PER_CELL_UPDATE,
+                          "achieved_GBs": step_bw, "peak_GBs": HBM_PEAK_GBS,
+                          "frac": step_bw / HBM_PEAK_GBS},
+        "kernel_ms_per_step": per_step,
+    }
+    s.close()
+    del u
+    torch.cuda.empty_cache()
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

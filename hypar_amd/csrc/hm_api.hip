@@ -1,0 +1,368 @@
+// hm_api.hip -- C-ABI of libhypar_mi355x.so (declared in include/hypar_mi355x.h).
+//
+// The solver object owns the per-stage scratch in HBM; the caller owns the
+// state arrays.  All launches go to the solver's stream and are
+// asynchronous unless the entry point returns a host value.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/hypar_mi355x.h"
+#include "hm_internal.h"
+
+struct hm_solver {
+  hm::Grid g;
+  hm::Scheme sc;
+  int device;
+  hipStream_t stream;
+  double *dxg;        // ghosted dxinv x|y|z on the device
+  double *fI[3];      // interface fluxes, 5*ncell each
+  double *FV;         // viscous fluxes, 12*ncell (viscous runs only)
+  double *S[2];       // RK stage states (double-buffered)
+  double *acc;        // RK accumulator
+  double *part;       // reduction partials (device)
+  double *part_host;  // reduction partials (pinned host)
+  unsigned npart;
+  double *tmp;        // layout-conversion staging
+  size_t tmp_elems;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+hm_status fail(hm_status code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HM_HIP(call)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return fail(HM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+hm_status check_launch(const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(HM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  return HM_OK;
+}
+
+// Uniform periodic grid x_i = i*(2*pi/n); ghost coordinates extrapolated
+// linearly (ReadArray.c:59-90); dxinv = 2/(x[i+1]-x[i-1]) inside and copies
+// of the edge value in the ghosts (InitialSolution.c:74-113).
+void uniform_dxinv(int n, int g, double *out) {
+  std::vector<double> X(n + 2 * g);
+  const double h = 2.0 * M_PI / n;
+  for (int i = 0; i < n; i++) X[g + i] = (double)i * h;
+  for (int i = 0; i < g; i++) {
+    int delta = g - i;
+    X[i] = X[g] + ((double)delta) * (X[g] - X[g + 1]);
+  }
+  for (int i = n + g; i < n + 2 * g; i++) {
+    int delta = i - (n + g - 1);
+    X[i] = X[n + g - 1] + ((double)delta) * (X[n + g - 1] - X[n + g - 2]);
+  }
+  for (int i = 0; i < n; i++) out[i + g] = 2.0 / (X[i + 1 + g] - X[i - 1 + g]);
+  for (int i = 0; i < g; i++) out[i] = out[g];
+  for (int i = n + g; i < n + 2 * g; i++) out[i] = out[n + g - 1];
+}
+
+hm_status dmalloc(double **p, size_t n) {
+  hipError_t e = hipMalloc((void **)p, n * sizeof(double));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(HM_ERR_NOMEM, "hipMalloc of " + std::to_string(n * sizeof(double)) + " bytes: " +
+                                  hipGetErrorString(e));
+  }
+  return HM_OK;
+}
+
+bool is_device_ptr(const void *p) {
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+hm_status check_state(const hm_solver *s, const void *p, const char *name) {
+  if (!s) return fail(HM_ERR_ARG, "solver is NULL");
+  if (!p) return fail(HM_ERR_ARG, std::string(name) + " is NULL");
+  if (!is_device_ptr(p)) return fail(HM_ERR_ARG, std::string(name) + " is not a device pointer");
+  return HM_OK;
+}
+
+// right-hand side pieces of the stage state U into fI / FV
+hm_status stage_fluxes(hm_solver *s, const double *U) {
+  for (int d = 0; d < 3; d++) hm::launch_iface(s->g, s->sc, d, U, s->fI[d], s->stream);
+  if (s->sc.Re > 0) hm::launch_visc_flux(s->g, s->sc, U, s->FV, s->dxg, s->stream);
+  return check_launch("interface/viscous flux kernels");
+}
+
+hm_status ensure_tmp(hm_solver *s, size_t n) {
+  if (s->tmp_elems >= n) return HM_OK;
+  if (s->tmp) (void)hipFree(s->tmp);
+  s->tmp = nullptr;
+  s->tmp_elems = 0;
+  hm_status st = dmalloc(&s->tmp, n);
+  if (st == HM_OK) s->tmp_elems = n;
+  return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *hm_last_error(void) { return g_err.c_str(); }
+const char *hm_version(void) { return "hypar-mi355x 0.1 (gfx950, fp64)"; }
+
+void hm_destroy(hm_solver *s) {
+  if (!s) return;
+  double *bufs[] = {s->dxg, s->fI[0], s->fI[1], s->fI[2], s->FV, s->S[0], s->S[1], s->acc, s->part, s->tmp};
+  for (double *b : bufs)
+    if (b) (void)hipFree(b);
+  if (s->part_host) (void)hipHostFree(s->part_host);
+  delete s;
+}
+
+hm_status hm_create(const hm_config *cfg, hm_solver **out) {
+  if (!cfg || !out) return fail(HM_ERR_ARG, "hm_create: NULL argument");
+  *out = nullptr;
+  for (int d = 0; d < 3; d++)
+    if (cfg->dim[d] < 5) return fail(HM_ERR_ARG, "hm_create: every grid dimension must be >= 5");
+  if (cfg->interp != HM_INTERP_COMPONENTS && cfg->interp != HM_INTERP_CHARACTERISTIC)
+    return fail(HM_ERR_UNSUPPORTED, "hm_create: unknown interpolation type");
+  if (cfg->upwind != HM_UPWIND_ROE && cfg->upwind != HM_UPWIND_RUSANOV)
+    return fail(HM_ERR_UNSUPPORTED, "hm_create: unknown upwinding scheme");
+  if (!(cfg->gamma > 1.0)) return fail(HM_ERR_ARG, "hm_create: gamma must exceed 1");
+  if (!(cfg->weno_eps > 0.0)) return fail(HM_ERR_ARG, "hm_create: weno_eps must be positive");
+  if (cfg->Re > 0 && !(cfg->Minf > 0 && cfg->Pr > 0))
+    return fail(HM_ERR_ARG, "hm_create: viscous runs need Minf > 0 and Pr > 0");
+
+  hm_solver *s = new hm_solver();
+  std::memset(s, 0, sizeof(*s));
+  for (int d = 0; d < 3; d++) s->g.n[d] = cfg->dim[d];
+  s->g.ncell = (size_t)cfg->dim[0] * cfg->dim[1] * cfg->dim[2];
+  s->sc.interp = cfg->interp == HM_INTERP_CHARACTERISTIC ? hm::INTERP_CHAR : hm::INTERP_COMP;
+  s->sc.upwind = cfg->upwind == HM_UPWIND_ROE ? hm::UPWIND_ROE : hm::UPWIND_RUSANOV;
+  s->sc.gamma = cfg->gamma;
+  s->sc.Re = cfg->Re > 0 ? cfg->Re / cfg->Minf : -1.0;
+  s->sc.Pr = cfg->Pr;
+  s->sc.eps = cfg->weno_eps;
+  if (hipGetDevice(&s->device) != hipSuccess) {
+    hm_destroy(s);
+    return fail(HM_ERR_HIP, "hm_create: no HIP device");
+  }
+  s->stream = nullptr;
+
+  const int g = 3;
+  size_t ndx = (size_t)(cfg->dim[0] + 2 * g) + (cfg->dim[1] + 2 * g) + (cfg->dim[2] + 2 * g);
+  std::vector<double> dxh(ndx);
+  if (cfg->dxinv) {
+    std::memcpy(dxh.data(), cfg->dxinv, ndx * sizeof(double));
+  } else {
+    size_t off = 0;
+    for (int d = 0; d < 3; d++) {
+      uniform_dxinv(cfg->dim[d], g, dxh.data() + off);
+      off += cfg->dim[d] + 2 * g;
+    }
+  }
+  const size_t n5 = 5 * s->g.ncell;
+  hm_status st = HM_OK;
+  if ((st = dmalloc(&s->dxg, ndx)) != HM_OK) goto bad;
+  for (int d = 0; d < 3; d++)
+    if ((st = dmalloc(&s->fI[d], n5)) != HM_OK) goto bad;
+  if (s->sc.Re > 0 && (st = dmalloc(&s->FV, 12 * s->g.ncell)) != HM_OK) goto bad;
+  if ((st = dmalloc(&s->S[0], n5)) != HM_OK) goto bad;
+  if ((st = dmalloc(&s->S[1], n5)) != HM_OK) goto bad;
+  if ((st = dmalloc(&s->acc, n5)) != HM_OK) goto bad;
+  s->npart = hm::reduction_blocks(n5);
+  if ((st = dmalloc(&s->part, s->npart)) != HM_OK) goto bad;
+  if (hipHostMalloc((void **)&s->part_host, s->npart * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    st = fail(HM_ERR_NOMEM, "hm_create: pinned host allocation failed");
+    goto bad;
+  }
+  if (hipMemcpy(s->dxg, dxh.data(), ndx * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+    st = fail(HM_ERR_HIP, "hm_create: copy of dxinv failed");
+    goto bad;
+  }
+  *out = s;
+  return HM_OK;
+bad:
+  hm_destroy(s);
+  return st;
+}
+
+hm_status hm_set_stream(hm_solver *s, void *stream) {
+  if (!s) return fail(HM_ERR_ARG, "hm_set_stream: solver is NULL");
+  s->stream = (hipStream_t)stream;
+  return HM_OK;
+}
+
+size_t hm_ncell(const hm_solver *s) { return s ? s->g.ncell : 0; }
+
+hm_status hm_upload(hm_solver *s, double *dev, const double *host, int layout) {
+  hm_status st = check_state(s, dev, "hm_upload: dev");
+  if (st != HM_OK) return st;
+  if (!host) return fail(HM_ERR_ARG, "hm_upload: host is NULL");
+  const size_t n5 = 5 * s->g.ncell;
+  if (layout == HM_LAYOUT_SOA_INTERIOR) {
+    HM_HIP(hipMemcpyAsync(dev, host, n5 * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  } else if (layout == HM_LAYOUT_AOS_INTERIOR || layout == HM_LAYOUT_REFERENCE) {
+    const int g = layout == HM_LAYOUT_REFERENCE ? 3 : 0;
+    const size_t n = 5 * (size_t)(s->g.n[0] + 2 * g) * (s->g.n[1] + 2 * g) * (s->g.n[2] + 2 * g);
+    if ((st = ensure_tmp(s, n)) != HM_OK) return st;
+    HM_HIP(hipMemcpyAsync(s->tmp, host, n * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    hm::launch_aos_to_soa(s->g, s->tmp, dev, g, s->stream);
+    if ((st = check_launch("hm_upload")) != HM_OK) return st;
+  } else {
+    return fail(HM_ERR_ARG, "hm_upload: unknown layout");
+  }
+  HM_HIP(hipStreamSynchronize(s->stream));
+  return HM_OK;
+}
+
+hm_status hm_download(hm_solver *s, double *host, const double *dev, int layout) {
+  hm_status st = check_state(s, dev, "hm_download: dev");
+  if (st != HM_OK) return st;
+  if (!host) return fail(HM_ERR_ARG, "hm_download: host is NULL");
+  const size_t n5 = 5 * s->g.ncell;
+  if (layout == HM_LAYOUT_SOA_INTERIOR) {
+    HM_HIP(hipMemcpyAsync(host, dev, n5 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  } else if (layout == HM_LAYOUT_AOS_INTERIOR || layout == HM_LAYOUT_REFERENCE) {
+    const int g = layout == HM_LAYOUT_REFERENCE ? 3 : 0;
+    const size_t n = 5 * (size_t)(s->g.n[0] + 2 * g) * (s->g.n[1] + 2 * g) * (s->g.n[2] + 2 * g);
+    if ((st = ensure_tmp(s, n)) != HM_OK) return st;
+    // ghost entries of the reference layout come back as zeros
+    HM_HIP(hipMemsetAsync(s->tmp, 0, n * sizeof(double), s->stream));
+    hm::launch_soa_to_aos(s->g, dev, s->tmp, g, s->stream);
+    if ((st = check_launch("hm_download")) != HM_OK) return st;
+    HM_HIP(hipMemcpyAsync(host, s->tmp, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  } else {
+    return fail(HM_ERR_ARG, "hm_download: unknown layout");
+  }
+  HM_HIP(hipStreamSynchronize(s->stream));
+  return HM_OK;
+}
+
+hm_status hm_rk4(hm_solver *s, double *u, double dt, int nsteps) {
+  hm_status st = check_state(s, u, "hm_rk4: u");
+  if (st != HM_OK) return st;
+  if (nsteps < 0) return fail(HM_ERR_ARG, "hm_rk4: nsteps < 0");
+  if (!std::isfinite(dt)) return fail(HM_ERR_ARG, "hm_rk4: dt is not finite");
+  // "rk 44" tableau (TimeExplicitRKInitialize.c:58-68)
+  const double A_sub[3] = {0.5, 0.5, 1.0};  // A[s+1][s]
+  const double b[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+  for (int it = 0; it < nsteps; it++) {
+    for (int stage = 0; stage < 4; stage++) {
+      const double *Ust = stage == 0 ? u : s->S[(stage - 1) % 2];
+      double *Snext = s->S[stage % 2];
+      if ((st = stage_fluxes(s, Ust)) != HM_OK) return st;
+      const double ca = stage < 3 ? dt * A_sub[stage] : 0.0;
+      const double cb = dt * b[stage];
+      hm::launch_stage(s->g, s->sc, s->fI[0], s->fI[1], s->fI[2], Ust, s->FV, u, Snext, s->acc,
+                       nullptr, hm::OUT_STAGE, s->dxg, stage, ca, cb, s->stream);
+      if ((st = check_launch("hm_rk4 stage")) != HM_OK) return st;
+    }
+  }
+  return HM_OK;
+}
+
+static hm_status rhs_part(hm_solver *s, double *out, const double *u, int mode, const char *name) {
+  hm_status st = check_state(s, u, name);
+  if (st != HM_OK) return st;
+  if ((st = check_state(s, out, name)) != HM_OK) return st;
+  if ((st = stage_fluxes(s, u)) != HM_OK) return st;
+  hm::launch_stage(s->g, s->sc, s->fI[0], s->fI[1], s->fI[2], u, s->FV, nullptr, nullptr, nullptr, out,
+                   mode, s->dxg, 0, 0.0, 0.0, s->stream);
+  return check_launch(name);
+}
+
+hm_status hm_rhs(hm_solver *s, double *rhs, const double *u) {
+  return rhs_part(s, rhs, u, hm::OUT_RHS, "hm_rhs");
+}
+hm_status hm_hyperbolic(hm_solver *s, double *hyp, const double *u) {
+  return rhs_part(s, hyp, u, hm::OUT_HYP, "hm_hyperbolic");
+}
+hm_status hm_parabolic(hm_solver *s, double *par, const double *u) {
+  return rhs_part(s, par, u, hm::OUT_PAR, "hm_parabolic");
+}
+
+hm_status hm_cfl(hm_solver *s, const double *u, double dt, double *cfl) {
+  hm_status st = check_state(s, u, "hm_cfl: u");
+  if (st != HM_OK) return st;
+  if (!cfl) return fail(HM_ERR_ARG, "hm_cfl: cfl is NULL");
+  unsigned nb = hm::reduction_blocks(s->g.ncell);
+  hm::launch_cfl(s->g, s->sc, u, s->part, nb, s->dxg, dt, s->stream);
+  if ((st = check_launch("hm_cfl")) != HM_OK) return st;
+  HM_HIP(hipMemcpyAsync(s->part_host, s->part, nb * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HM_HIP(hipStreamSynchronize(s->stream));
+  double m = 0.0;
+  for (unsigned i = 0; i < nb; i++) m = s->part_host[i] > m ? s->part_host[i] : m;
+  *cfl = m;
+  if (!std::isfinite(m)) return fail(HM_ERR_NONFINITE, "hm_cfl: state holds NaN/Inf");
+  return HM_OK;
+}
+
+hm_status hm_step_norm(hm_solver *s, const double *a, const double *b, double *norm) {
+  hm_status st = check_state(s, a, "hm_step_norm: a");
+  if (st != HM_OK) return st;
+  if ((st = check_state(s, b, "hm_step_norm: b")) != HM_OK) return st;
+  if (!norm) return fail(HM_ERR_ARG, "hm_step_norm: norm is NULL");
+  const size_t n5 = 5 * s->g.ncell;
+  unsigned nb = hm::reduction_blocks(n5);
+  hm::launch_sqdiff(a, b, s->part, nb, n5, s->stream);
+  if ((st = check_launch("hm_step_norm")) != HM_OK) return st;
+  HM_HIP(hipMemcpyAsync(s->part_host, s->part, nb * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HM_HIP(hipStreamSynchronize(s->stream));
+  double sum = 0.0;
+  for (unsigned i = 0; i < nb; i++) sum += s->part_host[i];
+  *norm = std::sqrt(sum / (double)s->g.ncell);
+  if (!std::isfinite(*norm)) return fail(HM_ERR_NONFINITE, "hm_step_norm: NaN/Inf");
+  return HM_OK;
+}
+
+hm_status hm_profile_kernels(hm_solver *s, const double *u, int reps, double *ms) {
+  hm_status st = check_state(s, u, "hm_profile_kernels: u");
+  if (st != HM_OK) return st;
+  if (!ms || reps < 1) return fail(HM_ERR_ARG, "hm_profile_kernels: bad arguments");
+  hipEvent_t e0, e1;
+  HM_HIP(hipEventCreate(&e0));
+  HM_HIP(hipEventCreate(&e1));
+  auto timed = [&](int slot, auto &&launch) -> hm_status {
+    launch();  // warm
+    if (hipEventRecord(e0, s->stream) != hipSuccess) return fail(HM_ERR_HIP, "event record");
+    for (int r = 0; r < reps; r++) launch();
+    if (hipEventRecord(e1, s->stream) != hipSuccess) return fail(HM_ERR_HIP, "event record");
+    if (hipEventSynchronize(e1) != hipSuccess) return fail(HM_ERR_HIP, "event sync");
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, e0, e1) != hipSuccess) return fail(HM_ERR_HIP, "elapsed time");
+    ms[slot] = (double)t / reps;
+    return check_launch("hm_profile_kernels");
+  };
+  for (int d = 0; d < 3 && st == HM_OK; d++)
+    st = timed(d, [&] { hm::launch_iface(s->g, s->sc, d, u, s->fI[d], s->stream); });
+  if (st == HM_OK) {
+    if (s->sc.Re > 0)
+      st = timed(3, [&] { hm::launch_visc_flux(s->g, s->sc, u, s->FV, s->dxg, s->stream); });
+    else
+      ms[3] = 0.0;
+  }
+  if (st == HM_OK)
+    st = timed(4, [&] {
+      hm::launch_stage(s->g, s->sc, s->fI[0], s->fI[1], s->fI[2], u, s->FV, nullptr, nullptr, nullptr,
+                       s->S[1], hm::OUT_RHS, s->dxg, 0, 0.0, 0.0, s->stream);
+    });
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return st;
+}
+
+}  // extern "C"

@@ -1,0 +1,187 @@
+// hm_device.h -- device math of the NS3D WENO5 path for gfx950 (FP64).
+//
+// Physics follows include/physicalmodels/navierstokes3d.h:98-470 of the
+// reference (flow variables, Euler flux, Roe average, eigenvalues and
+// eigenvectors of the flux Jacobian).  WENO5 follows the Jiang-Shu
+// formulation of include/interpolation.h:265-285 and
+// src/InterpolationFunctions/Interp1PrimFifthOrderWENO.c:146-158.
+//
+// Restatements (equal in exact arithmetic, rounding-level differences only):
+//  * The Jiang-Shu weights are applied as a single ratio
+//      sum_k a_k f_k / sum_k a_k,  a_k = c_k * prod_{j!=k} (b_j+eps)^2,
+//    which is w_k = (c_k/(b_k+eps)^2) / sum_j (c_j/(b_j+eps)^2) multiplied
+//    through by prod_j (b_j+eps)^2: one division per interpolant instead of
+//    four.
+//  * Eigenvector entries divided by a^2 or 2a^2 use one reciprocal.
+//  * Characteristic upwinding uses L*R = I: with uL = R cL and uR = R cR,
+//    R|Lambda|L (uR-uL)/2 = R |Lambda| (cR-cL)/2, so the interface flux is
+//    R (0.5 (fcL + fcR) - 0.5 |lambda| (ucR - ucL)).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace hm {
+
+constexpr int NV = 5;
+
+struct Prim {
+  double rho, vx, vy, vz, e, P;
+};
+
+// _NavierStokes3DGetFlowVar_ (navierstokes3d.h:98-108)
+__device__ __forceinline__ Prim flowvar(const double *u, double gamma) {
+  Prim q;
+  q.rho = u[0];
+  q.vx = (q.rho == 0) ? 0.0 : u[1] / q.rho;
+  q.vy = (q.rho == 0) ? 0.0 : u[2] / q.rho;
+  q.vz = (q.rho == 0) ? 0.0 : u[3] / q.rho;
+  q.e = u[4];
+  double vsq = q.vx * q.vx + q.vy * q.vy + q.vz * q.vz;
+  q.P = (q.e - 0.5 * q.rho * vsq) * (gamma - 1.0);
+  return q;
+}
+
+template <int DIR> __device__ __forceinline__ double vel(const Prim &q) {
+  return DIR == 0 ? q.vx : (DIR == 1 ? q.vy : q.vz);
+}
+
+// _NavierStokes3DSetFlux_ (navierstokes3d.h:118-139)
+template <int DIR> __device__ __forceinline__ void euler_flux(double f[NV], const Prim &q) {
+  const double vn = vel<DIR>(q);
+  const double m = q.rho * vn;
+  f[0] = m;
+  f[1] = m * q.vx;
+  f[2] = m * q.vy;
+  f[3] = m * q.vz;
+  f[DIR + 1] += q.P;
+  f[4] = (q.e + q.P) * vn;
+}
+
+// energy of the modified solution uC (NavierStokes3DModifiedSolution.c:65
+// with the gravity fields f = g = 1): P/(gamma-1) + rho|v|^2/2
+__device__ __forceinline__ double modified_energy(const Prim &q, double inv_gm1) {
+  return (q.P * inv_gm1) + (0.5 * q.rho * (q.vx * q.vx + q.vy * q.vy + q.vz * q.vz));
+}
+
+// _NavierStokes3DRoeAverage_ (navierstokes3d.h:144-169), returned as
+// primitive variables of the averaged state.
+__device__ __forceinline__ Prim roe_average(const double *uL, const double *uR, double gamma) {
+  const double gm1 = gamma - 1.0;
+  Prim l = flowvar(uL, gamma), r = flowvar(uR, gamma);
+  double HL = 0.5 * (l.vx * l.vx + l.vy * l.vy + l.vz * l.vz) + (gamma * l.P / l.rho) / gm1;
+  double HR = 0.5 * (r.vx * r.vx + r.vy * r.vy + r.vz * r.vz) + (gamma * r.P / r.rho) / gm1;
+  double tL = sqrt(l.rho), tR = sqrt(r.rho);
+  double inv = 1.0 / (tL + tR);
+  Prim a;
+  a.rho = tL * tR;
+  a.vx = (tL * l.vx + tR * r.vx) * inv;
+  a.vy = (tL * l.vy + tR * r.vy) * inv;
+  a.vz = (tL * l.vz + tR * r.vz) * inv;
+  double H = (tL * HL + tR * HR) * inv;
+  double vsq = a.vx * a.vx + a.vy * a.vy + a.vz * a.vz;
+  a.P = (H - 0.5 * vsq) * (a.rho * gm1) / gamma;
+  a.e = a.P / gm1 + 0.5 * a.rho * vsq;
+  return a;
+}
+
+// Eigen-decomposition of the x/y/z flux Jacobian at a state given by its
+// primitive variables (navierstokes3d.h:254-470).  Row r of L and column r
+// of R belong to eigenvalue lam[r]; the row order is the reference's.
+struct Eigen {
+  double L[NV][NV];
+  double R[NV][NV];  // R[row][col]
+  double lam[NV];
+};
+
+template <int DIR>
+__device__ __forceinline__ void eigen(Eigen &E, const Prim &q, double gamma) {
+  const double gm1 = gamma - 1.0;
+  const double vx = q.vx, vy = q.vy, vz = q.vz;
+  const double ek = 0.5 * (vx * vx + vy * vy + vz * vz);
+  const double a = sqrt(gamma * q.P / q.rho);
+  const double ia2 = 1.0 / (a * a);
+  const double ih2 = 0.5 * ia2;
+  const double h0 = a * a / gm1 + ek;
+  const double vn = DIR == 0 ? vx : (DIR == 1 ? vy : vz);
+  const int m = DIR + 1;  // row/column of the u-a wave
+  // entropy wave: row 0
+  E.L[0][0] = (a * a - gm1 * ek) * ia2;
+  E.L[0][1] = (gm1 * vx) * ia2;
+  E.L[0][2] = (gm1 * vy) * ia2;
+  E.L[0][3] = (gm1 * vz) * ia2;
+  E.L[0][4] = (-gm1) * ia2;
+  // u-a (row m) and u+a (row 4)
+  E.L[m][0] = (gm1 * ek + a * vn) * ih2;
+  E.L[4][0] = (gm1 * ek - a * vn) * ih2;
+  E.L[m][1] = ((-gm1) * vx) * ih2;
+  E.L[m][2] = ((-gm1) * vy) * ih2;
+  E.L[m][3] = ((-gm1) * vz) * ih2;
+  E.L[4][1] = E.L[m][1];
+  E.L[4][2] = E.L[m][2];
+  E.L[4][3] = E.L[m][3];
+  E.L[m][DIR + 1] = ((-gm1) * vn - a) * ih2;
+  E.L[4][DIR + 1] = ((-gm1) * vn + a) * ih2;
+  E.L[m][4] = gm1 * ih2;
+  E.L[4][4] = gm1 * ih2;
+  // R columns 0, m, 4
+  E.R[0][0] = 1.0; E.R[1][0] = vx; E.R[2][0] = vy; E.R[3][0] = vz; E.R[4][0] = ek;
+  E.R[0][m] = 1.0; E.R[1][m] = vx; E.R[2][m] = vy; E.R[3][m] = vz; E.R[4][m] = h0 - a * vn;
+  E.R[0][4] = 1.0; E.R[1][4] = vx; E.R[2][4] = vy; E.R[3][4] = vz; E.R[4][4] = h0 + a * vn;
+  E.R[DIR + 1][m] = vn - a;
+  E.R[DIR + 1][4] = vn + a;
+  // shear waves
+  if (DIR == 0) {
+    E.L[2][0] = vy;  E.L[2][1] = 0; E.L[2][2] = -1; E.L[2][3] = 0; E.L[2][4] = 0;
+    E.L[3][0] = -vz; E.L[3][1] = 0; E.L[3][2] = 0;  E.L[3][3] = 1; E.L[3][4] = 0;
+    E.R[0][2] = 0; E.R[1][2] = 0; E.R[2][2] = -1; E.R[3][2] = 0; E.R[4][2] = -vy;
+    E.R[0][3] = 0; E.R[1][3] = 0; E.R[2][3] = 0;  E.R[3][3] = 1; E.R[4][3] = vz;
+  } else if (DIR == 1) {
+    E.L[1][0] = -vx; E.L[1][1] = 1; E.L[1][2] = 0; E.L[1][3] = 0;  E.L[1][4] = 0;
+    E.L[3][0] = vz;  E.L[3][1] = 0; E.L[3][2] = 0; E.L[3][3] = -1; E.L[3][4] = 0;
+    E.R[0][1] = 0; E.R[1][1] = 1; E.R[2][1] = 0; E.R[3][1] = 0;  E.R[4][1] = vx;
+    E.R[0][3] = 0; E.R[1][3] = 0; E.R[2][3] = 0; E.R[3][3] = -1; E.R[4][3] = -vz;
+  } else {
+    E.L[1][0] = vx;  E.L[1][1] = -1; E.L[1][2] = 0; E.L[1][3] = 0; E.L[1][4] = 0;
+    E.L[2][0] = -vy; E.L[2][1] = 0;  E.L[2][2] = 1; E.L[2][3] = 0; E.L[2][4] = 0;
+    E.R[0][1] = 0; E.R[1][1] = -1; E.R[2][1] = 0; E.R[3][1] = 0; E.R[4][1] = -vx;
+    E.R[0][2] = 0; E.R[1][2] = 0;  E.R[2][2] = 1; E.R[3][2] = 0; E.R[4][2] = vy;
+  }
+  for (int r = 0; r < NV; r++) E.lam[r] = vn;
+  E.lam[m] = vn - a;
+  E.lam[4] = vn + a;
+}
+
+// |lambda| with Harten's entropy fix, delta = 1e-6
+// (NavierStokes3DUpwind.c:95-101)
+__device__ __forceinline__ double harten_abs(double l) {
+  const double delta = 0.000001, delta2 = delta * delta;
+  const double al = l < 0 ? -l : l;
+  return al < delta ? (l * l + delta2) / (2 * delta) : al;
+}
+
+// WENO5 interpolant with Jiang-Shu weights.  (m3,m2,m1,p1,p2) are the stencil
+// values the smoothness indicators use, (g3,g2,g1,q1,q2) the values that are
+// interpolated (they differ only for the solution, whose weights come from u
+// but whose interpolant uses the modified solution uC).
+__device__ __forceinline__ double weno5(double m3, double m2, double m1, double p1, double p2,
+                                        double g3, double g2, double g1, double q1, double q2,
+                                        double eps) {
+  const double c1312 = 13.0 / 12.0, c14 = 1.0 / 4.0, s6 = 1.0 / 6.0;
+  double t, r;
+  t = m3 - 2 * m2 + m1; r = m3 - 4 * m2 + 3 * m1;
+  double b1 = c1312 * t * t + c14 * r * r;
+  t = m2 - 2 * m1 + p1; r = m2 - p1;
+  double b2 = c1312 * t * t + c14 * r * r;
+  t = m1 - 2 * p1 + p2; r = 3 * m1 - 4 * p1 + p2;
+  double b3 = c1312 * t * t + c14 * r * r;
+  double s1 = (b1 + eps) * (b1 + eps);
+  double s2 = (b2 + eps) * (b2 + eps);
+  double s3 = (b3 + eps) * (b3 + eps);
+  double a1 = 0.1 * (s2 * s3), a2 = 0.6 * (s1 * s3), a3 = 0.3 * (s1 * s2);
+  double f1 = (2 * s6) * g3 + (-7 * s6) * g2 + (11 * s6) * g1;
+  double f2 = (-s6) * g2 + (5 * s6) * g1 + (2 * s6) * q1;
+  double f3 = (2 * s6) * g1 + (5 * s6) * q1 + (-s6) * q2;
+  return (a1 * f1 + a2 * f2 + a3 * f3) / (a1 + a2 + a3);
+}
+
+}  // namespace hm

@@ -1,0 +1,46 @@
+// hm_internal.h -- declarations shared by the C-ABI and the kernel launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+namespace hm {
+
+enum { INTERP_COMP = 0, INTERP_CHAR = 1 };
+enum { UPWIND_ROE = 0, UPWIND_RUSANOV = 1 };
+enum { OUT_STAGE = 0, OUT_RHS = 1, OUT_HYP = 2, OUT_PAR = 3 };
+
+struct Grid {
+  int n[3];
+  size_t ncell;
+};
+
+struct Scheme {
+  int interp, upwind;
+  double gamma;
+  double Re;  // as the solver uses it (physics.inp Re / Minf); <= 0 inviscid
+  double Pr;
+  double eps;
+};
+
+// interface fluxes along dir of the hm-layout state u into fI (5*ncell)
+void launch_iface(const Grid &g, const Scheme &sc, int dir, const double *u, double *fI,
+                  hipStream_t st);
+// viscous fluxes (12*ncell) of state u
+void launch_visc_flux(const Grid &g, const Scheme &sc, const double *u, double *FV,
+                      const double *dxg, hipStream_t st);
+// flux differences, viscous divergence and RK4 stage update (mode OUT_STAGE),
+// or the right-hand side / hyperbolic / parabolic term into `out`
+void launch_stage(const Grid &g, const Scheme &sc, const double *fIx, const double *fIy,
+                  const double *fIz, const double *Ustage, const double *FV, double *u, double *S,
+                  double *acc, double *out, int mode, const double *dxg, int stage, double ca,
+                  double cb, hipStream_t st);
+unsigned reduction_blocks(size_t n);
+void launch_cfl(const Grid &g, const Scheme &sc, const double *u, double *part, unsigned nb,
+                const double *dxg, double dt, hipStream_t st);
+void launch_sqdiff(const double *a, const double *b, double *part, unsigned nb, size_t n,
+                   hipStream_t st);
+void launch_aos_to_soa(const Grid &g, const double *aos, double *soa, int ghosts, hipStream_t st);
+void launch_soa_to_aos(const Grid &g, const double *soa, double *aos, int ghosts, hipStream_t st);
+
+}  // namespace hm

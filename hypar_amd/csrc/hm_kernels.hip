@@ -1,0 +1,565 @@
+// hm_kernels.hip -- gfx950 kernels of the NS3D WENO5 + RK4 path.
+//
+// State layout: structure of arrays without ghosts (see include/hypar_mi355x.h).
+// All boundaries are periodic; stencil indices wrap.
+//
+// Kernels per RK stage:
+//   k_iface<DIR,...>   interface fluxes F_{i-1/2} along DIR for every cell i
+//                      (Roe/Rusanov upwinding of WENO5 reconstructions);
+//                      HyperbolicFunction.c:75-93 with ReconstructHyperbolic
+//   k_visc_flux        viscous fluxes of x, y, z at every cell
+//                      (NavierStokes3DParabolicFunction.c:85-195)
+//   k_stage            flux differences in x, y, z, the viscous divergence,
+//                      rhs = -hyp + par, and the RK4 stage update in 3N form
+//                      (TimeRHSFunctionExplicit.c:88-93, TimeRK.c:126-195)
+#include <hip/hip_runtime.h>
+
+#include "hm_device.h"
+#include "hm_internal.h"
+
+namespace hm {
+
+__device__ __forceinline__ int wrap(int i, int n) {
+  // i in [-n, 2n)
+  return i < 0 ? i + n : (i >= n ? i - n : i);
+}
+
+// ---------------------------------------------------------------------------
+// interface flux kernel
+// ---------------------------------------------------------------------------
+template <int DIR, int INTERP, int UPW>
+__device__ __forceinline__ void iface_flux(const double (&U)[6][NV], double gamma, double eps,
+                                           double fI[NV]) {
+  const double inv_gm1 = 1.0 / (gamma - 1.0);
+  double F[6][NV];
+  double UC4[6];
+  for (int s = 0; s < 6; s++) {
+    Prim q = flowvar(U[s], gamma);
+    euler_flux<DIR>(F[s], q);
+    UC4[s] = modified_energy(q, inv_gm1);
+  }
+  // upwinding states are the cells adjacent to the interface (I-1, I)
+  Prim avg = roe_average(U[2], U[3], gamma);
+
+  if (INTERP == INTERP_CHAR) {
+    Eigen E;
+    eigen<DIR>(E, avg, gamma);
+    double alpha = 0.0;
+    if (UPW == UPWIND_RUSANOV) {
+      Prim l = flowvar(U[2], gamma), r = flowvar(U[3], gamma);
+      double aL = sqrt(gamma * l.P / l.rho) + fabs(vel<DIR>(l));
+      double aR = sqrt(gamma * r.P / r.rho) + fabs(vel<DIR>(r));
+      double aA = sqrt(gamma * avg.P / avg.rho) + fabs(vel<DIR>(avg));
+      alpha = fmax(fmax(aL, aR), fmax(aR, aA));
+    }
+    double g[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      double pf[6], pu[6], pc[6];
+#pragma unroll
+      for (int s = 0; s < 6; s++) {
+        double af = 0.0, au = 0.0;
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+          af += E.L[v][k] * F[s][k];
+          au += E.L[v][k] * U[s][k];
+        }
+        pf[s] = af;
+        pu[s] = au;
+        pc[s] = au + E.L[v][4] * (UC4[s] - U[s][4]);
+      }
+      double fcL = weno5(pf[0], pf[1], pf[2], pf[3], pf[4], pf[0], pf[1], pf[2], pf[3], pf[4], eps);
+      double fcR = weno5(pf[5], pf[4], pf[3], pf[2], pf[1], pf[5], pf[4], pf[3], pf[2], pf[1], eps);
+      double ucL = weno5(pu[0], pu[1], pu[2], pu[3], pu[4], pc[0], pc[1], pc[2], pc[3], pc[4], eps);
+      double ucR = weno5(pu[5], pu[4], pu[3], pu[2], pu[1], pc[5], pc[4], pc[3], pc[2], pc[1], eps);
+      double lam = (UPW == UPWIND_ROE) ? harten_abs(E.lam[v]) : alpha;
+      g[v] = 0.5 * (fcL + fcR) - lam * (0.5 * (ucR - ucL));
+    }
+#pragma unroll
+    for (int r = 0; r < NV; r++) {
+      double acc = 0.0;
+#pragma unroll
+      for (int c = 0; c < NV; c++) acc += E.R[r][c] * g[c];
+      fI[r] = acc;
+    }
+  } else {
+    double fL[NV], fR[NV], uL[NV], uR[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      double c0 = (v == 4) ? UC4[0] : U[0][v], c1 = (v == 4) ? UC4[1] : U[1][v];
+      double c2 = (v == 4) ? UC4[2] : U[2][v], c3 = (v == 4) ? UC4[3] : U[3][v];
+      double c4 = (v == 4) ? UC4[4] : U[4][v], c5 = (v == 4) ? UC4[5] : U[5][v];
+      fL[v] = weno5(F[0][v], F[1][v], F[2][v], F[3][v], F[4][v], F[0][v], F[1][v], F[2][v], F[3][v], F[4][v], eps);
+      fR[v] = weno5(F[5][v], F[4][v], F[3][v], F[2][v], F[1][v], F[5][v], F[4][v], F[3][v], F[2][v], F[1][v], eps);
+      uL[v] = weno5(U[0][v], U[1][v], U[2][v], U[3][v], U[4][v], c0, c1, c2, c3, c4, eps);
+      uR[v] = weno5(U[5][v], U[4][v], U[3][v], U[2][v], U[1][v], c5, c4, c3, c2, c1, eps);
+    }
+    if (UPW == UPWIND_RUSANOV) {
+      Prim l = flowvar(U[2], gamma), r = flowvar(U[3], gamma);
+      double aL = sqrt(gamma * l.P / l.rho) + fabs(vel<DIR>(l));
+      double aR = sqrt(gamma * r.P / r.rho) + fabs(vel<DIR>(r));
+      double aA = sqrt(gamma * avg.P / avg.rho) + fabs(vel<DIR>(avg));
+      double alpha = fmax(fmax(aL, aR), fmax(aR, aA));
+#pragma unroll
+      for (int v = 0; v < NV; v++) fI[v] = 0.5 * (fL[v] + fR[v]) - alpha * (0.5 * (uR[v] - uL[v]));
+    } else {
+      Eigen E;
+      eigen<DIR>(E, avg, gamma);
+      double c[NV];
+#pragma unroll
+      for (int r = 0; r < NV; r++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < NV; k++) acc += E.L[r][k] * (0.5 * (uR[k] - uL[k]));
+        c[r] = harten_abs(E.lam[r]) * acc;
+      }
+#pragma unroll
+      for (int r = 0; r < NV; r++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < NV; k++) acc += E.R[r][k] * c[k];
+        fI[r] = 0.5 * (fL[r] + fR[r]) - acc;
+      }
+    }
+  }
+}
+
+// one thread per interface i-1/2 (stored at the index of cell i)
+template <int DIR, int INTERP, int UPW>
+__global__ void __launch_bounds__(256) k_iface(const double *__restrict__ u, double *__restrict__ fI,
+                                               int nx, int ny, int nz, double gamma, double eps) {
+  const size_t ncell = (size_t)nx * ny * nz;
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ncell) return;
+  const int i = (int)(idx % nx);
+  const int j = (int)((idx / nx) % ny);
+  const int k = (int)(idx / ((size_t)nx * ny));
+  const int n = DIR == 0 ? nx : (DIR == 1 ? ny : nz);
+  const int pos = DIR == 0 ? i : (DIR == 1 ? j : k);
+  const size_t stride = DIR == 0 ? 1 : (DIR == 1 ? (size_t)nx : (size_t)nx * ny);
+  const size_t base = idx - (size_t)pos * stride;
+  double U[6][NV];
+#pragma unroll
+  for (int s = 0; s < 6; s++) {
+    const size_t c = base + (size_t)wrap(pos + s - 3, n) * stride;
+#pragma unroll
+    for (int v = 0; v < NV; v++) U[s][v] = u[(size_t)v * ncell + c];
+  }
+  double f[NV];
+  iface_flux<DIR, INTERP, UPW>(U, gamma, eps, f);
+#pragma unroll
+  for (int v = 0; v < NV; v++) fI[(size_t)v * ncell + idx] = f[v];
+}
+
+// ---------------------------------------------------------------------------
+// viscous terms
+// ---------------------------------------------------------------------------
+// Primitive variables used by the viscous term: (u, v, w, T) with
+// T = gamma*P/rho (NavierStokes3DParabolicFunction.c:85-108).
+struct QT {
+  double u, v, w, T;
+};
+
+__device__ __forceinline__ QT load_q(const double *__restrict__ U, size_t ncell, size_t c,
+                                     double gamma) {
+  double s[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) s[v] = U[(size_t)v * ncell + c];
+  Prim p = flowvar(s, gamma);
+  QT q;
+  q.u = p.vx; q.v = p.vy; q.w = p.vz;
+  q.T = gamma * p.P / p.rho;
+  return q;
+}
+
+// 4th-order central difference (FirstDerivativeFourthOrder.c:103)
+__device__ __forceinline__ double d4c(double fm2, double fm1, double fp1, double fp2) {
+  return (fm2 - 8 * fm1 + 8 * fp1 - fp2) * (1.0 / 12.0);
+}
+
+struct ViscParams {
+  double gamma, inv_Re, inv_gm1, inv_Pr;
+};
+
+// Viscous flux along DIR at a point from the primitive values and the
+// gradient components (derivatives already scaled by dxinv).
+// grad[c][e] = d(q_c)/dx_e for c in (u,v,w,T).
+template <int DIR>
+__device__ __forceinline__ void visc_flux(double fv[4], const QT &q, const double (&gr)[4][3],
+                                          double mu, const ViscParams &vp) {
+  const double two_third = 2.0 / 3.0;
+  const double mr = mu * vp.inv_Re;
+  const double kq = mu * vp.inv_Re * vp.inv_gm1 * vp.inv_Pr;
+  const double ux = gr[0][0], uy = gr[0][1], uz = gr[0][2];
+  const double vx = gr[1][0], vy = gr[1][1], vz = gr[1][2];
+  const double wx = gr[2][0], wy = gr[2][1], wz = gr[2][2];
+  double t1, t2, t3, qd;
+  if (DIR == 0) {
+    t1 = two_third * mr * (2 * ux - vy - wz);
+    t2 = mr * (uy + vx);
+    t3 = mr * (uz + wx);
+    qd = kq * gr[3][0];
+  } else if (DIR == 1) {
+    t1 = mr * (uy + vx);
+    t2 = two_third * mr * (-ux + 2 * vy - wz);
+    t3 = mr * (vz + wy);
+    qd = kq * gr[3][1];
+  } else {
+    t1 = mr * (uz + wx);
+    t2 = mr * (vz + wy);
+    t3 = two_third * mr * (-ux - vy + 2 * wz);
+    qd = kq * gr[3][2];
+  }
+  fv[0] = t1; fv[1] = t2; fv[2] = t3;
+  fv[3] = q.u * t1 + q.v * t2 + q.w * t3 + qd;
+}
+
+// Viscous fluxes of x, y, z at every cell: FV[(d*4+c)*ncell + cell],
+// c = momentum x, y, z, energy.  Gradients use the 4th-order central
+// difference with periodic neighbours, scaled by the cell's dxinv.
+__global__ void __launch_bounds__(256) k_visc_flux(const double *__restrict__ u, double *__restrict__ FV,
+                                                   int nx, int ny, int nz,
+                                                   const double *__restrict__ dxg,  // ghosted dxinv x|y|z
+                                                   ViscParams vp) {
+  const size_t ncell = (size_t)nx * ny * nz;
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ncell) return;
+  const int i = (int)(idx % nx);
+  const int j = (int)((idx / nx) % ny);
+  const int k = (int)(idx / ((size_t)nx * ny));
+  const int n[3] = {nx, ny, nz};
+  const int pos[3] = {i, j, k};
+  const size_t st[3] = {1, (size_t)nx, (size_t)nx * ny};
+  const double *dxi[3] = {dxg + 3, dxg + (nx + 6) + 3, dxg + (nx + 6) + (ny + 6) + 3};
+  QT q0 = load_q(u, ncell, idx, vp.gamma);
+  double gr[4][3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const size_t base = idx - (size_t)pos[d] * st[d];
+    QT qm2 = load_q(u, ncell, base + (size_t)wrap(pos[d] - 2, n[d]) * st[d], vp.gamma);
+    QT qm1 = load_q(u, ncell, base + (size_t)wrap(pos[d] - 1, n[d]) * st[d], vp.gamma);
+    QT qp1 = load_q(u, ncell, base + (size_t)wrap(pos[d] + 1, n[d]) * st[d], vp.gamma);
+    QT qp2 = load_q(u, ncell, base + (size_t)wrap(pos[d] + 2, n[d]) * st[d], vp.gamma);
+    const double s = dxi[d][pos[d]];
+    gr[0][d] = d4c(qm2.u, qm1.u, qp1.u, qp2.u) * s;
+    gr[1][d] = d4c(qm2.v, qm1.v, qp1.v, qp2.v) * s;
+    gr[2][d] = d4c(qm2.w, qm1.w, qp1.w, qp2.w) * s;
+    gr[3][d] = d4c(qm2.T, qm1.T, qp1.T, qp2.T) * s;
+  }
+  const double mu = exp(0.76 * log(q0.T));
+  double fv[4];
+  visc_flux<0>(fv, q0, gr, mu, vp);
+  for (int c = 0; c < 4; c++) FV[(size_t)(0 * 4 + c) * ncell + idx] = fv[c];
+  visc_flux<1>(fv, q0, gr, mu, vp);
+  for (int c = 0; c < 4; c++) FV[(size_t)(1 * 4 + c) * ncell + idx] = fv[c];
+  visc_flux<2>(fv, q0, gr, mu, vp);
+  for (int c = 0; c < 4; c++) FV[(size_t)(2 * 4 + c) * ncell + idx] = fv[c];
+}
+
+// Viscous flux along DIR at a ghost position gp in {-2,-1,n,n+1} of the line
+// through cell `base` (other indices interior).  The reference evaluates it
+// from its ghost layers: primitive values are the periodic copies, the
+// derivative along DIR uses the biased stencils of
+// FirstDerivativeFourthOrder.c:74-93/106-125 in the second-outermost layer
+// and the central one otherwise, scaled by the ghost dxinv (a copy of the
+// nearest interior value, InitialSolution.c:101-113), and the derivatives
+// across DIR are zero because the reference computes them only on lines whose
+// other indices are interior.
+template <int DIR>
+__device__ void visc_flux_ghost(double fv[4], const double *__restrict__ u, size_t ncell,
+                                size_t base, size_t stride, int n, int gp, double dx_ghost,
+                                const ViscParams &vp) {
+  QT f[5];  // positions gp-2 .. gp+2 (periodic copies)
+#pragma unroll
+  for (int s = 0; s < 5; s++) f[s] = load_q(u, ncell, base + (size_t)wrap(gp - 2 + s, n) * stride, vp.gamma);
+  double d[4];
+  if (gp == -2) {
+    // (-3 f(-3) - 10 f(-2) + 18 f(-1) - 6 f(0) + f(1)) / 12
+    QT f5 = load_q(u, ncell, base + (size_t)wrap(gp + 3, n) * stride, vp.gamma);
+    d[0] = (-3 * f[1].u - 10 * f[2].u + 18 * f[3].u - 6 * f[4].u + f5.u) * (1.0 / 12.0);
+    d[1] = (-3 * f[1].v - 10 * f[2].v + 18 * f[3].v - 6 * f[4].v + f5.v) * (1.0 / 12.0);
+    d[2] = (-3 * f[1].w - 10 * f[2].w + 18 * f[3].w - 6 * f[4].w + f5.w) * (1.0 / 12.0);
+    d[3] = (-3 * f[1].T - 10 * f[2].T + 18 * f[3].T - 6 * f[4].T + f5.T) * (1.0 / 12.0);
+  } else if (gp == n + 1) {
+    // (-f(n-2) + 6 f(n-1) - 18 f(n) + 10 f(n+1) + 3 f(n+2)) / 12
+    QT fm3 = load_q(u, ncell, base + (size_t)wrap(gp - 3, n) * stride, vp.gamma);
+    d[0] = (-fm3.u + 6 * f[0].u - 18 * f[1].u + 10 * f[2].u + 3 * f[3].u) * (1.0 / 12.0);
+    d[1] = (-fm3.v + 6 * f[0].v - 18 * f[1].v + 10 * f[2].v + 3 * f[3].v) * (1.0 / 12.0);
+    d[2] = (-fm3.w + 6 * f[0].w - 18 * f[1].w + 10 * f[2].w + 3 * f[3].w) * (1.0 / 12.0);
+    d[3] = (-fm3.T + 6 * f[0].T - 18 * f[1].T + 10 * f[2].T + 3 * f[3].T) * (1.0 / 12.0);
+  } else {
+    d[0] = d4c(f[0].u, f[1].u, f[3].u, f[4].u);
+    d[1] = d4c(f[0].v, f[1].v, f[3].v, f[4].v);
+    d[2] = d4c(f[0].w, f[1].w, f[3].w, f[4].w);
+    d[3] = d4c(f[0].T, f[1].T, f[3].T, f[4].T);
+  }
+  double gr[4][3];
+#pragma unroll
+  for (int c = 0; c < 4; c++)
+#pragma unroll
+    for (int e = 0; e < 3; e++) gr[c][e] = (e == DIR) ? d[c] * dx_ghost : 0.0;
+  const QT &q = f[2];
+  const double mu = exp(0.76 * log(q.T));
+  visc_flux<DIR>(fv, q, gr, mu, vp);
+}
+
+// d/dx_DIR of the viscous flux at cell idx, not yet scaled by dxinv:
+// (F(-2) - 8F(-1) + 8F(+1) - F(+2))/12 with ghost values where the stencil
+// leaves the interior.
+template <int DIR>
+__device__ __forceinline__ void visc_div_dir(double out[4], const double *__restrict__ u,
+                                             const double *__restrict__ FV, size_t ncell, size_t idx,
+                                             int pos, int n, size_t stride, const double *dxg_dir,
+                                             const ViscParams &vp) {
+  const size_t base = idx - (size_t)pos * stride;
+  double F[4][4];  // offsets -2,-1,+1,+2
+  const int offs[4] = {-2, -1, 1, 2};
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    const int gp = pos + offs[s];
+    if (gp >= 0 && gp < n) {
+      const size_t c = base + (size_t)gp * stride;
+#pragma unroll
+      for (int v = 0; v < 4; v++) F[s][v] = FV[(size_t)(DIR * 4 + v) * ncell + c];
+    } else {
+      // ghosted dxinv array: entry 0 is position -3
+      visc_flux_ghost<DIR>(F[s], u, ncell, base, stride, n, gp, dxg_dir[3 + gp], vp);
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 4; v++) out[v] = d4c(F[0][v], F[1][v], F[2][v], F[3][v]);
+}
+
+// ---------------------------------------------------------------------------
+// flux differences + viscous divergence + RK4 stage update (3N form)
+// ---------------------------------------------------------------------------
+// mode OUT_STAGE:
+// stage 0: acc = u + cb*k;   S = u + ca*k
+// stage 1,2: acc = acc + cb*k; S = u + ca*k
+// stage 3: u = acc + cb*k
+// with k = -hyp + par and hyp = (hx + hy) + hz.  Other modes store k, hyp
+// or par in `out` instead.
+__global__ void __launch_bounds__(256) k_stage(
+    const double *__restrict__ fIx, const double *__restrict__ fIy, const double *__restrict__ fIz,
+    const double *__restrict__ Ustage, const double *__restrict__ FV, double *__restrict__ u,
+    double *__restrict__ S, double *__restrict__ acc, double *__restrict__ out, int mode, int nx,
+    int ny, int nz, const double *__restrict__ dxg, ViscParams vp, int viscous, int stage, double ca,
+    double cb) {
+  const size_t ncell = (size_t)nx * ny * nz;
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ncell) return;
+  const int i = (int)(idx % nx);
+  const int j = (int)((idx / nx) % ny);
+  const int k = (int)(idx / ((size_t)nx * ny));
+  const double *dxi = dxg + 3, *dyi = dxg + (nx + 6) + 3, *dzi = dxg + (nx + 6) + (ny + 6) + 3;
+  const size_t ip = idx - i + wrap(i + 1, nx);
+  const size_t jp = idx + (size_t)(wrap(j + 1, ny) - j) * nx;
+  const size_t kp = idx + ((size_t)wrap(k + 1, nz) - k) * (size_t)nx * ny;
+  double par[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (viscous) {
+    double dv[4];
+    visc_div_dir<0>(dv, Ustage, FV, ncell, idx, i, nx, 1, dxg, vp);
+    for (int v = 0; v < 4; v++) par[v + 1] = dxi[i] * dv[v];
+    visc_div_dir<1>(dv, Ustage, FV, ncell, idx, j, ny, (size_t)nx, dxg + (nx + 6), vp);
+    for (int v = 0; v < 4; v++) par[v + 1] += dyi[j] * dv[v];
+    visc_div_dir<2>(dv, Ustage, FV, ncell, idx, k, nz, (size_t)nx * ny, dxg + (nx + 6) + (ny + 6), vp);
+    for (int v = 0; v < 4; v++) par[v + 1] += dzi[k] * dv[v];
+  }
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    const size_t o = (size_t)v * ncell;
+    double hyp = dxi[i] * (fIx[o + ip] - fIx[o + idx]);
+    hyp += dyi[j] * (fIy[o + jp] - fIy[o + idx]);
+    hyp += dzi[k] * (fIz[o + kp] - fIz[o + idx]);
+    const double r = (-hyp) + par[v];
+    if (mode != OUT_STAGE) {
+      out[o + idx] = mode == OUT_RHS ? r : (mode == OUT_HYP ? hyp : par[v]);
+      continue;
+    }
+    if (stage == 0) {
+      const double u0 = u[o + idx];
+      acc[o + idx] = u0 + cb * r;
+      S[o + idx] = u0 + ca * r;
+    } else if (stage < 3) {
+      acc[o + idx] = acc[o + idx] + cb * r;
+      S[o + idx] = u[o + idx] + ca * r;
+    } else {
+      u[o + idx] = acc[o + idx] + cb * r;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// reductions: CFL max and squared-difference sum
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_max(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// per-block partial maxima of the local CFL; NaN propagates as +Inf flag
+__global__ void __launch_bounds__(256) k_cfl(const double *__restrict__ u, double *__restrict__ part,
+                                             int nx, int ny, int nz, const double *__restrict__ dxg,
+                                             double gamma, double dt) {
+  const size_t ncell = (size_t)nx * ny * nz;
+  __shared__ double red[4];
+  double m = 0.0;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < ncell;
+       idx += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(idx % nx);
+    const int j = (int)((idx / nx) % ny);
+    const int k = (int)(idx / ((size_t)nx * ny));
+    double s[NV];
+    for (int v = 0; v < NV; v++) s[v] = u[(size_t)v * ncell + idx];
+    Prim q = flowvar(s, gamma);
+    const double c = sqrt(gamma * q.P / q.rho);
+    const double lx = (fabs(q.vx) + c) * dt * dxg[3 + i];
+    const double ly = (fabs(q.vy) + c) * dt * dxg[(nx + 6) + 3 + j];
+    const double lz = (fabs(q.vz) + c) * dt * dxg[(nx + 6) + (ny + 6) + 3 + k];
+    double l = fmax(fmax(lx, ly), lz);
+    if (!(l == l) || l > 1.0e300) l = __builtin_inf();
+    m = fmax(m, l);
+  }
+  m = wave_max(m);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) red[w] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = red[0];
+    for (int q = 1; q < (int)(blockDim.x >> 6); q++) r = fmax(r, red[q]);
+    part[blockIdx.x] = r;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_sqdiff(const double *__restrict__ a, const double *__restrict__ b,
+                                                double *__restrict__ part, size_t n) {
+  __shared__ double red[4];
+  double s = 0.0;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+       idx += (size_t)gridDim.x * blockDim.x) {
+    const double d = a[idx] - b[idx];
+    s += d * d;
+  }
+  s = wave_sum(s);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) red[w] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = 0.0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); q++) r += red[q];
+    part[blockIdx.x] = r;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// layout conversion (reference AoS with ghosts <-> hm SoA)
+// ---------------------------------------------------------------------------
+__global__ void k_aos_to_soa(const double *__restrict__ aos, double *__restrict__ soa, int nx, int ny,
+                             int nz, int g) {
+  const size_t ncell = (size_t)nx * ny * nz;
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ncell) return;
+  const int i = (int)(idx % nx);
+  const int j = (int)((idx / nx) % ny);
+  const int k = (int)(idx / ((size_t)nx * ny));
+  const size_t p = ((size_t)(k + g) * (ny + 2 * g) + (j + g)) * (nx + 2 * g) + (i + g);
+  for (int v = 0; v < NV; v++) soa[(size_t)v * ncell + idx] = aos[p * NV + v];
+}
+
+__global__ void k_soa_to_aos(const double *__restrict__ soa, double *__restrict__ aos, int nx, int ny,
+                             int nz, int g) {
+  const size_t ncell = (size_t)nx * ny * nz;
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ncell) return;
+  const int i = (int)(idx % nx);
+  const int j = (int)((idx / nx) % ny);
+  const int k = (int)(idx / ((size_t)nx * ny));
+  const size_t p = ((size_t)(k + g) * (ny + 2 * g) + (j + g)) * (nx + 2 * g) + (i + g);
+  for (int v = 0; v < NV; v++) aos[p * NV + v] = soa[(size_t)v * ncell + idx];
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline unsigned nblocks(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+template <int DIR, int INTERP, int UPW>
+static void launch_iface_t(const Grid &g, const double *u, double *fI, double gamma, double eps,
+                           hipStream_t st) {
+  hipLaunchKernelGGL((k_iface<DIR, INTERP, UPW>), dim3(nblocks(g.ncell, 256)), dim3(256), 0, st, u, fI,
+                     g.n[0], g.n[1], g.n[2], gamma, eps);
+}
+
+template <int DIR>
+static void launch_iface_d(const Grid &g, const Scheme &sc, const double *u, double *fI,
+                           hipStream_t st) {
+  if (sc.interp == INTERP_CHAR) {
+    if (sc.upwind == UPWIND_ROE) launch_iface_t<DIR, INTERP_CHAR, UPWIND_ROE>(g, u, fI, sc.gamma, sc.eps, st);
+    else launch_iface_t<DIR, INTERP_CHAR, UPWIND_RUSANOV>(g, u, fI, sc.gamma, sc.eps, st);
+  } else {
+    if (sc.upwind == UPWIND_ROE) launch_iface_t<DIR, INTERP_COMP, UPWIND_ROE>(g, u, fI, sc.gamma, sc.eps, st);
+    else launch_iface_t<DIR, INTERP_COMP, UPWIND_RUSANOV>(g, u, fI, sc.gamma, sc.eps, st);
+  }
+}
+
+void launch_iface(const Grid &g, const Scheme &sc, int dir, const double *u, double *fI,
+                  hipStream_t st) {
+  if (dir == 0) launch_iface_d<0>(g, sc, u, fI, st);
+  else if (dir == 1) launch_iface_d<1>(g, sc, u, fI, st);
+  else launch_iface_d<2>(g, sc, u, fI, st);
+}
+
+static ViscParams visc_params(const Scheme &sc) {
+  ViscParams vp;
+  vp.gamma = sc.gamma;
+  vp.inv_Re = sc.Re > 0 ? 1.0 / sc.Re : 0.0;
+  vp.inv_gm1 = 1.0 / (sc.gamma - 1.0);
+  vp.inv_Pr = 1.0 / sc.Pr;
+  return vp;
+}
+
+void launch_visc_flux(const Grid &g, const Scheme &sc, const double *u, double *FV,
+                      const double *dxg, hipStream_t st) {
+  hipLaunchKernelGGL(k_visc_flux, dim3(nblocks(g.ncell, 256)), dim3(256), 0, st, u, FV, g.n[0], g.n[1],
+                     g.n[2], dxg, visc_params(sc));
+}
+
+void launch_stage(const Grid &g, const Scheme &sc, const double *fIx, const double *fIy,
+                  const double *fIz, const double *Ustage, const double *FV, double *u, double *S,
+                  double *acc, double *out, int mode, const double *dxg, int stage, double ca,
+                  double cb, hipStream_t st) {
+  hipLaunchKernelGGL(k_stage, dim3(nblocks(g.ncell, 256)), dim3(256), 0, st, fIx, fIy, fIz, Ustage, FV,
+                     u, S, acc, out, mode, g.n[0], g.n[1], g.n[2], dxg, visc_params(sc),
+                     sc.Re > 0 ? 1 : 0, stage, ca, cb);
+}
+
+unsigned reduction_blocks(size_t n) {
+  unsigned b = nblocks(n, 256);
+  return b > 2048 ? 2048 : (b == 0 ? 1 : b);
+}
+
+void launch_cfl(const Grid &g, const Scheme &sc, const double *u, double *part, unsigned nb,
+                const double *dxg, double dt, hipStream_t st) {
+  hipLaunchKernelGGL(k_cfl, dim3(nb), dim3(256), 0, st, u, part, g.n[0], g.n[1], g.n[2], dxg, sc.gamma, dt);
+}
+
+void launch_sqdiff(const double *a, const double *b, double *part, unsigned nb, size_t n,
+                   hipStream_t st) {
+  hipLaunchKernelGGL(k_sqdiff, dim3(nb), dim3(256), 0, st, a, b, part, n);
+}
+
+void launch_aos_to_soa(const Grid &g, const double *aos, double *soa, int ghosts, hipStream_t st) {
+  hipLaunchKernelGGL(k_aos_to_soa, dim3(nblocks(g.ncell, 256)), dim3(256), 0, st, aos, soa, g.n[0], g.n[1],
+                     g.n[2], ghosts);
+}
+
+void launch_soa_to_aos(const Grid &g, const double *soa, double *aos, int ghosts, hipStream_t st) {
+  hipLaunchKernelGGL(k_soa_to_aos, dim3(nblocks(g.ncell, 256)), dim3(256), 0, st, soa, aos, g.n[0], g.n[1],
+                     g.n[2], ghosts);
+}
+
+}  // namespace hm

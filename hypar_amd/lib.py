@@ -1,0 +1,77 @@
+"""ctypes binding of libhypar_mi355x.so (C-ABI in include/hypar_mi355x.h).
+
+torch is imported before the library is loaded so that the process has a
+single HIP runtime (torch's bundled libamdhip64.so.7 satisfies the library's
+dependency by soname).  A missing library is an error, never a fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import torch  # noqa: F401  (must precede the CDLL load, see module docstring)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhypar_mi355x.so")
+
+HM_OK = 0
+HM_ERR_ARG, HM_ERR_HIP, HM_ERR_NONFINITE, HM_ERR_UNSUPPORTED, HM_ERR_NOMEM = 1, 2, 3, 4, 5
+INTERP = {"components": 0, "characteristic": 1}
+UPWIND = {"roe": 0, "rusanov": 1}
+LAYOUT_REFERENCE, LAYOUT_AOS_INTERIOR, LAYOUT_SOA_INTERIOR = 0, 1, 2
+
+
+class HMConfig(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int * 3), ("gamma", ctypes.c_double), ("Re", ctypes.c_double),
+                ("Minf", ctypes.c_double), ("Pr", ctypes.c_double), ("interp", ctypes.c_int),
+                ("upwind", ctypes.c_int), ("weno_eps", ctypes.c_double),
+                ("dxinv", ctypes.POINTER(ctypes.c_double))]
+
+
+class HMError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"hm error {code}: {msg}")
+        self.code = code
+
+
+_LIB = None
+
+# (name, argtypes, restype); every symbol declared in include/hypar_mi355x.h
+_VP, _DP, _SZ = ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t
+SIGNATURES = [
+    ("hm_create", [ctypes.POINTER(HMConfig), ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+    ("hm_destroy", [_VP], None),
+    ("hm_last_error", [], ctypes.c_char_p),
+    ("hm_version", [], ctypes.c_char_p),
+    ("hm_set_stream", [_VP, _VP], ctypes.c_int),
+    ("hm_ncell", [_VP], _SZ),
+    ("hm_upload", [_VP, _DP, _DP, ctypes.c_int], ctypes.c_int),
+    ("hm_download", [_VP, _DP, _DP, ctypes.c_int], ctypes.c_int),
+    ("hm_rk4", [_VP, _DP, ctypes.c_double, ctypes.c_int], ctypes.c_int),
+    ("hm_rhs", [_VP, _DP, _DP], ctypes.c_int),
+    ("hm_hyperbolic", [_VP, _DP, _DP], ctypes.c_int),
+    ("hm_parabolic", [_VP, _DP, _DP], ctypes.c_int),
+    ("hm_cfl", [_VP, _DP, ctypes.c_double, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    ("hm_step_norm", [_VP, _DP, _DP, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    ("hm_profile_kernels", [_VP, _DP, ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+]
+
+
+def load(path: str = LIB_PATH):
+    """Load the library (once) and declare every entry point."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: build it with __graft_entry__.build()")
+        L = ctypes.CDLL(path)
+        for name, args, res in SIGNATURES:
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _LIB = L
+    return _LIB
+
+
+def check(code: int):
+    if code != HM_OK:
+        raise HMError(code, load().hm_last_error().decode())

@@ -1,0 +1,152 @@
+"""Python layer over the C-ABI: an NS3D solver whose state lives in HBM as a
+torch float64 tensor in hm layout (SoA, no ghosts; include/hypar_mi355x.h).
+
+Mirrors the reference's solver object for this path: a grid with its
+physics and numerics fixed at construction (NavierStokes3DInitialize,
+WENOInitialize), then TimeRK steps (TimeIntegrate), the right-hand side
+(RHSFunction), its hyperbolic and parabolic parts, the CFL number
+(ComputeCFL) and the per-step norm (TimePostStep).
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import lib as _lib
+
+
+class NS3D:
+    """Navier-Stokes 3D WENO5 + RK4 on the current GPU."""
+
+    def __init__(self, dims: Sequence[int], interp: str = "components", upwind: str = "rusanov",
+                 Re: float = -1.0, Minf: float = 1.0, Pr: float = 0.72, gamma: float = 1.4,
+                 eps: float = 1e-6, dxinv: Optional[np.ndarray] = None):
+        if not torch.cuda.is_available():
+            raise RuntimeError("NS3D needs a GPU (torch.cuda.is_available() is False)")
+        self.L = _lib.load()
+        self.dims = tuple(int(d) for d in dims)
+        self.ncell = int(np.prod(self.dims))
+        cfg = _lib.HMConfig()
+        cfg.dim[:] = list(self.dims)
+        cfg.gamma, cfg.Re, cfg.Minf, cfg.Pr = gamma, Re, Minf, Pr
+        cfg.interp = _lib.INTERP[interp]
+        cfg.upwind = _lib.UPWIND[upwind]
+        cfg.weno_eps = eps
+        self._dx = None
+        if dxinv is not None:
+            self._dx = np.ascontiguousarray(dxinv, dtype=np.float64)
+            cfg.dxinv = self._dx.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        torch.cuda.init()
+        h = ctypes.c_void_p()
+        _lib.check(self.L.hm_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+
+    @classmethod
+    def from_workload(cls, w):
+        return cls(w.dims, w.interp, w.upwind, w.Re, w.Minf, w.Pr, w.gamma, w.weno_eps)
+
+    # ------------------------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.hm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _stream(self):
+        _lib.check(self.L.hm_set_stream(self.h, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+
+    def _check_state(self, t: torch.Tensor, name="u"):
+        if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()
+                and t.numel() == 5 * self.ncell):
+            raise ValueError(f"{name} must be a contiguous float64 CUDA tensor of {5 * self.ncell} elements")
+
+    def empty(self) -> torch.Tensor:
+        return torch.empty(5 * self.ncell, dtype=torch.float64, device="cuda")
+
+    # ------------------------------------------------------------------
+    def to_device(self, u: np.ndarray, layout: str = "aos") -> torch.Tensor:
+        """Upload a host state: (nz,ny,nx,5) interior AoS, or the reference's
+        ghosted AoS array (layout='reference')."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        t = self.empty()
+        self._stream()
+        code = {"aos": _lib.LAYOUT_AOS_INTERIOR, "reference": _lib.LAYOUT_REFERENCE,
+                "soa": _lib.LAYOUT_SOA_INTERIOR}[layout]
+        _lib.check(self.L.hm_upload(self.h, ctypes.c_void_p(t.data_ptr()),
+                                    ctypes.c_void_p(u.ctypes.data), code))
+        return t
+
+    def to_host(self, t: torch.Tensor) -> np.ndarray:
+        """Download to a (nz,ny,nx,5) interior AoS array."""
+        self._check_state(t)
+        nx, ny, nz = self.dims
+        out = np.empty((nz, ny, nx, 5), dtype=np.float64)
+        self._stream()
+        _lib.check(self.L.hm_download(self.h, ctypes.c_void_p(out.ctypes.data),
+                                      ctypes.c_void_p(t.data_ptr()), _lib.LAYOUT_AOS_INTERIOR))
+        return out
+
+    # ------------------------------------------------------------------
+    def rk4(self, u: torch.Tensor, dt: float, nsteps: int) -> torch.Tensor:
+        """nsteps RK4 steps in place (enqueued on torch's current stream)."""
+        self._check_state(u)
+        self._stream()
+        _lib.check(self.L.hm_rk4(self.h, ctypes.c_void_p(u.data_ptr()), float(dt), int(nsteps)))
+        return u
+
+    def _term(self, fn, u):
+        self._check_state(u)
+        out = self.empty()
+        self._stream()
+        _lib.check(fn(self.h, ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(u.data_ptr())))
+        return out
+
+    def rhs(self, u):
+        return self._term(self.L.hm_rhs, u)
+
+    def hyperbolic(self, u):
+        return self._term(self.L.hm_hyperbolic, u)
+
+    def parabolic(self, u):
+        return self._term(self.L.hm_parabolic, u)
+
+    def cfl(self, u: torch.Tensor, dt: float) -> float:
+        self._check_state(u)
+        self._stream()
+        out = ctypes.c_double()
+        _lib.check(self.L.hm_cfl(self.h, ctypes.c_void_p(u.data_ptr()), float(dt), ctypes.byref(out)))
+        return out.value
+
+    def step_norm(self, a: torch.Tensor, b: torch.Tensor) -> float:
+        self._check_state(a, "a")
+        self._check_state(b, "b")
+        self._stream()
+        out = ctypes.c_double()
+        _lib.check(self.L.hm_step_norm(self.h, ctypes.c_void_p(a.data_ptr()),
+                                       ctypes.c_void_p(b.data_ptr()), ctypes.byref(out)))
+        return out.value
+
+
+    def profile_kernels(self, u: torch.Tensor, reps: int = 5) -> dict:
+        """Mean duration (ms) of each kernel of one RK stage, HIP events on
+        the solver's stream."""
+        self._check_state(u)
+        self._stream()
+        ms = (ctypes.c_double * 5)()
+        _lib.check(self.L.hm_profile_kernels(self.h, ctypes.c_void_p(u.data_ptr()), int(reps), ms))
+        return {"iface_x": ms[0], "iface_y": ms[1], "iface_z": ms[2], "visc_flux": ms[3],
+                "stage": ms[4]}
+
+
+def soa_to_aos(t: torch.Tensor, dims) -> np.ndarray:
+    """hm-layout tensor -> (nz,ny,nx,5) numpy (host copy through torch)."""
+    nx, ny, nz = dims
+    return t.view(5, nz, ny, nx).permute(1, 2, 3, 0).contiguous().cpu().numpy()

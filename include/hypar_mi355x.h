@@ -1,0 +1,127 @@
+/* hypar_mi355x.h -- C-ABI of the MI355X-native NS3D WENO5 + RK4 path.
+ *
+ * libhypar_mi355x.so runs HyPar's explicit right-hand side and RK4 time step
+ * for the 3D compressible Navier-Stokes model (NavierStokes3D) on one gfx950
+ * GPU with hand-written HIP kernels.  Every entry point takes plain pointers
+ * and sizes and returns an hm_status (0 = success); it never exits the
+ * process.  hm_last_error() describes the most recent failure of the calling
+ * thread.
+ *
+ * Device layout ("hm layout"): structure of arrays without ghost points,
+ *   u[v*ncell + (k*ny + j)*nx + i],  v = 0..4 (rho, rho*u, rho*v, rho*w, e),
+ *   ncell = nx*ny*nz, x fastest.
+ * Ghost points of the reference layout are not stored: every boundary is
+ * periodic (BCPeriodicU) and the kernels wrap indices, which gives the values
+ * the reference's ghost fill would.  The reference CPU layout (AoS, variables
+ * fastest, 3 ghost layers) is accepted at the host transfer entry points.
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *   hm_rk4 ............. HyPar::TimeIntegrate = TimeRK (include/hypar.h:220,
+ *                        src/TimeIntegration/TimeRK.c:35-202)
+ *   hm_rhs ............. TimeIntegration::RHSFunction = TimeRHSFunctionExplicit
+ *                        (src/TimeIntegration/TimeRHSFunctionExplicit.c:30-98)
+ *   hm_hyperbolic ...... HyPar::HyperbolicFunction (include/hypar.h:250-253,
+ *                        src/HyParFunctions/HyperbolicFunction.c:31-126)
+ *   hm_parabolic ....... HyPar::ParabolicFunction = NavierStokes3DParabolicFunction
+ *                        (include/hypar.h:256, src/PhysicalModels/NavierStokes3D/
+ *                        NavierStokes3DParabolicFunction.c:50-325)
+ *   hm_cfl ............. HyPar::ComputeCFL = NavierStokes3DComputeCFL
+ *                        (include/hypar.h:269, NavierStokes3DComputeCFL.c:16-55)
+ *   hm_step_norm ....... TimePostStep norm (src/TimeIntegration/TimePostStep.c:43-63)
+ */
+#ifndef HYPAR_MI355X_H
+#define HYPAR_MI355X_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  HM_OK = 0,
+  HM_ERR_ARG = 1,         /* invalid argument or size                    */
+  HM_ERR_HIP = 2,         /* a HIP runtime call failed                   */
+  HM_ERR_NONFINITE = 3,   /* a reduction found NaN/Inf in the state      */
+  HM_ERR_UNSUPPORTED = 4, /* scheme or option not implemented            */
+  HM_ERR_NOMEM = 5        /* device allocation failed                    */
+} hm_status;
+
+/* hyp_interp_type of solver.inp (src/Simulation/ReadInputs.c) */
+typedef enum { HM_INTERP_COMPONENTS = 0, HM_INTERP_CHARACTERISTIC = 1 } hm_interp;
+/* upwinding of physics.inp (NavierStokes3DInitialize.c:162-333) */
+typedef enum { HM_UPWIND_ROE = 0, HM_UPWIND_RUSANOV = 1 } hm_upwind;
+/* host array layouts accepted by the transfer calls */
+typedef enum {
+  HM_LAYOUT_REFERENCE = 0, /* AoS, variables fastest, 3 ghost layers (HyPar CPU) */
+  HM_LAYOUT_AOS_INTERIOR = 1,/* AoS, variables fastest, interior only          */
+  HM_LAYOUT_SOA_INTERIOR = 2 /* hm layout                                        */
+} hm_layout;
+
+typedef struct {
+  int dim[3];          /* interior grid size (each >= 5)                          */
+  double gamma;        /* physics.inp gamma                                       */
+  double Re;           /* physics.inp Re; <= 0 is inviscid.  Divided by Minf as
+                          NavierStokes3DInitialize.c:368 does                     */
+  double Minf;         /* physics.inp Minf                                        */
+  double Pr;           /* physics.inp Pr                                          */
+  int interp;          /* hm_interp                                               */
+  int upwind;          /* hm_upwind                                               */
+  double weno_eps;     /* weno.inp epsilon (Jiang-Shu weights, p = 2)             */
+  const double *dxinv; /* optional host array, HyPar::dxinv layout: x, y, z
+                          concatenated, each with 3 ghost entries per side.
+                          NULL: uniform periodic grid x_i = 2*pi*i/n, metrics
+                          computed exactly as InitialSolution.c:74-113        */
+} hm_config;
+
+typedef struct hm_solver hm_solver;
+
+/* Create a solver bound to the current HIP device; allocates its scratch. */
+hm_status hm_create(const hm_config *cfg, hm_solver **out);
+/* Free a solver (NULL is ignored). */
+void hm_destroy(hm_solver *s);
+/* Message of the last failure on the calling thread ("" if none). */
+const char *hm_last_error(void);
+/* Library version string. */
+const char *hm_version(void);
+
+/* Use `stream` (a hipStream_t; NULL = default stream) for all launches. */
+hm_status hm_set_stream(hm_solver *s, void *stream);
+/* Number of interior cells and doubles in one hm-layout state. */
+size_t hm_ncell(const hm_solver *s);
+
+/* Host <-> device state transfer (synchronous).  `dev` is an hm-layout
+ * device array of 5*ncell doubles; `host` uses `layout`. */
+hm_status hm_upload(hm_solver *s, double *dev, const double *host, int layout);
+hm_status hm_download(hm_solver *s, double *host, const double *dev, int layout);
+
+/* nsteps classical RK4 steps of size dt on the hm-layout device state u,
+ * in place (TimeRK with the "rk 44" tableau). */
+hm_status hm_rk4(hm_solver *s, double *u, double dt, int nsteps);
+
+/* rhs = -hyperbolic + parabolic (+ zero source) of the device state u. */
+hm_status hm_rhs(hm_solver *s, double *rhs, const double *u);
+/* hyperbolic term alone: sum over x,y,z of dxinv*(F_{i+1/2}-F_{i-1/2}). */
+hm_status hm_hyperbolic(hm_solver *s, double *hyp, const double *u);
+/* viscous term alone (zero when Re <= 0). */
+hm_status hm_parabolic(hm_solver *s, double *par, const double *u);
+
+/* Max over cells and directions of (|v_d|+c)*dt*dxinv_d. Sets *cfl. Returns
+ * HM_ERR_NONFINITE when the state holds NaN/Inf. */
+hm_status hm_cfl(hm_solver *s, const double *u, double dt, double *cfl);
+/* sqrt(sum((a-b)^2) / ncell) over all 5 variables (TimePostStep norm). */
+hm_status hm_step_norm(hm_solver *s, const double *a, const double *b, double *norm);
+
+/* Kernel timing for roofline reports (no counterpart in the reference; the
+ * reference's GPU_STAT build prints per-kernel cudaEvent times,
+ * HyperbolicFunction_GPU.cu:517-576).  Launches each kernel of one RK stage
+ * `reps` times on the solver's stream with state u and stores the mean
+ * duration in milliseconds, measured with HIP events on that stream:
+ * ms[0..2] interface fluxes x, y, z; ms[3] viscous fluxes (0 if inviscid);
+ * ms[4] flux difference + stage update.  u is not modified. */
+hm_status hm_profile_kernels(hm_solver *s, const double *u, int reps, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

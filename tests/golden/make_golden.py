@@ -1,0 +1,150 @@
+"""Write golden vectors for the NS3D WENO5-RK4 path by running the reference.
+
+Runs the reference HyPar binary built from the reference sources by
+``oracle/Makefile.ref`` (into ``oracle/_ref/HyPar``) on decks written by
+``hypar_amd.decks`` and stores inputs and outputs as small ``.npz`` fixtures
+in this directory.  Run it in the build container (the reference is not
+present on GPU machines):
+
+    python tests/golden/make_golden.py            # all fixtures
+    python tests/golden/make_golden.py c0_16_s2   # one fixture
+
+The reference runs single-threaded: its OpenMP build shares the eigenvector
+scratch arrays of the characteristic WENO weights between threads
+(src/InterpolationFunctions/WENOFifthOrderCalculateWeights.c:796-833 declares
+``L`` and ``uavg`` outside the ``omp parallel for`` and does not list them as
+private), so multi-threaded characteristic runs are not reproducible.
+
+Small grids store the full initial and final interior states.  Large grids
+store the workload (seed, sizes, dt) plus checksums and a strided subsample of
+the final state; the initial state is regenerated from the seed.
+"""
+from __future__ import annotations
+
+import dataclasses
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from hypar_amd import decks  # noqa: E402
+
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "HyPar")
+
+
+def _workload(kind: str, dims, n_iter: int) -> decks.Workload:
+    if kind == "c0":
+        w = decks.config0(dims[0], n_iter)
+    elif kind == "c1":
+        w = decks.config1(dims[0], n_iter)
+    else:
+        raise ValueError(kind)
+    return dataclasses.replace(w, dims=tuple(dims), n_iter=n_iter, dt=None)
+
+
+# name -> (scheme, dims, steps, store_full)
+FIXTURES = {
+    "c0_16_s1": ("c0", (16, 16, 16), 1, True),
+    "c0_16_s10": ("c0", (16, 16, 16), 10, True),
+    "c1_16_s1": ("c1", (16, 16, 16), 1, True),
+    "c1_16_s10": ("c1", (16, 16, 16), 10, True),
+    "c0_ragged_s3": ("c0", (20, 12, 16), 3, True),
+    "c1_ragged_s3": ("c1", (12, 16, 20), 3, True),
+    "c0_64_s100": ("c0", (64, 64, 64), 100, False),
+    "c1_32_s20": ("c1", (32, 32, 32), 20, False),
+    "c1_64_s200": ("c1", (64, 64, 64), 200, False),
+}
+
+ENSEMBLE = {
+    # three members of uneven sizes, shared dt and n_iter
+    "ens3_s4": ([(16, 12, 20), (12, 20, 16), (20, 16, 12)], 4),
+}
+
+
+def run_reference(deck_dir: str) -> float:
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    t = time.time()
+    r = subprocess.run([REF_BIN], cwd=deck_dir, capture_output=True, text=True, env=env)
+    if r.returncode != 0:
+        raise RuntimeError(f"reference failed in {deck_dir}:\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+    return time.time() - t
+
+
+def summary(u: np.ndarray) -> dict:
+    return {"sum": [float(v) for v in u.reshape(-1, 5).sum(axis=0)],
+            "l2": [float(v) for v in np.sqrt((u.reshape(-1, 5) ** 2).sum(axis=0))]}
+
+
+def subsample(u: np.ndarray, step: int = 4) -> np.ndarray:
+    return np.ascontiguousarray(u[::step, ::step, ::step, :])
+
+
+def make_single(name: str):
+    kind, dims, steps, full = FIXTURES[name]
+    w = _workload(kind, dims, steps)
+    with tempfile.TemporaryDirectory() as d:
+        u0 = decks.write_deck(d, w, screen_op_iter=max(1, steps))
+        wall = run_reference(d)
+        _, _, uN = decks.read_op_bin(os.path.join(d, "op.bin"))
+    meta = {"name": name, "scheme": kind, "dims": list(dims), "steps": steps, "dt": w.dt,
+            "seed": w.seed, "interp": w.interp, "upwind": w.upwind, "Re": w.Re,
+            "Minf": w.Minf, "Pr": w.Pr, "gamma": w.gamma, "eps": w.weno_eps,
+            "field": w.field, "reference_wall_s": wall, "reference_threads": 1,
+            "u0_summary": summary(u0), "uN_summary": summary(uN)}
+    arrays = {"meta": np.array(json.dumps(meta))}
+    if full:
+        arrays["u0"] = u0
+        arrays["uN"] = uN
+    else:
+        arrays["uN_sub4"] = subsample(uN)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
+    print(f"{name}: dims={dims} steps={steps} dt={w.dt:.6g} reference {wall:.1f}s", flush=True)
+
+
+def make_ensemble(name: str):
+    sizes, steps = ENSEMBLE[name]
+    members = [dataclasses.replace(decks.config0(s[0], steps), dims=tuple(s), seed=20262000 + i,
+                                   dt=None) for i, s in enumerate(sizes)]
+    u0s = [decks.initial_state(m) for m in members]
+    dt = min(decks.cfl_dt(u, m.dims) for u, m in zip(u0s, members))
+    with tempfile.TemporaryDirectory() as d:
+        decks.write_ensemble_deck(d, members, steps, dt)
+        wall = run_reference(d)
+        outs = []
+        for i in range(len(members)):
+            _, _, uN = decks.read_op_bin(os.path.join(d, f"op_{decks.member_index(i, len(members))}.bin"))
+            outs.append(uN)
+    meta = {"name": name, "sizes": [list(s) for s in sizes], "steps": steps, "dt": dt,
+            "seeds": [m.seed for m in members], "reference_wall_s": wall,
+            "reference_threads": 1}
+    arrays = {"meta": np.array(json.dumps(meta))}
+    for i, (u0, uN) in enumerate(zip(u0s, outs)):
+        arrays[f"u0_{i}"] = u0
+        arrays[f"uN_{i}"] = uN
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
+    print(f"{name}: sizes={sizes} steps={steps} dt={dt:.6g} reference {wall:.1f}s", flush=True)
+
+
+def main(argv):
+    if not os.path.exists(REF_BIN):
+        raise SystemExit(f"{REF_BIN} missing: make -f oracle/Makefile.ref REF=/root/reference")
+    names = argv[1:] or (list(FIXTURES) + list(ENSEMBLE))
+    for n in names:
+        if n in FIXTURES:
+            make_single(n)
+        elif n in ENSEMBLE:
+            make_ensemble(n)
+        else:
+            raise SystemExit(f"unknown fixture {n}")
+
+
+if __name__ == "__main__":
+    main(sys.argv)

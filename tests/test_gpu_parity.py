@@ -1,0 +1,194 @@
+"""Parity of the HIP path (through the C-ABI) with the reference.
+
+Tolerance (the path's stated bar): relative L2 over all interior cells and
+all 5 variables <= 1e-12 against the reference's CPU path.  Reference
+outputs come from golden vectors written by the reference binary; at sizes
+without fixtures the CPU oracle (bit-identical to the reference on every
+fixture) is the checker, and at 256^3 size-independent properties are
+checked too.
+"""
+import ctypes
+import dataclasses
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, load_fixture, workload_from_meta
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _solver(w):
+    from hypar_amd.solver import NS3D
+    return NS3D.from_workload(w)
+
+
+FULL = ["c0_16_s1", "c0_16_s10", "c1_16_s1", "c1_16_s10", "c0_ragged_s3", "c1_ragged_s3"]
+
+
+@pytest.mark.parametrize("name", FULL)
+def test_rk4_matches_reference_fixture(name):
+    from hypar_amd import decks
+    meta, z = load_fixture(name)
+    w = workload_from_meta(meta)
+    _say(f"[gpu] fixture {name} dims={w.dims} steps={meta['steps']}")
+    s = _solver(w)
+    u = s.to_device(z["u0"])
+    s.rk4(u, w.dt, meta["steps"])
+    got = s.to_host(u)
+    err = decks.rel_l2(got, z["uN"])
+    _say(f"[gpu] {name}: rel L2 = {err:.3e}")
+    assert err <= TOL
+
+
+@pytest.mark.parametrize("name", ["c1_32_s20", "c0_64_s100", "c1_64_s200"])
+def test_rk4_long_runs_match_reference(name):
+    from hypar_amd import decks
+    if not os.path.exists(os.path.join(GOLDEN, name + ".npz")):
+        pytest.skip(f"fixture {name} not generated")
+    meta, z = load_fixture(name)
+    w = workload_from_meta(meta)
+    _say(f"[gpu] fixture {name} dims={w.dims} steps={meta['steps']}")
+    s = _solver(w)
+    u = s.to_device(decks.initial_state(w))
+    s.rk4(u, w.dt, meta["steps"])
+    got = s.to_host(u)
+    err = decks.rel_l2(got[::4, ::4, ::4, :], z["uN_sub4"])
+    sums = got.reshape(-1, 5).sum(axis=0)
+    _say(f"[gpu] {name}: rel L2 (every 4th cell) = {err:.3e}")
+    assert err <= TOL
+    assert np.allclose(sums, meta["uN_summary"]["sum"], rtol=1e-11, atol=1e-9)
+    l2 = np.sqrt((got.reshape(-1, 5) ** 2).sum(axis=0))
+    assert np.allclose(l2, meta["uN_summary"]["l2"], rtol=1e-12)
+
+
+def test_ensemble_members_match_reference():
+    from hypar_amd import decks
+    from hypar_amd.solver import NS3D
+    meta, z = load_fixture("ens3_s4")
+    for i, dims in enumerate(meta["sizes"]):
+        _say(f"[gpu] ensemble member {i} dims={dims}")
+        s = NS3D(tuple(dims))
+        u = s.to_device(z[f"u0_{i}"])
+        s.rk4(u, meta["dt"], meta["steps"])
+        assert decks.rel_l2(s.to_host(u), z[f"uN_{i}"]) <= TOL
+
+
+SCHEMES = [("components", "rusanov", -1.0), ("components", "roe", -1.0),
+           ("characteristic", "rusanov", -1.0), ("characteristic", "roe", -1.0),
+           ("characteristic", "roe", 1600.0), ("components", "rusanov", 100.0)]
+
+
+@pytest.mark.parametrize("interp,upwind,Re", SCHEMES)
+def test_terms_match_oracle(interp, upwind, Re):
+    from hypar_amd import decks
+    from oracle.oracle import Oracle
+    w = dataclasses.replace(decks.config1(16, 1), dims=(18, 13, 21), interp=interp, upwind=upwind,
+                            Re=Re, dt=None)
+    _say(f"[gpu] terms {interp}/{upwind}/Re={Re} dims={w.dims}")
+    u0 = decks.initial_state(w)
+    o = Oracle.for_workload(w)
+    s = _solver(w)
+    u = s.to_device(u0)
+    for term in ("hyperbolic", "parabolic", "rhs"):
+        got = s.to_host(getattr(s, term)(u))
+        ref = getattr(o, term)(u0)
+        if np.linalg.norm(ref) == 0:
+            assert np.max(np.abs(got)) == 0
+        else:
+            assert decks.rel_l2(got, ref) <= 1e-13, term
+
+
+def test_cfl_and_step_norm_match_oracle():
+    from hypar_amd import decks
+    from oracle.oracle import Oracle
+    w = dataclasses.replace(decks.config0(16, 1), dims=(16, 20, 12), dt=None)
+    u0, dt = decks.prepare(w)
+    s = _solver(w)
+    u = s.to_device(u0)
+    cfl = s.cfl(u, dt)
+    assert abs(cfl - Oracle.for_workload(w).cfl(u0, dt)) <= 1e-15
+    assert abs(cfl - 0.5) < 1e-12  # dt was chosen for CFL 0.5
+    v = u.clone()
+    s.rk4(v, dt, 1)
+    norm = s.step_norm(v, u)
+    ref = np.sqrt(((s.to_host(v) - u0) ** 2).sum() / np.prod(w.dims))
+    assert abs(norm - ref) <= 1e-14 * ref
+
+
+def test_nonfinite_state_is_reported():
+    from hypar_amd import decks, lib
+    w = decks.config0(8, 1)
+    s = _solver(w)
+    u0 = decks.initial_state(w)
+    u0[3, 2, 1, 0] = np.nan
+    u = s.to_device(u0)
+    with pytest.raises(lib.HMError) as e:
+        s.cfl(u, 0.01)
+    assert e.value.code == lib.HM_ERR_NONFINITE
+
+
+def test_bad_arguments_are_rejected():
+    from hypar_amd import decks, lib
+    w = decks.config0(8, 1)
+    s = _solver(w)
+    with pytest.raises(ValueError):
+        s.rk4(torch.zeros(10, dtype=torch.float64, device="cuda"), 0.01, 1)
+    host = np.zeros(5 * 8 ** 3)
+    code = s.L.hm_rk4(s.h, ctypes.c_void_p(host.ctypes.data), 0.01, 1)
+    assert code == lib.HM_ERR_ARG
+
+
+# ------------------------------------------------------------ full size
+def test_full_size_rhs_matches_oracle_256():
+    """One right-hand side of the production workload (256^3, characteristic
+    WENO5 + Roe + viscous) against the oracle on the host."""
+    from hypar_amd import decks
+    from oracle.oracle import Oracle
+    w = decks.config1(256, 1)
+    _say("[gpu] 256^3 rhs vs oracle: generating input")
+    u0 = decks.initial_state(w)
+    s = _solver(w)
+    u = s.to_device(u0)
+    _say("[gpu] 256^3 rhs on GPU")
+    got = s.to_host(s.rhs(u))
+    del u
+    torch.cuda.empty_cache()
+    _say("[gpu] 256^3 rhs on the CPU oracle")
+    ref = Oracle.for_workload(w).rhs(u0)
+    err = decks.rel_l2(got, ref)
+    _say(f"[gpu] 256^3 rhs rel L2 = {err:.3e}")
+    assert err <= 1e-13
+
+
+def test_full_size_invariants_256():
+    """256^3 inviscid: a uniform state is preserved exactly; the rhs
+    conserves mass/momentum/energy; shifting the field shifts the rhs."""
+    from hypar_amd import decks
+    w = decks.config0(256, 1)
+    s = _solver(w)
+    # free stream
+    uni = np.zeros((256, 256, 256, 5))
+    uni[...] = [1.1, 0.2, -0.1, 0.05, 2.0]
+    r = s.rhs(s.to_device(uni))
+    assert float(r.abs().max()) == 0.0
+    u0 = decks.initial_state(w)
+    u = s.to_device(u0)
+    r = s.rhs(u).view(5, -1)
+    tot = r.sum(dim=1)
+    scale = r.abs().sum(dim=1)
+    assert bool(torch.all(tot.abs() <= 1e-12 * scale))
+    us = s.to_device(np.roll(u0, 8, axis=1))
+    rs = s.rhs(us).view(5, 256, 256, 256)
+    rr = torch.roll(r.view(5, 256, 256, 256), 8, dims=2)
+    assert float((rs - rr).abs().max()) <= 1e-11 * float(r.abs().max())
