@@ -68,6 +68,20 @@ def cpu_baseline(n=64, steps=2):
             "sample": f"CPU oracle, same scheme on a periodic {n}^3 grid, {steps} RK4 steps, {t:.2f} s"}
 
 
+def copy_probe(mib=1024, reps=3):
+    """A plain device copy of known size (torch copy_ of `mib` MiB), the
+    calibration run for the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = mib * 1024 * 1024 // 8
+    a = torch.rand(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8,
+                      "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,7 +89,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py: this system has one GPU; --gpus {args.gpus} is not supported "
               f"(the path would shard as described in DESIGN.md).", file=sys.stderr)
@@ -120,8 +139,12 @@ def main():
     else:
         bytes_launch = (15 * 8 + 3 * 40) * ncell
     achieved = bytes_launch / (kt[dom] / 1e3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{args.n}.json")
+    if os.path.exists(tfile):  # rocprofv3 FETCH_SIZE/WRITE_SIZE passes, copy-calibrated
+        traffic = json.load(open(tfile))["bytes_per_launch"].get(dom)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "avg_launch_ms": kt[dom], "algorithmic_bytes_per_launch": bytes_launch}
     step_bw = value * STEP_BYTES_PER_CELL_UPDATE / 1e9
     line = {

@@ -329,6 +329,110 @@ hm_status hm_step_norm(hm_solver *s, const double *a, const double *b, double *n
   return HM_OK;
 }
 
+hm_status hm_malloc(double **dev, size_t n) {
+  if (!dev) return fail(HM_ERR_ARG, "hm_malloc: NULL output");
+  return dmalloc(dev, n ? n : 1);
+}
+
+void hm_free(double *dev) {
+  if (dev) (void)hipFree(dev);
+}
+
+hm_status hm_copy_to_device(double *dev, const double *host, size_t n) {
+  if (!dev || !host) return fail(HM_ERR_ARG, "hm_copy_to_device: NULL pointer");
+  HM_HIP(hipMemcpy(dev, host, n * sizeof(double), hipMemcpyHostToDevice));
+  return HM_OK;
+}
+
+hm_status hm_copy_to_host(double *host, const double *dev, size_t n) {
+  if (!dev || !host) return fail(HM_ERR_ARG, "hm_copy_to_host: NULL pointer");
+  HM_HIP(hipMemcpy(host, dev, n * sizeof(double), hipMemcpyDeviceToHost));
+  return HM_OK;
+}
+
+hm_status hm_copy_device(hm_solver *s, double *dst, const double *src, size_t n) {
+  hm_status st = check_state(s, dst, "hm_copy_device: dst");
+  if (st != HM_OK) return st;
+  if ((st = check_state(s, src, "hm_copy_device: src")) != HM_OK) return st;
+  HM_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+  return HM_OK;
+}
+
+hm_status hm_synchronize(hm_solver *s) {
+  if (!s) return fail(HM_ERR_ARG, "hm_synchronize: solver is NULL");
+  HM_HIP(hipStreamSynchronize(s->stream));
+  return HM_OK;
+}
+
+static hm_status check_dir(int dir, const char *name) {
+  if (dir < 0 || dir > 2) return fail(HM_ERR_ARG, std::string(name) + ": dir must be 0, 1 or 2");
+  return HM_OK;
+}
+
+hm_status hm_flux(hm_solver *s, double *f, const double *u, int dir) {
+  hm_status st;
+  if ((st = check_dir(dir, "hm_flux")) != HM_OK) return st;
+  if ((st = check_state(s, f, "hm_flux: f")) != HM_OK) return st;
+  if ((st = check_state(s, u, "hm_flux: u")) != HM_OK) return st;
+  hm::launch_flux(s->g, s->sc, dir, u, f, s->stream);
+  return check_launch("hm_flux");
+}
+
+hm_status hm_modified_solution(hm_solver *s, double *uC, const double *u) {
+  hm_status st;
+  if ((st = check_state(s, uC, "hm_modified_solution: uC")) != HM_OK) return st;
+  if ((st = check_state(s, u, "hm_modified_solution: u")) != HM_OK) return st;
+  hm::launch_modsol(s->g, s->sc, u, uC, s->stream);
+  return check_launch("hm_modified_solution");
+}
+
+hm_status hm_interpolate(hm_solver *s, double *fI, const double *fC, const double *u, int upw,
+                         int dir, int uflag) {
+  hm_status st;
+  if ((st = check_dir(dir, "hm_interpolate")) != HM_OK) return st;
+  if (upw == 0) return fail(HM_ERR_ARG, "hm_interpolate: upw must be nonzero");
+  if ((st = check_state(s, fI, "hm_interpolate: fI")) != HM_OK) return st;
+  if ((st = check_state(s, fC, "hm_interpolate: fC")) != HM_OK) return st;
+  if ((st = check_state(s, u, "hm_interpolate: u")) != HM_OK) return st;
+  hm::launch_interp(s->g, s->sc, dir, fI, fC, u, upw, uflag ? 1 : 0, s->stream);
+  return check_launch("hm_interpolate");
+}
+
+hm_status hm_upwind(hm_solver *s, double *fI, const double *fL, const double *fR,
+                    const double *uL, const double *uR, const double *u, int dir) {
+  hm_status st;
+  if ((st = check_dir(dir, "hm_upwind")) != HM_OK) return st;
+  const void *ptrs[] = {fI, fL, fR, uL, uR, u};
+  for (const void *p : ptrs)
+    if ((st = check_state(s, p, "hm_upwind: array")) != HM_OK) return st;
+  hm::launch_upwind(s->g, s->sc, dir, fI, fL, fR, uL, uR, u, s->stream);
+  return check_launch("hm_upwind");
+}
+
+hm_status hm_flux_difference(hm_solver *s, double *hyp, const double *fI, int dir, int accumulate) {
+  hm_status st;
+  if ((st = check_dir(dir, "hm_flux_difference")) != HM_OK) return st;
+  if ((st = check_state(s, hyp, "hm_flux_difference: hyp")) != HM_OK) return st;
+  if ((st = check_state(s, fI, "hm_flux_difference: fI")) != HM_OK) return st;
+  hm::launch_fluxdiff(s->g, dir, hyp, fI, s->dxg, accumulate ? 1 : 0, s->stream);
+  return check_launch("hm_flux_difference");
+}
+
+hm_status hm_axpy(hm_solver *s, const double *x, double a, double *y, size_t n) {
+  hm_status st;
+  if ((st = check_state(s, x, "hm_axpy: x")) != HM_OK) return st;
+  if ((st = check_state(s, y, "hm_axpy: y")) != HM_OK) return st;
+  hm::launch_axpy(x, a, y, n, s->stream);
+  return check_launch("hm_axpy");
+}
+
+hm_status hm_fill(hm_solver *s, double *x, double value, size_t n) {
+  hm_status st;
+  if ((st = check_state(s, x, "hm_fill: x")) != HM_OK) return st;
+  hm::launch_fill(x, value, n, s->stream);
+  return check_launch("hm_fill");
+}
+
 hm_status hm_profile_kernels(hm_solver *s, const double *u, int reps, double *ms) {
   hm_status st = check_state(s, u, "hm_profile_kernels: u");
   if (st != HM_OK) return st;
@@ -355,10 +459,12 @@ hm_status hm_profile_kernels(hm_solver *s, const double *u, int reps, double *ms
     else
       ms[3] = 0.0;
   }
+  // the stage kernel as an RK step runs it (stage 1: reads u and acc,
+  // writes acc and the next stage state); u itself is not written
   if (st == HM_OK)
     st = timed(4, [&] {
-      hm::launch_stage(s->g, s->sc, s->fI[0], s->fI[1], s->fI[2], u, s->FV, nullptr, nullptr, nullptr,
-                       s->S[1], hm::OUT_RHS, s->dxg, 0, 0.0, 0.0, s->stream);
+      hm::launch_stage(s->g, s->sc, s->fI[0], s->fI[1], s->fI[2], u, s->FV, const_cast<double *>(u),
+                       s->S[1], s->acc, nullptr, hm::OUT_STAGE, s->dxg, 1, 0.0, 0.0, s->stream);
     });
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

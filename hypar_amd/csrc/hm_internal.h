@@ -43,4 +43,17 @@ void launch_sqdiff(const double *a, const double *b, double *part, unsigned nb, 
 void launch_aos_to_soa(const Grid &g, const double *aos, double *soa, int ghosts, hipStream_t st);
 void launch_soa_to_aos(const Grid &g, const double *soa, double *aos, int ghosts, hipStream_t st);
 
+// per-slot operators (hm_slots.hip)
+void launch_flux(const Grid &g, const Scheme &sc, int dir, const double *u, double *f, hipStream_t st);
+void launch_modsol(const Grid &g, const Scheme &sc, const double *u, double *uC, hipStream_t st);
+void launch_interp(const Grid &g, const Scheme &sc, int dir, double *fI, const double *fC,
+                   const double *u, int upw, int uflag, hipStream_t st);
+void launch_upwind(const Grid &g, const Scheme &sc, int dir, double *fI, const double *fL,
+                   const double *fR, const double *uL, const double *uR, const double *u,
+                   hipStream_t st);
+void launch_fluxdiff(const Grid &g, int dir, double *hyp, const double *fI, const double *dxg,
+                     int accumulate, hipStream_t st);
+void launch_axpy(const double *x, double a, double *y, size_t n, hipStream_t st);
+void launch_fill(double *x, double value, size_t n, hipStream_t st);
+
 }  // namespace hm

@@ -55,6 +55,7 @@ class Workload:
     seed: int = 20260
     field: str = "random_phase"       # random_phase | taylor_green
     dt: Optional[float] = None        # filled from the CFL when None
+    time_scheme_type: str = "44"      # solver.inp time_scheme_type (rk class)
 
     @property
     def viscous(self) -> bool:
@@ -273,7 +274,8 @@ def _weno_flags(weno_type: str):
 
 
 def write_solver_inp(path: str, dims_list, n_iter: int, dt: float, interp: str,
-                     screen_op_iter: int = 1, file_op_iter: Optional[int] = None):
+                     screen_op_iter: int = 1, file_op_iter: Optional[int] = None,
+                     time_scheme_type: str = "44"):
     size = " ".join(" ".join(str(n) for n in d) for d in dims_list)
     with open(os.path.join(path, "solver.inp"), "w") as f:
         f.write("begin\n")
@@ -285,7 +287,7 @@ def write_solver_inp(path: str, dims_list, n_iter: int, dt: float, interp: str,
         f.write(f"  n_iter              {n_iter}\n")
         f.write("  restart_iter        0\n")
         f.write("  time_scheme         rk\n")
-        f.write("  time_scheme_type    44\n")
+        f.write(f"  time_scheme_type    {time_scheme_type}\n")
         f.write("  hyp_space_scheme    weno5\n")
         f.write(f"  hyp_interp_type     {interp}\n")
         f.write("  par_space_type      nonconservative-2stage\n")
@@ -356,7 +358,8 @@ def write_deck(path: str, w: Workload, u0: Optional[np.ndarray] = None,
         u0, _ = prepare(w)
     elif w.dt is None:
         w.dt = cfl_dt(u0, w.dims, w.gamma, w.cfl)
-    write_solver_inp(path, [w.dims], w.n_iter, w.dt, w.interp, screen_op_iter)
+    write_solver_inp(path, [w.dims], w.n_iter, w.dt, w.interp, screen_op_iter,
+                     time_scheme_type=w.time_scheme_type)
     write_physics_inp(path, w)
     write_weno_inp(path, w)
     write_boundary_inp(path)

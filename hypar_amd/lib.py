@@ -54,6 +54,19 @@ SIGNATURES = [
     ("hm_cfl", [_VP, _DP, ctypes.c_double, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     ("hm_step_norm", [_VP, _DP, _DP, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     ("hm_profile_kernels", [_VP, _DP, ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    ("hm_malloc", [ctypes.POINTER(ctypes.c_void_p), _SZ], ctypes.c_int),
+    ("hm_free", [_DP], None),
+    ("hm_copy_to_device", [_DP, _DP, _SZ], ctypes.c_int),
+    ("hm_copy_to_host", [_DP, _DP, _SZ], ctypes.c_int),
+    ("hm_copy_device", [_VP, _DP, _DP, _SZ], ctypes.c_int),
+    ("hm_synchronize", [_VP], ctypes.c_int),
+    ("hm_flux", [_VP, _DP, _DP, ctypes.c_int], ctypes.c_int),
+    ("hm_modified_solution", [_VP, _DP, _DP], ctypes.c_int),
+    ("hm_interpolate", [_VP, _DP, _DP, _DP, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    ("hm_upwind", [_VP, _DP, _DP, _DP, _DP, _DP, _DP, ctypes.c_int], ctypes.c_int),
+    ("hm_flux_difference", [_VP, _DP, _DP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    ("hm_axpy", [_VP, _DP, ctypes.c_double, _DP, _SZ], ctypes.c_int),
+    ("hm_fill", [_VP, _DP, ctypes.c_double, _SZ], ctypes.c_int),
 ]
 
 

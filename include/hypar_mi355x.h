@@ -28,6 +28,7 @@
  *   hm_cfl ............. HyPar::ComputeCFL = NavierStokes3DComputeCFL
  *                        (include/hypar.h:269, NavierStokes3DComputeCFL.c:16-55)
  *   hm_step_norm ....... TimePostStep norm (src/TimeIntegration/TimePostStep.c:43-63)
+ *   hm_flux ... hm_fill  the per-slot operators (see below)
  */
 #ifndef HYPAR_MI355X_H
 #define HYPAR_MI355X_H
@@ -48,15 +49,15 @@ typedef enum {
 } hm_status;
 
 /* hyp_interp_type of solver.inp (src/Simulation/ReadInputs.c) */
-typedef enum { HM_INTERP_COMPONENTS = 0, HM_INTERP_CHARACTERISTIC = 1 } hm_interp;
+typedef enum { HM_INTERP_COMPONENTS = 0, HM_INTERP_CHARACTERISTIC = 1 } hm_interp_kind;
 /* upwinding of physics.inp (NavierStokes3DInitialize.c:162-333) */
-typedef enum { HM_UPWIND_ROE = 0, HM_UPWIND_RUSANOV = 1 } hm_upwind;
+typedef enum { HM_UPWIND_ROE = 0, HM_UPWIND_RUSANOV = 1 } hm_upwind_kind;
 /* host array layouts accepted by the transfer calls */
 typedef enum {
   HM_LAYOUT_REFERENCE = 0, /* AoS, variables fastest, 3 ghost layers (HyPar CPU) */
   HM_LAYOUT_AOS_INTERIOR = 1,/* AoS, variables fastest, interior only          */
   HM_LAYOUT_SOA_INTERIOR = 2 /* hm layout                                        */
-} hm_layout;
+} hm_layout_kind;
 
 typedef struct {
   int dim[3];          /* interior grid size (each >= 5)                          */
@@ -65,8 +66,8 @@ typedef struct {
                           NavierStokes3DInitialize.c:368 does                     */
   double Minf;         /* physics.inp Minf                                        */
   double Pr;           /* physics.inp Pr                                          */
-  int interp;          /* hm_interp                                               */
-  int upwind;          /* hm_upwind                                               */
+  int interp;          /* hm_interp_kind                                             */
+  int upwind;          /* hm_upwind_kind                                             */
   double weno_eps;     /* weno.inp epsilon (Jiang-Shu weights, p = 2)             */
   const double *dxinv; /* optional host array, HyPar::dxinv layout: x, y, z
                           concatenated, each with 3 ghost entries per side.
@@ -111,6 +112,48 @@ hm_status hm_parabolic(hm_solver *s, double *par, const double *u);
 hm_status hm_cfl(hm_solver *s, const double *u, double dt, double *cfl);
 /* sqrt(sum((a-b)^2) / ncell) over all 5 variables (TimePostStep norm). */
 hm_status hm_step_norm(hm_solver *s, const double *a, const double *b, double *norm);
+
+/* ---- device memory (so that host code needs no HIP headers) ---------- */
+/* Allocate / free n doubles of device memory on the current device. */
+hm_status hm_malloc(double **dev, size_t n);
+void hm_free(double *dev);
+/* Synchronous copies of n doubles: host->device, device->host, device->device. */
+hm_status hm_copy_to_device(double *dev, const double *host, size_t n);
+hm_status hm_copy_to_host(double *host, const double *dev, size_t n);
+hm_status hm_copy_device(hm_solver *s, double *dst, const double *src, size_t n);
+/* Wait for all work enqueued on the solver's stream. */
+hm_status hm_synchronize(hm_solver *s);
+
+/* ---- per-slot operators -------------------------------------------------
+ * HyPar composes its hyperbolic term from function-pointer slots
+ * (ReconstructHyperbolic, src/HyParFunctions/HyperbolicFunction.c:167-222).
+ * Each entry below is one slot on hm-layout device arrays; the scheme
+ * (component-wise/characteristic, Roe/Rusanov) is the solver's.  Interface
+ * arrays store interface i-1/2 at the index of cell i (periodic lines hold
+ * n interfaces). */
+/* FFunction = NavierStokes3DFlux (NavierStokes3DFlux.c:23-57; include/hypar.h:276) */
+hm_status hm_flux(hm_solver *s, double *f, const double *u, int dir);
+/* UFunction = NavierStokes3DModifiedSolution (NavierStokes3DModifiedSolution.c:31-70;
+ * include/hypar.h:321) */
+hm_status hm_modified_solution(hm_solver *s, double *uC, const double *u);
+/* InterpolateInterfacesHyp = Interp1PrimFifthOrderWENO[Char] with the weights
+ * of SetInterpLimiterVar = WENOFifthOrderCalculateWeights[Char]
+ * (include/hypar.h:224,234; Interp1PrimFifthOrderWENO.c:74-168,
+ * Interp1PrimFifthOrderWENOChar.c:86-198).  upw > 0 left-biased, < 0
+ * right-biased; uflag = 1 when fC is the (modified) solution, whose weights
+ * then come from the raw solution u. */
+hm_status hm_interpolate(hm_solver *s, double *fI, const double *fC, const double *u, int upw,
+                         int dir, int uflag);
+/* Upwind = NavierStokes3DUpwindRoe / NavierStokes3DUpwindRusanov
+ * (NavierStokes3DUpwind.c:40-124,349-420; include/hypar.h:295) */
+hm_status hm_upwind(hm_solver *s, double *fI, const double *fL, const double *fR,
+                    const double *uL, const double *uR, const double *u, int dir);
+/* hyp (+)= dxinv*(F_{i+1/2}-F_{i-1/2}) along dir (HyperbolicFunction.c:94-101) */
+hm_status hm_flux_difference(hm_solver *s, double *hyp, const double *fI, int dir, int accumulate);
+/* y += a*x and x = value on n doubles (_ArrayAXPY_, _ArraySetValue_,
+ * include/arrayfunctions.h:271-275) */
+hm_status hm_axpy(hm_solver *s, const double *x, double a, double *y, size_t n);
+hm_status hm_fill(hm_solver *s, double *x, double value, size_t n);
 
 /* Kernel timing for roofline reports (no counterpart in the reference; the
  * reference's GPU_STAT build prints per-kernel cudaEvent times,

@@ -38,4 +38,5 @@ def workload_from_meta(meta):
     return decks.Workload(meta["name"], tuple(meta["dims"]), interp=meta["interp"],
                           upwind=meta["upwind"], Re=meta["Re"], Minf=meta["Minf"], Pr=meta["Pr"],
                           gamma=meta["gamma"], weno_eps=meta["eps"], n_iter=meta["steps"],
-                          seed=meta["seed"], field=meta["field"], dt=meta["dt"])
+                          seed=meta["seed"], field=meta["field"], dt=meta["dt"],
+                          time_scheme_type=meta.get("time_scheme_type", "44"))

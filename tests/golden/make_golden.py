@@ -41,6 +41,9 @@ REF_BIN = os.path.join(ROOT, "oracle", "_ref", "HyPar")
 
 
 def _workload(kind: str, dims, n_iter: int) -> decks.Workload:
+    if kind == "c0_ssprk3":
+        return dataclasses.replace(decks.config0(dims[0], n_iter), dims=tuple(dims), n_iter=n_iter,
+                                   dt=None, time_scheme_type="ssprk3")
     if kind == "c0":
         w = decks.config0(dims[0], n_iter)
     elif kind == "c1":
@@ -61,6 +64,7 @@ FIXTURES = {
     "c0_64_s100": ("c0", (64, 64, 64), 100, False),
     "c1_32_s20": ("c1", (32, 32, 32), 20, False),
     "c1_64_s200": ("c1", (64, 64, 64), 200, False),
+    "c0_ssprk3_16_s4": ("c0_ssprk3", (16, 16, 16), 4, True),
 }
 
 ENSEMBLE = {
@@ -97,7 +101,8 @@ def make_single(name: str):
     meta = {"name": name, "scheme": kind, "dims": list(dims), "steps": steps, "dt": w.dt,
             "seed": w.seed, "interp": w.interp, "upwind": w.upwind, "Re": w.Re,
             "Minf": w.Minf, "Pr": w.Pr, "gamma": w.gamma, "eps": w.weno_eps,
-            "field": w.field, "reference_wall_s": wall, "reference_threads": 1,
+            "field": w.field, "time_scheme_type": w.time_scheme_type,
+            "reference_wall_s": wall, "reference_threads": 1,
             "u0_summary": summary(u0), "uN_summary": summary(uN)}
     arrays = {"meta": np.array(json.dumps(meta))}
     if full:
