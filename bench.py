@@ -26,9 +26,17 @@ sys.path.insert(0, ROOT)
 METRIC = "NS3D WENO5-RK4 FP64 cell-updates/s at 256³; adjoint/forward time ratio"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
-# algorithmic HBM bytes of one interface-flux launch per cell: read the 5
-# conserved variables once, write the 5 interface-flux components once
-IFACE_BYTES_PER_CELL = 80
+# algorithmic HBM bytes per cell of one launch of each kernel of an RK stage
+# (5 FP64 conserved variables = 40 B; viscous runs add the viscous flux of
+# the pass direction, 4 components = 32 B, and the z pass writes the four
+# primitive variables of the new state, 32 B):
+#   pass_x     read U_s, write R
+#   pass_y     read U_s and R, write R
+#   pass_z     read U_s, R, u, acc; write acc, U_{s+1}   (stage-1 form)
+#   visc_flux  read Q (4 primitives), write the viscous fluxes of x, y, z (12)
+#   prims      read u, write Q
+PASS_BYTES = {"pass_x": (80, 32), "pass_y": (120, 32), "pass_z": (240, 64), "visc_flux": (128, 0),
+              "prims": (72, 0)}
 # algorithmic HBM bytes per cell-update of a fused 3N RK4 step (5 FP64
 # variables; stage 1 reads u, writes acc and U2; stages 2-3 read U_s, u, acc,
 # write acc, U_{s+1}; stage 4 reads U4, acc, writes u)
@@ -68,6 +76,52 @@ def cpu_baseline(n=64, steps=2):
             "sample": f"CPU oracle, same scheme on a periodic {n}^3 grid, {steps} RK4 steps, {t:.2f} s"}
 
 
+def adjoint_ratio(n=128, nsteps=50, every=10):
+    """Adjoint/forward time ratio on the differentiation workload: inviscid
+    NS3D on a periodic n^3 grid, component-wise WENO5-JS + Rusanov, RK4 at
+    CFL 0.5; J = kinetic energy after nsteps steps, gradient with respect to
+    the initial density perturbation, a checkpoint every `every` steps.
+    Ratio = (forward with checkpoint stores + seed of dJ/du_N + reverse
+    sweep with segment recomputation) / (plain forward of the same steps),
+    both timed with events on the solver's stream."""
+    import torch
+
+    from hypar_amd import adjoint, decks
+    from hypar_amd.solver import NS3D
+    w = decks.config3(n, nsteps)
+    base, drho = adjoint.density_control_problem(w)
+    rho, vel, p = base
+    u0 = adjoint.conserved_state(rho + drho, vel, p)
+    dt = decks.cfl_dt(u0.view(5, n, n, n).permute(1, 2, 3, 0).cpu().numpy(), w.dims)
+    s = NS3D(w.dims, "components", "rusanov")
+    dV = adjoint.cell_volumes(w.dims)
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    u = u0.clone()
+    s.rk4(u, dt, 2)  # warm
+    u = u0.clone()
+    torch.cuda.synchronize()
+    ev[0].record(stream)
+    s.rk4(u, dt, nsteps)
+    ev[1].record(stream)
+    v = u0.clone()
+    ev[2].record(stream)
+    s.adjoint_forward(v, dt, nsteps, every)
+    uN = v.detach().requires_grad_(True)
+    (lam,) = torch.autograd.grad(adjoint.kinetic_energy(uN, w.dims, dV), uN)
+    lam = lam.contiguous()
+    s.adjoint_reverse(lam)
+    ev[3].record(stream)
+    torch.cuda.synchronize()
+    fwd, adj = ev[0].elapsed_time(ev[1]), ev[2].elapsed_time(ev[3])
+    same = bool(torch.equal(u, v))
+    s.close()
+    return {"workload": f"inviscid NS3D periodic {n}^3, WENO5-JS components + Rusanov, RK4, {nsteps} steps, "
+                        f"checkpoint every {every}, dJ/du0 of the final kinetic energy",
+            "forward_ms": fwd, "adjoint_ms": adj, "ratio": adj / fwd,
+            "forward_states_agree": same}
+
+
 def copy_probe(mib=1024, reps=3):
     """A plain device copy of known size (torch copy_ of `mib` MiB), the
     calibration run for the FETCH_SIZE / WRITE_SIZE counters."""
@@ -89,6 +143,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -97,7 +152,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this system has one GPU; --gpus {args.gpus} is not supported "
-              f"(the path would shard as described in DESIGN.md).", file=sys.stderr)
+              f"(no multi-GPU path is built; every figure is for one GPU).", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -130,22 +185,25 @@ def main():
 
     # per-kernel durations (HIP events on the solver's stream)
     kt = s.profile_kernels(u, reps=5)
-    per_step = {k: 4 * v for k, v in kt.items()}  # each kernel runs once per stage
+    per_step = {k: 4 * v for k, v in kt.items() if k != "prims"}  # once per stage
     dom = max(per_step, key=per_step.get)
-    if dom.startswith("iface"):
-        bytes_launch = IFACE_BYTES_PER_CELL * ncell
-    elif dom == "visc_flux":
-        bytes_launch = (40 + 96) * ncell
-    else:
-        bytes_launch = (15 * 8 + 3 * 40) * ncell
+    base, visc_extra = PASS_BYTES[dom]
+    bytes_launch = (base + visc_extra) * ncell
     achieved = bytes_launch / (kt[dom] / 1e3) / 1e9
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{args.n}.json")
-    if os.path.exists(tfile):  # rocprofv3 FETCH_SIZE/WRITE_SIZE passes, copy-calibrated
-        traffic = json.load(open(tfile))["bytes_per_launch"].get(dom)
+    prof = {}
+    pfile = os.path.join(ROOT, "profiles", f"counters_{args.n}.json")
+    if os.path.exists(pfile):  # rocprofv3 PMC passes of this workload (see the file)
+        prof = json.load(open(pfile))
+    traffic = prof.get("hbm_bytes_per_launch", {}).get(dom)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "avg_launch_ms": kt[dom], "algorithmic_bytes_per_launch": bytes_launch}
+    flops = prof.get("fp64_flops_per_launch", {}).get(dom)
+    roofline_fp64 = None
+    if flops:
+        tf = flops / (kt[dom] / 1e3) / 1e12
+        roofline_fp64 = {"bound": "fp64-valu", "kernel": dom, "achieved": tf, "peak": FP64_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": tf / FP64_PEAK_TFLOPS, "flops_per_launch": flops}
     step_bw = value * STEP_BYTES_PER_CELL_UPDATE / 1e9
     line = {
         "metric": METRIC,
@@ -164,6 +222,7 @@ def main():
                                f"4th-order viscous (Re 1600, Pr 0.72, Mach 0.1), RK4 CFL 0.5",
                    "grid": [args.n] * 3, "dt": dt, "final_cfl": cfl},
         "roofline": roofline,
+        "roofline_fp64": roofline_fp64,
         "step_roofline": {"bytes_per_cell_update": STEP_BYTES_PER_CELL_UPDATE,
                           "achieved_GBs": step_bw, "peak_GBs": HBM_PEAK_GBS,
                           "frac": step_bw / HBM_PEAK_GBS},
@@ -172,6 +231,8 @@ def main():
     s.close()
     del u
     torch.cuda.empty_cache()
+    if not args.no_adjoint:
+        line["adjoint"] = adjoint_ratio()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

@@ -15,6 +15,10 @@
 #include "hm_internal.h"
 
 struct hm_solver {
+  hm_solver() : g(), sc(), device(0), stream(nullptr), dxg(nullptr), fI{nullptr, nullptr, nullptr},
+                FV(nullptr), FVg{nullptr, nullptr, nullptr}, Q(nullptr), S{nullptr, nullptr},
+                acc(nullptr), part(nullptr), part_host(nullptr), npart(0), tmp(nullptr), tmp_elems(0),
+                tape_nsteps(0), tape_every(0), tape_dt(0.0) {}
   hm::Grid g;
   hm::Scheme sc;
   int device;
@@ -22,6 +26,8 @@ struct hm_solver {
   double *dxg;        // ghosted dxinv x|y|z on the device
   double *fI[3];      // interface fluxes, 5*ncell each
   double *FV;         // viscous fluxes, 12*ncell (viscous runs only)
+  double *FVg[3];     // ghost-layer viscous fluxes per direction (viscous only)
+  double *Q;          // primitive variables u,v,w,T, 4*ncell (viscous only)
   double *S[2];       // RK stage states (double-buffered)
   double *acc;        // RK accumulator
   double *part;       // reduction partials (device)
@@ -29,6 +35,10 @@ struct hm_solver {
   unsigned npart;
   double *tmp;        // layout-conversion staging
   size_t tmp_elems;
+  // adjoint tape: checkpoints, segment stage states, scratch
+  std::vector<double *> ckpt, stages, vscratch;
+  int tape_nsteps, tape_every;
+  double tape_dt;
 };
 
 namespace {
@@ -101,9 +111,40 @@ hm_status check_state(const hm_solver *s, const void *p, const char *name) {
 
 // right-hand side pieces of the stage state U into fI / FV
 hm_status stage_fluxes(hm_solver *s, const double *U) {
-  for (int d = 0; d < 3; d++) hm::launch_iface(s->g, s->sc, d, U, s->fI[d], s->stream);
-  if (s->sc.Re > 0) hm::launch_visc_flux(s->g, s->sc, U, s->FV, s->dxg, s->stream);
+  const bool visc = s->sc.Re > 0;
+  for (int d = 0; d < 3; d++) hm::launch_iface(s->g, s->sc, d, U, s->fI[d], visc ? s->Q : nullptr, s->stream);
+  if (visc) hm::launch_visc_flux(s->g, s->sc, s->Q, s->FV, s->FVg, s->dxg, s->stream);
   return check_launch("interface/viscous flux kernels");
+}
+
+// "rk 44" tableau (TimeExplicitRKInitialize.c:58-68)
+const double RK_A_SUB[3] = {0.5, 0.5, 1.0};  // A[s+1][s]
+const double RK_B[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+
+// One RK4 step of u in place through the fused direction passes.  Stage s
+// reads Sx[s-1] (u for s = 0) and writes Sx[s]; the last stage writes u.
+// Viscous runs need the primitive variables of u in Q on entry and leave
+// those of the new u there.
+hm_status rk4_step(hm_solver *s, double *u, double dt, double *const Sx[3]) {
+  const bool visc = s->sc.Re > 0;
+  for (int stage = 0; stage < 4; stage++) {
+    hm::PassBuffers b;
+    b.U = stage == 0 ? u : Sx[stage - 1];
+    b.R = s->fI[0];
+    b.FV = s->FV;
+    b.FVg = s->FVg;
+    b.u = u;
+    b.S = stage < 3 ? Sx[stage] : nullptr;
+    b.acc = s->acc;
+    b.out = nullptr;
+    b.Q = s->Q;
+    if (visc) hm::launch_visc_flux(s->g, s->sc, s->Q, s->FV, s->FVg, s->dxg, s->stream);
+    const double ca = stage < 3 ? dt * RK_A_SUB[stage] : 0.0, cb = dt * RK_B[stage];
+    for (int d = 0; d < 3; d++) hm::launch_pass(s->g, s->sc, d, b, hm::OUT_STAGE, stage, ca, cb, s->dxg, s->stream);
+    hm_status st = check_launch("RK4 stage passes");
+    if (st != HM_OK) return st;
+  }
+  return HM_OK;
 }
 
 hm_status ensure_tmp(hm_solver *s, size_t n) {
@@ -125,10 +166,13 @@ const char *hm_version(void) { return "hypar-mi355x 0.1 (gfx950, fp64)"; }
 
 void hm_destroy(hm_solver *s) {
   if (!s) return;
-  double *bufs[] = {s->dxg, s->fI[0], s->fI[1], s->fI[2], s->FV, s->S[0], s->S[1], s->acc, s->part, s->tmp};
+  double *bufs[] = {s->dxg, s->fI[0], s->fI[1], s->fI[2], s->FV, s->FVg[0], s->FVg[1], s->FVg[2], s->Q,
+                    s->S[0], s->S[1], s->acc, s->part, s->tmp};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   if (s->part_host) (void)hipHostFree(s->part_host);
+  for (auto *v : {&s->ckpt, &s->stages, &s->vscratch})
+    for (double *b : *v) (void)hipFree(b);
   delete s;
 }
 
@@ -147,7 +191,6 @@ hm_status hm_create(const hm_config *cfg, hm_solver **out) {
     return fail(HM_ERR_ARG, "hm_create: viscous runs need Minf > 0 and Pr > 0");
 
   hm_solver *s = new hm_solver();
-  std::memset(s, 0, sizeof(*s));
   for (int d = 0; d < 3; d++) s->g.n[d] = cfg->dim[d];
   s->g.ncell = (size_t)cfg->dim[0] * cfg->dim[1] * cfg->dim[2];
   s->sc.interp = cfg->interp == HM_INTERP_CHARACTERISTIC ? hm::INTERP_CHAR : hm::INTERP_COMP;
@@ -179,7 +222,12 @@ hm_status hm_create(const hm_config *cfg, hm_solver **out) {
   if ((st = dmalloc(&s->dxg, ndx)) != HM_OK) goto bad;
   for (int d = 0; d < 3; d++)
     if ((st = dmalloc(&s->fI[d], n5)) != HM_OK) goto bad;
-  if (s->sc.Re > 0 && (st = dmalloc(&s->FV, 12 * s->g.ncell)) != HM_OK) goto bad;
+  if (s->sc.Re > 0) {
+    if ((st = dmalloc(&s->FV, 12 * s->g.ncell)) != HM_OK) goto bad;
+    if ((st = dmalloc(&s->Q, 4 * s->g.ncell)) != HM_OK) goto bad;
+    for (int d = 0; d < 3; d++)
+      if ((st = dmalloc(&s->FVg[d], hm::visc_ghost_size(s->g, d))) != HM_OK) goto bad;
+  }
   if ((st = dmalloc(&s->S[0], n5)) != HM_OK) goto bad;
   if ((st = dmalloc(&s->S[1], n5)) != HM_OK) goto bad;
   if ((st = dmalloc(&s->acc, n5)) != HM_OK) goto bad;
@@ -257,21 +305,11 @@ hm_status hm_rk4(hm_solver *s, double *u, double dt, int nsteps) {
   if (st != HM_OK) return st;
   if (nsteps < 0) return fail(HM_ERR_ARG, "hm_rk4: nsteps < 0");
   if (!std::isfinite(dt)) return fail(HM_ERR_ARG, "hm_rk4: dt is not finite");
-  // "rk 44" tableau (TimeExplicitRKInitialize.c:58-68)
-  const double A_sub[3] = {0.5, 0.5, 1.0};  // A[s+1][s]
-  const double b[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
-  for (int it = 0; it < nsteps; it++) {
-    for (int stage = 0; stage < 4; stage++) {
-      const double *Ust = stage == 0 ? u : s->S[(stage - 1) % 2];
-      double *Snext = s->S[stage % 2];
-      if ((st = stage_fluxes(s, Ust)) != HM_OK) return st;
-      const double ca = stage < 3 ? dt * A_sub[stage] : 0.0;
-      const double cb = dt * b[stage];
-      hm::launch_stage(s->g, s->sc, s->fI[0], s->fI[1], s->fI[2], Ust, s->FV, u, Snext, s->acc,
-                       nullptr, hm::OUT_STAGE, s->dxg, stage, ca, cb, s->stream);
-      if ((st = check_launch("hm_rk4 stage")) != HM_OK) return st;
-    }
-  }
+  if (nsteps == 0) return HM_OK;
+  if (s->sc.Re > 0) hm::launch_prims(s->g, s->sc, u, s->Q, s->stream);
+  double *const Sx[3] = {s->S[0], s->S[1], s->S[0]};
+  for (int it = 0; it < nsteps; it++)
+    if ((st = rk4_step(s, u, dt, Sx)) != HM_OK) return st;
   return HM_OK;
 }
 
@@ -280,13 +318,30 @@ static hm_status rhs_part(hm_solver *s, double *out, const double *u, int mode, 
   if (st != HM_OK) return st;
   if ((st = check_state(s, out, name)) != HM_OK) return st;
   if ((st = stage_fluxes(s, u)) != HM_OK) return st;
-  hm::launch_stage(s->g, s->sc, s->fI[0], s->fI[1], s->fI[2], u, s->FV, nullptr, nullptr, nullptr, out,
-                   mode, s->dxg, 0, 0.0, 0.0, s->stream);
+  hm::launch_stage(s->g, s->sc, s->fI[0], s->fI[1], s->fI[2], s->FV, s->FVg, nullptr, nullptr, nullptr,
+                   out, mode, s->dxg, 0, 0.0, 0.0, s->stream);
   return check_launch(name);
 }
 
 hm_status hm_rhs(hm_solver *s, double *rhs, const double *u) {
-  return rhs_part(s, rhs, u, hm::OUT_RHS, "hm_rhs");
+  hm_status st = check_state(s, u, "hm_rhs: u");
+  if (st != HM_OK) return st;
+  if ((st = check_state(s, rhs, "hm_rhs: rhs")) != HM_OK) return st;
+  if (rhs == u) return fail(HM_ERR_ARG, "hm_rhs: rhs must not alias u");
+  const bool visc = s->sc.Re > 0;
+  if (visc) {
+    hm::launch_prims(s->g, s->sc, u, s->Q, s->stream);
+    hm::launch_visc_flux(s->g, s->sc, s->Q, s->FV, s->FVg, s->dxg, s->stream);
+  }
+  hm::PassBuffers b;
+  b.U = u;
+  b.R = s->fI[0];
+  b.FV = s->FV;
+  b.FVg = s->FVg;
+  b.u = b.S = b.acc = b.Q = nullptr;
+  b.out = rhs;
+  for (int d = 0; d < 3; d++) hm::launch_pass(s->g, s->sc, d, b, hm::OUT_RHS, 0, 0.0, 0.0, s->dxg, s->stream);
+  return check_launch("hm_rhs");
 }
 hm_status hm_hyperbolic(hm_solver *s, double *hyp, const double *u) {
   return rhs_part(s, hyp, u, hm::OUT_HYP, "hm_hyperbolic");
@@ -433,6 +488,140 @@ hm_status hm_fill(hm_solver *s, double *x, double value, size_t n) {
   return check_launch("hm_fill");
 }
 
+// ---------------------------------------------------------------------------
+// discrete adjoint
+// ---------------------------------------------------------------------------
+
+static hm_status ensure_buffers(std::vector<double *> &v, size_t count, size_t n) {
+  while (v.size() < count) {
+    double *p = nullptr;
+    hm_status st = dmalloc(&p, n);
+    if (st != HM_OK) return st;
+    v.push_back(p);
+  }
+  return HM_OK;
+}
+
+// one RK4 step of u in place keeping the stage states U1, U2, U3
+static hm_status rk4_step_store(hm_solver *s, double *u, double dt, double *const U[3]) {
+  if (s->sc.Re > 0) hm::launch_prims(s->g, s->sc, u, s->Q, s->stream);
+  return rk4_step(s, u, dt, U);
+}
+
+// lam <- step adjoint given the stage states U[0..3] (U[0] = u_n)
+static hm_status rk4_step_vjp(hm_solver *s, double *const U[4], double dt, double *lam) {
+  const size_t n5 = 5 * s->g.ncell;
+  hm_status st = ensure_buffers(s->vscratch, 3, n5);
+  if (st != HM_OK) return st;
+  double *mu = s->vscratch[0], *nu = s->vscratch[1], *L = s->vscratch[2];
+  HM_HIP(hipMemcpyAsync(L, lam, n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+  for (int stage = 3; stage >= 0; stage--) {
+    // mu = b_s lam' + a_{s+1,s} nu_{s+1}
+    HM_HIP(hipMemcpyAsync(mu, lam, n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+    hm::launch_axpby(nu, stage < 3 ? dt * RK_A_SUB[stage] : 0.0, mu, dt * RK_B[stage], n5, s->stream);
+    hm::launch_rhs_vjp(s->g, s->sc, U[stage], mu, nu, s->dxg, s->stream);
+    hm::launch_axpy(nu, 1.0, L, n5, s->stream);
+    if ((st = check_launch("rk4_step_vjp")) != HM_OK) return st;
+  }
+  HM_HIP(hipMemcpyAsync(lam, L, n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+  return HM_OK;
+}
+
+static hm_status check_vjp(hm_solver *s, const char *name) {
+  if (!s) return fail(HM_ERR_ARG, std::string(name) + ": solver is NULL");
+  if (!hm::vjp_supported(s->g, s->sc))
+    return fail(HM_ERR_UNSUPPORTED, std::string(name) +
+                                        ": the adjoint is built for the inviscid component-wise WENO5 + "
+                                        "Rusanov scheme with grid sides <= 512");
+  return HM_OK;
+}
+
+hm_status hm_rhs_vjp(hm_solver *s, double *out, const double *u, const double *lam) {
+  hm_status st;
+  if ((st = check_vjp(s, "hm_rhs_vjp")) != HM_OK) return st;
+  if ((st = check_state(s, out, "hm_rhs_vjp: out")) != HM_OK) return st;
+  if ((st = check_state(s, u, "hm_rhs_vjp: u")) != HM_OK) return st;
+  if ((st = check_state(s, lam, "hm_rhs_vjp: lam")) != HM_OK) return st;
+  hm::launch_rhs_vjp(s->g, s->sc, u, lam, out, s->dxg, s->stream);
+  return check_launch("hm_rhs_vjp");
+}
+
+hm_status hm_rk4_vjp(hm_solver *s, const double *u_n, double dt, double *lam) {
+  hm_status st;
+  if ((st = check_vjp(s, "hm_rk4_vjp")) != HM_OK) return st;
+  if ((st = check_state(s, u_n, "hm_rk4_vjp: u_n")) != HM_OK) return st;
+  if ((st = check_state(s, lam, "hm_rk4_vjp: lam")) != HM_OK) return st;
+  const size_t n5 = 5 * s->g.ncell;
+  if ((st = ensure_buffers(s->stages, 4, n5)) != HM_OK) return st;
+  double *U[4] = {s->stages[0], s->stages[1], s->stages[2], s->stages[3]};
+  HM_HIP(hipMemcpyAsync(U[0], u_n, n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+  // a throw-away copy advanced through the step yields the stage states
+  if ((st = ensure_buffers(s->vscratch, 4, n5)) != HM_OK) return st;
+  double *w = s->vscratch[3];
+  HM_HIP(hipMemcpyAsync(w, u_n, n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+  double *const Us[3] = {U[1], U[2], U[3]};
+  if ((st = rk4_step_store(s, w, dt, Us)) != HM_OK) return st;
+  return rk4_step_vjp(s, U, dt, lam);
+}
+
+hm_status hm_adjoint_forward(hm_solver *s, double *u, double dt, int nsteps, int every) {
+  hm_status st;
+  if ((st = check_vjp(s, "hm_adjoint_forward")) != HM_OK) return st;
+  if ((st = check_state(s, u, "hm_adjoint_forward: u")) != HM_OK) return st;
+  if (nsteps < 1 || every < 1) return fail(HM_ERR_ARG, "hm_adjoint_forward: nsteps and checkpoint_every must be >= 1");
+  const size_t n5 = 5 * s->g.ncell;
+  const int nck = (nsteps + every - 1) / every;
+  if ((st = ensure_buffers(s->ckpt, (size_t)nck, n5)) != HM_OK) return st;
+  for (int n = 0; n < nsteps; n++) {
+    if (n % every == 0)
+      HM_HIP(hipMemcpyAsync(s->ckpt[n / every], u, n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+    if ((st = hm_rk4(s, u, dt, 1)) != HM_OK) return st;
+  }
+  s->tape_nsteps = nsteps;
+  s->tape_every = every;
+  s->tape_dt = dt;
+  return HM_OK;
+}
+
+hm_status hm_adjoint_reverse(hm_solver *s, double *lam) {
+  hm_status st;
+  if ((st = check_vjp(s, "hm_adjoint_reverse")) != HM_OK) return st;
+  if ((st = check_state(s, lam, "hm_adjoint_reverse: lam")) != HM_OK) return st;
+  if (s->tape_nsteps < 1) return fail(HM_ERR_ARG, "hm_adjoint_reverse: no forward sweep recorded");
+  const int N = s->tape_nsteps, K = s->tape_every;
+  const double dt = s->tape_dt;
+  const size_t n5 = 5 * s->g.ncell;
+  if ((st = ensure_buffers(s->stages, (size_t)4 * K, n5)) != HM_OK) return st;
+  const int nck = (N + K - 1) / K;
+  for (int seg = nck - 1; seg >= 0; seg--) {
+    const int n0 = seg * K, n1 = (n0 + K < N) ? n0 + K : N;
+    // recompute the segment, keeping every step's stage states
+    for (int n = n0; n < n1; n++) {
+      double **U = &s->stages[(size_t)4 * (n - n0)];
+      // U[0] of the first step is the checkpoint; later steps find u_n in
+      // their slot, advanced there by the step before
+      if (n == n0)
+        HM_HIP(hipMemcpyAsync(U[0], s->ckpt[seg], n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+      if (n + 1 < n1) {
+        // advance a copy into the next step's U0 slot
+        double *next = s->stages[(size_t)4 * (n - n0 + 1)];
+        HM_HIP(hipMemcpyAsync(next, U[0], n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+        double *const Us[3] = {U[1], U[2], U[3]};
+        if ((st = rk4_step_store(s, next, dt, Us)) != HM_OK) return st;
+      } else {
+        if ((st = ensure_buffers(s->vscratch, 4, n5)) != HM_OK) return st;
+        double *w = s->vscratch[3];
+        HM_HIP(hipMemcpyAsync(w, U[0], n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+        double *const Us[3] = {U[1], U[2], U[3]};
+        if ((st = rk4_step_store(s, w, dt, Us)) != HM_OK) return st;
+      }
+    }
+    for (int n = n1 - 1; n >= n0; n--)
+      if ((st = rk4_step_vjp(s, &s->stages[(size_t)4 * (n - n0)], dt, lam)) != HM_OK) return st;
+  }
+  return HM_OK;
+}
+
 hm_status hm_profile_kernels(hm_solver *s, const double *u, int reps, double *ms) {
   hm_status st = check_state(s, u, "hm_profile_kernels: u");
   if (st != HM_OK) return st;
@@ -451,21 +640,34 @@ hm_status hm_profile_kernels(hm_solver *s, const double *u, int reps, double *ms
     ms[slot] = (double)t / reps;
     return check_launch("hm_profile_kernels");
   };
-  for (int d = 0; d < 3 && st == HM_OK; d++)
-    st = timed(d, [&] { hm::launch_iface(s->g, s->sc, d, u, s->fI[d], s->stream); });
-  if (st == HM_OK) {
-    if (s->sc.Re > 0)
-      st = timed(3, [&] { hm::launch_visc_flux(s->g, s->sc, u, s->FV, s->dxg, s->stream); });
-    else
-      ms[3] = 0.0;
+  const bool visc = s->sc.Re > 0;
+  if (visc) {
+    hm::launch_prims(s->g, s->sc, u, s->Q, s->stream);
+    hm::launch_visc_flux(s->g, s->sc, s->Q, s->FV, s->FVg, s->dxg, s->stream);
   }
-  // the stage kernel as an RK step runs it (stage 1: reads u and acc,
-  // writes acc and the next stage state); u itself is not written
-  if (st == HM_OK)
-    st = timed(4, [&] {
-      hm::launch_stage(s->g, s->sc, s->fI[0], s->fI[1], s->fI[2], u, s->FV, const_cast<double *>(u),
-                       s->S[1], s->acc, nullptr, hm::OUT_STAGE, s->dxg, 1, 0.0, 0.0, s->stream);
-    });
+  // the passes as stage 1 of an RK step runs them: the z pass reads u and
+  // acc and writes acc, the next stage state (and Q); u is not written
+  HM_HIP(hipMemsetAsync(s->acc, 0, 5 * s->g.ncell * sizeof(double), s->stream));
+  hm::PassBuffers b;
+  b.U = u;
+  b.R = s->fI[0];
+  b.FV = s->FV;
+  b.FVg = s->FVg;
+  b.u = const_cast<double *>(u);
+  b.S = s->S[1];
+  b.acc = s->acc;
+  b.out = nullptr;
+  b.Q = s->fI[1];  // primitive variables of the stage state go to scratch
+  for (int d = 0; d < 3 && st == HM_OK; d++)
+    st = timed(d, [&] { hm::launch_pass(s->g, s->sc, d, b, hm::OUT_STAGE, 1, 0.0, 0.0, s->dxg, s->stream); });
+  if (st == HM_OK) {
+    if (visc) {
+      st = timed(3, [&] { hm::launch_visc_flux(s->g, s->sc, s->Q, s->FV, s->FVg, s->dxg, s->stream); });
+      if (st == HM_OK) st = timed(4, [&] { hm::launch_prims(s->g, s->sc, u, s->fI[1], s->stream); });
+    } else {
+      ms[3] = ms[4] = 0.0;
+    }
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return st;

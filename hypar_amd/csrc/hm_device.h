@@ -184,4 +184,52 @@ __device__ __forceinline__ double weno5(double m3, double m2, double m1, double 
   return (a1 * f1 + a2 * f2 + a3 * f3) / (a1 + a2 + a3);
 }
 
+// periodic index, i in [-n, 2n)
+__device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Remap the hardware block id so that every XCD works through one
+// contiguous range of logical blocks: neighbouring tiles, which share
+// stencil rows, then hit the same L2.
+__device__ __forceinline__ size_t xcd_block() {
+  const unsigned b = blockIdx.x, nb = gridDim.x;
+  const unsigned q = nb >> 3, r = nb & 7u, x = b & 7u, k = b >> 3;
+  return (size_t)((x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k);
+}
+
+// 4th-order central difference (FirstDerivativeFourthOrder.c:103)
+__device__ __forceinline__ double d4c(double fm2, double fm1, double fp1, double fp2) {
+  return (fm2 - 8 * fm1 + 8 * fp1 - fp2) * (1.0 / 12.0);
+}
+
+// d/dx_DIR of the viscous flux at cell idx, not yet scaled by dxinv:
+// (F(-2) - 8F(-1) + 8F(+1) - F(+2))/12 with the ghost-layer values of FVg
+// where the stencil leaves the interior.  FV[(DIR*4+c)*ncell + cell];
+// FVg[(slot*4+c)*nplane + line], slot 0..3 for positions -2,-1,n,n+1 of the
+// line (line = j+ny*k along x, i+nx*k along y, i+nx*j along z).
+template <int DIR>
+__device__ __forceinline__ void visc_div_dir(double out[4], const double *__restrict__ FV,
+                                             const double *__restrict__ FVg, size_t ncell, size_t idx,
+                                             int pos, int n, size_t stride, size_t line) {
+  const size_t base = idx - (size_t)pos * stride;
+  const size_t nplane = ncell / n;
+  double F[4][4];  // offsets -2,-1,+1,+2
+  const int offs[4] = {-2, -1, 1, 2};
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    const int gp = pos + offs[s];
+    if (gp >= 0 && gp < n) {
+      const size_t c = base + (size_t)gp * stride;
+#pragma unroll
+      for (int v = 0; v < 4; v++) F[s][v] = FV[(size_t)(DIR * 4 + v) * ncell + c];
+    } else {
+      const int slot = gp < 0 ? gp + 2 : gp - n + 2;
+#pragma unroll
+      for (int v = 0; v < 4; v++) F[s][v] = FVg[(size_t)(slot * 4 + v) * nplane + line];
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 4; v++) out[v] = d4c(F[0][v], F[1][v], F[2][v], F[3][v]);
+}
+
 }  // namespace hm

@@ -23,16 +23,20 @@ struct Scheme {
   double eps;
 };
 
-// interface fluxes along dir of the hm-layout state u into fI (5*ncell)
-void launch_iface(const Grid &g, const Scheme &sc, int dir, const double *u, double *fI,
+// interface fluxes along dir of the hm-layout state u into fI (5*ncell); the
+// x pass also stores the primitive variables (u,v,w,T) into Q (4*ncell)
+// when Q is not null
+void launch_iface(const Grid &g, const Scheme &sc, int dir, const double *u, double *fI, double *Q,
                   hipStream_t st);
-// viscous fluxes (12*ncell) of state u
-void launch_visc_flux(const Grid &g, const Scheme &sc, const double *u, double *FV,
+// doubles of the ghost viscous-flux array of direction dir
+size_t visc_ghost_size(const Grid &g, int dir);
+// viscous fluxes (12*ncell) and ghost-layer viscous fluxes from Q
+void launch_visc_flux(const Grid &g, const Scheme &sc, const double *Q, double *FV, double *const FVg[3],
                       const double *dxg, hipStream_t st);
 // flux differences, viscous divergence and RK4 stage update (mode OUT_STAGE),
 // or the right-hand side / hyperbolic / parabolic term into `out`
 void launch_stage(const Grid &g, const Scheme &sc, const double *fIx, const double *fIy,
-                  const double *fIz, const double *Ustage, const double *FV, double *u, double *S,
+                  const double *fIz, const double *FV, double *const FVg[3], double *u, double *S,
                   double *acc, double *out, int mode, const double *dxg, int stage, double ca,
                   double cb, hipStream_t st);
 unsigned reduction_blocks(size_t n);
@@ -55,5 +59,29 @@ void launch_fluxdiff(const Grid &g, int dir, double *hyp, const double *fI, cons
                      int accumulate, hipStream_t st);
 void launch_axpy(const double *x, double a, double *y, size_t n, hipStream_t st);
 void launch_fill(double *x, double value, size_t n, hipStream_t st);
+
+// fused direction passes (hm_pass.hip)
+struct PassBuffers {
+  const double *U;        // stage state
+  double *R;              // rhs accumulator, 5*ncell
+  const double *FV;       // viscous fluxes (viscous runs)
+  double *const *FVg;     // ghost-layer viscous fluxes per direction (viscous runs)
+  double *u, *S, *acc;    // RK4 state, next stage state, accumulator (z pass, OUT_STAGE)
+  double *out;            // right-hand side (z pass, OUT_RHS)
+  double *Q;              // primitive variables of the new state (z pass, viscous OUT_STAGE)
+};
+// pass along dir: x initialises R, y accumulates, z finishes the stage
+// (mode OUT_STAGE with stage, ca = dt*A[s+1][s], cb = dt*b[s]) or writes
+// the right-hand side (mode OUT_RHS)
+void launch_pass(const Grid &g, const Scheme &sc, int dir, const PassBuffers &b, int mode, int stage, double ca,
+                 double cb, const double *dxg, hipStream_t st);
+// primitive variables (u,v,w,T) of u into Q
+void launch_prims(const Grid &g, const Scheme &sc, const double *u, double *Q, hipStream_t st);
+
+// discrete adjoint (hm_adjoint.hip)
+bool vjp_supported(const Grid &g, const Scheme &sc);
+void launch_rhs_vjp(const Grid &g, const Scheme &sc, const double *u, const double *lam, double *out,
+                    const double *dxg, hipStream_t st);
+void launch_axpby(const double *x, double a, double *y, double b, size_t n, hipStream_t st);
 
 }  // namespace hm

@@ -7,8 +7,10 @@
 //   k_iface<DIR,...>   interface fluxes F_{i-1/2} along DIR for every cell i
 //                      (Roe/Rusanov upwinding of WENO5 reconstructions);
 //                      HyperbolicFunction.c:75-93 with ReconstructHyperbolic
-//   k_visc_flux        viscous fluxes of x, y, z at every cell
+//   k_visc_flux        viscous fluxes of x, y, z at every cell from the
+//                      primitive variables the x pass stores
 //                      (NavierStokes3DParabolicFunction.c:85-195)
+//   k_visc_ghost<DIR>  viscous fluxes at the reference's ghost layers
 //   k_stage            flux differences in x, y, z, the viscous divergence,
 //                      rhs = -hyp + par, and the RK4 stage update in 3N form
 //                      (TimeRHSFunctionExplicit.c:88-93, TimeRK.c:126-195)
@@ -18,11 +20,6 @@
 #include "hm_internal.h"
 
 namespace hm {
-
-__device__ __forceinline__ int wrap(int i, int n) {
-  // i in [-n, 2n)
-  return i < 0 ? i + n : (i >= n ? i - n : i);
-}
 
 // ---------------------------------------------------------------------------
 // interface flux kernel
@@ -124,12 +121,15 @@ __device__ __forceinline__ void iface_flux(const double (&U)[6][NV], double gamm
   }
 }
 
-// one thread per interface i-1/2 (stored at the index of cell i)
+// one thread per interface i-1/2 (stored at the index of cell i).  The
+// x-direction pass also stores the primitive variables (u, v, w, T) of its
+// cell for the viscous kernels (Q != nullptr).
 template <int DIR, int INTERP, int UPW>
 __global__ void __launch_bounds__(256) k_iface(const double *__restrict__ u, double *__restrict__ fI,
-                                               int nx, int ny, int nz, double gamma, double eps) {
+                                               double *__restrict__ Q, int nx, int ny, int nz,
+                                               double gamma, double eps) {
   const size_t ncell = (size_t)nx * ny * nz;
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t idx = xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= ncell) return;
   const int i = (int)(idx % nx);
   const int j = (int)((idx / nx) % ny);
@@ -145,6 +145,13 @@ __global__ void __launch_bounds__(256) k_iface(const double *__restrict__ u, dou
 #pragma unroll
     for (int v = 0; v < NV; v++) U[s][v] = u[(size_t)v * ncell + c];
   }
+  if (Q) {
+    Prim q = flowvar(U[3], gamma);
+    Q[idx] = q.vx;
+    Q[ncell + idx] = q.vy;
+    Q[2 * ncell + idx] = q.vz;
+    Q[3 * ncell + idx] = gamma * q.P / q.rho;
+  }
   double f[NV];
   iface_flux<DIR, INTERP, UPW>(U, gamma, eps, f);
 #pragma unroll
@@ -155,26 +162,19 @@ __global__ void __launch_bounds__(256) k_iface(const double *__restrict__ u, dou
 // viscous terms
 // ---------------------------------------------------------------------------
 // Primitive variables used by the viscous term: (u, v, w, T) with
-// T = gamma*P/rho (NavierStokes3DParabolicFunction.c:85-108).
+// T = gamma*P/rho (NavierStokes3DParabolicFunction.c:85-108), stored as
+// four arrays Q[c*ncell + cell].
 struct QT {
   double u, v, w, T;
 };
 
-__device__ __forceinline__ QT load_q(const double *__restrict__ U, size_t ncell, size_t c,
-                                     double gamma) {
-  double s[NV];
-#pragma unroll
-  for (int v = 0; v < NV; v++) s[v] = U[(size_t)v * ncell + c];
-  Prim p = flowvar(s, gamma);
+__device__ __forceinline__ QT load_q(const double *__restrict__ Q, size_t ncell, size_t c) {
   QT q;
-  q.u = p.vx; q.v = p.vy; q.w = p.vz;
-  q.T = gamma * p.P / p.rho;
+  q.u = Q[c];
+  q.v = Q[ncell + c];
+  q.w = Q[2 * ncell + c];
+  q.T = Q[3 * ncell + c];
   return q;
-}
-
-// 4th-order central difference (FirstDerivativeFourthOrder.c:103)
-__device__ __forceinline__ double d4c(double fm2, double fm1, double fp1, double fp2) {
-  return (fm2 - 8 * fm1 + 8 * fp1 - fp2) * (1.0 / 12.0);
 }
 
 struct ViscParams {
@@ -217,12 +217,12 @@ __device__ __forceinline__ void visc_flux(double fv[4], const QT &q, const doubl
 // Viscous fluxes of x, y, z at every cell: FV[(d*4+c)*ncell + cell],
 // c = momentum x, y, z, energy.  Gradients use the 4th-order central
 // difference with periodic neighbours, scaled by the cell's dxinv.
-__global__ void __launch_bounds__(256) k_visc_flux(const double *__restrict__ u, double *__restrict__ FV,
+__global__ void __launch_bounds__(256) k_visc_flux(const double *__restrict__ Q, double *__restrict__ FV,
                                                    int nx, int ny, int nz,
                                                    const double *__restrict__ dxg,  // ghosted dxinv x|y|z
                                                    ViscParams vp) {
   const size_t ncell = (size_t)nx * ny * nz;
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t idx = xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= ncell) return;
   const int i = (int)(idx % nx);
   const int j = (int)((idx / nx) % ny);
@@ -231,15 +231,15 @@ __global__ void __launch_bounds__(256) k_visc_flux(const double *__restrict__ u,
   const int pos[3] = {i, j, k};
   const size_t st[3] = {1, (size_t)nx, (size_t)nx * ny};
   const double *dxi[3] = {dxg + 3, dxg + (nx + 6) + 3, dxg + (nx + 6) + (ny + 6) + 3};
-  QT q0 = load_q(u, ncell, idx, vp.gamma);
+  QT q0 = load_q(Q, ncell, idx);
   double gr[4][3];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     const size_t base = idx - (size_t)pos[d] * st[d];
-    QT qm2 = load_q(u, ncell, base + (size_t)wrap(pos[d] - 2, n[d]) * st[d], vp.gamma);
-    QT qm1 = load_q(u, ncell, base + (size_t)wrap(pos[d] - 1, n[d]) * st[d], vp.gamma);
-    QT qp1 = load_q(u, ncell, base + (size_t)wrap(pos[d] + 1, n[d]) * st[d], vp.gamma);
-    QT qp2 = load_q(u, ncell, base + (size_t)wrap(pos[d] + 2, n[d]) * st[d], vp.gamma);
+    QT qm2 = load_q(Q, ncell, base + (size_t)wrap(pos[d] - 2, n[d]) * st[d]);
+    QT qm1 = load_q(Q, ncell, base + (size_t)wrap(pos[d] - 1, n[d]) * st[d]);
+    QT qp1 = load_q(Q, ncell, base + (size_t)wrap(pos[d] + 1, n[d]) * st[d]);
+    QT qp2 = load_q(Q, ncell, base + (size_t)wrap(pos[d] + 2, n[d]) * st[d]);
     const double s = dxi[d][pos[d]];
     gr[0][d] = d4c(qm2.u, qm1.u, qp1.u, qp2.u) * s;
     gr[1][d] = d4c(qm2.v, qm1.v, qp1.v, qp2.v) * s;
@@ -256,8 +256,8 @@ __global__ void __launch_bounds__(256) k_visc_flux(const double *__restrict__ u,
   for (int c = 0; c < 4; c++) FV[(size_t)(2 * 4 + c) * ncell + idx] = fv[c];
 }
 
-// Viscous flux along DIR at a ghost position gp in {-2,-1,n,n+1} of the line
-// through cell `base` (other indices interior).  The reference evaluates it
+// Viscous flux along DIR at the ghost positions gp in {-2,-1,n,n+1} of every
+// line along DIR (other indices interior), as the reference evaluates it
 // from its ghost layers: primitive values are the periodic copies, the
 // derivative along DIR uses the biased stencils of
 // FirstDerivativeFourthOrder.c:74-93/106-125 in the second-outermost layer
@@ -265,69 +265,59 @@ __global__ void __launch_bounds__(256) k_visc_flux(const double *__restrict__ u,
 // nearest interior value, InitialSolution.c:101-113), and the derivatives
 // across DIR are zero because the reference computes them only on lines whose
 // other indices are interior.
+// Output FVg[(slot*4 + c)*nplane + line], slot 0..3 for gp = -2,-1,n,n+1.
 template <int DIR>
-__device__ void visc_flux_ghost(double fv[4], const double *__restrict__ u, size_t ncell,
-                                size_t base, size_t stride, int n, int gp, double dx_ghost,
-                                const ViscParams &vp) {
-  QT f[5];  // positions gp-2 .. gp+2 (periodic copies)
+__global__ void __launch_bounds__(256) k_visc_ghost(const double *__restrict__ Q, double *__restrict__ FVg,
+                                                    int nx, int ny, int nz, const double *__restrict__ dxg,
+                                                    ViscParams vp) {
+  const size_t ncell = (size_t)nx * ny * nz;
+  const int n = DIR == 0 ? nx : (DIR == 1 ? ny : nz);
+  const size_t nplane = ncell / n;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 4 * nplane) return;
+  const int slot = (int)(t / nplane);
+  const size_t line = t % nplane;
+  size_t base, stride;
+  if (DIR == 0) { base = line * nx; stride = 1; }                          // line = j + ny*k
+  else if (DIR == 1) { base = (line % nx) + (line / nx) * (size_t)nx * ny; stride = nx; }  // i + nx*k
+  else { base = line; stride = (size_t)nx * ny; }                           // i + nx*j
+  const int gp = slot < 2 ? slot - 2 : n + slot - 2;
+  const double *dxd = dxg + (DIR >= 1 ? nx + 6 : 0) + (DIR == 2 ? ny + 6 : 0);
+  QT f[7];  // positions gp-3 .. gp+3 (periodic copies)
 #pragma unroll
-  for (int s = 0; s < 5; s++) f[s] = load_q(u, ncell, base + (size_t)wrap(gp - 2 + s, n) * stride, vp.gamma);
+  for (int s = 0; s < 7; s++) f[s] = load_q(Q, ncell, base + (size_t)wrap(gp - 3 + s, n) * stride);
   double d[4];
+  const double r12 = 1.0 / 12.0;
   if (gp == -2) {
     // (-3 f(-3) - 10 f(-2) + 18 f(-1) - 6 f(0) + f(1)) / 12
-    QT f5 = load_q(u, ncell, base + (size_t)wrap(gp + 3, n) * stride, vp.gamma);
-    d[0] = (-3 * f[1].u - 10 * f[2].u + 18 * f[3].u - 6 * f[4].u + f5.u) * (1.0 / 12.0);
-    d[1] = (-3 * f[1].v - 10 * f[2].v + 18 * f[3].v - 6 * f[4].v + f5.v) * (1.0 / 12.0);
-    d[2] = (-3 * f[1].w - 10 * f[2].w + 18 * f[3].w - 6 * f[4].w + f5.w) * (1.0 / 12.0);
-    d[3] = (-3 * f[1].T - 10 * f[2].T + 18 * f[3].T - 6 * f[4].T + f5.T) * (1.0 / 12.0);
+    d[0] = (-3 * f[2].u - 10 * f[3].u + 18 * f[4].u - 6 * f[5].u + f[6].u) * r12;
+    d[1] = (-3 * f[2].v - 10 * f[3].v + 18 * f[4].v - 6 * f[5].v + f[6].v) * r12;
+    d[2] = (-3 * f[2].w - 10 * f[3].w + 18 * f[4].w - 6 * f[5].w + f[6].w) * r12;
+    d[3] = (-3 * f[2].T - 10 * f[3].T + 18 * f[4].T - 6 * f[5].T + f[6].T) * r12;
   } else if (gp == n + 1) {
     // (-f(n-2) + 6 f(n-1) - 18 f(n) + 10 f(n+1) + 3 f(n+2)) / 12
-    QT fm3 = load_q(u, ncell, base + (size_t)wrap(gp - 3, n) * stride, vp.gamma);
-    d[0] = (-fm3.u + 6 * f[0].u - 18 * f[1].u + 10 * f[2].u + 3 * f[3].u) * (1.0 / 12.0);
-    d[1] = (-fm3.v + 6 * f[0].v - 18 * f[1].v + 10 * f[2].v + 3 * f[3].v) * (1.0 / 12.0);
-    d[2] = (-fm3.w + 6 * f[0].w - 18 * f[1].w + 10 * f[2].w + 3 * f[3].w) * (1.0 / 12.0);
-    d[3] = (-fm3.T + 6 * f[0].T - 18 * f[1].T + 10 * f[2].T + 3 * f[3].T) * (1.0 / 12.0);
+    d[0] = (-f[0].u + 6 * f[1].u - 18 * f[2].u + 10 * f[3].u + 3 * f[4].u) * r12;
+    d[1] = (-f[0].v + 6 * f[1].v - 18 * f[2].v + 10 * f[3].v + 3 * f[4].v) * r12;
+    d[2] = (-f[0].w + 6 * f[1].w - 18 * f[2].w + 10 * f[3].w + 3 * f[4].w) * r12;
+    d[3] = (-f[0].T + 6 * f[1].T - 18 * f[2].T + 10 * f[3].T + 3 * f[4].T) * r12;
   } else {
-    d[0] = d4c(f[0].u, f[1].u, f[3].u, f[4].u);
-    d[1] = d4c(f[0].v, f[1].v, f[3].v, f[4].v);
-    d[2] = d4c(f[0].w, f[1].w, f[3].w, f[4].w);
-    d[3] = d4c(f[0].T, f[1].T, f[3].T, f[4].T);
+    d[0] = d4c(f[1].u, f[2].u, f[4].u, f[5].u);
+    d[1] = d4c(f[1].v, f[2].v, f[4].v, f[5].v);
+    d[2] = d4c(f[1].w, f[2].w, f[4].w, f[5].w);
+    d[3] = d4c(f[1].T, f[2].T, f[4].T, f[5].T);
   }
+  const double dxgh = dxd[3 + gp];  // entry 0 is position -3
   double gr[4][3];
 #pragma unroll
   for (int c = 0; c < 4; c++)
 #pragma unroll
-    for (int e = 0; e < 3; e++) gr[c][e] = (e == DIR) ? d[c] * dx_ghost : 0.0;
-  const QT &q = f[2];
+    for (int e = 0; e < 3; e++) gr[c][e] = (e == DIR) ? d[c] * dxgh : 0.0;
+  const QT &q = f[3];
   const double mu = exp(0.76 * log(q.T));
+  double fv[4];
   visc_flux<DIR>(fv, q, gr, mu, vp);
-}
-
-// d/dx_DIR of the viscous flux at cell idx, not yet scaled by dxinv:
-// (F(-2) - 8F(-1) + 8F(+1) - F(+2))/12 with ghost values where the stencil
-// leaves the interior.
-template <int DIR>
-__device__ __forceinline__ void visc_div_dir(double out[4], const double *__restrict__ u,
-                                             const double *__restrict__ FV, size_t ncell, size_t idx,
-                                             int pos, int n, size_t stride, const double *dxg_dir,
-                                             const ViscParams &vp) {
-  const size_t base = idx - (size_t)pos * stride;
-  double F[4][4];  // offsets -2,-1,+1,+2
-  const int offs[4] = {-2, -1, 1, 2};
 #pragma unroll
-  for (int s = 0; s < 4; s++) {
-    const int gp = pos + offs[s];
-    if (gp >= 0 && gp < n) {
-      const size_t c = base + (size_t)gp * stride;
-#pragma unroll
-      for (int v = 0; v < 4; v++) F[s][v] = FV[(size_t)(DIR * 4 + v) * ncell + c];
-    } else {
-      // ghosted dxinv array: entry 0 is position -3
-      visc_flux_ghost<DIR>(F[s], u, ncell, base, stride, n, gp, dxg_dir[3 + gp], vp);
-    }
-  }
-#pragma unroll
-  for (int v = 0; v < 4; v++) out[v] = d4c(F[0][v], F[1][v], F[2][v], F[3][v]);
+  for (int c = 0; c < 4; c++) FVg[(size_t)(slot * 4 + c) * nplane + line] = fv[c];
 }
 
 // ---------------------------------------------------------------------------
@@ -338,31 +328,32 @@ __device__ __forceinline__ void visc_div_dir(double out[4], const double *__rest
 // stage 1,2: acc = acc + cb*k; S = u + ca*k
 // stage 3: u = acc + cb*k
 // with k = -hyp + par and hyp = (hx + hy) + hz.  Other modes store k, hyp
-// or par in `out` instead.
+// or par in `out` instead.  FVg holds the ghost viscous fluxes of x, y, z
+// (4 slots x 4 components x plane each).
 __global__ void __launch_bounds__(256) k_stage(
     const double *__restrict__ fIx, const double *__restrict__ fIy, const double *__restrict__ fIz,
-    const double *__restrict__ Ustage, const double *__restrict__ FV, double *__restrict__ u,
-    double *__restrict__ S, double *__restrict__ acc, double *__restrict__ out, int mode, int nx,
-    int ny, int nz, const double *__restrict__ dxg, ViscParams vp, int viscous, int stage, double ca,
-    double cb) {
+    const double *__restrict__ FV, const double *__restrict__ FVgx, const double *__restrict__ FVgy,
+    const double *__restrict__ FVgz, double *__restrict__ u, double *__restrict__ S,
+    double *__restrict__ acc, double *__restrict__ out, int mode, int nx, int ny, int nz,
+    const double *__restrict__ dxg, int viscous, int stage, double ca, double cb) {
   const size_t ncell = (size_t)nx * ny * nz;
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t idx = xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= ncell) return;
   const int i = (int)(idx % nx);
   const int j = (int)((idx / nx) % ny);
   const int k = (int)(idx / ((size_t)nx * ny));
   const double *dxi = dxg + 3, *dyi = dxg + (nx + 6) + 3, *dzi = dxg + (nx + 6) + (ny + 6) + 3;
   const size_t ip = idx - i + wrap(i + 1, nx);
-  const size_t jp = idx + (size_t)(wrap(j + 1, ny) - j) * nx;
-  const size_t kp = idx + ((size_t)wrap(k + 1, nz) - k) * (size_t)nx * ny;
+  const size_t jp = idx - (size_t)j * nx + (size_t)wrap(j + 1, ny) * nx;
+  const size_t kp = idx - (size_t)k * nx * ny + (size_t)wrap(k + 1, nz) * nx * ny;
   double par[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
   if (viscous) {
     double dv[4];
-    visc_div_dir<0>(dv, Ustage, FV, ncell, idx, i, nx, 1, dxg, vp);
+    visc_div_dir<0>(dv, FV, FVgx, ncell, idx, i, nx, 1, (size_t)j + (size_t)ny * k);
     for (int v = 0; v < 4; v++) par[v + 1] = dxi[i] * dv[v];
-    visc_div_dir<1>(dv, Ustage, FV, ncell, idx, j, ny, (size_t)nx, dxg + (nx + 6), vp);
+    visc_div_dir<1>(dv, FV, FVgy, ncell, idx, j, ny, (size_t)nx, (size_t)i + (size_t)nx * k);
     for (int v = 0; v < 4; v++) par[v + 1] += dyi[j] * dv[v];
-    visc_div_dir<2>(dv, Ustage, FV, ncell, idx, k, nz, (size_t)nx * ny, dxg + (nx + 6) + (ny + 6), vp);
+    visc_div_dir<2>(dv, FV, FVgz, ncell, idx, k, nz, (size_t)nx * ny, (size_t)i + (size_t)nx * j);
     for (int v = 0; v < 4; v++) par[v + 1] += dzi[k] * dv[v];
   }
 #pragma unroll
@@ -488,29 +479,29 @@ __global__ void k_soa_to_aos(const double *__restrict__ soa, double *__restrict_
 static inline unsigned nblocks(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 
 template <int DIR, int INTERP, int UPW>
-static void launch_iface_t(const Grid &g, const double *u, double *fI, double gamma, double eps,
-                           hipStream_t st) {
-  hipLaunchKernelGGL((k_iface<DIR, INTERP, UPW>), dim3(nblocks(g.ncell, 256)), dim3(256), 0, st, u, fI,
+static void launch_iface_t(const Grid &g, const double *u, double *fI, double *Q, double gamma,
+                           double eps, hipStream_t st) {
+  hipLaunchKernelGGL((k_iface<DIR, INTERP, UPW>), dim3(nblocks(g.ncell, 256)), dim3(256), 0, st, u, fI, Q,
                      g.n[0], g.n[1], g.n[2], gamma, eps);
 }
 
 template <int DIR>
-static void launch_iface_d(const Grid &g, const Scheme &sc, const double *u, double *fI,
+static void launch_iface_d(const Grid &g, const Scheme &sc, const double *u, double *fI, double *Q,
                            hipStream_t st) {
   if (sc.interp == INTERP_CHAR) {
-    if (sc.upwind == UPWIND_ROE) launch_iface_t<DIR, INTERP_CHAR, UPWIND_ROE>(g, u, fI, sc.gamma, sc.eps, st);
-    else launch_iface_t<DIR, INTERP_CHAR, UPWIND_RUSANOV>(g, u, fI, sc.gamma, sc.eps, st);
+    if (sc.upwind == UPWIND_ROE) launch_iface_t<DIR, INTERP_CHAR, UPWIND_ROE>(g, u, fI, Q, sc.gamma, sc.eps, st);
+    else launch_iface_t<DIR, INTERP_CHAR, UPWIND_RUSANOV>(g, u, fI, Q, sc.gamma, sc.eps, st);
   } else {
-    if (sc.upwind == UPWIND_ROE) launch_iface_t<DIR, INTERP_COMP, UPWIND_ROE>(g, u, fI, sc.gamma, sc.eps, st);
-    else launch_iface_t<DIR, INTERP_COMP, UPWIND_RUSANOV>(g, u, fI, sc.gamma, sc.eps, st);
+    if (sc.upwind == UPWIND_ROE) launch_iface_t<DIR, INTERP_COMP, UPWIND_ROE>(g, u, fI, Q, sc.gamma, sc.eps, st);
+    else launch_iface_t<DIR, INTERP_COMP, UPWIND_RUSANOV>(g, u, fI, Q, sc.gamma, sc.eps, st);
   }
 }
 
-void launch_iface(const Grid &g, const Scheme &sc, int dir, const double *u, double *fI,
+void launch_iface(const Grid &g, const Scheme &sc, int dir, const double *u, double *fI, double *Q,
                   hipStream_t st) {
-  if (dir == 0) launch_iface_d<0>(g, sc, u, fI, st);
-  else if (dir == 1) launch_iface_d<1>(g, sc, u, fI, st);
-  else launch_iface_d<2>(g, sc, u, fI, st);
+  if (dir == 0) launch_iface_d<0>(g, sc, u, fI, Q, st);
+  else if (dir == 1) launch_iface_d<1>(g, sc, u, fI, nullptr, st);
+  else launch_iface_d<2>(g, sc, u, fI, nullptr, st);
 }
 
 static ViscParams visc_params(const Scheme &sc) {
@@ -522,19 +513,28 @@ static ViscParams visc_params(const Scheme &sc) {
   return vp;
 }
 
-void launch_visc_flux(const Grid &g, const Scheme &sc, const double *u, double *FV,
+size_t visc_ghost_size(const Grid &g, int dir) { return 16 * (g.ncell / g.n[dir]); }
+
+void launch_visc_flux(const Grid &g, const Scheme &sc, const double *Q, double *FV, double *const FVg[3],
                       const double *dxg, hipStream_t st) {
-  hipLaunchKernelGGL(k_visc_flux, dim3(nblocks(g.ncell, 256)), dim3(256), 0, st, u, FV, g.n[0], g.n[1],
-                     g.n[2], dxg, visc_params(sc));
+  const ViscParams vp = visc_params(sc);
+  hipLaunchKernelGGL(k_visc_flux, dim3(nblocks(g.ncell, 256)), dim3(256), 0, st, Q, FV, g.n[0], g.n[1],
+                     g.n[2], dxg, vp);
+  hipLaunchKernelGGL(k_visc_ghost<0>, dim3(nblocks(4 * (g.ncell / g.n[0]), 256)), dim3(256), 0, st, Q,
+                     FVg[0], g.n[0], g.n[1], g.n[2], dxg, vp);
+  hipLaunchKernelGGL(k_visc_ghost<1>, dim3(nblocks(4 * (g.ncell / g.n[1]), 256)), dim3(256), 0, st, Q,
+                     FVg[1], g.n[0], g.n[1], g.n[2], dxg, vp);
+  hipLaunchKernelGGL(k_visc_ghost<2>, dim3(nblocks(4 * (g.ncell / g.n[2]), 256)), dim3(256), 0, st, Q,
+                     FVg[2], g.n[0], g.n[1], g.n[2], dxg, vp);
 }
 
 void launch_stage(const Grid &g, const Scheme &sc, const double *fIx, const double *fIy,
-                  const double *fIz, const double *Ustage, const double *FV, double *u, double *S,
+                  const double *fIz, const double *FV, double *const FVg[3], double *u, double *S,
                   double *acc, double *out, int mode, const double *dxg, int stage, double ca,
                   double cb, hipStream_t st) {
-  hipLaunchKernelGGL(k_stage, dim3(nblocks(g.ncell, 256)), dim3(256), 0, st, fIx, fIy, fIz, Ustage, FV,
-                     u, S, acc, out, mode, g.n[0], g.n[1], g.n[2], dxg, visc_params(sc),
-                     sc.Re > 0 ? 1 : 0, stage, ca, cb);
+  hipLaunchKernelGGL(k_stage, dim3(nblocks(g.ncell, 256)), dim3(256), 0, st, fIx, fIy, fIz, FV,
+                     FVg ? FVg[0] : nullptr, FVg ? FVg[1] : nullptr, FVg ? FVg[2] : nullptr, u, S, acc,
+                     out, mode, g.n[0], g.n[1], g.n[2], dxg, sc.Re > 0 ? 1 : 0, stage, ca, cb);
 }
 
 unsigned reduction_blocks(size_t n) {

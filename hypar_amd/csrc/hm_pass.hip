@@ -1,0 +1,638 @@
+// hm_pass.hip -- fused direction passes of the NS3D WENO5 right-hand side
+// and RK4 stage (gfx950, FP64).
+//
+// One RK stage is three launches, one per direction (plus the viscous-flux
+// kernel of hm_kernels.hip first when the run is viscous):
+//   pass x:  R  = -dxinv (F_{i+1/2} - F_{i-1/2}) + dxinv d/dx FV_x
+//   pass y:  R  = R - dyinv (G_{j+1/2} - G_{j-1/2}) + dyinv d/dy FV_y
+//   pass z:  r  = R - dzinv (H_{k+1/2} - H_{k-1/2}) + dzinv d/dz FV_z,
+//            then the RK4 stage update in 3N form (or r itself for hm_rhs)
+// i.e. rhs = -hyp + par of TimeRHSFunctionExplicit.c:88-93 with the
+// hyperbolic term of HyperbolicFunction.c:75-101 and the parabolic term of
+// NavierStokes3DParabolicFunction.c:50-325, summed direction by direction.
+//
+// Each pass stages the cells of its pencils in LDS once, evaluates every
+// cell's flux and modified energy once, computes the interface fluxes
+// (WENO5-JS, component-wise or characteristic, with Roe or Rusanov
+// upwinding) from LDS, and differences them in LDS: interface fluxes never
+// reach HBM.
+//  * x pass: one wavefront per x-line, chunks of 64 interfaces along x.
+//  * y, z passes: a block owns 64 consecutive columns (x fastest) of the
+//    plane across y (z), which may span several x-rows of that plane, and
+//    marches along y (z) four rows per step, with a 9-row ring of cell data
+//    and a 5-row ring of interface fluxes in LDS.
+// Output of cell c needs the interfaces c-1/2 and c+1/2; a step emits the
+// cells one row (lane) behind its interfaces, and the last cell of a line,
+// whose right interface is the periodic image of interface 0, is emitted
+// after the march.
+//
+// Restatements relative to the reference arithmetic (equal in exact
+// arithmetic; rounding-level differences only):
+//  * 1/rho, the Jiang-Shu normalisation and the Roe-average weights use a
+//    Newton-Raphson-refined reciprocal (two steps r <- r + r(1 - d r) from
+//    the hardware estimate); velocities are m * (1/rho).
+//  * The mass flux of a cell is its momentum m_d (the reference forms
+//    rho * (m_d / rho)).
+//  * The smoothness indicators are evaluated as 4*beta with 4*eps, which
+//    scales every (beta_k + eps)^2 by 16 and leaves the weights unchanged;
+//    the second differences shared by the left- and right-biased
+//    indicators of an interface are formed once.
+//  * The characteristic projections use the structure of the left
+//    eigenvectors (navierstokes3d.h:288-374): with
+//    w = v.x_{1..3} - x_4, rows 0, d and 4 are
+//      L00 x0 + (g-1)/a^2 w,  Ld0 x0 - (g-1)/(2a^2) w -/+ a/(2a^2) x_d,
+//    and the two shear rows have two non-zero entries; the product with the
+//    right eigenvectors (navierstokes3d.h:384-470) is expanded the same
+//    way.  The speed of sound of the Roe state is sqrt((g-1)(H - |v|^2/2)),
+//    the reference's sqrt(g P / rho) with P from H.
+#include <hip/hip_runtime.h>
+
+#include "hm_device.h"
+#include "hm_internal.h"
+
+namespace hm {
+
+namespace {
+
+constexpr int NQ = 10;  // per-cell LDS data: U0..U4, F1..F4 (F0 = U_d), modified energy
+
+// 1/d, d > 0 and normal: hardware estimate refined by two Newton-Raphson
+// steps, each of which squares the relative error.
+__device__ __forceinline__ double nr_rcp(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = __builtin_fma(-d, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-d, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+
+struct PassConst {
+  int n[3];
+  size_t ncell;
+  double gamma, gm1, inv_gm1, eps4;
+  const double *dxg;  // ghosted dxinv x|y|z
+};
+
+struct PassIO {
+  const double *U;    // stage state
+  double *R;          // rhs accumulator
+  const double *FV;   // viscous fluxes of all directions, FV[(d*4+c)*ncell + cell]
+  const double *FVg;  // ghost-layer viscous fluxes of the pass direction
+  double *u, *S, *acc, *out, *Q;
+  int visc, mode, stage;
+  double ca, cb;
+};
+
+// per-cell data of the pass direction (NavierStokes3DFlux.c:23-57 and
+// NavierStokes3DModifiedSolution.c:31-70)
+template <int DIR>
+__device__ __forceinline__ void cell_data(const double u[NV], const PassConst &k, double c[NQ]) {
+  const double rho = u[0];
+  const double ir = rho == 0.0 ? 0.0 : nr_rcp(rho);
+  const double vx = u[1] * ir, vy = u[2] * ir, vz = u[3] * ir;
+  const double vsq = vx * vx + vy * vy + vz * vz;
+  const double P = (u[4] - 0.5 * rho * vsq) * k.gm1;
+  const double vn = DIR == 0 ? vx : (DIR == 1 ? vy : vz);
+  const double m = u[DIR + 1];
+#pragma unroll
+  for (int q = 0; q < NV; q++) c[q] = u[q];
+  c[5] = m * vx + (DIR == 0 ? P : 0.0);
+  c[6] = m * vy + (DIR == 1 ? P : 0.0);
+  c[7] = m * vz + (DIR == 2 ? P : 0.0);
+  c[8] = (u[4] + P) * vn;
+  c[9] = (P * k.inv_gm1) + (0.5 * rho * vsq);
+}
+
+// Jiang-Shu WENO5 on both sides of one interface.  b[0..5] are the values of
+// cells I-3..I+2 the smoothness indicators use, g[0..5] the values that are
+// interpolated (include/interpolation.h:265-285,
+// Interp1PrimFifthOrderWENO.c:146-158; the right-biased stencil is the
+// mirror image, WENOFifthOrderCalculateWeights.c:166-177).
+__device__ __forceinline__ double weno_combine(double b1, double b2, double b3, double g3, double g2,
+                                               double g1, double q1, double q2, double eps4) {
+  const double s6 = 1.0 / 6.0;
+  const double s1 = (b1 + eps4) * (b1 + eps4);
+  const double s2 = (b2 + eps4) * (b2 + eps4);
+  const double s3 = (b3 + eps4) * (b3 + eps4);
+  const double a1 = 0.1 * (s2 * s3), a2 = 0.6 * (s1 * s3), a3 = 0.3 * (s1 * s2);
+  const double f1 = (2 * s6) * g3 + (-7 * s6) * g2 + (11 * s6) * g1;
+  const double f2 = (-s6) * g2 + (5 * s6) * g1 + (2 * s6) * q1;
+  const double f3 = (2 * s6) * g1 + (5 * s6) * q1 + (-s6) * q2;
+  return (a1 * f1 + a2 * f2 + a3 * f3) * nr_rcp(a1 + a2 + a3);
+}
+
+__device__ __forceinline__ void weno5_lr(const double (&b)[6], const double (&g)[6], double eps4, double &vl,
+                                         double &vr) {
+  const double c13 = 13.0 / 3.0;  // 4 * 13/12
+  const double d1 = b[0] - 2 * b[1] + b[2], d2 = b[1] - 2 * b[2] + b[3];
+  const double d3 = b[2] - 2 * b[3] + b[4], d4 = b[3] - 2 * b[4] + b[5];
+  const double q2 = c13 * (d2 * d2), q3 = c13 * (d3 * d3);
+  double e;
+  e = b[0] - 4 * b[1] + 3 * b[2];
+  const double l1 = c13 * (d1 * d1) + e * e;
+  e = b[1] - b[3];
+  const double l2 = q2 + e * e;
+  e = 3 * b[2] - 4 * b[3] + b[4];
+  const double l3 = q3 + e * e;
+  e = b[5] - 4 * b[4] + 3 * b[3];
+  const double r1 = c13 * (d4 * d4) + e * e;
+  e = b[4] - b[2];
+  const double r2 = q3 + e * e;
+  e = 3 * b[3] - 4 * b[2] + b[1];
+  const double r3 = q2 + e * e;
+  vl = weno_combine(l1, l2, l3, g[0], g[1], g[2], g[3], g[4], eps4);
+  vr = weno_combine(r1, r2, r3, g[5], g[4], g[3], g[2], g[1], eps4);
+}
+
+// Roe-averaged state of two cells (navierstokes3d.h:144-169) and the
+// eigen-structure constants of the DIR flux Jacobian there.
+struct RoeState {
+  double vx, vy, vz, vn, a, ek, h0;
+  double L00, Lm0, L40, k0, kh, aih;  // left-eigenvector entries (see top)
+  double cL, cR;                      // sound speeds + |v_d| of the two cells
+};
+
+template <int DIR>
+__device__ __forceinline__ RoeState roe_state(const double uL[NV], const double uR[NV], const PassConst &k) {
+  RoeState s;
+  const double rL = uL[0], rR = uR[0];
+  const double iL = rL == 0.0 ? 0.0 : nr_rcp(rL), iR = rR == 0.0 ? 0.0 : nr_rcp(rR);
+  const double vxL = uL[1] * iL, vyL = uL[2] * iL, vzL = uL[3] * iL;
+  const double vxR = uR[1] * iR, vyR = uR[2] * iR, vzR = uR[3] * iR;
+  const double qL = vxL * vxL + vyL * vyL + vzL * vzL, qR = vxR * vxR + vyR * vyR + vzR * vzR;
+  const double PL = (uL[4] - 0.5 * rL * qL) * k.gm1, PR = (uR[4] - 0.5 * rR * qR) * k.gm1;
+  const double c2L = k.gamma * PL * iL, c2R = k.gamma * PR * iR;
+  const double HL = 0.5 * qL + c2L * k.inv_gm1, HR = 0.5 * qR + c2R * k.inv_gm1;
+  const double tL = sqrt(rL), tR = sqrt(rR);
+  const double inv = nr_rcp(tL + tR);
+  s.vx = (tL * vxL + tR * vxR) * inv;
+  s.vy = (tL * vyL + tR * vyR) * inv;
+  s.vz = (tL * vzL + tR * vzR) * inv;
+  const double H = (tL * HL + tR * HR) * inv;
+  s.ek = 0.5 * (s.vx * s.vx + s.vy * s.vy + s.vz * s.vz);
+  const double a2 = (H - s.ek) * k.gm1;
+  s.a = sqrt(a2);
+  s.vn = DIR == 0 ? s.vx : (DIR == 1 ? s.vy : s.vz);
+  s.h0 = a2 * k.inv_gm1 + s.ek;
+  const double ia2 = nr_rcp(a2), ih2 = 0.5 * ia2;
+  const double gek = k.gm1 * s.ek, avn = s.a * s.vn;
+  s.L00 = (a2 - gek) * ia2;
+  s.Lm0 = (gek + avn) * ih2;
+  s.L40 = (gek - avn) * ih2;
+  s.k0 = k.gm1 * ia2;
+  s.kh = k.gm1 * ih2;
+  s.aih = s.a * ih2;
+  const double vnL = DIR == 0 ? vxL : (DIR == 1 ? vyL : vzL);
+  const double vnR = DIR == 0 ? vxR : (DIR == 1 ? vyR : vzR);
+  s.cL = sqrt(c2L) + fabs(vnL);
+  s.cR = sqrt(c2R) + fabs(vnR);
+  return s;
+}
+
+// |lambda| with Harten's entropy fix, delta = 1e-6 (NavierStokes3DUpwind.c:95-101)
+__device__ __forceinline__ double harten(double l) {
+  const double delta = 0.000001, delta2 = delta * delta, inv2d = 0.5 / delta;
+  const double al = fabs(l);
+  return al < delta ? (l * l + delta2) * inv2d : al;
+}
+
+// rows 0, d, 4 of L x (u-a wave in row d = DIR+1)
+template <int DIR>
+__device__ __forceinline__ void proj_acoustic(const RoeState &s, double x0, double x1, double x2, double x3,
+                                              double x4, double &p0, double &pm, double &p4) {
+  const double w = s.vx * x1 + s.vy * x2 + s.vz * x3 - x4;
+  const double xm = DIR == 0 ? x1 : (DIR == 1 ? x2 : x3);
+  const double t = s.kh * w;
+  p0 = s.L00 * x0 + s.k0 * w;
+  pm = s.Lm0 * x0 - (t + s.aih * xm);
+  p4 = s.L40 * x0 + (s.aih * xm - t);
+}
+
+// the two shear rows of L x, in row order
+template <int DIR>
+__device__ __forceinline__ void proj_shear(const RoeState &s, double x0, double x1, double x2, double x3,
+                                           double &pa, double &pb) {
+  if (DIR == 0) {        // rows 2, 3: (vy, 0, -1, 0, 0), (-vz, 0, 0, 1, 0)
+    pa = s.vy * x0 - x2;
+    pb = x3 - s.vz * x0;
+  } else if (DIR == 1) { // rows 1, 3: (-vx, 1, 0, 0, 0), (vz, 0, 0, -1, 0)
+    pa = x1 - s.vx * x0;
+    pb = s.vz * x0 - x3;
+  } else {               // rows 1, 2: (vx, -1, 0, 0, 0), (-vy, 0, 1, 0, 0)
+    pa = s.vx * x0 - x1;
+    pb = x2 - s.vy * x0;
+  }
+}
+
+// f = R g, with g0 (entropy), gm (u-a), g4 (u+a), ga, gb (shear rows in
+// row order)
+template <int DIR>
+__device__ __forceinline__ void right_mult(const RoeState &s, double g0, double gm, double g4, double ga,
+                                           double gb, double f[NV]) {
+  const double sum = g0 + gm + g4;
+  const double dif = s.a * (g4 - gm);
+  f[0] = sum;
+  f[1] = s.vx * sum;
+  f[2] = s.vy * sum;
+  f[3] = s.vz * sum;
+  f[DIR + 1] += dif;
+  double e = s.ek * g0 + s.h0 * (gm + g4) + s.vn * dif;
+  if (DIR == 0) {        // columns 2, 3: (0,0,-1,0,-vy), (0,0,0,1,vz)
+    f[2] -= ga;
+    f[3] += gb;
+    e += s.vz * gb - s.vy * ga;
+  } else if (DIR == 1) { // columns 1, 3: (0,1,0,0,vx), (0,0,0,-1,-vz)
+    f[1] += ga;
+    f[3] -= gb;
+    e += s.vx * ga - s.vz * gb;
+  } else {               // columns 1, 2: (0,-1,0,0,-vx), (0,0,1,0,vy)
+    f[1] -= ga;
+    f[2] += gb;
+    e += s.vy * gb - s.vx * ga;
+  }
+  f[4] = e;
+}
+
+// Interface flux at I-1/2 from the LDS data of cells I-3..I+2;
+// ld(s, q) returns quantity q of cell I-3+s.
+template <int DIR, int INTERP, int UPW, class LD>
+__device__ __forceinline__ void iface(const LD &ld, const PassConst &k, double fI[NV]) {
+  double uL[NV], uR[NV];
+#pragma unroll
+  for (int q = 0; q < NV; q++) {
+    uL[q] = ld(2, q);
+    uR[q] = ld(3, q);
+  }
+  const RoeState s = roe_state<DIR>(uL, uR, k);
+  const double aA = s.a + fabs(s.vn);
+  const double alpha = fmax(fmax(s.cL, s.cR), fmax(s.cR, aA));  // NavierStokes3DUpwind.c:349-420
+
+  if (INTERP == INTERP_CHAR) {
+    // rows 0, d, 4 (WENOFifthOrderCalculateWeightsChar, Interp1PrimFifthOrderWENOChar.c:86-198)
+    double f0[6], fm[6], f4[6], u0[6], um[6], u4[6], c0[6], cm[6], c4[6];
+#pragma unroll
+    for (int p = 0; p < 6; p++) {
+      const double x0 = ld(p, 0), x1 = ld(p, 1), x2 = ld(p, 2), x3 = ld(p, 3), x4 = ld(p, 4);
+      const double F0 = ld(p, DIR + 1), F1 = ld(p, 5), F2 = ld(p, 6), F3 = ld(p, 7), F4 = ld(p, 8);
+      const double dC = ld(p, 9) - x4;
+      proj_acoustic<DIR>(s, F0, F1, F2, F3, F4, f0[p], fm[p], f4[p]);
+      proj_acoustic<DIR>(s, x0, x1, x2, x3, x4, u0[p], um[p], u4[p]);
+      c0[p] = u0[p] - s.k0 * dC;
+      cm[p] = um[p] + s.kh * dC;
+      c4[p] = u4[p] + s.kh * dC;
+    }
+    double fl, fr, ul, ur;
+    const double l0 = UPW == UPWIND_ROE ? harten(s.vn) : alpha;
+    const double lm = UPW == UPWIND_ROE ? harten(s.vn - s.a) : alpha;
+    const double l4 = UPW == UPWIND_ROE ? harten(s.vn + s.a) : alpha;
+    weno5_lr(f0, f0, k.eps4, fl, fr);
+    weno5_lr(u0, c0, k.eps4, ul, ur);
+    const double g0 = 0.5 * (fl + fr) - l0 * (0.5 * (ur - ul));
+    weno5_lr(fm, fm, k.eps4, fl, fr);
+    weno5_lr(um, cm, k.eps4, ul, ur);
+    const double gm = 0.5 * (fl + fr) - lm * (0.5 * (ur - ul));
+    weno5_lr(f4, f4, k.eps4, fl, fr);
+    weno5_lr(u4, c4, k.eps4, ul, ur);
+    const double g4 = 0.5 * (fl + fr) - l4 * (0.5 * (ur - ul));
+    // shear rows: the modified energy does not enter them (L[row][4] = 0)
+    double fa[6], fb[6], ua[6], ub[6];
+#pragma unroll
+    for (int p = 0; p < 6; p++) {
+      proj_shear<DIR>(s, ld(p, DIR + 1), ld(p, 5), ld(p, 6), ld(p, 7), fa[p], fb[p]);
+      proj_shear<DIR>(s, ld(p, 0), ld(p, 1), ld(p, 2), ld(p, 3), ua[p], ub[p]);
+    }
+    const double ls = UPW == UPWIND_ROE ? harten(s.vn) : alpha;
+    weno5_lr(fa, fa, k.eps4, fl, fr);
+    weno5_lr(ua, ua, k.eps4, ul, ur);
+    const double ga = 0.5 * (fl + fr) - ls * (0.5 * (ur - ul));
+    weno5_lr(fb, fb, k.eps4, fl, fr);
+    weno5_lr(ub, ub, k.eps4, ul, ur);
+    const double gb = 0.5 * (fl + fr) - ls * (0.5 * (ur - ul));
+    right_mult<DIR>(s, g0, gm, g4, ga, gb, fI);
+  } else {
+    // component-wise (Interp1PrimFifthOrderWENO.c:74-168)
+    double fh[NV], du[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      double b[6], g[6];
+#pragma unroll
+      for (int p = 0; p < 6; p++) b[p] = ld(p, v == 0 ? DIR + 1 : 4 + v);
+      double fl, fr, ul, ur;
+      weno5_lr(b, b, k.eps4, fl, fr);
+#pragma unroll
+      for (int p = 0; p < 6; p++) {
+        b[p] = ld(p, v);
+        g[p] = v == 4 ? ld(p, 9) : b[p];
+      }
+      weno5_lr(b, g, k.eps4, ul, ur);
+      fh[v] = 0.5 * (fl + fr);
+      du[v] = 0.5 * (ur - ul);
+    }
+    if (UPW == UPWIND_RUSANOV) {
+#pragma unroll
+      for (int v = 0; v < NV; v++) fI[v] = fh[v] - alpha * du[v];
+    } else {
+      // R |Lambda| L (uR - uL)/2 (NavierStokes3DUpwind.c:40-124)
+      double p0, pm, p4, pa, pb;
+      proj_acoustic<DIR>(s, du[0], du[1], du[2], du[3], du[4], p0, pm, p4);
+      proj_shear<DIR>(s, du[0], du[1], du[2], du[3], pa, pb);
+      const double hs = harten(s.vn);
+      double d[NV];
+      right_mult<DIR>(s, hs * p0, harten(s.vn - s.a) * pm, harten(s.vn + s.a) * p4, hs * pa, hs * pb, d);
+#pragma unroll
+      for (int v = 0; v < NV; v++) fI[v] = fh[v] - d[v];
+    }
+  }
+}
+
+// Finish cell idx of the pass: flux difference, viscous divergence of the
+// pass direction, and the pass's output.
+template <int DIR>
+__device__ __forceinline__ void emit(const PassConst &k, const PassIO &io, size_t idx, int pos, size_t line,
+                                     const double (&fm)[NV], const double (&fp)[NV]) {
+  const int n = k.n[DIR];
+  const double dxi = k.dxg[(DIR >= 1 ? k.n[0] + 6 : 0) + (DIR == 2 ? k.n[1] + 6 : 0) + 3 + pos];
+  double par[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (io.visc) {
+    const size_t stride = DIR == 0 ? 1 : (DIR == 1 ? (size_t)k.n[0] : (size_t)k.n[0] * k.n[1]);
+    double dv[4];
+    visc_div_dir<DIR>(dv, io.FV, io.FVg, k.ncell, idx, pos, n, stride, line);
+#pragma unroll
+    for (int c = 0; c < 4; c++) par[c + 1] = dxi * dv[c];
+  }
+  double r[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    const double h = dxi * (fp[v] - fm[v]);
+    const size_t o = (size_t)v * k.ncell + idx;
+    r[v] = DIR == 0 ? (-h) + par[v] : (io.R[o] - h) + par[v];
+  }
+  if (DIR < 2 || io.mode == OUT_RHS) {
+    double *dst = DIR < 2 ? io.R : io.out;
+#pragma unroll
+    for (int v = 0; v < NV; v++) dst[(size_t)v * k.ncell + idx] = r[v];
+    return;
+  }
+  // RK4 stage update in 3N form (TimeRK.c:126-195)
+  double nu[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    const size_t o = (size_t)v * k.ncell + idx;
+    if (io.stage == 0) {
+      const double u0 = io.u[o];
+      io.acc[o] = u0 + io.cb * r[v];
+      nu[v] = u0 + io.ca * r[v];
+      io.S[o] = nu[v];
+    } else if (io.stage < 3) {
+      io.acc[o] = io.acc[o] + io.cb * r[v];
+      nu[v] = io.u[o] + io.ca * r[v];
+      io.S[o] = nu[v];
+    } else {
+      nu[v] = io.acc[o] + io.cb * r[v];
+      io.u[o] = nu[v];
+    }
+  }
+  if (io.visc) {
+    // primitive variables (u, v, w, T) of the new state for the next
+    // viscous-flux evaluation (NavierStokes3DParabolicFunction.c:85-108)
+    const Prim q = flowvar(nu, k.gamma);
+    io.Q[idx] = q.vx;
+    io.Q[k.ncell + idx] = q.vy;
+    io.Q[2 * k.ncell + idx] = q.vz;
+    io.Q[3 * k.ncell + idx] = k.gamma * q.P / q.rho;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// x pass: one wavefront per x-line
+// ---------------------------------------------------------------------------
+constexpr int XCH = 64;        // interfaces per chunk
+constexpr int XNC = XCH + 6;   // cells per chunk
+constexpr int XPAD = 72;
+
+template <int INTERP, int UPW>
+__global__ void __launch_bounds__(256) k_pass_x(PassConst k, PassIO io) {
+  __shared__ double sC[4][NQ][XPAD];
+  __shared__ double sF[4][NV][XCH + 1];  // slot 0: fI[x0-1]; slot 1+l: fI[x0+l]
+  __shared__ double sF0[4][NV];          // fI[0] of the line
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nx = k.n[0];
+  const size_t nlines = (size_t)k.n[1] * k.n[2];
+  const size_t line = (size_t)blockIdx.x * 4 + w;
+  const bool active = line < nlines;
+  const size_t base = (active ? line : 0) * (size_t)nx;
+  int x0 = 0;
+  for (; x0 < nx; x0 += XCH) {
+    if (x0 > 0) {
+      // carry fI[x0-1] into slot 0 (the previous chunk's output phase is done)
+      if (lane < NV) sF[w][lane][0] = sF[w][lane][XCH];
+    }
+    const int ncells = min(XNC, nx + 6 - x0);
+    for (int c = lane; c < ncells; c += 64) {
+      const size_t cell = base + wrap(x0 - 3 + c, nx);
+      double u[NV], cd[NQ];
+#pragma unroll
+      for (int v = 0; v < NV; v++) u[v] = io.U[(size_t)v * k.ncell + cell];
+      cell_data<0>(u, k, cd);
+#pragma unroll
+      for (int q = 0; q < NQ; q++) sC[w][q][c] = cd[q];
+    }
+    __syncthreads();
+    if (active && x0 + lane < nx) {
+      auto ld = [&](int s, int q) { return sC[w][q][lane + s]; };
+      double f[NV];
+      iface<0, INTERP, UPW>(ld, k, f);
+#pragma unroll
+      for (int v = 0; v < NV; v++) sF[w][v][lane + 1] = f[v];
+      if (x0 == 0 && lane == 0)
+#pragma unroll
+        for (int v = 0; v < NV; v++) sF0[w][v] = f[v];
+    }
+    __syncthreads();
+    const int cell = x0 - 1 + lane;
+    if (active && cell >= 0 && cell <= nx - 2) {
+      double fm[NV], fp[NV];
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        fm[v] = sF[w][v][lane];
+        fp[v] = sF[w][v][lane + 1];
+      }
+      emit<0>(k, io, base + cell, cell, line, fm, fp);
+    }
+    __syncthreads();
+  }
+  // last cell: fI[nx-1] of the last chunk, fI[nx] = fI[0]
+  x0 -= XCH;
+  if (active && lane == 0) {
+    double fm[NV], fp[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      fm[v] = sF[w][v][nx - x0];
+      fp[v] = sF0[w][v];
+    }
+    emit<0>(k, io, base + nx - 1, nx - 1, line, fm, fp);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// y, z passes: 64 x-columns marching along DIR
+// ---------------------------------------------------------------------------
+constexpr int MW = 64;  // columns
+constexpr int MR = 4;   // rows per step (one per wavefront)
+constexpr int NRING = 9;
+constexpr int NFR = 5;
+
+template <int DIR, int INTERP, int UPW>
+__global__ void __launch_bounds__(256) k_pass_m(PassConst k, PassIO io) {
+  __shared__ double sC[NQ][NRING][MW];
+  __shared__ double sF[NV][NFR][MW];
+  __shared__ double sF0[NV][MW];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nx = k.n[0], ny = k.n[1];
+  const int n = k.n[DIR];
+  // the 64 lanes take 64 consecutive columns c = i + nx*o of the plane
+  // across DIR (o = z index in the y pass, y index in the z pass); c is
+  // also the line index of the ghost-layer viscous fluxes
+  const size_t ncol = k.ncell / n;
+  const size_t c = (size_t)blockIdx.x * MW + lane;
+  const bool col = c < ncol;
+  const size_t line = col ? c : ncol - 1;
+  const size_t o = line / nx;
+  const size_t stride = DIR == 1 ? (size_t)nx : (size_t)nx * ny;
+  const size_t cbase = DIR == 1 ? o * nx * ny + (line - o * nx) : line;  // cell at position 0
+
+  auto load_row = [&](int r) {  // r in [-3, n+5)
+    const size_t cell = cbase + (size_t)wrap(r, n) * stride;
+    double u[NV], cd[NQ];
+#pragma unroll
+    for (int v = 0; v < NV; v++) u[v] = io.U[(size_t)v * k.ncell + cell];
+    cell_data<DIR>(u, k, cd);
+    const int slot = (r + NRING) % NRING;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) sC[q][slot][lane] = cd[q];
+  };
+
+  // rows -3..1
+  for (int r = w; r < 5; r += MR) load_row(r - 3);
+  for (int t0 = 0; t0 < n; t0 += MR) {
+    load_row(t0 + 2 + w);
+    __syncthreads();
+    const int t = t0 + w;
+    if (t < n) {
+      const int sl0 = (t - 3 + NRING) % NRING;
+      auto ld = [&](int s, int q) {
+        int sl = sl0 + s;
+        sl = sl >= NRING ? sl - NRING : sl;
+        return sC[q][sl][lane];
+      };
+      double f[NV];
+      iface<DIR, INTERP, UPW>(ld, k, f);
+      const int fs = t % NFR;
+#pragma unroll
+      for (int v = 0; v < NV; v++) sF[v][fs][lane] = f[v];
+      if (t == 0)
+#pragma unroll
+        for (int v = 0; v < NV; v++) sF0[v][lane] = f[v];
+    }
+    __syncthreads();
+    const int e = t0 - 1 + w;  // row emitted by this wavefront
+    if (col && e >= 0 && e <= n - 2) {
+      double fm[NV], fp[NV];
+      const int a = e % NFR, b = (e + 1) % NFR;
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        fm[v] = sF[v][a][lane];
+        fp[v] = sF[v][b][lane];
+      }
+      emit<DIR>(k, io, cbase + (size_t)e * stride, e, line, fm, fp);
+    }
+  }
+  // last row: fI[n-1] and fI[n] = fI[0]
+  if (w == 0 && col) {
+    double fm[NV], fp[NV];
+    const int a = (n - 1) % NFR;
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      fm[v] = sF[v][a][lane];
+      fp[v] = sF0[v][lane];
+    }
+    emit<DIR>(k, io, cbase + (size_t)(n - 1) * stride, n - 1, line, fm, fp);
+  }
+}
+
+// primitive variables (u, v, w, T) of a state
+__global__ void __launch_bounds__(256) k_prims(const double *__restrict__ u, double *__restrict__ Q, size_t ncell,
+                                               double gamma) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ncell) return;
+  double s[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) s[v] = u[(size_t)v * ncell + idx];
+  const Prim q = flowvar(s, gamma);
+  Q[idx] = q.vx;
+  Q[ncell + idx] = q.vy;
+  Q[2 * ncell + idx] = q.vz;
+  Q[3 * ncell + idx] = gamma * q.P / q.rho;
+}
+
+template <int DIR, int INTERP, int UPW>
+void launch_t(const Grid &g, const PassConst &k, const PassIO &io, hipStream_t st) {
+  if (DIR == 0) {
+    const size_t nlines = (size_t)g.n[1] * g.n[2];
+    hipLaunchKernelGGL((k_pass_x<INTERP, UPW>), dim3((unsigned)((nlines + 3) / 4)), dim3(256), 0, st, k, io);
+  } else {
+    const size_t nb = (g.ncell / g.n[DIR] + MW - 1) / MW;
+    hipLaunchKernelGGL((k_pass_m<DIR == 0 ? 1 : DIR, INTERP, UPW>), dim3((unsigned)nb), dim3(256), 0, st, k, io);
+  }
+}
+
+template <int DIR>
+void launch_d(const Grid &g, const Scheme &sc, const PassConst &k, const PassIO &io, hipStream_t st) {
+  if (sc.interp == INTERP_CHAR) {
+    if (sc.upwind == UPWIND_ROE) launch_t<DIR, INTERP_CHAR, UPWIND_ROE>(g, k, io, st);
+    else launch_t<DIR, INTERP_CHAR, UPWIND_RUSANOV>(g, k, io, st);
+  } else {
+    if (sc.upwind == UPWIND_ROE) launch_t<DIR, INTERP_COMP, UPWIND_ROE>(g, k, io, st);
+    else launch_t<DIR, INTERP_COMP, UPWIND_RUSANOV>(g, k, io, st);
+  }
+}
+
+}  // namespace
+
+void launch_pass(const Grid &g, const Scheme &sc, int dir, const PassBuffers &b, int mode, int stage, double ca,
+                 double cb, const double *dxg, hipStream_t st) {
+  PassConst k;
+  for (int d = 0; d < 3; d++) k.n[d] = g.n[d];
+  k.ncell = g.ncell;
+  k.gamma = sc.gamma;
+  k.gm1 = sc.gamma - 1.0;
+  k.inv_gm1 = 1.0 / (sc.gamma - 1.0);
+  k.eps4 = 4.0 * sc.eps;
+  k.dxg = dxg;
+  PassIO io;
+  io.U = b.U;
+  io.R = b.R;
+  io.FV = b.FV;
+  io.FVg = b.FVg[dir];
+  io.u = b.u;
+  io.S = b.S;
+  io.acc = b.acc;
+  io.out = b.out;
+  io.Q = b.Q;
+  io.visc = sc.Re > 0 ? 1 : 0;
+  io.mode = mode;
+  io.stage = stage;
+  io.ca = ca;
+  io.cb = cb;
+  if (dir == 0) launch_d<0>(g, sc, k, io, st);
+  else if (dir == 1) launch_d<1>(g, sc, k, io, st);
+  else launch_d<2>(g, sc, k, io, st);
+}
+
+void launch_prims(const Grid &g, const Scheme &sc, const double *u, double *Q, hipStream_t st) {
+  hipLaunchKernelGGL(k_prims, dim3((unsigned)((g.ncell + 255) / 256)), dim3(256), 0, st, u, Q, g.ncell, sc.gamma);
+}
+
+}  // namespace hm

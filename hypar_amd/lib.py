@@ -67,6 +67,10 @@ SIGNATURES = [
     ("hm_flux_difference", [_VP, _DP, _DP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     ("hm_axpy", [_VP, _DP, ctypes.c_double, _DP, _SZ], ctypes.c_int),
     ("hm_fill", [_VP, _DP, ctypes.c_double, _SZ], ctypes.c_int),
+    ("hm_rhs_vjp", [_VP, _DP, _DP, _DP], ctypes.c_int),
+    ("hm_rk4_vjp", [_VP, _DP, ctypes.c_double, _DP], ctypes.c_int),
+    ("hm_adjoint_forward", [_VP, _DP, ctypes.c_double, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    ("hm_adjoint_reverse", [_VP, _DP], ctypes.c_int),
 ]
 
 

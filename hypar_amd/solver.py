@@ -43,6 +43,7 @@ class NS3D:
         h = ctypes.c_void_p()
         _lib.check(self.L.hm_create(ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h
+        self.tape_id = 0  # generation of the adjoint tape (one per solver)
 
     @classmethod
     def from_workload(cls, w):
@@ -135,6 +136,43 @@ class NS3D:
         return out.value
 
 
+    # ------------------------------------------------------------------
+    # discrete adjoint (inviscid component-wise WENO5 + Rusanov)
+    def rhs_vjp(self, u: torch.Tensor, lam: torch.Tensor) -> torch.Tensor:
+        """(d rhs/d u)^T lam at state u."""
+        self._check_state(u)
+        self._check_state(lam, "lam")
+        out = self.empty()
+        self._stream()
+        _lib.check(self.L.hm_rhs_vjp(self.h, ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(u.data_ptr()),
+                                     ctypes.c_void_p(lam.data_ptr())))
+        return out
+
+    def rk4_vjp(self, u_n: torch.Tensor, dt: float, lam: torch.Tensor) -> torch.Tensor:
+        """lam <- (d u_{n+1}/d u_n)^T lam for one RK4 step (in place)."""
+        self._check_state(u_n)
+        self._check_state(lam, "lam")
+        self._stream()
+        _lib.check(self.L.hm_rk4_vjp(self.h, ctypes.c_void_p(u_n.data_ptr()), float(dt),
+                                     ctypes.c_void_p(lam.data_ptr())))
+        return lam
+
+    def adjoint_forward(self, u: torch.Tensor, dt: float, nsteps: int, every: int) -> torch.Tensor:
+        """nsteps RK4 steps in place, checkpointing every `every` steps."""
+        self._check_state(u)
+        self._stream()
+        self.tape_id += 1
+        _lib.check(self.L.hm_adjoint_forward(self.h, ctypes.c_void_p(u.data_ptr()), float(dt), int(nsteps),
+                                             int(every)))
+        return u
+
+    def adjoint_reverse(self, lam: torch.Tensor) -> torch.Tensor:
+        """dJ/du_N -> dJ/du_0 through the last adjoint_forward (in place)."""
+        self._check_state(lam, "lam")
+        self._stream()
+        _lib.check(self.L.hm_adjoint_reverse(self.h, ctypes.c_void_p(lam.data_ptr())))
+        return lam
+
     def profile_kernels(self, u: torch.Tensor, reps: int = 5) -> dict:
         """Mean duration (ms) of each kernel of one RK stage, HIP events on
         the solver's stream."""
@@ -142,8 +180,7 @@ class NS3D:
         self._stream()
         ms = (ctypes.c_double * 5)()
         _lib.check(self.L.hm_profile_kernels(self.h, ctypes.c_void_p(u.data_ptr()), int(reps), ms))
-        return {"iface_x": ms[0], "iface_y": ms[1], "iface_z": ms[2], "visc_flux": ms[3],
-                "stage": ms[4]}
+        return {"pass_x": ms[0], "pass_y": ms[1], "pass_z": ms[2], "visc_flux": ms[3], "prims": ms[4]}
 
 
 def soa_to_aos(t: torch.Tensor, dims) -> np.ndarray:

@@ -100,7 +100,8 @@ hm_status hm_download(hm_solver *s, double *host, const double *dev, int layout)
  * in place (TimeRK with the "rk 44" tableau). */
 hm_status hm_rk4(hm_solver *s, double *u, double dt, int nsteps);
 
-/* rhs = -hyperbolic + parabolic (+ zero source) of the device state u. */
+/* rhs = -hyperbolic + parabolic (+ zero source) of the device state u.
+ * rhs and u must be distinct arrays (HM_ERR_ARG otherwise). */
 hm_status hm_rhs(hm_solver *s, double *rhs, const double *u);
 /* hyperbolic term alone: sum over x,y,z of dxinv*(F_{i+1/2}-F_{i-1/2}). */
 hm_status hm_hyperbolic(hm_solver *s, double *hyp, const double *u);
@@ -155,13 +156,32 @@ hm_status hm_flux_difference(hm_solver *s, double *hyp, const double *fI, int di
 hm_status hm_axpy(hm_solver *s, const double *x, double a, double *y, size_t n);
 hm_status hm_fill(hm_solver *s, double *x, double value, size_t n);
 
+/* ---- discrete adjoint ------------------------------------------------
+ * No counterpart in the reference (it has no adjoint).  Reverse-mode
+ * derivatives of this library's own discretisation; built for the inviscid
+ * component-wise WENO5-JS + Rusanov scheme with grid sides <= 512
+ * (HM_ERR_UNSUPPORTED otherwise).  lam, out: hm-layout device arrays. */
+/* out = (d rhs / d u)^T lam at state u */
+hm_status hm_rhs_vjp(hm_solver *s, double *out, const double *u, const double *lam);
+/* lam <- (d u_{n+1} / d u_n)^T lam for one RK4 step of size dt from u_n */
+hm_status hm_rk4_vjp(hm_solver *s, const double *u_n, double dt, double *lam);
+/* Forward sweep of nsteps RK4 steps in place on u, keeping a checkpoint of
+ * the state every `checkpoint_every` steps in HBM (the solver owns them). */
+hm_status hm_adjoint_forward(hm_solver *s, double *u, double dt, int nsteps, int checkpoint_every);
+/* Reverse sweep of the last hm_adjoint_forward: lam holds dJ/du_N on entry
+ * and dJ/du_0 on return.  Each checkpoint segment is recomputed once with
+ * its RK stage states kept in HBM, then stepped back. */
+hm_status hm_adjoint_reverse(hm_solver *s, double *lam);
+
 /* Kernel timing for roofline reports (no counterpart in the reference; the
  * reference's GPU_STAT build prints per-kernel cudaEvent times,
  * HyperbolicFunction_GPU.cu:517-576).  Launches each kernel of one RK stage
  * `reps` times on the solver's stream with state u and stores the mean
  * duration in milliseconds, measured with HIP events on that stream:
- * ms[0..2] interface fluxes x, y, z; ms[3] viscous fluxes (0 if inviscid);
- * ms[4] flux difference + stage update.  u is not modified. */
+ * ms[0..2] the fused x, y, z passes of an RK stage (the z pass in its
+ * stage-1 form, which reads u and the accumulator and writes the next stage
+ * state); ms[3] viscous fluxes and ms[4] primitive variables (0 if
+ * inviscid).  u is not modified. */
 hm_status hm_profile_kernels(hm_solver *s, const double *u, int reps, double *ms);
 
 #ifdef __cplusplus

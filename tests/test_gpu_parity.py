@@ -4,7 +4,7 @@ Tolerance (the path's stated bar): relative L2 over all interior cells and
 all 5 variables <= 1e-12 against the reference's CPU path.  Reference
 outputs come from golden vectors written by the reference binary; at sizes
 without fixtures the CPU oracle (bit-identical to the reference on every
-fixture) is the checker, and at 256^3 size-independent properties are
+fixture) supplies the expected values, and at 256^3 size-independent properties are
 checked too.
 """
 import ctypes
@@ -106,7 +106,34 @@ def test_terms_match_oracle(interp, upwind, Re):
         if np.linalg.norm(ref) == 0:
             assert np.max(np.abs(got)) == 0
         else:
-            assert decks.rel_l2(got, ref) <= 1e-13, term
+            # the fused rhs sums the directions in another order than the
+            # separate terms; the cancellation in -hyp + par amplifies that
+            # rounding, so it gets the path's bar, the terms a tighter one
+            assert decks.rel_l2(got, ref) <= (TOL if term == "rhs" else 1e-13), term
+
+
+@pytest.mark.parametrize("dims", [(100, 6, 7), (7, 70, 6), (6, 7, 67), (5, 5, 5), (130, 9, 5)])
+@pytest.mark.parametrize("interp,upwind,Re", [("characteristic", "roe", 1600.0), ("components", "rusanov", -1.0),
+                                              ("components", "roe", -1.0), ("characteristic", "rusanov", 100.0)])
+def test_fused_passes_on_ragged_lines(dims, interp, upwind, Re):
+    """The fused direction passes on x-lines longer than one 64-interface
+    chunk and not a multiple of it, y/z lines whose length is not a multiple
+    of the 4-row march step, and the smallest grid: rhs and 3 RK4 steps
+    against the oracle."""
+    from hypar_amd import decks
+    from oracle.oracle import Oracle
+    w = dataclasses.replace(decks.config1(16, 1), dims=dims, interp=interp, upwind=upwind, Re=Re, dt=None)
+    _say(f"[gpu] fused passes {interp}/{upwind}/Re={Re} dims={dims}")
+    u0, dt = decks.prepare(w)
+    o = Oracle.for_workload(w)
+    s = _solver(w)
+    u = s.to_device(u0)
+    err = decks.rel_l2(s.to_host(s.rhs(u)), o.rhs(u0))
+    assert err <= TOL, err
+    s.rk4(u, dt, 3)
+    err = decks.rel_l2(s.to_host(u), o.rk4(u0, dt, 3))
+    _say(f"[gpu]   3 steps rel L2 = {err:.3e}")
+    assert err <= TOL
 
 
 def test_cfl_and_step_norm_match_oracle():
@@ -147,6 +174,10 @@ def test_bad_arguments_are_rejected():
     host = np.zeros(5 * 8 ** 3)
     code = s.L.hm_rk4(s.h, ctypes.c_void_p(host.ctypes.data), 0.01, 1)
     assert code == lib.HM_ERR_ARG
+    # the fused passes read neighbours of u while they write rhs
+    u = s.to_device(decks.initial_state(w))
+    code = s.L.hm_rhs(s.h, ctypes.c_void_p(u.data_ptr()), ctypes.c_void_p(u.data_ptr()))
+    assert code == lib.HM_ERR_ARG and b"alias" in s.L.hm_last_error()
 
 
 # ------------------------------------------------------------ full size
