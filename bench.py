@@ -122,6 +122,66 @@ def adjoint_ratio(n=128, nsteps=50, every=10):
             "forward_states_agree": same}
 
 
+def ensemble_throughput(nsteps=10):
+    """Batched-ensemble throughput: 64 inviscid NS3D members on periodic
+    grids whose sides are drawn from {32, 48, 64, 96, 128}, each with its own
+    random-phase state and its own dt (CFL 0.5); component-wise WENO5-JS +
+    Rusanov, RK4.  Tile shape and persistent grid are autotuned first; the
+    timed region replays the captured RK4 step nsteps times.  Beside it, the
+    same members advanced one after another through the single-grid path
+    (the reference's serial member loop), timed over the same steps."""
+    import torch
+
+    from hypar_amd import decks
+    from hypar_amd.ensemble import Ensemble
+    from hypar_amd.solver import NS3D
+    ws = decks.ensemble_config2()
+    u0s, dts = [], []
+    for w in ws:
+        u0, dt = decks.prepare(w)
+        u0s.append(u0)
+        dts.append(dt)
+    e = Ensemble([w.dims for w in ws])
+    u = e.to_device(u0s)
+    e.set_dt(dts)
+    shape, grid = e.tune(u, reps=2)
+    stream = torch.cuda.current_stream()
+    e.rk4(u, 1)  # capture + warm
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    e.rk4(u, nsteps)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1)
+    ncell = e.ncell
+    finite = bool(torch.isfinite(u).all())
+    e.close()
+    del u
+    torch.cuda.empty_cache()
+    serial = 0.0
+    for w, u0, dt in zip(ws, u0s, dts):
+        s = NS3D.from_workload(w)
+        v = s.to_device(u0)
+        s.rk4(v, dt, 1)
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        s.rk4(v, dt, nsteps)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        serial += ev0.elapsed_time(ev1)
+        s.close()
+        del v
+    torch.cuda.empty_cache()
+    return {"workload": "64 inviscid NS3D members, sides from {32,48,64,96,128}, own dt each (CFL 0.5), "
+                        "WENO5-JS components + Rusanov, RK4",
+            "members": len(ws), "cells": ncell, "steps": nsteps, "ms_per_step": ms / nsteps,
+            "value": ncell * nsteps / (ms / 1e3), "unit": "cell-updates/s",
+            "tiles": {"shape": shape, "grid": list(grid)}, "finite": finite,
+            "serial_members_ms_per_step": serial / nsteps,
+            "speedup_vs_serial_members": serial / ms}
+
+
 def copy_probe(mib=1024, reps=3):
     """A plain device copy of known size (torch copy_ of `mib` MiB), the
     calibration run for the FETCH_SIZE / WRITE_SIZE counters."""
@@ -144,6 +204,7 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -233,6 +294,8 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_adjoint:
         line["adjoint"] = adjoint_ratio()
+    if not args.no_ensemble:
+        line["ensemble"] = ensemble_throughput()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

@@ -673,4 +673,307 @@ hm_status hm_profile_kernels(hm_solver *s, const double *u, int reps, double *ms
   return st;
 }
 
+
+// ---------------------------------------------------------------------------
+// batched ensemble
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+struct hm_ensemble {
+  hm::Scheme sc;
+  int device = 0;
+  hipStream_t stream = nullptr;  // caller's stream (launches)
+  hipStream_t cap = nullptr;     // private stream for graph capture
+  int nmem = 0;
+  std::vector<hm::EnsMember> mem;
+  hm::EnsMember *mem_dev = nullptr;
+  double *dxg = nullptr;
+  size_t ncell = 0;
+  double *R = nullptr, *S[2] = {nullptr, nullptr}, *acc = nullptr;
+  unsigned *item0[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  unsigned nitems[2][3] = {{0, 0, 0}, {0, 0, 0}};
+  int shape = 0;
+  unsigned grid[3] = {0, 0, 0};
+  bool dt_set = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const double *graph_u = nullptr;
+  int graph_shape = -1;
+  unsigned graph_grid[3] = {0, 0, 0};
+};
+
+namespace {
+
+void ens_drop_graph(hm_ensemble *e) {
+  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  e->exec = nullptr;
+  e->graph = nullptr;
+  e->graph_u = nullptr;
+}
+
+// the 12 launches of one RK4 step of all members (stage s reads S[(s-1)%2],
+// writes S[s%2]; the last stage writes u)
+void ens_step_launches(hm_ensemble *e, double *u, hipStream_t st) {
+  for (int stage = 0; stage < 4; stage++) {
+    const double *U = stage == 0 ? u : e->S[(stage - 1) % 2];
+    double *Sn = stage < 3 ? e->S[stage % 2] : nullptr;
+    for (int d = 0; d < 3; d++)
+      hm::launch_ens_pass(e->sc, d, e->shape, e->mem_dev, e->nmem, e->item0[e->shape][d], e->nitems[e->shape][d],
+                          e->grid[d], U, e->R, u, Sn, e->acc, stage, st);
+  }
+}
+
+hm_status ens_capture(hm_ensemble *e, double *u) {
+  ens_drop_graph(e);
+  HM_HIP(hipStreamBeginCapture(e->cap, hipStreamCaptureModeThreadLocal));
+  ens_step_launches(e, u, e->cap);
+  hipError_t err = hipStreamEndCapture(e->cap, &e->graph);
+  if (err != hipSuccess) return fail(HM_ERR_HIP, std::string("hm_ensemble: graph capture: ") + hipGetErrorString(err));
+  HM_HIP(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+  e->graph_u = u;
+  e->graph_shape = e->shape;
+  for (int d = 0; d < 3; d++) e->graph_grid[d] = e->grid[d];
+  return HM_OK;
+}
+
+hm_status ens_check(const hm_ensemble *e, const void *u, const char *name) {
+  if (!e) return fail(HM_ERR_ARG, std::string(name) + ": ensemble is NULL");
+  if (!u) return fail(HM_ERR_ARG, std::string(name) + ": state is NULL");
+  if (!is_device_ptr(u)) return fail(HM_ERR_ARG, std::string(name) + ": state is not a device pointer");
+  return HM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void hm_ensemble_destroy(hm_ensemble *e) {
+  if (!e) return;
+  ens_drop_graph(e);
+  if (e->cap) (void)hipStreamDestroy(e->cap);
+  double *bufs[] = {e->dxg, e->R, e->S[0], e->S[1], e->acc};
+  for (double *b : bufs)
+    if (b) (void)hipFree(b);
+  if (e->mem_dev) (void)hipFree(e->mem_dev);
+  for (int sh = 0; sh < 2; sh++)
+    for (int d = 0; d < 3; d++)
+      if (e->item0[sh][d]) (void)hipFree(e->item0[sh][d]);
+  delete e;
+}
+
+hm_status hm_ensemble_create(const hm_config *members, int nmembers, hm_ensemble **out) {
+  if (!members || !out || nmembers < 1) return fail(HM_ERR_ARG, "hm_ensemble_create: bad arguments");
+  *out = nullptr;
+  const hm_config &c0 = members[0];
+  for (int m = 0; m < nmembers; m++) {
+    const hm_config &c = members[m];
+    for (int d = 0; d < 3; d++)
+      if (c.dim[d] < 5) return fail(HM_ERR_ARG, "hm_ensemble_create: every grid dimension must be >= 5");
+    if (c.interp != c0.interp || c.upwind != c0.upwind || c.gamma != c0.gamma || c.weno_eps != c0.weno_eps)
+      return fail(HM_ERR_ARG, "hm_ensemble_create: members must share interpolation, upwinding, gamma and eps");
+    if (c.Re > 0)
+      return fail(HM_ERR_UNSUPPORTED, "hm_ensemble_create: batched members are inviscid (Re <= 0)");
+  }
+  if (c0.interp != HM_INTERP_COMPONENTS && c0.interp != HM_INTERP_CHARACTERISTIC)
+    return fail(HM_ERR_UNSUPPORTED, "hm_ensemble_create: unknown interpolation type");
+  if (c0.upwind != HM_UPWIND_ROE && c0.upwind != HM_UPWIND_RUSANOV)
+    return fail(HM_ERR_UNSUPPORTED, "hm_ensemble_create: unknown upwinding scheme");
+  if (!(c0.gamma > 1.0)) return fail(HM_ERR_ARG, "hm_ensemble_create: gamma must exceed 1");
+  if (!(c0.weno_eps > 0.0)) return fail(HM_ERR_ARG, "hm_ensemble_create: weno_eps must be positive");
+
+  hm_ensemble *e = new hm_ensemble();
+  e->sc.interp = c0.interp == HM_INTERP_CHARACTERISTIC ? hm::INTERP_CHAR : hm::INTERP_COMP;
+  e->sc.upwind = c0.upwind == HM_UPWIND_ROE ? hm::UPWIND_ROE : hm::UPWIND_RUSANOV;
+  e->sc.gamma = c0.gamma;
+  e->sc.Re = -1.0;
+  e->sc.Pr = c0.Pr;
+  e->sc.eps = c0.weno_eps;
+  e->nmem = nmembers;
+  hm_status st = HM_OK;
+  std::vector<double> dxh;
+  std::vector<size_t> dxoff(nmembers);
+  e->mem.resize(nmembers);
+  for (int m = 0; m < nmembers; m++) {
+    const hm_config &c = members[m];
+    hm::EnsMember &M = e->mem[m];
+    for (int d = 0; d < 3; d++) M.n[d] = c.dim[d];
+    M.ncell = (size_t)c.dim[0] * c.dim[1] * c.dim[2];
+    M.off = e->ncell;
+    M.dt = 0.0;
+    e->ncell += M.ncell;
+    const size_t ndx = (size_t)(c.dim[0] + 6) + (c.dim[1] + 6) + (c.dim[2] + 6);
+    dxoff[m] = dxh.size();
+    dxh.resize(dxh.size() + ndx);
+    if (c.dxinv) {
+      std::memcpy(dxh.data() + dxoff[m], c.dxinv, ndx * sizeof(double));
+    } else {
+      size_t o = dxoff[m];
+      for (int d = 0; d < 3; d++) {
+        uniform_dxinv(c.dim[d], 3, dxh.data() + o);
+        o += c.dim[d] + 6;
+      }
+    }
+  }
+  const size_t n5 = 5 * e->ncell;
+  if (hipGetDevice(&e->device) != hipSuccess) {
+    st = fail(HM_ERR_HIP, "hm_ensemble_create: no HIP device");
+    goto bad;
+  }
+  if ((st = dmalloc(&e->dxg, dxh.size())) != HM_OK) goto bad;
+  if ((st = dmalloc(&e->R, n5)) != HM_OK) goto bad;
+  if ((st = dmalloc(&e->S[0], n5)) != HM_OK) goto bad;
+  if ((st = dmalloc(&e->S[1], n5)) != HM_OK) goto bad;
+  if ((st = dmalloc(&e->acc, n5)) != HM_OK) goto bad;
+  if (hipMemcpy(e->dxg, dxh.data(), dxh.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+    st = fail(HM_ERR_HIP, "hm_ensemble_create: copy of dxinv failed");
+    goto bad;
+  }
+  for (int m = 0; m < nmembers; m++) e->mem[m].dxg = e->dxg + dxoff[m];
+  if (hipMalloc((void **)&e->mem_dev, nmembers * sizeof(hm::EnsMember)) != hipSuccess) {
+    st = fail(HM_ERR_NOMEM, "hm_ensemble_create: member table");
+    goto bad;
+  }
+  if (hipMemcpy(e->mem_dev, e->mem.data(), nmembers * sizeof(hm::EnsMember), hipMemcpyHostToDevice) != hipSuccess) {
+    st = fail(HM_ERR_HIP, "hm_ensemble_create: copy of the member table failed");
+    goto bad;
+  }
+  for (int sh = 0; sh < 2; sh++)
+    for (int d = 0; d < 3; d++) {
+      std::vector<unsigned> pre(nmembers + 1, 0);
+      for (int m = 0; m < nmembers; m++) pre[m + 1] = pre[m] + hm::ens_items_per_member(d, sh, e->mem[m].n);
+      e->nitems[sh][d] = pre[nmembers];
+      if (hipMalloc((void **)&e->item0[sh][d], pre.size() * sizeof(unsigned)) != hipSuccess ||
+          hipMemcpy(e->item0[sh][d], pre.data(), pre.size() * sizeof(unsigned), hipMemcpyHostToDevice) !=
+              hipSuccess) {
+        st = fail(HM_ERR_NOMEM, "hm_ensemble_create: work lists");
+        goto bad;
+      }
+    }
+  if (hipStreamCreate(&e->cap) != hipSuccess) {
+    st = fail(HM_ERR_HIP, "hm_ensemble_create: stream creation failed");
+    goto bad;
+  }
+  *out = e;
+  return HM_OK;
+bad:
+  hm_ensemble_destroy(e);
+  return st;
+}
+
+size_t hm_ensemble_ncell(const hm_ensemble *e) { return e ? e->ncell : 0; }
+
+hm_status hm_ensemble_member(const hm_ensemble *e, int m, size_t *offset, size_t *ncell) {
+  if (!e || m < 0 || m >= e->nmem) return fail(HM_ERR_ARG, "hm_ensemble_member: bad member index");
+  if (offset) *offset = e->mem[m].off;
+  if (ncell) *ncell = e->mem[m].ncell;
+  return HM_OK;
+}
+
+hm_status hm_ensemble_set_stream(hm_ensemble *e, void *stream) {
+  if (!e) return fail(HM_ERR_ARG, "hm_ensemble_set_stream: ensemble is NULL");
+  e->stream = (hipStream_t)stream;
+  return HM_OK;
+}
+
+hm_status hm_ensemble_set_dt(hm_ensemble *e, const double *dt) {
+  if (!e || !dt) return fail(HM_ERR_ARG, "hm_ensemble_set_dt: NULL argument");
+  for (int m = 0; m < e->nmem; m++) {
+    if (!std::isfinite(dt[m])) return fail(HM_ERR_ARG, "hm_ensemble_set_dt: dt is not finite");
+    e->mem[m].dt = dt[m];
+  }
+  // ordered on the caller's stream before any later step
+  HM_HIP(hipMemcpyAsync(e->mem_dev, e->mem.data(), e->nmem * sizeof(hm::EnsMember), hipMemcpyHostToDevice,
+                        e->stream));
+  HM_HIP(hipStreamSynchronize(e->stream));
+  e->dt_set = true;
+  return HM_OK;
+}
+
+hm_status hm_ensemble_set_tiles(hm_ensemble *e, int shape, const unsigned *grid) {
+  if (!e || (shape != 0 && shape != 1)) return fail(HM_ERR_ARG, "hm_ensemble_set_tiles: shape must be 0 or 1");
+  e->shape = shape;
+  for (int d = 0; d < 3; d++) e->grid[d] = grid ? grid[d] : 0;
+  return HM_OK;
+}
+
+hm_status hm_ensemble_get_tiles(const hm_ensemble *e, int *shape, unsigned *grid) {
+  if (!e) return fail(HM_ERR_ARG, "hm_ensemble_get_tiles: ensemble is NULL");
+  if (shape) *shape = e->shape;
+  if (grid)
+    for (int d = 0; d < 3; d++) grid[d] = e->grid[d];
+  return HM_OK;
+}
+
+hm_status hm_ensemble_rk4(hm_ensemble *e, double *u, int nsteps) {
+  hm_status st = ens_check(e, u, "hm_ensemble_rk4");
+  if (st != HM_OK) return st;
+  if (nsteps < 0) return fail(HM_ERR_ARG, "hm_ensemble_rk4: nsteps < 0");
+  if (!e->dt_set) return fail(HM_ERR_ARG, "hm_ensemble_rk4: set the members' dt first (hm_ensemble_set_dt)");
+  if (nsteps == 0) return HM_OK;
+  bool stale = !e->exec || e->graph_u != u || e->graph_shape != e->shape;
+  for (int d = 0; d < 3; d++) stale = stale || e->graph_grid[d] != e->grid[d];
+  if (stale && (st = ens_capture(e, u)) != HM_OK) return st;
+  for (int it = 0; it < nsteps; it++) HM_HIP(hipGraphLaunch(e->exec, e->stream));
+  return HM_OK;
+}
+
+hm_status hm_ensemble_tune(hm_ensemble *e, const double *u, int reps, int *shape, unsigned *grid) {
+  hm_status st = ens_check(e, u, "hm_ensemble_tune");
+  if (st != HM_OK) return st;
+  if (!e->dt_set) return fail(HM_ERR_ARG, "hm_ensemble_tune: set the members' dt first (hm_ensemble_set_dt)");
+  if (reps < 1) reps = 1;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->device) != hipSuccess || cus < 1)
+    cus = 256;
+  const size_t n5 = 5 * e->ncell;
+  double *w = nullptr;
+  if ((st = dmalloc(&w, n5)) != HM_OK) return st;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  double best = 1e300;
+  int best_shape = 0;
+  unsigned best_grid[3] = {0, 0, 0};
+  const unsigned mults[4] = {0, 1, 2, 4};  // 0: one block per work item
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    st = fail(HM_ERR_HIP, "hm_ensemble_tune: event creation failed");
+  }
+  for (int sh = 0; sh < 2 && st == HM_OK; sh++)
+    for (unsigned mi = 0; mi < 4 && st == HM_OK; mi++) {
+      const unsigned per_cu = sh == 0 ? 2 : 4;  // resident blocks per CU at 2 waves per SIMD
+      e->shape = sh;
+      for (int d = 0; d < 3; d++) e->grid[d] = mults[mi] * per_cu * (unsigned)cus;
+      // candidates time `reps` graph replays of one step on a scratch copy
+      if (hipMemcpyAsync(w, u, n5 * sizeof(double), hipMemcpyDeviceToDevice, e->stream) != hipSuccess) {
+        st = fail(HM_ERR_HIP, "hm_ensemble_tune: copy failed");
+        break;
+      }
+      if ((st = hm_ensemble_rk4(e, w, 1)) != HM_OK) break;  // capture + warm
+      (void)hipEventRecord(e0, e->stream);
+      if ((st = hm_ensemble_rk4(e, w, reps)) != HM_OK) break;
+      (void)hipEventRecord(e1, e->stream);
+      if (hipEventSynchronize(e1) != hipSuccess) {
+        st = fail(HM_ERR_HIP, "hm_ensemble_tune: event sync failed");
+        break;
+      }
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best) {
+        best = ms;
+        best_shape = sh;
+        for (int d = 0; d < 3; d++) best_grid[d] = e->grid[d];
+      }
+    }
+  ens_drop_graph(e);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(w);
+  e->shape = best_shape;
+  for (int d = 0; d < 3; d++) e->grid[d] = best_grid[d];
+  if (st != HM_OK) return st;
+  if (shape) *shape = best_shape;
+  if (grid)
+    for (int d = 0; d < 3; d++) grid[d] = best_grid[d];
+  return HM_OK;
+}
+
 }  // extern "C"

@@ -78,6 +78,23 @@ void launch_pass(const Grid &g, const Scheme &sc, int dir, const PassBuffers &b,
 // primitive variables (u,v,w,T) of u into Q
 void launch_prims(const Grid &g, const Scheme &sc, const double *u, double *Q, hipStream_t st);
 
+// batched ensemble passes (hm_pass.hip): inviscid members of one scheme
+struct EnsMember {
+  int n[3];
+  size_t ncell;
+  size_t off;         // first cell of the member; its state is u[5*off .. 5*(off+ncell))
+  const double *dxg;  // ghosted dxinv x|y|z of the member (device)
+  double dt;          // the member's time step
+};
+// work items of one member in pass dir with tile shape `shape` (0: 4
+// wavefronts per block, 1: 2 wavefronts per block)
+unsigned ens_items_per_member(int dir, int shape, const int n[3]);
+// one pass of RK stage `stage` over all members; grid = number of
+// persistent blocks (0: one block per work item)
+void launch_ens_pass(const Scheme &sc, int dir, int shape, const EnsMember *mem, int nmem, const unsigned *item0,
+                     unsigned nitems, unsigned grid, const double *U, double *R, double *u, double *S, double *acc,
+                     int stage, hipStream_t st);
+
 // discrete adjoint (hm_adjoint.hip)
 bool vjp_supported(const Grid &g, const Scheme &sc);
 void launch_rhs_vjp(const Grid &g, const Scheme &sc, const double *u, const double *lam, double *out,

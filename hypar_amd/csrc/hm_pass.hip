@@ -56,6 +56,7 @@ namespace {
 
 constexpr int NQ = 10;  // per-cell LDS data: U0..U4, F1..F4 (F0 = U_d), modified energy
 
+
 // 1/d, d > 0 and normal: hardware estimate refined by two Newton-Raphson
 // steps, each of which squares the relative error.
 __device__ __forceinline__ double nr_rcp(double d) {
@@ -404,29 +405,35 @@ __device__ __forceinline__ void emit(const PassConst &k, const PassIO &io, size_
 }
 
 // ---------------------------------------------------------------------------
-// x pass: one wavefront per x-line
+// x pass: one wavefront per x-line, LPB lines per block
 // ---------------------------------------------------------------------------
 constexpr int XCH = 64;        // interfaces per chunk
 constexpr int XNC = XCH + 6;   // cells per chunk
 constexpr int XPAD = 72;
 
-template <int INTERP, int UPW>
-__global__ void __launch_bounds__(256) k_pass_x(PassConst k, PassIO io) {
-  __shared__ double sC[4][NQ][XPAD];
-  __shared__ double sF[4][NV][XCH + 1];  // slot 0: fI[x0-1]; slot 1+l: fI[x0+l]
-  __shared__ double sF0[4][NV];          // fI[0] of the line
+template <int LPB>
+struct XShared {
+  double sC[LPB][NQ][XPAD];
+  double sF[LPB][NV][XCH + 1];  // slot 0: fI[x0-1]; slot 1+l: fI[x0+l]
+  double sF0[LPB][NV];          // fI[0] of the line
+};
+
+// lines LPB*group .. LPB*group + LPB-1 of the grid (line = j + ny*k)
+template <int INTERP, int UPW, int LPB>
+__device__ __forceinline__ void pass_x_body(const PassConst &k, const PassIO &io, size_t group,
+                                            XShared<LPB> &sh) {
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int nx = k.n[0];
   const size_t nlines = (size_t)k.n[1] * k.n[2];
-  const size_t line = (size_t)blockIdx.x * 4 + w;
+  const size_t line = group * LPB + w;
   const bool active = line < nlines;
   const size_t base = (active ? line : 0) * (size_t)nx;
   int x0 = 0;
   for (; x0 < nx; x0 += XCH) {
     if (x0 > 0) {
       // carry fI[x0-1] into slot 0 (the previous chunk's output phase is done)
-      if (lane < NV) sF[w][lane][0] = sF[w][lane][XCH];
+      if (lane < NV) sh.sF[w][lane][0] = sh.sF[w][lane][XCH];
     }
     const int ncells = min(XNC, nx + 6 - x0);
     for (int c = lane; c < ncells; c += 64) {
@@ -436,18 +443,18 @@ __global__ void __launch_bounds__(256) k_pass_x(PassConst k, PassIO io) {
       for (int v = 0; v < NV; v++) u[v] = io.U[(size_t)v * k.ncell + cell];
       cell_data<0>(u, k, cd);
 #pragma unroll
-      for (int q = 0; q < NQ; q++) sC[w][q][c] = cd[q];
+      for (int q = 0; q < NQ; q++) sh.sC[w][q][c] = cd[q];
     }
     __syncthreads();
     if (active && x0 + lane < nx) {
-      auto ld = [&](int s, int q) { return sC[w][q][lane + s]; };
+      auto ld = [&](int s, int q) { return sh.sC[w][q][lane + s]; };
       double f[NV];
       iface<0, INTERP, UPW>(ld, k, f);
 #pragma unroll
-      for (int v = 0; v < NV; v++) sF[w][v][lane + 1] = f[v];
+      for (int v = 0; v < NV; v++) sh.sF[w][v][lane + 1] = f[v];
       if (x0 == 0 && lane == 0)
 #pragma unroll
-        for (int v = 0; v < NV; v++) sF0[w][v] = f[v];
+        for (int v = 0; v < NV; v++) sh.sF0[w][v] = f[v];
     }
     __syncthreads();
     const int cell = x0 - 1 + lane;
@@ -455,8 +462,8 @@ __global__ void __launch_bounds__(256) k_pass_x(PassConst k, PassIO io) {
       double fm[NV], fp[NV];
 #pragma unroll
       for (int v = 0; v < NV; v++) {
-        fm[v] = sF[w][v][lane];
-        fp[v] = sF[w][v][lane + 1];
+        fm[v] = sh.sF[w][v][lane];
+        fp[v] = sh.sF[w][v][lane + 1];
       }
       emit<0>(k, io, base + cell, cell, line, fm, fp);
     }
@@ -468,26 +475,32 @@ __global__ void __launch_bounds__(256) k_pass_x(PassConst k, PassIO io) {
     double fm[NV], fp[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) {
-      fm[v] = sF[w][v][nx - x0];
-      fp[v] = sF0[w][v];
+      fm[v] = sh.sF[w][v][nx - x0];
+      fp[v] = sh.sF0[w][v];
     }
     emit<0>(k, io, base + nx - 1, nx - 1, line, fm, fp);
   }
 }
 
 // ---------------------------------------------------------------------------
-// y, z passes: 64 x-columns marching along DIR
+// y, z passes: 64 columns marching along DIR, MR rows per step
 // ---------------------------------------------------------------------------
 constexpr int MW = 64;  // columns
-constexpr int MR = 4;   // rows per step (one per wavefront)
-constexpr int NRING = 9;
-constexpr int NFR = 5;
 
-template <int DIR, int INTERP, int UPW>
-__global__ void __launch_bounds__(256) k_pass_m(PassConst k, PassIO io) {
-  __shared__ double sC[NQ][NRING][MW];
-  __shared__ double sF[NV][NFR][MW];
-  __shared__ double sF0[NV][MW];
+template <int MR>
+struct MShared {
+  static constexpr int NRING = MR + 5;  // cell rows t0-3 .. t0+MR+1
+  static constexpr int NFR = MR + 1;    // interface rows t0-1 .. t0+MR-1
+  double sC[NQ][NRING][MW];
+  double sF[NV][NFR][MW];
+  double sF0[NV][MW];
+};
+
+// columns MW*group .. MW*group + MW-1 of the plane across DIR
+template <int DIR, int INTERP, int UPW, int MR>
+__device__ __forceinline__ void pass_m_body(const PassConst &k, const PassIO &io, size_t group,
+                                            MShared<MR> &sh) {
+  constexpr int NRING = MShared<MR>::NRING, NFR = MShared<MR>::NFR;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int nx = k.n[0], ny = k.n[1];
@@ -496,14 +509,14 @@ __global__ void __launch_bounds__(256) k_pass_m(PassConst k, PassIO io) {
   // across DIR (o = z index in the y pass, y index in the z pass); c is
   // also the line index of the ghost-layer viscous fluxes
   const size_t ncol = k.ncell / n;
-  const size_t c = (size_t)blockIdx.x * MW + lane;
+  const size_t c = group * MW + lane;
   const bool col = c < ncol;
   const size_t line = col ? c : ncol - 1;
   const size_t o = line / nx;
   const size_t stride = DIR == 1 ? (size_t)nx : (size_t)nx * ny;
   const size_t cbase = DIR == 1 ? o * nx * ny + (line - o * nx) : line;  // cell at position 0
 
-  auto load_row = [&](int r) {  // r in [-3, n+5)
+  auto load_row = [&](int r) {  // r in [-3, n+MR+1]
     const size_t cell = cbase + (size_t)wrap(r, n) * stride;
     double u[NV], cd[NQ];
 #pragma unroll
@@ -511,7 +524,7 @@ __global__ void __launch_bounds__(256) k_pass_m(PassConst k, PassIO io) {
     cell_data<DIR>(u, k, cd);
     const int slot = (r + NRING) % NRING;
 #pragma unroll
-    for (int q = 0; q < NQ; q++) sC[q][slot][lane] = cd[q];
+    for (int q = 0; q < NQ; q++) sh.sC[q][slot][lane] = cd[q];
   };
 
   // rows -3..1
@@ -525,16 +538,16 @@ __global__ void __launch_bounds__(256) k_pass_m(PassConst k, PassIO io) {
       auto ld = [&](int s, int q) {
         int sl = sl0 + s;
         sl = sl >= NRING ? sl - NRING : sl;
-        return sC[q][sl][lane];
+        return sh.sC[q][sl][lane];
       };
       double f[NV];
       iface<DIR, INTERP, UPW>(ld, k, f);
       const int fs = t % NFR;
 #pragma unroll
-      for (int v = 0; v < NV; v++) sF[v][fs][lane] = f[v];
+      for (int v = 0; v < NV; v++) sh.sF[v][fs][lane] = f[v];
       if (t == 0)
 #pragma unroll
-        for (int v = 0; v < NV; v++) sF0[v][lane] = f[v];
+        for (int v = 0; v < NV; v++) sh.sF0[v][lane] = f[v];
     }
     __syncthreads();
     const int e = t0 - 1 + w;  // row emitted by this wavefront
@@ -543,8 +556,8 @@ __global__ void __launch_bounds__(256) k_pass_m(PassConst k, PassIO io) {
       const int a = e % NFR, b = (e + 1) % NFR;
 #pragma unroll
       for (int v = 0; v < NV; v++) {
-        fm[v] = sF[v][a][lane];
-        fp[v] = sF[v][b][lane];
+        fm[v] = sh.sF[v][a][lane];
+        fp[v] = sh.sF[v][b][lane];
       }
       emit<DIR>(k, io, cbase + (size_t)e * stride, e, line, fm, fp);
     }
@@ -555,10 +568,100 @@ __global__ void __launch_bounds__(256) k_pass_m(PassConst k, PassIO io) {
     const int a = (n - 1) % NFR;
 #pragma unroll
     for (int v = 0; v < NV; v++) {
-      fm[v] = sF[v][a][lane];
-      fp[v] = sF0[v][lane];
+      fm[v] = sh.sF[v][a][lane];
+      fp[v] = sh.sF0[v][lane];
     }
     emit<DIR>(k, io, cbase + (size_t)(n - 1) * stride, n - 1, line, fm, fp);
+  }
+}
+
+template <int INTERP, int UPW>
+__global__ void __launch_bounds__(256) k_pass_x(PassConst k, PassIO io) {
+  __shared__ XShared<4> sh;
+  pass_x_body<INTERP, UPW, 4>(k, io, blockIdx.x, sh);
+}
+
+template <int DIR, int INTERP, int UPW>
+__global__ void __launch_bounds__(256) k_pass_m(PassConst k, PassIO io) {
+  __shared__ MShared<4> sh;
+  pass_m_body<DIR, INTERP, UPW, 4>(k, io, blockIdx.x, sh);
+}
+
+// ---------------------------------------------------------------------------
+// batched passes: all members of an ensemble in one persistent launch
+// ---------------------------------------------------------------------------
+struct EnsArgs {
+  const EnsMember *mem;    // member table
+  int nmem;
+  const unsigned *item0;   // first work item of each member, nmem + 1 entries
+  unsigned nitems;
+  double gamma, gm1, inv_gm1, eps4;
+  const double *U;         // stage states of all members (member m at 5*off)
+  double *R, *u, *S, *acc;
+  int mode, stage;
+};
+
+// the member of a work item and the item's index within the member
+__device__ __forceinline__ int ens_member(const EnsArgs &a, unsigned item) {
+  int lo = 0, hi = a.nmem - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (a.item0[mid] <= item) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ void ens_setup(const EnsArgs &a, int m, PassConst &k, PassIO &io) {
+  const EnsMember M = a.mem[m];
+  for (int d = 0; d < 3; d++) k.n[d] = M.n[d];
+  k.ncell = M.ncell;
+  k.gamma = a.gamma;
+  k.gm1 = a.gm1;
+  k.inv_gm1 = a.inv_gm1;
+  k.eps4 = a.eps4;
+  k.dxg = M.dxg;
+  const size_t o = 5 * M.off;
+  io.U = a.U + o;
+  io.R = a.R + o;
+  io.FV = nullptr;
+  io.FVg = nullptr;
+  io.u = a.u + o;
+  io.S = a.S ? a.S + o : nullptr;
+  io.acc = a.acc + o;
+  io.out = nullptr;
+  io.Q = nullptr;
+  io.visc = 0;
+  io.mode = a.mode;
+  io.stage = a.stage;
+  const double A_sub[3] = {0.5, 0.5, 1.0}, b[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+  io.ca = a.stage < 3 ? M.dt * A_sub[a.stage] : 0.0;
+  io.cb = M.dt * b[a.stage];
+}
+
+template <int INTERP, int UPW, int LPB>
+__global__ void __launch_bounds__(64 * LPB) k_ens_x(EnsArgs a) {
+  __shared__ XShared<LPB> sh;
+  for (unsigned item = blockIdx.x; item < a.nitems; item += gridDim.x) {
+    const int m = ens_member(a, item);
+    PassConst k;
+    PassIO io;
+    ens_setup(a, m, k, io);
+    pass_x_body<INTERP, UPW, LPB>(k, io, item - a.item0[m], sh);
+    __syncthreads();
+  }
+}
+
+template <int DIR, int INTERP, int UPW, int MR>
+__global__ void __launch_bounds__(64 * MR) k_ens_m(EnsArgs a) {
+  __shared__ MShared<MR> sh;
+  for (unsigned item = blockIdx.x; item < a.nitems; item += gridDim.x) {
+    const int m = ens_member(a, item);
+    PassConst k;
+    PassIO io;
+    ens_setup(a, m, k, io);
+    pass_m_body<DIR, INTERP, UPW, MR>(k, io, item - a.item0[m], sh);
+    __syncthreads();
   }
 }
 
@@ -629,6 +732,61 @@ void launch_pass(const Grid &g, const Scheme &sc, int dir, const PassBuffers &b,
   if (dir == 0) launch_d<0>(g, sc, k, io, st);
   else if (dir == 1) launch_d<1>(g, sc, k, io, st);
   else launch_d<2>(g, sc, k, io, st);
+}
+
+
+namespace {
+template <int INTERP, int UPW>
+void ens_launch_t(int dir, int shape, const EnsArgs &a, unsigned grid, hipStream_t st) {
+  if (dir == 0) {
+    if (shape == 0) hipLaunchKernelGGL((k_ens_x<INTERP, UPW, 4>), dim3(grid), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_ens_x<INTERP, UPW, 2>), dim3(grid), dim3(128), 0, st, a);
+  } else if (dir == 1) {
+    if (shape == 0) hipLaunchKernelGGL((k_ens_m<1, INTERP, UPW, 4>), dim3(grid), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_ens_m<1, INTERP, UPW, 2>), dim3(grid), dim3(128), 0, st, a);
+  } else {
+    if (shape == 0) hipLaunchKernelGGL((k_ens_m<2, INTERP, UPW, 4>), dim3(grid), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_ens_m<2, INTERP, UPW, 2>), dim3(grid), dim3(128), 0, st, a);
+  }
+}
+}  // namespace
+
+unsigned ens_items_per_member(int dir, int shape, const int n[3]) {
+  const size_t ncell = (size_t)n[0] * n[1] * n[2];
+  if (dir == 0) {
+    const size_t lpb = shape == 0 ? 4 : 2;
+    return (unsigned)(((size_t)n[1] * n[2] + lpb - 1) / lpb);
+  }
+  return (unsigned)((ncell / n[dir] + MW - 1) / MW);
+}
+
+void launch_ens_pass(const Scheme &sc, int dir, int shape, const EnsMember *mem, int nmem, const unsigned *item0,
+                     unsigned nitems, unsigned grid, const double *U, double *R, double *u, double *S, double *acc,
+                     int stage, hipStream_t st) {
+  EnsArgs a;
+  a.mem = mem;
+  a.nmem = nmem;
+  a.item0 = item0;
+  a.nitems = nitems;
+  a.gamma = sc.gamma;
+  a.gm1 = sc.gamma - 1.0;
+  a.inv_gm1 = 1.0 / (sc.gamma - 1.0);
+  a.eps4 = 4.0 * sc.eps;
+  a.U = U;
+  a.R = R;
+  a.u = u;
+  a.S = S;
+  a.acc = acc;
+  a.mode = OUT_STAGE;
+  a.stage = stage;
+  if (grid == 0 || grid > nitems) grid = nitems;
+  if (sc.interp == INTERP_CHAR) {
+    if (sc.upwind == UPWIND_ROE) ens_launch_t<INTERP_CHAR, UPWIND_ROE>(dir, shape, a, grid, st);
+    else ens_launch_t<INTERP_CHAR, UPWIND_RUSANOV>(dir, shape, a, grid, st);
+  } else {
+    if (sc.upwind == UPWIND_ROE) ens_launch_t<INTERP_COMP, UPWIND_ROE>(dir, shape, a, grid, st);
+    else ens_launch_t<INTERP_COMP, UPWIND_RUSANOV>(dir, shape, a, grid, st);
+  }
 }
 
 void launch_prims(const Grid &g, const Scheme &sc, const double *u, double *Q, hipStream_t st) {

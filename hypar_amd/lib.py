@@ -71,6 +71,18 @@ SIGNATURES = [
     ("hm_rk4_vjp", [_VP, _DP, ctypes.c_double, _DP], ctypes.c_int),
     ("hm_adjoint_forward", [_VP, _DP, ctypes.c_double, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     ("hm_adjoint_reverse", [_VP, _DP], ctypes.c_int),
+    ("hm_ensemble_create", [ctypes.POINTER(HMConfig), ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)],
+     ctypes.c_int),
+    ("hm_ensemble_destroy", [_VP], None),
+    ("hm_ensemble_ncell", [_VP], _SZ),
+    ("hm_ensemble_member", [_VP, ctypes.c_int, ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)], ctypes.c_int),
+    ("hm_ensemble_set_stream", [_VP, _VP], ctypes.c_int),
+    ("hm_ensemble_set_dt", [_VP, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    ("hm_ensemble_rk4", [_VP, _DP, ctypes.c_int], ctypes.c_int),
+    ("hm_ensemble_set_tiles", [_VP, ctypes.c_int, ctypes.POINTER(ctypes.c_uint)], ctypes.c_int),
+    ("hm_ensemble_get_tiles", [_VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint)], ctypes.c_int),
+    ("hm_ensemble_tune", [_VP, _DP, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint)],
+     ctypes.c_int),
 ]
 
 

@@ -184,6 +184,43 @@ hm_status hm_adjoint_reverse(hm_solver *s, double *lam);
  * inviscid).  u is not modified. */
 hm_status hm_profile_kernels(hm_solver *s, const double *u, int reps, double *ms);
 
+/* ------------------------------------------------------------------------
+ * Batched ensembles.
+ *
+ * Replaces the member loop of ensemble_simulations
+ * (src/TimeIntegration/TimeRK.c:130-193, include/ensemble_simulations.h,
+ * src/Simulation/ReadInputs.c:189-207), which advances the nsims members
+ * one after another with a shared dt.  Here all members, of uneven grid
+ * sizes and each with its own dt, advance together: every direction pass
+ * of an RK stage is one persistent launch over the (member, tile) work
+ * items of all members, and one RK4 step is a captured HIP graph replayed
+ * per step.  Members share the scheme (interpolation, upwinding, gamma,
+ * eps) and are inviscid.  The batched state holds member m's hm-layout
+ * state at u[5*offset_m .. 5*(offset_m + ncell_m)).
+ * ------------------------------------------------------------------------ */
+typedef struct hm_ensemble hm_ensemble;
+
+hm_status hm_ensemble_create(const hm_config *members, int nmembers, hm_ensemble **out);
+void hm_ensemble_destroy(hm_ensemble *e);
+/* total cells of all members */
+size_t hm_ensemble_ncell(const hm_ensemble *e);
+/* first cell and cell count of member m in the batched state */
+hm_status hm_ensemble_member(const hm_ensemble *e, int m, size_t *offset, size_t *ncell);
+hm_status hm_ensemble_set_stream(hm_ensemble *e, void *stream);
+/* per-member time steps dt[0..nmembers-1] (synchronous) */
+hm_status hm_ensemble_set_dt(hm_ensemble *e, const double *dt);
+/* nsteps RK4 steps of every member, in place (graph replays on the stream) */
+hm_status hm_ensemble_rk4(hm_ensemble *e, double *u, int nsteps);
+/* Tile shape (0: blocks of 4 wavefronts = 4 x-lines / 4 rows per march
+ * step; 1: blocks of 2) and persistent grid sizes of the x, y, z passes
+ * (0: one block per work item). */
+hm_status hm_ensemble_set_tiles(hm_ensemble *e, int shape, const unsigned *grid);
+hm_status hm_ensemble_get_tiles(const hm_ensemble *e, int *shape, unsigned *grid);
+/* Autotuner: times `reps` graph-replayed steps of every candidate tile
+ * shape and grid size on a scratch copy of u (u is not modified), keeps
+ * the fastest and returns it. */
+hm_status hm_ensemble_tune(hm_ensemble *e, const double *u, int reps, int *shape, unsigned *grid);
+
 #ifdef __cplusplus
 }
 #endif
