@@ -12,10 +12,15 @@
 // The interface-flux VJP spreads g over the 6 stencil cells as a flux-adjoint
 // part (later multiplied by the flux Jacobian transpose of the cell, once per
 // cell) and a direct solution-adjoint part (WENO of the solution, Rusanov
-// speed).  One workgroup owns one grid line along the direction: the line
-// lives in LDS, each thread is one interface, and the six stencil
-// contributions are accumulated in six conflict-free passes (pass s touches
-// cell i-3+s of interface i), so the result is deterministic.
+// speed).  One workgroup owns one grid line along the direction: the line's
+// states, fluxes and modified energies live in LDS (sized to the line), each
+// thread is one interface, and each thread writes its six stencil
+// contributions to LDS slots that the owning cell then sums in a fixed
+// order, so the result is deterministic.
+//
+// Within an RK4 step the stage adjoint mu_s = dt b_s lam + dt a_{s+1,s}
+// nu_{s+1} is formed on the fly from lam and the previous stage's nu, and
+// the last direction adds the finished nu_s into the step adjoint.
 //
 // Non-smooth points (the max of the Rusanov speeds, |v|) take the derivative
 // of the branch the forward evaluation selects.
@@ -30,10 +35,19 @@ constexpr int VJP_LINE_MAX = 512;  // longest grid line an adjoint workgroup hol
 
 __device__ __forceinline__ int wrapn(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
+// 1/d, d > 0 and normal: hardware estimate and two Newton-Raphson steps
+__device__ __forceinline__ double vjp_rcp(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = __builtin_fma(-d, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-d, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+
 // reverse of weno5(m3..p2; g3..q2): adds d out/d m (smoothness stencil) to mb
-// and d out/d g (interpolated values) to gb, scaled by ob
-__device__ __forceinline__ void weno5_vjp(const double m[5], const double g[5], double eps, double ob,
-                                          double mb[5], double gb[5]) {
+// and d out/d g (interpolated values) to gb, scaled by ob; returns out
+__device__ __forceinline__ double weno5_vjp(const double m[5], const double g[5], double eps, double ob,
+                                            double mb[5], double gb[5]) {
   const double c1312 = 13.0 / 12.0, c14 = 0.25, s6 = 1.0 / 6.0;
   const double t1 = m[0] - 2 * m[1] + m[2], r1 = m[0] - 4 * m[1] + 3 * m[2];
   const double t2 = m[1] - 2 * m[2] + m[3], r2 = m[1] - m[3];
@@ -44,7 +58,7 @@ __device__ __forceinline__ void weno5_vjp(const double m[5], const double g[5], 
   const double e1 = b1 + eps, e2 = b2 + eps, e3 = b3 + eps;
   const double s1 = e1 * e1, s2 = e2 * e2, s3 = e3 * e3;
   const double a1 = 0.1 * (s2 * s3), a2 = 0.6 * (s1 * s3), a3 = 0.3 * (s1 * s2);
-  const double inv = 1.0 / (a1 + a2 + a3);
+  const double inv = vjp_rcp(a1 + a2 + a3);
   const double w1 = a1 * inv, w2 = a2 * inv, w3 = a3 * inv;
   const double f1 = (2 * s6) * g[0] + (-7 * s6) * g[1] + (11 * s6) * g[2];
   const double f2 = (-s6) * g[1] + (5 * s6) * g[2] + (2 * s6) * g[3];
@@ -56,10 +70,12 @@ __device__ __forceinline__ void weno5_vjp(const double m[5], const double g[5], 
   gb[2] += ob * (w1 * (11 * s6) + w2 * (5 * s6) + w3 * (2 * s6));
   gb[3] += ob * (w2 * (2 * s6) + w3 * (5 * s6));
   gb[4] += ob * (w3 * (-s6));
-  // weights: d out/d b_k = -2 w_k (f_k - out) / (b_k + eps)
-  const double bb1 = -2.0 * ob * w1 * (f1 - out) / e1;
-  const double bb2 = -2.0 * ob * w2 * (f2 - out) / e2;
-  const double bb3 = -2.0 * ob * w3 * (f3 - out) / e3;
+  // weights: d out/d b_k = -2 w_k (f_k - out) / (b_k + eps), with
+  // 1/e_k from one reciprocal of e1 e2 e3
+  const double re = vjp_rcp(e1 * e2 * e3);
+  const double bb1 = -2.0 * ob * w1 * (f1 - out) * (re * (e2 * e3));
+  const double bb2 = -2.0 * ob * w2 * (f2 - out) * (re * (e1 * e3));
+  const double bb3 = -2.0 * ob * w3 * (f3 - out) * (re * (e1 * e2));
   const double tt1 = bb1 * (2 * c1312) * t1, rr1 = bb1 * (2 * c14) * r1;
   const double tt2 = bb2 * (2 * c1312) * t2, rr2 = bb2 * (2 * c14) * r2;
   const double tt3 = bb3 * (2 * c1312) * t3, rr3 = bb3 * (2 * c14) * r3;
@@ -68,6 +84,7 @@ __device__ __forceinline__ void weno5_vjp(const double m[5], const double g[5], 
   mb[2] += tt1 + 3 * rr1 - 2 * tt2 + tt3 + 3 * rr3;
   mb[3] += tt2 - rr2 - 2 * tt3 - 4 * rr3;
   mb[4] += tt3 + rr3;
+  return out;
 }
 
 // gradient of c + |v_d| of one state with respect to its conserved variables
@@ -150,76 +167,90 @@ __device__ __forceinline__ void flux_jac_T(const double *u, double gamma, const 
   ub[4] += fb[1 + DIR] * gm1 + fb[4] * gamma * vn;
 }
 
-// out (+)= (d rhs_DIR / d u)^T lam along one direction, one workgroup per line
+// nu (+)= (d rhs_DIR / d u)^T mu along one direction, one workgroup per line,
+// with mu = cb*lam + ca*nu_in (nu_in may be null).  mode VJP_WRITE stores,
+// VJP_ADD accumulates into nu, VJP_ADD_L accumulates into nu and adds the
+// finished nu into L, VJP_ADD_LSET sets L = lam + the finished nu.  LDS (dynamic, 23 doubles per cell of the line):
+// state [5][n], cell flux [5][n], modified energy [n], stencil
+// contributions of the flux and solution adjoints [6][n] each.
+enum { VJP_WRITE = 0, VJP_ADD = 1, VJP_ADD_L = 2, VJP_ADD_LSET = 3 };
+
 template <int DIR>
 __global__ void __launch_bounds__(VJP_LINE_MAX) k_rhs_vjp(const double *__restrict__ u, const double *__restrict__ lam,
-                                                      double *__restrict__ out, int nx, int ny, int nz,
-                                                      const double *__restrict__ dxg, double gamma,
-                                                      double eps, int accumulate) {
+                                                      const double *__restrict__ nu_in, double ca, double cb,
+                                                      double *__restrict__ nu, double *__restrict__ L, int mode,
+                                                      int nx, int ny, int nz, const double *__restrict__ dxg,
+                                                      double gamma, double eps) {
+  extern __shared__ double smem[];
   const size_t ncell = (size_t)nx * ny * nz;
   const int n = DIR == 0 ? nx : (DIR == 1 ? ny : nz);
-  const size_t line = blockIdx.x;
+  // neighbouring lines share cache lines along x: keep them on one XCD
+  const size_t line = xcd_block();
   size_t base, stride;
   if (DIR == 0) { base = line * nx; stride = 1; }
   else if (DIR == 1) { base = (line % nx) + (line / nx) * (size_t)nx * ny; stride = nx; }
   else { base = line; stride = (size_t)nx * ny; }
   const double *dxd = dxg + 3 + (DIR >= 1 ? nx + 6 : 0) + (DIR == 2 ? ny + 6 : 0);
-  __shared__ double sU[NV][VJP_LINE_MAX];
-  __shared__ double sF[NV][VJP_LINE_MAX];  // flux adjoint accumulator
-  __shared__ double sB[NV][VJP_LINE_MAX];  // solution adjoint accumulator
+  double *sU = smem;            // [5][n]
+  double *sFl = sU + 5 * n;     // [5][n]
+  double *sC4 = sFl + 5 * n;    // [n]
+  double *cF = sC4 + n;         // [6][n]
+  double *cU = cF + 6 * n;      // [6][n]
   const int t = threadIdx.x;
   const bool active = t < n;
+  const double inv_gm1 = 1.0 / (gamma - 1.0);
+  const int tm = active ? wrapn(t - 1, n) : 0;
   double g[NV] = {0, 0, 0, 0, 0};
   if (active) {
-    const size_t c = base + (size_t)t * stride, cm = base + (size_t)wrapn(t - 1, n) * stride;
+    const size_t c = base + (size_t)t * stride, cm = base + (size_t)tm * stride;
+    double us[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) {
-      sU[v][t] = u[(size_t)v * ncell + c];
-      sF[v][t] = 0.0;
-      sB[v][t] = 0.0;
-      g[v] = dxd[t] * lam[(size_t)v * ncell + c] - dxd[wrapn(t - 1, n)] * lam[(size_t)v * ncell + cm];
+      us[v] = u[(size_t)v * ncell + c];
+      sU[v * n + t] = us[v];
+      const size_t o = (size_t)v * ncell;
+      const double mu = cb * lam[o + c] + (nu_in ? ca * nu_in[o + c] : 0.0);
+      const double mum = cb * lam[o + cm] + (nu_in ? ca * nu_in[o + cm] : 0.0);
+      g[v] = dxd[t] * mu - dxd[tm] * mum;
     }
+    const Prim q = flowvar(us, gamma);
+    double fl[NV];
+    euler_flux<DIR>(fl, q);
+#pragma unroll
+    for (int v = 0; v < NV; v++) sFl[v * n + t] = fl[v];
+    sC4[t] = modified_energy(q, inv_gm1);
   }
   __syncthreads();
-  // per-cell primitive variables of the 6 stencil cells
-  Prim q[6];
   int cs[6];
   double alpha = 0.0, abar = 0.0;
   int sel = 0;
-  const double inv_gm1 = 1.0 / (gamma - 1.0);
   if (active) {
 #pragma unroll
-    for (int s = 0; s < 6; s++) {
-      cs[s] = wrapn(t + s - 3, n);
-      double us[NV];
-#pragma unroll
-      for (int v = 0; v < NV; v++) us[v] = sU[v][cs[s]];
-      q[s] = flowvar(us, gamma);
-    }
+    for (int s = 0; s < 6; s++) cs[s] = wrapn(t + s - 3, n);
     double u2[NV], u3[NV];
 #pragma unroll
-    for (int v = 0; v < NV; v++) { u2[v] = sU[v][cs[2]]; u3[v] = sU[v][cs[3]]; }
+    for (int v = 0; v < NV; v++) { u2[v] = sU[v * n + cs[2]]; u3[v] = sU[v * n + cs[3]]; }
+    const Prim q2 = flowvar(u2, gamma), q3 = flowvar(u3, gamma);
     const Prim a = roe_average(u2, u3, gamma);
-    const double aL = sqrt(gamma * q[2].P / q[2].rho) + fabs(vel<DIR>(q[2]));
-    const double aR = sqrt(gamma * q[3].P / q[3].rho) + fabs(vel<DIR>(q[3]));
+    const double aL = sqrt(gamma * q2.P / q2.rho) + fabs(vel<DIR>(q2));
+    const double aR = sqrt(gamma * q3.P / q3.rho) + fabs(vel<DIR>(q3));
     const double aA = sqrt(gamma * a.P / a.rho) + fabs(vel<DIR>(a));
     const double m01 = fmax(aL, aR), m12 = fmax(aR, aA);
     alpha = fmax(m01, m12);
     // alpha = max(max(aL,aR), max(aR,aA)): derivative of the selected term
     sel = m01 >= m12 ? (aL >= aR ? 0 : 1) : (aR >= aA ? 1 : 2);
   }
+  double accF[NV], accU[NV];
 #pragma unroll
   for (int v = 0; v < NV; v++) {
-    double fb[6] = {0, 0, 0, 0, 0, 0}, ub[6] = {0, 0, 0, 0, 0, 0};
     if (active) {
+      double fb[6] = {0, 0, 0, 0, 0, 0}, ub[6] = {0, 0, 0, 0, 0, 0};
       double F[6], U[6], C[6];
 #pragma unroll
       for (int s = 0; s < 6; s++) {
-        double fl[NV];
-        euler_flux<DIR>(fl, q[s]);
-        F[s] = fl[v];
-        U[s] = sU[v][cs[s]];
-        C[s] = v == 4 ? modified_energy(q[s], inv_gm1) : U[s];
+        F[s] = sFl[v * n + cs[s]];
+        U[s] = sU[v * n + cs[s]];
+        C[s] = v == 4 ? sC4[cs[s]] : U[s];
       }
       double mL[5], gL[5], mR[5], gR[5];
       double mbL[5] = {0, 0, 0, 0, 0}, gbL[5] = {0, 0, 0, 0, 0};
@@ -239,56 +270,71 @@ __global__ void __launch_bounds__(VJP_LINE_MAX) k_rhs_vjp(const double *__restri
       // same function of u, so both adjoints land on u)
 #pragma unroll
       for (int k = 0; k < 5; k++) { mL[k] = U[k]; gL[k] = C[k]; mR[k] = U[5 - k]; gR[k] = C[5 - k]; }
-      const double uLv = weno5(mL[0], mL[1], mL[2], mL[3], mL[4], gL[0], gL[1], gL[2], gL[3], gL[4], eps);
-      const double uRv = weno5(mR[0], mR[1], mR[2], mR[3], mR[4], gR[0], gR[1], gR[2], gR[3], gR[4], eps);
+      const double uLv = weno5_vjp(mL, gL, eps, 0.5 * alpha * g[v], mbL, gbL);
+      const double uRv = weno5_vjp(mR, gR, eps, -0.5 * alpha * g[v], mbR, gbR);
       abar += -0.5 * (uRv - uLv) * g[v];
-      weno5_vjp(mL, gL, eps, 0.5 * alpha * g[v], mbL, gbL);
-      weno5_vjp(mR, gR, eps, -0.5 * alpha * g[v], mbR, gbR);
 #pragma unroll
       for (int k = 0; k < 5; k++) {
         ub[k] += mbL[k] + gbL[k];
         ub[5 - k] += mbR[k] + gbR[k];
       }
-    }
-    // six conflict-free accumulation passes: pass s adds to cell i-3+s
+      // slot s of interface t belongs to cell t-3+s
 #pragma unroll
-    for (int s = 0; s < 6; s++) {
-      if (active) {
-        sF[v][cs[s]] += fb[s];
-        sB[v][cs[s]] += ub[s];
+      for (int s = 0; s < 6; s++) {
+        cF[s * n + t] = fb[s];
+        cU[s * n + t] = ub[s];
       }
-      __syncthreads();
     }
-  }
-  // Rusanov speed adjoint onto the two cells next to the interface
-  double ub2[NV] = {0, 0, 0, 0, 0}, ub3[NV] = {0, 0, 0, 0, 0};
-  if (active) {
-    double u2[NV], u3[NV];
+    __syncthreads();
+    // cell t sums slot s of interface t+3-s, s = 0..5, in that order
+    accF[v] = accU[v] = 0.0;
+    if (active) {
 #pragma unroll
-    for (int v = 0; v < NV; v++) { u2[v] = sU[v][cs[2]]; u3[v] = sU[v][cs[3]]; }
+      for (int s = 0; s < 6; s++) {
+        const int i = wrapn(t + 3 - s, n);
+        accF[v] += cF[s * n + i];
+        accU[v] += cU[s * n + i];
+      }
+    }
+    __syncthreads();
+  }
+  // Rusanov speed adjoint onto the two cells next to interface t: cell t-1
+  // (slot in cF) and cell t (slot in cU)
+  if (active) {
+    double u2[NV], u3[NV], ub2[NV] = {0, 0, 0, 0, 0}, ub3[NV] = {0, 0, 0, 0, 0};
+#pragma unroll
+    for (int v = 0; v < NV; v++) { u2[v] = sU[v * n + cs[2]]; u3[v] = sU[v * n + cs[3]]; }
     if (sel == 0) speed_vjp<DIR>(u2, gamma, abar, ub2);
     else if (sel == 1) speed_vjp<DIR>(u3, gamma, abar, ub3);
     else roe_speed_vjp<DIR>(u2, u3, gamma, abar, ub2, ub3);
-    for (int v = 0; v < NV; v++) sB[v][cs[2]] += ub2[v];
-  }
-  __syncthreads();
-  if (active)
-    for (int v = 0; v < NV; v++) sB[v][cs[3]] += ub3[v];
-  __syncthreads();
-  if (active) {
-    double uc[NV], fb[NV], ub[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) {
-      uc[v] = sU[v][t];
-      fb[v] = sF[v][t];
-      ub[v] = sB[v][t];
+      cF[v * n + t] = ub2[v];
+      cU[v * n + t] = ub3[v];
     }
-    flux_jac_T<DIR>(uc, gamma, fb, ub);
+  }
+  __syncthreads();
+  if (active) {
+    const int tp = wrapn(t + 1, n);
+    double uc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      uc[v] = sU[v * n + t];
+      accU[v] += cF[v * n + tp] + cU[v * n + t];
+    }
+    flux_jac_T<DIR>(uc, gamma, accF, accU);
     const size_t c = base + (size_t)t * stride;
 #pragma unroll
     for (int v = 0; v < NV; v++) {
       const size_t o = (size_t)v * ncell + c;
-      out[o] = accumulate ? out[o] + ub[v] : ub[v];
+      if (mode == VJP_WRITE) {
+        nu[o] = accU[v];
+      } else {
+        const double r = nu[o] + accU[v];
+        nu[o] = r;
+        if (mode == VJP_ADD_L) L[o] += r;
+        else if (mode == VJP_ADD_LSET) L[o] = lam[o] + r;
+      }
     }
   }
 }
@@ -301,24 +347,27 @@ __global__ void k_axpby(const double *__restrict__ x, double a, double *__restri
 
 // ---------------------------------------------------------------------------
 bool vjp_supported(const Grid &g, const Scheme &sc) {
-  return sc.interp == INTERP_COMP && sc.upwind == UPWIND_RUSANOV && !(sc.Re > 0) && g.n[0] <= VJP_LINE_MAX &&
+  return sc.interp == INTERP_COMP && sc.upwind == UPWIND_RUSANOV && sc.weno == WENO_JS && !(sc.Re > 0) &&
+         g.n[0] <= VJP_LINE_MAX &&
          g.n[1] <= VJP_LINE_MAX && g.n[2] <= VJP_LINE_MAX;
 }
 
-void launch_rhs_vjp(const Grid &g, const Scheme &sc, const double *u, const double *lam, double *out,
-                    const double *dxg, hipStream_t st) {
+void launch_rhs_vjp(const Grid &g, const Scheme &sc, const double *u, const double *lam, const double *nu_in,
+                    double ca, double cb, double *nu, double *L, bool set_L, const double *dxg, hipStream_t st) {
   for (int d = 0; d < 3; d++) {
     const unsigned lines = (unsigned)(g.ncell / g.n[d]);
     const unsigned threads = (unsigned)((g.n[d] + 63) / 64 * 64);
+    const size_t lds = (size_t)23 * g.n[d] * sizeof(double);
+    const int mode = d == 0 ? VJP_WRITE : (d == 2 && L ? (set_L ? VJP_ADD_LSET : VJP_ADD_L) : VJP_ADD);
     if (d == 0)
-      hipLaunchKernelGGL(k_rhs_vjp<0>, dim3(lines), dim3(threads), 0, st, u, lam, out, g.n[0], g.n[1], g.n[2],
-                         dxg, sc.gamma, sc.eps, 0);
+      hipLaunchKernelGGL(k_rhs_vjp<0>, dim3(lines), dim3(threads), lds, st, u, lam, nu_in, ca, cb, nu, L, mode,
+                         g.n[0], g.n[1], g.n[2], dxg, sc.gamma, sc.eps);
     else if (d == 1)
-      hipLaunchKernelGGL(k_rhs_vjp<1>, dim3(lines), dim3(threads), 0, st, u, lam, out, g.n[0], g.n[1], g.n[2],
-                         dxg, sc.gamma, sc.eps, 1);
+      hipLaunchKernelGGL(k_rhs_vjp<1>, dim3(lines), dim3(threads), lds, st, u, lam, nu_in, ca, cb, nu, L, mode,
+                         g.n[0], g.n[1], g.n[2], dxg, sc.gamma, sc.eps);
     else
-      hipLaunchKernelGGL(k_rhs_vjp<2>, dim3(lines), dim3(threads), 0, st, u, lam, out, g.n[0], g.n[1], g.n[2],
-                         dxg, sc.gamma, sc.eps, 1);
+      hipLaunchKernelGGL(k_rhs_vjp<2>, dim3(lines), dim3(threads), lds, st, u, lam, nu_in, ca, cb, nu, L, mode,
+                         g.n[0], g.n[1], g.n[2], dxg, sc.gamma, sc.eps);
   }
 }
 

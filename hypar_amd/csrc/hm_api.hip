@@ -39,6 +39,10 @@ struct hm_solver {
   std::vector<double *> ckpt, stages, vscratch;
   int tape_nsteps, tape_every;
   double tape_dt;
+  // reverse sweep: a second stream recomputes the next segment while the
+  // solver's stream runs the adjoint of the current one
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_ready[2] = {nullptr, nullptr}, ev_done[2] = {nullptr, nullptr}, ev_start = nullptr;
 };
 
 namespace {
@@ -173,6 +177,9 @@ void hm_destroy(hm_solver *s) {
   if (s->part_host) (void)hipHostFree(s->part_host);
   for (auto *v : {&s->ckpt, &s->stages, &s->vscratch})
     for (double *b : *v) (void)hipFree(b);
+  for (hipEvent_t e : {s->ev_ready[0], s->ev_ready[1], s->ev_done[0], s->ev_done[1], s->ev_start})
+    if (e) (void)hipEventDestroy(e);
+  if (s->aux) (void)hipStreamDestroy(s->aux);
   delete s;
 }
 
@@ -187,6 +194,8 @@ hm_status hm_create(const hm_config *cfg, hm_solver **out) {
     return fail(HM_ERR_UNSUPPORTED, "hm_create: unknown upwinding scheme");
   if (!(cfg->gamma > 1.0)) return fail(HM_ERR_ARG, "hm_create: gamma must exceed 1");
   if (!(cfg->weno_eps > 0.0)) return fail(HM_ERR_ARG, "hm_create: weno_eps must be positive");
+  if (cfg->weno_type < HM_WENO_JS || cfg->weno_type > HM_WENO_YC)
+    return fail(HM_ERR_UNSUPPORTED, "hm_create: unknown WENO weight formulation");
   if (cfg->Re > 0 && !(cfg->Minf > 0 && cfg->Pr > 0))
     return fail(HM_ERR_ARG, "hm_create: viscous runs need Minf > 0 and Pr > 0");
 
@@ -199,6 +208,7 @@ hm_status hm_create(const hm_config *cfg, hm_solver **out) {
   s->sc.Re = cfg->Re > 0 ? cfg->Re / cfg->Minf : -1.0;
   s->sc.Pr = cfg->Pr;
   s->sc.eps = cfg->weno_eps;
+  s->sc.weno = cfg->weno_type;
   if (hipGetDevice(&s->device) != hipSuccess) {
     hm_destroy(s);
     return fail(HM_ERR_HIP, "hm_create: no HIP device");
@@ -508,22 +518,24 @@ static hm_status rk4_step_store(hm_solver *s, double *u, double dt, double *cons
   return rk4_step(s, u, dt, U);
 }
 
-// lam <- step adjoint given the stage states U[0..3] (U[0] = u_n)
-static hm_status rk4_step_vjp(hm_solver *s, double *const U[4], double dt, double *lam) {
+// Step adjoint given the stage states U[0..3] (U[0] = u_n):
+// lam_out = lam + sum_s nu_s, nu_s = J(U_s)^T mu_s,
+// mu_s = dt b_s lam + dt a_{s+1,s} nu_{s+1} (formed inside the VJP kernels);
+// lam_out must differ from lam
+static hm_status rk4_step_vjp(hm_solver *s, double *const U[4], double dt, const double *lam, double *lam_out) {
   const size_t n5 = 5 * s->g.ncell;
   hm_status st = ensure_buffers(s->vscratch, 3, n5);
   if (st != HM_OK) return st;
-  double *mu = s->vscratch[0], *nu = s->vscratch[1], *L = s->vscratch[2];
-  HM_HIP(hipMemcpyAsync(L, lam, n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+  double *nuA = s->vscratch[0], *nuB = s->vscratch[1];
+  const double *nu_in = nullptr;
+  double *nu_out = nuA;
   for (int stage = 3; stage >= 0; stage--) {
-    // mu = b_s lam' + a_{s+1,s} nu_{s+1}
-    HM_HIP(hipMemcpyAsync(mu, lam, n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
-    hm::launch_axpby(nu, stage < 3 ? dt * RK_A_SUB[stage] : 0.0, mu, dt * RK_B[stage], n5, s->stream);
-    hm::launch_rhs_vjp(s->g, s->sc, U[stage], mu, nu, s->dxg, s->stream);
-    hm::launch_axpy(nu, 1.0, L, n5, s->stream);
+    const double ca = stage < 3 ? dt * RK_A_SUB[stage] : 0.0, cb = dt * RK_B[stage];
+    hm::launch_rhs_vjp(s->g, s->sc, U[stage], lam, nu_in, ca, cb, nu_out, lam_out, stage == 3, s->dxg, s->stream);
     if ((st = check_launch("rk4_step_vjp")) != HM_OK) return st;
+    nu_in = nu_out;
+    nu_out = nu_out == nuA ? nuB : nuA;
   }
-  HM_HIP(hipMemcpyAsync(lam, L, n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
   return HM_OK;
 }
 
@@ -531,7 +543,7 @@ static hm_status check_vjp(hm_solver *s, const char *name) {
   if (!s) return fail(HM_ERR_ARG, std::string(name) + ": solver is NULL");
   if (!hm::vjp_supported(s->g, s->sc))
     return fail(HM_ERR_UNSUPPORTED, std::string(name) +
-                                        ": the adjoint is built for the inviscid component-wise WENO5 + "
+                                        ": the adjoint is built for the inviscid component-wise WENO5-JS + "
                                         "Rusanov scheme with grid sides <= 512");
   return HM_OK;
 }
@@ -542,7 +554,8 @@ hm_status hm_rhs_vjp(hm_solver *s, double *out, const double *u, const double *l
   if ((st = check_state(s, out, "hm_rhs_vjp: out")) != HM_OK) return st;
   if ((st = check_state(s, u, "hm_rhs_vjp: u")) != HM_OK) return st;
   if ((st = check_state(s, lam, "hm_rhs_vjp: lam")) != HM_OK) return st;
-  hm::launch_rhs_vjp(s->g, s->sc, u, lam, out, s->dxg, s->stream);
+  if (out == u || out == lam) return fail(HM_ERR_ARG, "hm_rhs_vjp: out must not alias u or lam");
+  hm::launch_rhs_vjp(s->g, s->sc, u, lam, nullptr, 0.0, 1.0, out, nullptr, false, s->dxg, s->stream);
   return check_launch("hm_rhs_vjp");
 }
 
@@ -561,8 +574,15 @@ hm_status hm_rk4_vjp(hm_solver *s, const double *u_n, double dt, double *lam) {
   HM_HIP(hipMemcpyAsync(w, u_n, n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
   double *const Us[3] = {U[1], U[2], U[3]};
   if ((st = rk4_step_store(s, w, dt, Us)) != HM_OK) return st;
-  return rk4_step_vjp(s, U, dt, lam);
+  double *L = s->vscratch[2];
+  if ((st = rk4_step_vjp(s, U, dt, lam, L)) != HM_OK) return st;
+  HM_HIP(hipMemcpyAsync(lam, L, n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+  return HM_OK;
 }
+
+// Two sets of segment stage states: segment seg uses set seg % 2; step m of
+// a segment keeps u_n, U1, U2, U3 at stages[set*4K + 4m .. +3].
+static double **segment_stages(hm_solver *s, int seg, int K) { return &s->stages[(size_t)4 * K * (seg % 2)]; }
 
 hm_status hm_adjoint_forward(hm_solver *s, double *u, double dt, int nsteps, int every) {
   hm_status st;
@@ -572,14 +592,45 @@ hm_status hm_adjoint_forward(hm_solver *s, double *u, double dt, int nsteps, int
   const size_t n5 = 5 * s->g.ncell;
   const int nck = (nsteps + every - 1) / every;
   if ((st = ensure_buffers(s->ckpt, (size_t)nck, n5)) != HM_OK) return st;
+  if ((st = ensure_buffers(s->stages, (size_t)8 * every, n5)) != HM_OK) return st;
+  const int nlast = (nck - 1) * every;
   for (int n = 0; n < nsteps; n++) {
     if (n % every == 0)
       HM_HIP(hipMemcpyAsync(s->ckpt[n / every], u, n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
-    if ((st = hm_rk4(s, u, dt, 1)) != HM_OK) return st;
+    if (n < nlast) {
+      if ((st = hm_rk4(s, u, dt, 1)) != HM_OK) return st;
+    } else {
+      // the last segment keeps its stage states: the reverse sweep starts on
+      // them without recomputation
+      double **U = segment_stages(s, nck - 1, every) + (size_t)4 * (n - nlast);
+      HM_HIP(hipMemcpyAsync(U[0], u, n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+      double *const Us[3] = {U[1], U[2], U[3]};
+      if ((st = rk4_step_store(s, u, dt, Us)) != HM_OK) return st;
+    }
   }
   s->tape_nsteps = nsteps;
   s->tape_every = every;
   s->tape_dt = dt;
+  return HM_OK;
+}
+
+// recompute segment seg from its checkpoint, keeping every step's stage states
+static hm_status recompute_segment(hm_solver *s, int seg, int n0, int n1, double dt) {
+  const size_t n5 = 5 * s->g.ncell;
+  const int K = s->tape_every;
+  double **S = segment_stages(s, seg, K);
+  hm_status st = ensure_buffers(s->vscratch, 4, n5);
+  if (st != HM_OK) return st;
+  for (int n = n0; n < n1; n++) {
+    double **U = S + (size_t)4 * (n - n0);
+    // U[0] of the first step is the checkpoint; later steps find u_n in
+    // their slot, advanced there by the step before
+    if (n == n0) HM_HIP(hipMemcpyAsync(U[0], s->ckpt[seg], n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+    double *next = n + 1 < n1 ? S[(size_t)4 * (n - n0 + 1)] : s->vscratch[3];
+    HM_HIP(hipMemcpyAsync(next, U[0], n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+    double *const Us[3] = {U[1], U[2], U[3]};
+    if ((st = rk4_step_store(s, next, dt, Us)) != HM_OK) return st;
+  }
   return HM_OK;
 }
 
@@ -591,34 +642,43 @@ hm_status hm_adjoint_reverse(hm_solver *s, double *lam) {
   const int N = s->tape_nsteps, K = s->tape_every;
   const double dt = s->tape_dt;
   const size_t n5 = 5 * s->g.ncell;
-  if ((st = ensure_buffers(s->stages, (size_t)4 * K, n5)) != HM_OK) return st;
+  if ((st = ensure_buffers(s->stages, (size_t)8 * K, n5)) != HM_OK) return st;
+  if ((st = ensure_buffers(s->vscratch, 4, n5)) != HM_OK) return st;
+  if (!s->aux) HM_HIP(hipStreamCreate(&s->aux));  // default priority
+  for (hipEvent_t *e : {&s->ev_ready[0], &s->ev_ready[1], &s->ev_done[0], &s->ev_done[1], &s->ev_start})
+    if (!*e) HM_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
   const int nck = (N + K - 1) / K;
+  hipStream_t main = s->stream;
+  double *cur = lam;
+  // the recomputation stream starts after everything queued so far
+  HM_HIP(hipEventRecord(s->ev_start, main));
+  HM_HIP(hipStreamWaitEvent(s->aux, s->ev_start, 0));
   for (int seg = nck - 1; seg >= 0; seg--) {
     const int n0 = seg * K, n1 = (n0 + K < N) ? n0 + K : N;
-    // recompute the segment, keeping every step's stage states
-    for (int n = n0; n < n1; n++) {
-      double **U = &s->stages[(size_t)4 * (n - n0)];
-      // U[0] of the first step is the checkpoint; later steps find u_n in
-      // their slot, advanced there by the step before
-      if (n == n0)
-        HM_HIP(hipMemcpyAsync(U[0], s->ckpt[seg], n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
-      if (n + 1 < n1) {
-        // advance a copy into the next step's U0 slot
-        double *next = s->stages[(size_t)4 * (n - n0 + 1)];
-        HM_HIP(hipMemcpyAsync(next, U[0], n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
-        double *const Us[3] = {U[1], U[2], U[3]};
-        if ((st = rk4_step_store(s, next, dt, Us)) != HM_OK) return st;
-      } else {
-        if ((st = ensure_buffers(s->vscratch, 4, n5)) != HM_OK) return st;
-        double *w = s->vscratch[3];
-        HM_HIP(hipMemcpyAsync(w, U[0], n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
-        double *const Us[3] = {U[1], U[2], U[3]};
-        if ((st = rk4_step_store(s, w, dt, Us)) != HM_OK) return st;
-      }
+    if (seg > 0) {
+      // recompute the previous segment into the other set on the second
+      // stream, once the adjoint that last read that set is done
+      const int o = (seg - 1) % 2;
+      if (seg + 1 < nck) HM_HIP(hipStreamWaitEvent(s->aux, s->ev_done[o], 0));
+      s->stream = s->aux;
+      st = recompute_segment(s, seg - 1, n0 - K, n0, dt);
+      s->stream = main;
+      if (st != HM_OK) return st;
+      HM_HIP(hipEventRecord(s->ev_ready[o], s->aux));
     }
-    for (int n = n1 - 1; n >= n0; n--)
-      if ((st = rk4_step_vjp(s, &s->stages[(size_t)4 * (n - n0)], dt, lam)) != HM_OK) return st;
+    // the last segment's states come from the forward sweep; the others
+    // from the recomputation queued in the previous iteration
+    if (seg + 1 < nck) HM_HIP(hipStreamWaitEvent(main, s->ev_ready[seg % 2], 0));
+    double **S = segment_stages(s, seg, K);
+    for (int n = n1 - 1; n >= n0; n--) {
+      // the step adjoint alternates between the caller's array and scratch
+      double *out = cur == lam ? s->vscratch[2] : lam;
+      if ((st = rk4_step_vjp(s, S + (size_t)4 * (n - n0), dt, cur, out)) != HM_OK) return st;
+      cur = out;
+    }
+    HM_HIP(hipEventRecord(s->ev_done[seg % 2], main));
   }
+  if (cur != lam) HM_HIP(hipMemcpyAsync(lam, cur, n5 * sizeof(double), hipMemcpyDeviceToDevice, main));
   return HM_OK;
 }
 
@@ -774,6 +834,8 @@ hm_status hm_ensemble_create(const hm_config *members, int nmembers, hm_ensemble
       return fail(HM_ERR_ARG, "hm_ensemble_create: members must share interpolation, upwinding, gamma and eps");
     if (c.Re > 0)
       return fail(HM_ERR_UNSUPPORTED, "hm_ensemble_create: batched members are inviscid (Re <= 0)");
+    if (c.weno_type != HM_WENO_JS)
+      return fail(HM_ERR_UNSUPPORTED, "hm_ensemble_create: batched members use Jiang-Shu weights");
   }
   if (c0.interp != HM_INTERP_COMPONENTS && c0.interp != HM_INTERP_CHARACTERISTIC)
     return fail(HM_ERR_UNSUPPORTED, "hm_ensemble_create: unknown interpolation type");
@@ -789,6 +851,7 @@ hm_status hm_ensemble_create(const hm_config *members, int nmembers, hm_ensemble
   e->sc.Re = -1.0;
   e->sc.Pr = c0.Pr;
   e->sc.eps = c0.weno_eps;
+  e->sc.weno = hm::WENO_JS;
   e->nmem = nmembers;
   hm_status st = HM_OK;
   std::vector<double> dxh;

@@ -11,7 +11,7 @@
 //      sum_k a_k f_k / sum_k a_k,  a_k = c_k * prod_{j!=k} (b_j+eps)^2,
 //    which is w_k = (c_k/(b_k+eps)^2) / sum_j (c_j/(b_j+eps)^2) multiplied
 //    through by prod_j (b_j+eps)^2: one division per interpolant instead of
-//    four.
+//    four; the WENO-Z / Yamaleev-Carpenter weights likewise.
 //  * Eigenvector entries divided by a^2 or 2a^2 use one reciprocal.
 //  * Characteristic upwinding uses L*R = I: with uL = R cL and uR = R cR,
 //    R|Lambda|L (uR-uL)/2 = R |Lambda| (cR-cL)/2, so the interface flux is
@@ -159,28 +159,57 @@ __device__ __forceinline__ double harten_abs(double l) {
   return al < delta ? (l * l + delta2) / (2 * delta) : al;
 }
 
-// WENO5 interpolant with Jiang-Shu weights.  (m3,m2,m1,p1,p2) are the stencil
-// values the smoothness indicators use, (g3,g2,g1,q1,q2) the values that are
+// WENO5 interpolant.  (m3,m2,m1,p1,p2) are the stencil values the
+// smoothness indicators use, (g3,g2,g1,q1,q2) the values that are
 // interpolated (they differ only for the solution, whose weights come from u
-// but whose interpolant uses the modified solution uC).
+// but whose interpolant uses the modified solution uC).  `type` selects the
+// weights (0 Jiang-Shu, 1 mapped, 2 WENO-Z, 3 Yamaleev-Carpenter;
+// include/interpolation.h:265-500): JS, Z and YC are applied in ratio form
+// (one division), the mapped weights as the reference forms them.
 __device__ __forceinline__ double weno5(double m3, double m2, double m1, double p1, double p2,
                                         double g3, double g2, double g1, double q1, double q2,
-                                        double eps) {
+                                        double eps, int type = 0) {
   const double c1312 = 13.0 / 12.0, c14 = 1.0 / 4.0, s6 = 1.0 / 6.0;
   double t, r;
   t = m3 - 2 * m2 + m1; r = m3 - 4 * m2 + 3 * m1;
-  double b1 = c1312 * t * t + c14 * r * r;
+  const double b1 = c1312 * t * t + c14 * r * r;
   t = m2 - 2 * m1 + p1; r = m2 - p1;
-  double b2 = c1312 * t * t + c14 * r * r;
+  const double b2 = c1312 * t * t + c14 * r * r;
   t = m1 - 2 * p1 + p2; r = 3 * m1 - 4 * p1 + p2;
-  double b3 = c1312 * t * t + c14 * r * r;
-  double s1 = (b1 + eps) * (b1 + eps);
-  double s2 = (b2 + eps) * (b2 + eps);
-  double s3 = (b3 + eps) * (b3 + eps);
-  double a1 = 0.1 * (s2 * s3), a2 = 0.6 * (s1 * s3), a3 = 0.3 * (s1 * s2);
-  double f1 = (2 * s6) * g3 + (-7 * s6) * g2 + (11 * s6) * g1;
-  double f2 = (-s6) * g2 + (5 * s6) * g1 + (2 * s6) * q1;
-  double f3 = (2 * s6) * g1 + (5 * s6) * q1 + (-s6) * q2;
+  const double b3 = c1312 * t * t + c14 * r * r;
+  const double f1 = (2 * s6) * g3 + (-7 * s6) * g2 + (11 * s6) * g1;
+  const double f2 = (-s6) * g2 + (5 * s6) * g1 + (2 * s6) * q1;
+  const double f3 = (2 * s6) * g1 + (5 * s6) * q1 + (-s6) * q2;
+  const double s1 = (b1 + eps) * (b1 + eps);
+  const double s2 = (b2 + eps) * (b2 + eps);
+  const double s3 = (b3 + eps) * (b3 + eps);
+  if (type == 1) {
+    // mapped (Henrick, Aslam & Powers): JS weights w mapped by
+    // w (c + c^2 - 3 c w + w^2) / (c^2 + w (1 - 2c)), normalised again
+    const double c1 = 0.1, c2 = 0.6, c3 = 0.3;
+    const double a1 = c1 / s1, a2 = c2 / s2, a3 = c3 / s3;
+    const double in = 1.0 / (a1 + a2 + a3);
+    const double w1 = a1 * in, w2 = a2 * in, w3 = a3 * in;
+    const double d1 = w1 * (c1 + c1 * c1 - 3 * c1 * w1 + w1 * w1) / (c1 * c1 + w1 * (1.0 - 2.0 * c1));
+    const double d2 = w2 * (c2 + c2 * c2 - 3 * c2 * w2 + w2 * w2) / (c2 * c2 + w2 * (1.0 - 2.0 * c2));
+    const double d3 = w3 * (c3 + c3 * c3 - 3 * c3 * w3 + w3 * w3) / (c3 * c3 + w3 * (1.0 - 2.0 * c3));
+    return (d1 * f1 + d2 * f2 + d3 * f3) / (d1 + d2 + d3);
+  }
+  if (type == 2 || type == 3) {
+    // a_k = c_k (1 + (tau/(b_k+eps))^2), multiplied through by s1 s2 s3
+    double tau;
+    if (type == 2) {
+      tau = fabs(b3 - b1);
+    } else {
+      const double d4 = m3 - 4 * m2 + 6 * m1 - 4 * p1 + p2;
+      tau = d4 * d4;
+    }
+    const double t2 = tau * tau;
+    const double a1 = 0.1 * ((s1 + t2) * (s2 * s3)), a2 = 0.6 * ((s2 + t2) * (s1 * s3)),
+                 a3 = 0.3 * ((s3 + t2) * (s1 * s2));
+    return (a1 * f1 + a2 * f2 + a3 * f3) / (a1 + a2 + a3);
+  }
+  const double a1 = 0.1 * (s2 * s3), a2 = 0.6 * (s1 * s3), a3 = 0.3 * (s1 * s2);
   return (a1 * f1 + a2 * f2 + a3 * f3) / (a1 + a2 + a3);
 }
 
@@ -202,18 +231,17 @@ __device__ __forceinline__ double d4c(double fm2, double fm1, double fp1, double
   return (fm2 - 8 * fm1 + 8 * fp1 - fp2) * (1.0 / 12.0);
 }
 
-// d/dx_DIR of the viscous flux at cell idx, not yet scaled by dxinv:
-// (F(-2) - 8F(-1) + 8F(+1) - F(+2))/12 with the ghost-layer values of FVg
-// where the stencil leaves the interior.  FV[(DIR*4+c)*ncell + cell];
-// FVg[(slot*4+c)*nplane + line], slot 0..3 for positions -2,-1,n,n+1 of the
-// line (line = j+ny*k along x, i+nx*k along y, i+nx*j along z).
+// Viscous flux along DIR at offsets -2,-1,+1,+2 from cell idx along DIR,
+// from FV inside the grid and from the ghost-layer values of FVg where the
+// stencil leaves it.  FV[(DIR*4+c)*ncell + cell]; FVg[(slot*4+c)*nplane +
+// line], slot 0..3 for positions -2,-1,n,n+1 of the line (line = j+ny*k
+// along x, i+nx*k along y, i+nx*j along z).
 template <int DIR>
-__device__ __forceinline__ void visc_div_dir(double out[4], const double *__restrict__ FV,
-                                             const double *__restrict__ FVg, size_t ncell, size_t idx,
-                                             int pos, int n, size_t stride, size_t line) {
+__device__ __forceinline__ void visc_load_dir(double F[4][4], const double *__restrict__ FV,
+                                              const double *__restrict__ FVg, size_t ncell, size_t idx, int pos,
+                                              int n, size_t stride, size_t line) {
   const size_t base = idx - (size_t)pos * stride;
   const size_t nplane = ncell / n;
-  double F[4][4];  // offsets -2,-1,+1,+2
   const int offs[4] = {-2, -1, 1, 2};
 #pragma unroll
   for (int s = 0; s < 4; s++) {
@@ -228,6 +256,16 @@ __device__ __forceinline__ void visc_div_dir(double out[4], const double *__rest
       for (int v = 0; v < 4; v++) F[s][v] = FVg[(size_t)(slot * 4 + v) * nplane + line];
     }
   }
+}
+
+// d/dx_DIR of the viscous flux at cell idx, not yet scaled by dxinv:
+// (F(-2) - 8F(-1) + 8F(+1) - F(+2))/12
+template <int DIR>
+__device__ __forceinline__ void visc_div_dir(double out[4], const double *__restrict__ FV,
+                                             const double *__restrict__ FVg, size_t ncell, size_t idx,
+                                             int pos, int n, size_t stride, size_t line) {
+  double F[4][4];
+  visc_load_dir<DIR>(F, FV, FVg, ncell, idx, pos, n, stride, line);
 #pragma unroll
   for (int v = 0; v < 4; v++) out[v] = d4c(F[0][v], F[1][v], F[2][v], F[3][v]);
 }

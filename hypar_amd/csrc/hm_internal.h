@@ -9,6 +9,7 @@ namespace hm {
 enum { INTERP_COMP = 0, INTERP_CHAR = 1 };
 enum { UPWIND_ROE = 0, UPWIND_RUSANOV = 1 };
 enum { OUT_STAGE = 0, OUT_RHS = 1, OUT_HYP = 2, OUT_PAR = 3 };
+enum { WENO_JS = 0, WENO_MAPPED = 1, WENO_Z = 2, WENO_YC = 3 };
 
 struct Grid {
   int n[3];
@@ -21,6 +22,7 @@ struct Scheme {
   double Re;  // as the solver uses it (physics.inp Re / Minf); <= 0 inviscid
   double Pr;
   double eps;
+  int weno;   // WENO_*
 };
 
 // interface fluxes along dir of the hm-layout state u into fI (5*ncell); the
@@ -97,8 +99,11 @@ void launch_ens_pass(const Scheme &sc, int dir, int shape, const EnsMember *mem,
 
 // discrete adjoint (hm_adjoint.hip)
 bool vjp_supported(const Grid &g, const Scheme &sc);
-void launch_rhs_vjp(const Grid &g, const Scheme &sc, const double *u, const double *lam, double *out,
-                    const double *dxg, hipStream_t st);
+// nu = (d rhs/d u)^T mu at u with mu = cb*lam + ca*nu_in (nu_in may be
+// null); when L is not null the finished nu is also added into L (set_L:
+// L = lam + nu)
+void launch_rhs_vjp(const Grid &g, const Scheme &sc, const double *u, const double *lam, const double *nu_in,
+                    double ca, double cb, double *nu, double *L, bool set_L, const double *dxg, hipStream_t st);
 void launch_axpby(const double *x, double a, double *y, double b, size_t n, hipStream_t st);
 
 }  // namespace hm

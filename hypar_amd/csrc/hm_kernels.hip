@@ -25,7 +25,7 @@ namespace hm {
 // interface flux kernel
 // ---------------------------------------------------------------------------
 template <int DIR, int INTERP, int UPW>
-__device__ __forceinline__ void iface_flux(const double (&U)[6][NV], double gamma, double eps,
+__device__ __forceinline__ void iface_flux(const double (&U)[6][NV], double gamma, double eps, int weno,
                                            double fI[NV]) {
   const double inv_gm1 = 1.0 / (gamma - 1.0);
   double F[6][NV];
@@ -65,10 +65,10 @@ __device__ __forceinline__ void iface_flux(const double (&U)[6][NV], double gamm
         pu[s] = au;
         pc[s] = au + E.L[v][4] * (UC4[s] - U[s][4]);
       }
-      double fcL = weno5(pf[0], pf[1], pf[2], pf[3], pf[4], pf[0], pf[1], pf[2], pf[3], pf[4], eps);
-      double fcR = weno5(pf[5], pf[4], pf[3], pf[2], pf[1], pf[5], pf[4], pf[3], pf[2], pf[1], eps);
-      double ucL = weno5(pu[0], pu[1], pu[2], pu[3], pu[4], pc[0], pc[1], pc[2], pc[3], pc[4], eps);
-      double ucR = weno5(pu[5], pu[4], pu[3], pu[2], pu[1], pc[5], pc[4], pc[3], pc[2], pc[1], eps);
+      double fcL = weno5(pf[0], pf[1], pf[2], pf[3], pf[4], pf[0], pf[1], pf[2], pf[3], pf[4], eps, weno);
+      double fcR = weno5(pf[5], pf[4], pf[3], pf[2], pf[1], pf[5], pf[4], pf[3], pf[2], pf[1], eps, weno);
+      double ucL = weno5(pu[0], pu[1], pu[2], pu[3], pu[4], pc[0], pc[1], pc[2], pc[3], pc[4], eps, weno);
+      double ucR = weno5(pu[5], pu[4], pu[3], pu[2], pu[1], pc[5], pc[4], pc[3], pc[2], pc[1], eps, weno);
       double lam = (UPW == UPWIND_ROE) ? harten_abs(E.lam[v]) : alpha;
       g[v] = 0.5 * (fcL + fcR) - lam * (0.5 * (ucR - ucL));
     }
@@ -86,10 +86,10 @@ __device__ __forceinline__ void iface_flux(const double (&U)[6][NV], double gamm
       double c0 = (v == 4) ? UC4[0] : U[0][v], c1 = (v == 4) ? UC4[1] : U[1][v];
       double c2 = (v == 4) ? UC4[2] : U[2][v], c3 = (v == 4) ? UC4[3] : U[3][v];
       double c4 = (v == 4) ? UC4[4] : U[4][v], c5 = (v == 4) ? UC4[5] : U[5][v];
-      fL[v] = weno5(F[0][v], F[1][v], F[2][v], F[3][v], F[4][v], F[0][v], F[1][v], F[2][v], F[3][v], F[4][v], eps);
-      fR[v] = weno5(F[5][v], F[4][v], F[3][v], F[2][v], F[1][v], F[5][v], F[4][v], F[3][v], F[2][v], F[1][v], eps);
-      uL[v] = weno5(U[0][v], U[1][v], U[2][v], U[3][v], U[4][v], c0, c1, c2, c3, c4, eps);
-      uR[v] = weno5(U[5][v], U[4][v], U[3][v], U[2][v], U[1][v], c5, c4, c3, c2, c1, eps);
+      fL[v] = weno5(F[0][v], F[1][v], F[2][v], F[3][v], F[4][v], F[0][v], F[1][v], F[2][v], F[3][v], F[4][v], eps, weno);
+      fR[v] = weno5(F[5][v], F[4][v], F[3][v], F[2][v], F[1][v], F[5][v], F[4][v], F[3][v], F[2][v], F[1][v], eps, weno);
+      uL[v] = weno5(U[0][v], U[1][v], U[2][v], U[3][v], U[4][v], c0, c1, c2, c3, c4, eps, weno);
+      uR[v] = weno5(U[5][v], U[4][v], U[3][v], U[2][v], U[1][v], c5, c4, c3, c2, c1, eps, weno);
     }
     if (UPW == UPWIND_RUSANOV) {
       Prim l = flowvar(U[2], gamma), r = flowvar(U[3], gamma);
@@ -127,7 +127,7 @@ __device__ __forceinline__ void iface_flux(const double (&U)[6][NV], double gamm
 template <int DIR, int INTERP, int UPW>
 __global__ void __launch_bounds__(256) k_iface(const double *__restrict__ u, double *__restrict__ fI,
                                                double *__restrict__ Q, int nx, int ny, int nz,
-                                               double gamma, double eps) {
+                                               double gamma, double eps, int weno) {
   const size_t ncell = (size_t)nx * ny * nz;
   const size_t idx = xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= ncell) return;
@@ -153,7 +153,7 @@ __global__ void __launch_bounds__(256) k_iface(const double *__restrict__ u, dou
     Q[3 * ncell + idx] = gamma * q.P / q.rho;
   }
   double f[NV];
-  iface_flux<DIR, INTERP, UPW>(U, gamma, eps, f);
+  iface_flux<DIR, INTERP, UPW>(U, gamma, eps, weno, f);
 #pragma unroll
   for (int v = 0; v < NV; v++) fI[(size_t)v * ncell + idx] = f[v];
 }
@@ -217,43 +217,82 @@ __device__ __forceinline__ void visc_flux(double fv[4], const QT &q, const doubl
 // Viscous fluxes of x, y, z at every cell: FV[(d*4+c)*ncell + cell],
 // c = momentum x, y, z, energy.  Gradients use the 4th-order central
 // difference with periodic neighbours, scaled by the cell's dxinv.
-__global__ void __launch_bounds__(256) k_visc_flux(const double *__restrict__ Q, double *__restrict__ FV,
-                                                   int nx, int ny, int nz,
-                                                   const double *__restrict__ dxg,  // ghosted dxinv x|y|z
-                                                   ViscParams vp) {
+//
+// A block owns a VTX x VTY tile of (x, y) columns and marches through VKS
+// z-planes.  Each thread keeps its column's (u, v, w, T) at k-2..k+2 in
+// registers (the z-derivative stencil, one new plane loaded per step); the
+// tile's plane k with a 2-cell halo in x and y is staged in LDS for the x-
+// and y-derivatives.  Every plane of Q is thus read from HBM about once per
+// tile, instead of once per stencil point.
+constexpr int VTX = 64, VTY = 4, VKS = 16;
+constexpr int VHX = VTX + 4, VHY = VTY + 4;
+
+__global__ void __launch_bounds__(VTX * VTY) k_visc_flux(const double *__restrict__ Q, double *__restrict__ FV,
+                                                         int nx, int ny, int nz,
+                                                         const double *__restrict__ dxg,  // ghosted dxinv x|y|z
+                                                         ViscParams vp) {
+  __shared__ double sQ[4][VHY][VHX];
   const size_t ncell = (size_t)nx * ny * nz;
-  const size_t idx = xcd_block() * blockDim.x + threadIdx.x;
-  if (idx >= ncell) return;
-  const int i = (int)(idx % nx);
-  const int j = (int)((idx / nx) % ny);
-  const int k = (int)(idx / ((size_t)nx * ny));
-  const int n[3] = {nx, ny, nz};
-  const int pos[3] = {i, j, k};
-  const size_t st[3] = {1, (size_t)nx, (size_t)nx * ny};
-  const double *dxi[3] = {dxg + 3, dxg + (nx + 6) + 3, dxg + (nx + 6) + (ny + 6) + 3};
-  QT q0 = load_q(Q, ncell, idx);
-  double gr[4][3];
+  const size_t plane = (size_t)nx * ny;
+  const int ntx = (nx + VTX - 1) / VTX, nty = (ny + VTY - 1) / VTY;
+  const int b = blockIdx.x;
+  const int tx = b % ntx, ty = (b / ntx) % nty, kz = b / (ntx * nty);
+  const int x0 = tx * VTX, y0 = ty * VTY, k0 = kz * VKS;
+  const int lx = threadIdx.x % VTX, ly = threadIdx.x / VTX;
+  const int i = x0 + lx, j = y0 + ly;
+  const bool own = i < nx && j < ny;
+  const int ic = own ? i : 0, jc = own ? j : 0;
+  const size_t col = (size_t)jc * nx + ic;
+  const double dxi = dxg[3 + ic], dyi = dxg[(nx + 6) + 3 + jc];
+  const double *dzi = dxg + (nx + 6) + (ny + 6) + 3;
+  const int k1 = min(k0 + VKS, nz);
+  // window: planes k-2..k+2 of this column
+  QT q[5];
 #pragma unroll
-  for (int d = 0; d < 3; d++) {
-    const size_t base = idx - (size_t)pos[d] * st[d];
-    QT qm2 = load_q(Q, ncell, base + (size_t)wrap(pos[d] - 2, n[d]) * st[d]);
-    QT qm1 = load_q(Q, ncell, base + (size_t)wrap(pos[d] - 1, n[d]) * st[d]);
-    QT qp1 = load_q(Q, ncell, base + (size_t)wrap(pos[d] + 1, n[d]) * st[d]);
-    QT qp2 = load_q(Q, ncell, base + (size_t)wrap(pos[d] + 2, n[d]) * st[d]);
-    const double s = dxi[d][pos[d]];
-    gr[0][d] = d4c(qm2.u, qm1.u, qp1.u, qp2.u) * s;
-    gr[1][d] = d4c(qm2.v, qm1.v, qp1.v, qp2.v) * s;
-    gr[2][d] = d4c(qm2.w, qm1.w, qp1.w, qp2.w) * s;
-    gr[3][d] = d4c(qm2.T, qm1.T, qp1.T, qp2.T) * s;
+  for (int s = 0; s < 4; s++) q[s + 1] = load_q(Q, ncell, col + (size_t)wrap(k0 - 2 + s, nz) * plane);
+  for (int k = k0; k < k1; k++) {
+#pragma unroll
+    for (int s = 0; s < 4; s++) q[s] = q[s + 1];
+    q[4] = load_q(Q, ncell, col + (size_t)wrap(k + 2, nz) * plane);
+    // tile of plane k with halo
+    const size_t pk = (size_t)k * plane;
+    for (int c = threadIdx.x; c < VHX * VHY; c += VTX * VTY) {
+      const int hx = c % VHX, hy = c / VHX;
+      // x0 - 2 + hx spans [x0 - 2, x0 + VTX + 1], which can exceed 2 nx
+      const int gi = (x0 - 2 + hx + 2 * nx) % nx, gj = wrap(y0 - 2 + hy, ny);
+      const QT h = load_q(Q, ncell, pk + (size_t)gj * nx + gi);
+      sQ[0][hy][hx] = h.u;
+      sQ[1][hy][hx] = h.v;
+      sQ[2][hy][hx] = h.w;
+      sQ[3][hy][hx] = h.T;
+    }
+    __syncthreads();
+    if (own) {
+      double gr[4][3];
+      const double dzk = dzi[k];
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const double *r = &sQ[c][ly + 2][0];
+        gr[c][0] = d4c(r[lx], r[lx + 1], r[lx + 3], r[lx + 4]) * dxi;
+        gr[c][1] = d4c(sQ[c][ly][lx + 2], sQ[c][ly + 1][lx + 2], sQ[c][ly + 3][lx + 2], sQ[c][ly + 4][lx + 2]) * dyi;
+      }
+      gr[0][2] = d4c(q[0].u, q[1].u, q[3].u, q[4].u) * dzk;
+      gr[1][2] = d4c(q[0].v, q[1].v, q[3].v, q[4].v) * dzk;
+      gr[2][2] = d4c(q[0].w, q[1].w, q[3].w, q[4].w) * dzk;
+      gr[3][2] = d4c(q[0].T, q[1].T, q[3].T, q[4].T) * dzk;
+      const QT &q0 = q[2];
+      const double mu = exp(0.76 * log(q0.T));
+      const size_t idx = pk + col;
+      double fv[4];
+      visc_flux<0>(fv, q0, gr, mu, vp);
+      for (int c = 0; c < 4; c++) FV[(size_t)(0 * 4 + c) * ncell + idx] = fv[c];
+      visc_flux<1>(fv, q0, gr, mu, vp);
+      for (int c = 0; c < 4; c++) FV[(size_t)(1 * 4 + c) * ncell + idx] = fv[c];
+      visc_flux<2>(fv, q0, gr, mu, vp);
+      for (int c = 0; c < 4; c++) FV[(size_t)(2 * 4 + c) * ncell + idx] = fv[c];
+    }
+    __syncthreads();
   }
-  const double mu = exp(0.76 * log(q0.T));
-  double fv[4];
-  visc_flux<0>(fv, q0, gr, mu, vp);
-  for (int c = 0; c < 4; c++) FV[(size_t)(0 * 4 + c) * ncell + idx] = fv[c];
-  visc_flux<1>(fv, q0, gr, mu, vp);
-  for (int c = 0; c < 4; c++) FV[(size_t)(1 * 4 + c) * ncell + idx] = fv[c];
-  visc_flux<2>(fv, q0, gr, mu, vp);
-  for (int c = 0; c < 4; c++) FV[(size_t)(2 * 4 + c) * ncell + idx] = fv[c];
 }
 
 // Viscous flux along DIR at the ghost positions gp in {-2,-1,n,n+1} of every
@@ -480,20 +519,20 @@ static inline unsigned nblocks(size_t n, unsigned bs) { return (unsigned)((n + b
 
 template <int DIR, int INTERP, int UPW>
 static void launch_iface_t(const Grid &g, const double *u, double *fI, double *Q, double gamma,
-                           double eps, hipStream_t st) {
+                           double eps, int weno, hipStream_t st) {
   hipLaunchKernelGGL((k_iface<DIR, INTERP, UPW>), dim3(nblocks(g.ncell, 256)), dim3(256), 0, st, u, fI, Q,
-                     g.n[0], g.n[1], g.n[2], gamma, eps);
+                     g.n[0], g.n[1], g.n[2], gamma, eps, weno);
 }
 
 template <int DIR>
 static void launch_iface_d(const Grid &g, const Scheme &sc, const double *u, double *fI, double *Q,
                            hipStream_t st) {
   if (sc.interp == INTERP_CHAR) {
-    if (sc.upwind == UPWIND_ROE) launch_iface_t<DIR, INTERP_CHAR, UPWIND_ROE>(g, u, fI, Q, sc.gamma, sc.eps, st);
-    else launch_iface_t<DIR, INTERP_CHAR, UPWIND_RUSANOV>(g, u, fI, Q, sc.gamma, sc.eps, st);
+    if (sc.upwind == UPWIND_ROE) launch_iface_t<DIR, INTERP_CHAR, UPWIND_ROE>(g, u, fI, Q, sc.gamma, sc.eps, sc.weno, st);
+    else launch_iface_t<DIR, INTERP_CHAR, UPWIND_RUSANOV>(g, u, fI, Q, sc.gamma, sc.eps, sc.weno, st);
   } else {
-    if (sc.upwind == UPWIND_ROE) launch_iface_t<DIR, INTERP_COMP, UPWIND_ROE>(g, u, fI, Q, sc.gamma, sc.eps, st);
-    else launch_iface_t<DIR, INTERP_COMP, UPWIND_RUSANOV>(g, u, fI, Q, sc.gamma, sc.eps, st);
+    if (sc.upwind == UPWIND_ROE) launch_iface_t<DIR, INTERP_COMP, UPWIND_ROE>(g, u, fI, Q, sc.gamma, sc.eps, sc.weno, st);
+    else launch_iface_t<DIR, INTERP_COMP, UPWIND_RUSANOV>(g, u, fI, Q, sc.gamma, sc.eps, sc.weno, st);
   }
 }
 
@@ -518,8 +557,8 @@ size_t visc_ghost_size(const Grid &g, int dir) { return 16 * (g.ncell / g.n[dir]
 void launch_visc_flux(const Grid &g, const Scheme &sc, const double *Q, double *FV, double *const FVg[3],
                       const double *dxg, hipStream_t st) {
   const ViscParams vp = visc_params(sc);
-  hipLaunchKernelGGL(k_visc_flux, dim3(nblocks(g.ncell, 256)), dim3(256), 0, st, Q, FV, g.n[0], g.n[1],
-                     g.n[2], dxg, vp);
+  const unsigned nb = (unsigned)(((g.n[0] + VTX - 1) / VTX) * ((g.n[1] + VTY - 1) / VTY) * ((g.n[2] + VKS - 1) / VKS));
+  hipLaunchKernelGGL(k_visc_flux, dim3(nb), dim3(VTX * VTY), 0, st, Q, FV, g.n[0], g.n[1], g.n[2], dxg, vp);
   hipLaunchKernelGGL(k_visc_ghost<0>, dim3(nblocks(4 * (g.ncell / g.n[0]), 256)), dim3(256), 0, st, Q,
                      FVg[0], g.n[0], g.n[1], g.n[2], dxg, vp);
   hipLaunchKernelGGL(k_visc_ghost<1>, dim3(nblocks(4 * (g.ncell / g.n[1]), 256)), dim3(256), 0, st, Q,

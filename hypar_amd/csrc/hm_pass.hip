@@ -104,24 +104,58 @@ __device__ __forceinline__ void cell_data(const double u[NV], const PassConst &k
   c[9] = (P * k.inv_gm1) + (0.5 * rho * vsq);
 }
 
-// Jiang-Shu WENO5 on both sides of one interface.  b[0..5] are the values of
-// cells I-3..I+2 the smoothness indicators use, g[0..5] the values that are
-// interpolated (include/interpolation.h:265-285,
+// WENO5 on both sides of one interface.  b[0..5] are the values of cells
+// I-3..I+2 the smoothness indicators use, g[0..5] the values that are
+// interpolated (include/interpolation.h:265-500,
 // Interp1PrimFifthOrderWENO.c:146-158; the right-biased stencil is the
-// mirror image, WENOFifthOrderCalculateWeights.c:166-177).
-__device__ __forceinline__ double weno_combine(double b1, double b2, double b3, double g3, double g2,
+// mirror image, WENOFifthOrderCalculateWeights.c:166-177).  The indicators
+// arrive scaled by 4 (l1..l3 = 4 beta_k) together with eps4 = 4 eps; W
+// selects the weights:
+//  * WENO_JS:     a_k = c_k / (beta_k + eps)^2
+//  * WENO_Z:      a_k = c_k (1 + (tau / (beta_k + eps))^2), tau = |beta_1 - beta_3|
+//  * WENO_YC:     the same with tau = (m3 - 4 m2 + 6 m1 - 4 p1 + p2)^2
+//  * WENO_MAPPED: the JS weights w mapped by
+//                 w (c + c^2 - 3 c w + w^2) / (c^2 + w (1 - 2c)) and renormalised
+// All are applied in ratio form with one reciprocal (two for the mapped).
+template <int W>
+__device__ __forceinline__ double weno_combine(double l1, double l2, double l3, double d4, double g3, double g2,
                                                double g1, double q1, double q2, double eps4) {
   const double s6 = 1.0 / 6.0;
-  const double s1 = (b1 + eps4) * (b1 + eps4);
-  const double s2 = (b2 + eps4) * (b2 + eps4);
-  const double s3 = (b3 + eps4) * (b3 + eps4);
-  const double a1 = 0.1 * (s2 * s3), a2 = 0.6 * (s1 * s3), a3 = 0.3 * (s1 * s2);
+  const double s1 = (l1 + eps4) * (l1 + eps4);
+  const double s2 = (l2 + eps4) * (l2 + eps4);
+  const double s3 = (l3 + eps4) * (l3 + eps4);
   const double f1 = (2 * s6) * g3 + (-7 * s6) * g2 + (11 * s6) * g1;
   const double f2 = (-s6) * g2 + (5 * s6) * g1 + (2 * s6) * q1;
   const double f3 = (2 * s6) * g1 + (5 * s6) * q1 + (-s6) * q2;
+  double a1, a2, a3;
+  if (W == WENO_Z || W == WENO_YC) {
+    // tau scales like beta for WENO-Z; the YC tau does not, hence 4 tau
+    const double tau = W == WENO_Z ? fabs(l3 - l1) : 4.0 * (d4 * d4);
+    const double t2 = tau * tau;
+    a1 = 0.1 * ((s1 + t2) * (s2 * s3));
+    a2 = 0.6 * ((s2 + t2) * (s1 * s3));
+    a3 = 0.3 * ((s3 + t2) * (s1 * s2));
+  } else {
+    a1 = 0.1 * (s2 * s3);
+    a2 = 0.6 * (s1 * s3);
+    a3 = 0.3 * (s1 * s2);
+  }
+  if (W == WENO_MAPPED) {
+    const double c1 = 0.1, c2 = 0.6, c3 = 0.3;
+    const double in = nr_rcp(a1 + a2 + a3);
+    const double w1 = a1 * in, w2 = a2 * in, w3 = a3 * in;
+    const double n1 = w1 * (c1 + c1 * c1 - 3 * c1 * w1 + w1 * w1), e1 = c1 * c1 + w1 * (1.0 - 2.0 * c1);
+    const double n2 = w2 * (c2 + c2 * c2 - 3 * c2 * w2 + w2 * w2), e2 = c2 * c2 + w2 * (1.0 - 2.0 * c2);
+    const double n3 = w3 * (c3 + c3 * c3 - 3 * c3 * w3 + w3 * w3), e3 = c3 * c3 + w3 * (1.0 - 2.0 * c3);
+    // n_k / e_k multiplied through by e1 e2 e3
+    a1 = n1 * (e2 * e3);
+    a2 = n2 * (e1 * e3);
+    a3 = n3 * (e1 * e2);
+  }
   return (a1 * f1 + a2 * f2 + a3 * f3) * nr_rcp(a1 + a2 + a3);
 }
 
+template <int W>
 __device__ __forceinline__ void weno5_lr(const double (&b)[6], const double (&g)[6], double eps4, double &vl,
                                          double &vr) {
   const double c13 = 13.0 / 3.0;  // 4 * 13/12
@@ -141,8 +175,11 @@ __device__ __forceinline__ void weno5_lr(const double (&b)[6], const double (&g)
   const double r2 = q3 + e * e;
   e = 3 * b[3] - 4 * b[2] + b[1];
   const double r3 = q2 + e * e;
-  vl = weno_combine(l1, l2, l3, g[0], g[1], g[2], g[3], g[4], eps4);
-  vr = weno_combine(r1, r2, r3, g[5], g[4], g[3], g[2], g[1], eps4);
+  // 4th undivided differences of the two stencils (YC weights only)
+  const double tl = W == WENO_YC ? b[0] - 4 * b[1] + 6 * b[2] - 4 * b[3] + b[4] : 0.0;
+  const double tr = W == WENO_YC ? b[5] - 4 * b[4] + 6 * b[3] - 4 * b[2] + b[1] : 0.0;
+  vl = weno_combine<W>(l1, l2, l3, tl, g[0], g[1], g[2], g[3], g[4], eps4);
+  vr = weno_combine<W>(r1, r2, r3, tr, g[5], g[4], g[3], g[2], g[1], eps4);
 }
 
 // Roe-averaged state of two cells (navierstokes3d.h:144-169) and the
@@ -256,7 +293,7 @@ __device__ __forceinline__ void right_mult(const RoeState &s, double g0, double 
 
 // Interface flux at I-1/2 from the LDS data of cells I-3..I+2;
 // ld(s, q) returns quantity q of cell I-3+s.
-template <int DIR, int INTERP, int UPW, class LD>
+template <int DIR, int INTERP, int UPW, int W, class LD>
 __device__ __forceinline__ void iface(const LD &ld, const PassConst &k, double fI[NV]) {
   double uL[NV], uR[NV];
 #pragma unroll
@@ -286,14 +323,14 @@ __device__ __forceinline__ void iface(const LD &ld, const PassConst &k, double f
     const double l0 = UPW == UPWIND_ROE ? harten(s.vn) : alpha;
     const double lm = UPW == UPWIND_ROE ? harten(s.vn - s.a) : alpha;
     const double l4 = UPW == UPWIND_ROE ? harten(s.vn + s.a) : alpha;
-    weno5_lr(f0, f0, k.eps4, fl, fr);
-    weno5_lr(u0, c0, k.eps4, ul, ur);
+    weno5_lr<W>(f0, f0, k.eps4, fl, fr);
+    weno5_lr<W>(u0, c0, k.eps4, ul, ur);
     const double g0 = 0.5 * (fl + fr) - l0 * (0.5 * (ur - ul));
-    weno5_lr(fm, fm, k.eps4, fl, fr);
-    weno5_lr(um, cm, k.eps4, ul, ur);
+    weno5_lr<W>(fm, fm, k.eps4, fl, fr);
+    weno5_lr<W>(um, cm, k.eps4, ul, ur);
     const double gm = 0.5 * (fl + fr) - lm * (0.5 * (ur - ul));
-    weno5_lr(f4, f4, k.eps4, fl, fr);
-    weno5_lr(u4, c4, k.eps4, ul, ur);
+    weno5_lr<W>(f4, f4, k.eps4, fl, fr);
+    weno5_lr<W>(u4, c4, k.eps4, ul, ur);
     const double g4 = 0.5 * (fl + fr) - l4 * (0.5 * (ur - ul));
     // shear rows: the modified energy does not enter them (L[row][4] = 0)
     double fa[6], fb[6], ua[6], ub[6];
@@ -303,11 +340,11 @@ __device__ __forceinline__ void iface(const LD &ld, const PassConst &k, double f
       proj_shear<DIR>(s, ld(p, 0), ld(p, 1), ld(p, 2), ld(p, 3), ua[p], ub[p]);
     }
     const double ls = UPW == UPWIND_ROE ? harten(s.vn) : alpha;
-    weno5_lr(fa, fa, k.eps4, fl, fr);
-    weno5_lr(ua, ua, k.eps4, ul, ur);
+    weno5_lr<W>(fa, fa, k.eps4, fl, fr);
+    weno5_lr<W>(ua, ua, k.eps4, ul, ur);
     const double ga = 0.5 * (fl + fr) - ls * (0.5 * (ur - ul));
-    weno5_lr(fb, fb, k.eps4, fl, fr);
-    weno5_lr(ub, ub, k.eps4, ul, ur);
+    weno5_lr<W>(fb, fb, k.eps4, fl, fr);
+    weno5_lr<W>(ub, ub, k.eps4, ul, ur);
     const double gb = 0.5 * (fl + fr) - ls * (0.5 * (ur - ul));
     right_mult<DIR>(s, g0, gm, g4, ga, gb, fI);
   } else {
@@ -319,13 +356,13 @@ __device__ __forceinline__ void iface(const LD &ld, const PassConst &k, double f
 #pragma unroll
       for (int p = 0; p < 6; p++) b[p] = ld(p, v == 0 ? DIR + 1 : 4 + v);
       double fl, fr, ul, ur;
-      weno5_lr(b, b, k.eps4, fl, fr);
+      weno5_lr<W>(b, b, k.eps4, fl, fr);
 #pragma unroll
       for (int p = 0; p < 6; p++) {
         b[p] = ld(p, v);
         g[p] = v == 4 ? ld(p, 9) : b[p];
       }
-      weno5_lr(b, g, k.eps4, ul, ur);
+      weno5_lr<W>(b, g, k.eps4, ul, ur);
       fh[v] = 0.5 * (fl + fr);
       du[v] = 0.5 * (ur - ul);
     }
@@ -419,7 +456,7 @@ struct XShared {
 };
 
 // lines LPB*group .. LPB*group + LPB-1 of the grid (line = j + ny*k)
-template <int INTERP, int UPW, int LPB>
+template <int INTERP, int UPW, int LPB, int W>
 __device__ __forceinline__ void pass_x_body(const PassConst &k, const PassIO &io, size_t group,
                                             XShared<LPB> &sh) {
   const int lane = threadIdx.x & 63;
@@ -449,7 +486,7 @@ __device__ __forceinline__ void pass_x_body(const PassConst &k, const PassIO &io
     if (active && x0 + lane < nx) {
       auto ld = [&](int s, int q) { return sh.sC[w][q][lane + s]; };
       double f[NV];
-      iface<0, INTERP, UPW>(ld, k, f);
+      iface<0, INTERP, UPW, W>(ld, k, f);
 #pragma unroll
       for (int v = 0; v < NV; v++) sh.sF[w][v][lane + 1] = f[v];
       if (x0 == 0 && lane == 0)
@@ -497,7 +534,7 @@ struct MShared {
 };
 
 // columns MW*group .. MW*group + MW-1 of the plane across DIR
-template <int DIR, int INTERP, int UPW, int MR>
+template <int DIR, int INTERP, int UPW, int MR, int W>
 __device__ __forceinline__ void pass_m_body(const PassConst &k, const PassIO &io, size_t group,
                                             MShared<MR> &sh) {
   constexpr int NRING = MShared<MR>::NRING, NFR = MShared<MR>::NFR;
@@ -541,7 +578,7 @@ __device__ __forceinline__ void pass_m_body(const PassConst &k, const PassIO &io
         return sh.sC[q][sl][lane];
       };
       double f[NV];
-      iface<DIR, INTERP, UPW>(ld, k, f);
+      iface<DIR, INTERP, UPW, W>(ld, k, f);
       const int fs = t % NFR;
 #pragma unroll
       for (int v = 0; v < NV; v++) sh.sF[v][fs][lane] = f[v];
@@ -575,16 +612,16 @@ __device__ __forceinline__ void pass_m_body(const PassConst &k, const PassIO &io
   }
 }
 
-template <int INTERP, int UPW>
+template <int INTERP, int UPW, int W>
 __global__ void __launch_bounds__(256) k_pass_x(PassConst k, PassIO io) {
   __shared__ XShared<4> sh;
-  pass_x_body<INTERP, UPW, 4>(k, io, blockIdx.x, sh);
+  pass_x_body<INTERP, UPW, 4, W>(k, io, blockIdx.x, sh);
 }
 
-template <int DIR, int INTERP, int UPW>
+template <int DIR, int INTERP, int UPW, int W>
 __global__ void __launch_bounds__(256) k_pass_m(PassConst k, PassIO io) {
   __shared__ MShared<4> sh;
-  pass_m_body<DIR, INTERP, UPW, 4>(k, io, blockIdx.x, sh);
+  pass_m_body<DIR, INTERP, UPW, 4, W>(k, io, blockIdx.x, sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -647,7 +684,7 @@ __global__ void __launch_bounds__(64 * LPB) k_ens_x(EnsArgs a) {
     PassConst k;
     PassIO io;
     ens_setup(a, m, k, io);
-    pass_x_body<INTERP, UPW, LPB>(k, io, item - a.item0[m], sh);
+    pass_x_body<INTERP, UPW, LPB, WENO_JS>(k, io, item - a.item0[m], sh);
     __syncthreads();
   }
 }
@@ -660,7 +697,7 @@ __global__ void __launch_bounds__(64 * MR) k_ens_m(EnsArgs a) {
     PassConst k;
     PassIO io;
     ens_setup(a, m, k, io);
-    pass_m_body<DIR, INTERP, UPW, MR>(k, io, item - a.item0[m], sh);
+    pass_m_body<DIR, INTERP, UPW, MR, WENO_JS>(k, io, item - a.item0[m], sh);
     __syncthreads();
   }
 }
@@ -680,25 +717,33 @@ __global__ void __launch_bounds__(256) k_prims(const double *__restrict__ u, dou
   Q[3 * ncell + idx] = gamma * q.P / q.rho;
 }
 
-template <int DIR, int INTERP, int UPW>
+template <int DIR, int INTERP, int UPW, int W>
 void launch_t(const Grid &g, const PassConst &k, const PassIO &io, hipStream_t st) {
   if (DIR == 0) {
     const size_t nlines = (size_t)g.n[1] * g.n[2];
-    hipLaunchKernelGGL((k_pass_x<INTERP, UPW>), dim3((unsigned)((nlines + 3) / 4)), dim3(256), 0, st, k, io);
+    hipLaunchKernelGGL((k_pass_x<INTERP, UPW, W>), dim3((unsigned)((nlines + 3) / 4)), dim3(256), 0, st, k, io);
   } else {
     const size_t nb = (g.ncell / g.n[DIR] + MW - 1) / MW;
-    hipLaunchKernelGGL((k_pass_m<DIR == 0 ? 1 : DIR, INTERP, UPW>), dim3((unsigned)nb), dim3(256), 0, st, k, io);
+    hipLaunchKernelGGL((k_pass_m<DIR == 0 ? 1 : DIR, INTERP, UPW, W>), dim3((unsigned)nb), dim3(256), 0, st, k, io);
   }
+}
+
+template <int DIR, int INTERP, int UPW>
+void launch_w(const Grid &g, const Scheme &sc, const PassConst &k, const PassIO &io, hipStream_t st) {
+  if (sc.weno == WENO_MAPPED) launch_t<DIR, INTERP, UPW, WENO_MAPPED>(g, k, io, st);
+  else if (sc.weno == WENO_Z) launch_t<DIR, INTERP, UPW, WENO_Z>(g, k, io, st);
+  else if (sc.weno == WENO_YC) launch_t<DIR, INTERP, UPW, WENO_YC>(g, k, io, st);
+  else launch_t<DIR, INTERP, UPW, WENO_JS>(g, k, io, st);
 }
 
 template <int DIR>
 void launch_d(const Grid &g, const Scheme &sc, const PassConst &k, const PassIO &io, hipStream_t st) {
   if (sc.interp == INTERP_CHAR) {
-    if (sc.upwind == UPWIND_ROE) launch_t<DIR, INTERP_CHAR, UPWIND_ROE>(g, k, io, st);
-    else launch_t<DIR, INTERP_CHAR, UPWIND_RUSANOV>(g, k, io, st);
+    if (sc.upwind == UPWIND_ROE) launch_w<DIR, INTERP_CHAR, UPWIND_ROE>(g, sc, k, io, st);
+    else launch_w<DIR, INTERP_CHAR, UPWIND_RUSANOV>(g, sc, k, io, st);
   } else {
-    if (sc.upwind == UPWIND_ROE) launch_t<DIR, INTERP_COMP, UPWIND_ROE>(g, k, io, st);
-    else launch_t<DIR, INTERP_COMP, UPWIND_RUSANOV>(g, k, io, st);
+    if (sc.upwind == UPWIND_ROE) launch_w<DIR, INTERP_COMP, UPWIND_ROE>(g, sc, k, io, st);
+    else launch_w<DIR, INTERP_COMP, UPWIND_RUSANOV>(g, sc, k, io, st);
   }
 }
 

@@ -75,7 +75,7 @@ __global__ void __launch_bounds__(256) k_modsol(const double *__restrict__ u, do
 template <int DIR, int CHAR>
 __global__ void __launch_bounds__(256) k_interp(double *__restrict__ fI, const double *__restrict__ fC,
                                                 const double *__restrict__ u, int upw, int uflag,
-                                                int nx, int ny, int nz, double gamma, double eps) {
+                                                int nx, int ny, int nz, double gamma, double eps, int weno) {
   const size_t ncell = (size_t)nx * ny * nz;
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= ncell) return;
@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(256) k_interp(double *__restrict__ fI, const d
   if (!CHAR) {
     for (int v = 0; v < NV; v++)
       fI[(size_t)v * ncell + idx] = weno5(sm[0][v], sm[1][v], sm[2][v], sm[3][v], sm[4][v], val[0][v],
-                                          val[1][v], val[2][v], val[3][v], val[4][v], eps);
+                                          val[1][v], val[2][v], val[3][v], val[4][v], eps, weno);
     return;
   }
   double ul[NV], ur[NV];
@@ -117,7 +117,7 @@ __global__ void __launch_bounds__(256) k_interp(double *__restrict__ fI, const d
       ps[s] = a;
       pv[s] = b;
     }
-    fc[r] = weno5(ps[0], ps[1], ps[2], ps[3], ps[4], pv[0], pv[1], pv[2], pv[3], pv[4], eps);
+    fc[r] = weno5(ps[0], ps[1], ps[2], ps[3], ps[4], pv[0], pv[1], pv[2], pv[3], pv[4], eps, weno);
   }
   for (int r = 0; r < NV; r++) {
     double a = 0.0;
@@ -221,10 +221,10 @@ static void interp_d(const Grid &g, const Scheme &sc, double *fI, const double *
                      int upw, int uflag, hipStream_t st) {
   if (sc.interp == INTERP_CHAR)
     hipLaunchKernelGGL((k_interp<DIR, 1>), dim3(nb(g.ncell)), dim3(256), 0, st, fI, fC, u, upw, uflag,
-                       g.n[0], g.n[1], g.n[2], sc.gamma, sc.eps);
+                       g.n[0], g.n[1], g.n[2], sc.gamma, sc.eps, sc.weno);
   else
     hipLaunchKernelGGL((k_interp<DIR, 0>), dim3(nb(g.ncell)), dim3(256), 0, st, fI, fC, u, upw, uflag,
-                       g.n[0], g.n[1], g.n[2], sc.gamma, sc.eps);
+                       g.n[0], g.n[1], g.n[2], sc.gamma, sc.eps, sc.weno);
 }
 
 void launch_interp(const Grid &g, const Scheme &sc, int dir, double *fI, const double *fC,

@@ -90,8 +90,8 @@ int hy_initialize_solvers(HyPar *s, const char *dir) {
     return hy_error("hyp_flux_split yes is not supported by the GPU path");
   s->interp = (WENOParameters *)calloc(1, sizeof(WENOParameters));
   if (hy_read_weno(dir, s->interp)) return 1;
-  if (s->interp->mapped || s->interp->borges || s->interp->yc || s->interp->no_limiting)
-    return hy_error("only Jiang-Shu WENO5 weights are built (mapped/borges/yc/no_limiting = 0)");
+  if (s->interp->no_limiting)
+    return hy_error("no_limiting (frozen optimal WENO5 weights) is not supported");
   s->HyperbolicFunction = HyperbolicFunction;
   s->SourceFunction = SourceFunction;
   s->InterpolateInterfacesHyp = Interp1PrimFifthOrderWENO;
@@ -169,6 +169,9 @@ int NavierStokes3DInitialize(void *sv, void *m, const char *dir) {
   cfg.interp = strcmp(s->interp_type, "characteristic") ? HM_INTERP_COMPONENTS : HM_INTERP_CHARACTERISTIC;
   cfg.upwind = strcmp(p->upw_choice, "roe") ? HM_UPWIND_RUSANOV : HM_UPWIND_ROE;
   cfg.weno_eps = s->interp->eps;
+  /* the reference's precedence (WENOFifthOrderCalculateWeights.c:55-57) */
+  cfg.weno_type = s->interp->yc ? HM_WENO_YC
+                  : (s->interp->borges ? HM_WENO_Z : (s->interp->mapped ? HM_WENO_MAPPED : HM_WENO_JS));
   cfg.dxinv = s->dxinv;
   if (hm_check(hm_create(&cfg, &s->dev), "hm_create")) return 1;
   if (alloc_device(s)) return 1;
@@ -250,8 +253,8 @@ int Interp1PrimFifthOrderWENO(double *fI, double *fC, double *u, double *x, int 
   return hm_check(hm_interpolate(((HyPar *)s)->dev, fI, fC, u, upw, dir, uflag), "InterpolateInterfacesHyp");
 }
 
-/* The device interpolation evaluates the Jiang-Shu weights of its stencil
- * from the same inputs the reference's weight pass would use, so there is
+/* The device interpolation evaluates the WENO weights of its stencil from
+ * the same inputs the reference's weight pass would use, so there is
  * nothing to precompute. */
 int WENOFifthOrderCalculateWeights(double *fC, double *uC, double *x, int dir, void *s, void *m) {
   (void)fC; (void)uC; (void)x; (void)dir; (void)s; (void)m;

@@ -18,6 +18,7 @@ HM_OK = 0
 HM_ERR_ARG, HM_ERR_HIP, HM_ERR_NONFINITE, HM_ERR_UNSUPPORTED, HM_ERR_NOMEM = 1, 2, 3, 4, 5
 INTERP = {"components": 0, "characteristic": 1}
 UPWIND = {"roe": 0, "rusanov": 1}
+WENO = {"js": 0, "mapped": 1, "borges": 2, "yc": 3}
 LAYOUT_REFERENCE, LAYOUT_AOS_INTERIOR, LAYOUT_SOA_INTERIOR = 0, 1, 2
 
 
@@ -25,7 +26,7 @@ class HMConfig(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int * 3), ("gamma", ctypes.c_double), ("Re", ctypes.c_double),
                 ("Minf", ctypes.c_double), ("Pr", ctypes.c_double), ("interp", ctypes.c_int),
                 ("upwind", ctypes.c_int), ("weno_eps", ctypes.c_double),
-                ("dxinv", ctypes.POINTER(ctypes.c_double))]
+                ("dxinv", ctypes.POINTER(ctypes.c_double)), ("weno_type", ctypes.c_int)]
 
 
 class HMError(RuntimeError):

@@ -23,7 +23,7 @@ class NS3D:
 
     def __init__(self, dims: Sequence[int], interp: str = "components", upwind: str = "rusanov",
                  Re: float = -1.0, Minf: float = 1.0, Pr: float = 0.72, gamma: float = 1.4,
-                 eps: float = 1e-6, dxinv: Optional[np.ndarray] = None):
+                 eps: float = 1e-6, dxinv: Optional[np.ndarray] = None, weno_type: str = "js"):
         if not torch.cuda.is_available():
             raise RuntimeError("NS3D needs a GPU (torch.cuda.is_available() is False)")
         self.L = _lib.load()
@@ -35,6 +35,7 @@ class NS3D:
         cfg.interp = _lib.INTERP[interp]
         cfg.upwind = _lib.UPWIND[upwind]
         cfg.weno_eps = eps
+        cfg.weno_type = _lib.WENO[weno_type]
         self._dx = None
         if dxinv is not None:
             self._dx = np.ascontiguousarray(dxinv, dtype=np.float64)
@@ -47,7 +48,7 @@ class NS3D:
 
     @classmethod
     def from_workload(cls, w):
-        return cls(w.dims, w.interp, w.upwind, w.Re, w.Minf, w.Pr, w.gamma, w.weno_eps)
+        return cls(w.dims, w.interp, w.upwind, w.Re, w.Minf, w.Pr, w.gamma, w.weno_eps, weno_type=w.weno_type)
 
     # ------------------------------------------------------------------
     def close(self):

@@ -52,6 +52,10 @@ typedef enum {
 typedef enum { HM_INTERP_COMPONENTS = 0, HM_INTERP_CHARACTERISTIC = 1 } hm_interp_kind;
 /* upwinding of physics.inp (NavierStokes3DInitialize.c:162-333) */
 typedef enum { HM_UPWIND_ROE = 0, HM_UPWIND_RUSANOV = 1 } hm_upwind_kind;
+/* WENO weight formulation of weno.inp (WENOInitialize.c:62-96; dispatch
+ * WENOFifthOrderCalculateWeights.c:55-57): Jiang-Shu, mapped (Henrick,
+ * Aslam & Powers), WENO-Z (Borges et al.), Yamaleev-Carpenter */
+typedef enum { HM_WENO_JS = 0, HM_WENO_MAPPED = 1, HM_WENO_Z = 2, HM_WENO_YC = 3 } hm_weno_kind;
 /* host array layouts accepted by the transfer calls */
 typedef enum {
   HM_LAYOUT_REFERENCE = 0, /* AoS, variables fastest, 3 ghost layers (HyPar CPU) */
@@ -68,11 +72,12 @@ typedef struct {
   double Pr;           /* physics.inp Pr                                          */
   int interp;          /* hm_interp_kind                                             */
   int upwind;          /* hm_upwind_kind                                             */
-  double weno_eps;     /* weno.inp epsilon (Jiang-Shu weights, p = 2)             */
+  double weno_eps;     /* weno.inp epsilon (p = 2)                                */
   const double *dxinv; /* optional host array, HyPar::dxinv layout: x, y, z
                           concatenated, each with 3 ghost entries per side.
                           NULL: uniform periodic grid x_i = 2*pi*i/n, metrics
                           computed exactly as InitialSolution.c:74-113        */
+  int weno_type;       /* hm_weno_kind (0, the zero-initialised value, is JS)     */
 } hm_config;
 
 typedef struct hm_solver hm_solver;

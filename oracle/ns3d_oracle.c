@@ -284,6 +284,63 @@ void orc_weno_js_weights(double m3, double m2, double m1, double p1, double p2, 
   weno_js(&w[0], &w[1], &w[2], 0.1, 0.6, 0.3, m3, m2, m1, p1, p2, eps);
 }
 
+/* Mapped weights of Henrick, Aslam & Powers (macro _WENOWeights_v_M_,
+ * include/interpolation.h:313-344): the JS weights mapped by
+ * g_k(w) = w (c_k + c_k^2 - 3 c_k w + w^2) / (c_k^2 + w (1 - 2 c_k)) and
+ * normalised again. */
+static inline void weno_mapped(double *w1, double *w2, double *w3, double c1, double c2, double c3,
+                               double m3, double m2, double m1, double p1, double p2, double eps) {
+  double a1, a2, a3, a_sum_inv;
+  weno_js(w1, w2, w3, c1, c2, c3, m3, m2, m1, p1, p2, eps);
+  a1 = *w1 * (c1 + c1 * c1 - 3 * c1 * *w1 + *w1 * *w1) / (c1 * c1 + *w1 * (1.0 - 2.0 * c1));
+  a2 = *w2 * (c2 + c2 * c2 - 3 * c2 * *w2 + *w2 * *w2) / (c2 * c2 + *w2 * (1.0 - 2.0 * c2));
+  a3 = *w3 * (c3 + c3 * c3 - 3 * c3 * *w3 + *w3 * *w3) / (c3 * c3 + *w3 * (1.0 - 2.0 * c3));
+  a_sum_inv = 1.0 / (a1 + a2 + a3);
+  *w1 = a1 * a_sum_inv;
+  *w2 = a2 * a_sum_inv;
+  *w3 = a3 * a_sum_inv;
+}
+
+/* WENO-Z of Borges et al. (tau = |b1 - b3|, macro _WENOWeights_v_Z_,
+ * include/interpolation.h:426-447) and of Yamaleev & Carpenter (tau = the
+ * squared 4th undivided difference, macro _WENOWeights_v_YC_,
+ * include/interpolation.h:479-500): a_k = c_k (1 + (tau/(b_k+eps))^2). */
+static inline void weno_tau(int yc, double *w1, double *w2, double *w3, double c1, double c2, double c3,
+                            double m3, double m2, double m1, double p1, double p2, double eps) {
+  double b1, b2, b3, a1, a2, a3, a_sum_inv, tau;
+  b1 = thirteen_by_twelve * (m3 - 2 * m2 + m1) * (m3 - 2 * m2 + m1) +
+       one_fourth * (m3 - 4 * m2 + 3 * m1) * (m3 - 4 * m2 + 3 * m1);
+  b2 = thirteen_by_twelve * (m2 - 2 * m1 + p1) * (m2 - 2 * m1 + p1) +
+       one_fourth * (m2 - p1) * (m2 - p1);
+  b3 = thirteen_by_twelve * (m1 - 2 * p1 + p2) * (m1 - 2 * p1 + p2) +
+       one_fourth * (3 * m1 - 4 * p1 + p2) * (3 * m1 - 4 * p1 + p2);
+  if (yc)
+    tau = (m3 - 4 * m2 + 6 * m1 - 4 * p1 + p2) * (m3 - 4 * m2 + 6 * m1 - 4 * p1 + p2);
+  else
+    tau = fabs(b3 - b1);
+  a1 = c1 * (1.0 + (tau / (b1 + eps)) * (tau / (b1 + eps)));
+  a2 = c2 * (1.0 + (tau / (b2 + eps)) * (tau / (b2 + eps)));
+  a3 = c3 * (1.0 + (tau / (b3 + eps)) * (tau / (b3 + eps)));
+  a_sum_inv = 1.0 / (a1 + a2 + a3);
+  *w1 = a1 * a_sum_inv;
+  *w2 = a2 * a_sum_inv;
+  *w3 = a3 * a_sum_inv;
+}
+
+/* dispatch on the weight formulation (WENOFifthOrderCalculateWeights.c:55-57, 79-81) */
+static inline void weno_weights(int type, double *w1, double *w2, double *w3, double c1, double c2, double c3,
+                                double m3, double m2, double m1, double p1, double p2, double eps) {
+  if (type == ORC_WENO_YC) weno_tau(1, w1, w2, w3, c1, c2, c3, m3, m2, m1, p1, p2, eps);
+  else if (type == ORC_WENO_Z) weno_tau(0, w1, w2, w3, c1, c2, c3, m3, m2, m1, p1, p2, eps);
+  else if (type == ORC_WENO_MAPPED) weno_mapped(w1, w2, w3, c1, c2, c3, m3, m2, m1, p1, p2, eps);
+  else weno_js(w1, w2, w3, c1, c2, c3, m3, m2, m1, p1, p2, eps);
+}
+
+void orc_weno_weights(int type, double m3, double m2, double m1, double p1, double p2, double eps,
+                      double w[3]) {
+  weno_weights(type, &w[0], &w[1], &w[2], 0.1, 0.6, 0.3, m3, m2, m1, p1, p2, eps);
+}
+
 /* fifth-order candidate interpolants and their convex combination,
  * component-wise form (Interp1PrimFifthOrderWENO.c:146-158) */
 static inline double weno_interp_comp(double w1, double w2, double w3, double fm3, double fm2,
@@ -382,13 +439,13 @@ static void interface_flux(const orc_params *p, const double *u, const double *u
   if (p->interp == ORC_INTERP_COMPONENTS) {
     for (int v = 0; v < NV; v++) {
       double w1LF, w2LF, w3LF, w1RF, w2RF, w3RF, w1LU, w2LU, w3LU, w1RU, w2RU, w3RU;
-      weno_js(&w1LF, &w2LF, &w3LF, c1, c2, c3, fC[qm3L * NV + v], fC[qm2L * NV + v],
+      weno_weights(p->weno_type, &w1LF, &w2LF, &w3LF, c1, c2, c3, fC[qm3L * NV + v], fC[qm2L * NV + v],
               fC[qm1L * NV + v], fC[qp1L * NV + v], fC[qp2L * NV + v], eps);
-      weno_js(&w1RF, &w2RF, &w3RF, c1, c2, c3, fC[qm3R * NV + v], fC[qm2R * NV + v],
+      weno_weights(p->weno_type, &w1RF, &w2RF, &w3RF, c1, c2, c3, fC[qm3R * NV + v], fC[qm2R * NV + v],
               fC[qm1R * NV + v], fC[qp1R * NV + v], fC[qp2R * NV + v], eps);
-      weno_js(&w1LU, &w2LU, &w3LU, c1, c2, c3, u[qm3L * NV + v], u[qm2L * NV + v],
+      weno_weights(p->weno_type, &w1LU, &w2LU, &w3LU, c1, c2, c3, u[qm3L * NV + v], u[qm2L * NV + v],
               u[qm1L * NV + v], u[qp1L * NV + v], u[qp2L * NV + v], eps);
-      weno_js(&w1RU, &w2RU, &w3RU, c1, c2, c3, u[qm3R * NV + v], u[qm2R * NV + v],
+      weno_weights(p->weno_type, &w1RU, &w2RU, &w3RU, c1, c2, c3, u[qm3R * NV + v], u[qm2R * NV + v],
               u[qm1R * NV + v], u[qp1R * NV + v], u[qp2R * NV + v], eps);
       uL[v] = weno_interp_comp(w1LU, w2LU, w3LU, uC[qm3L * NV + v], uC[qm2L * NV + v],
                                uC[qm1L * NV + v], uC[qp1L * NV + v], uC[qp2L * NV + v]);
@@ -423,10 +480,10 @@ static void interface_flux(const orc_params *p, const double *u, const double *u
       matvecmult(m1RU, L, u + NV * qm1R); matvecmult(p1RU, L, u + NV * qp1R);
       matvecmult(p2RU, L, u + NV * qp2R);
       for (int v = 0; v < NV; v++) {
-        weno_js(&wLF[0][v], &wLF[1][v], &wLF[2][v], c1, c2, c3, m3LF[v], m2LF[v], m1LF[v], p1LF[v], p2LF[v], eps);
-        weno_js(&wRF[0][v], &wRF[1][v], &wRF[2][v], c1, c2, c3, m3RF[v], m2RF[v], m1RF[v], p1RF[v], p2RF[v], eps);
-        weno_js(&wLU[0][v], &wLU[1][v], &wLU[2][v], c1, c2, c3, m3LU[v], m2LU[v], m1LU[v], p1LU[v], p2LU[v], eps);
-        weno_js(&wRU[0][v], &wRU[1][v], &wRU[2][v], c1, c2, c3, m3RU[v], m2RU[v], m1RU[v], p1RU[v], p2RU[v], eps);
+        weno_weights(p->weno_type, &wLF[0][v], &wLF[1][v], &wLF[2][v], c1, c2, c3, m3LF[v], m2LF[v], m1LF[v], p1LF[v], p2LF[v], eps);
+        weno_weights(p->weno_type, &wRF[0][v], &wRF[1][v], &wRF[2][v], c1, c2, c3, m3RF[v], m2RF[v], m1RF[v], p1RF[v], p2RF[v], eps);
+        weno_weights(p->weno_type, &wLU[0][v], &wLU[1][v], &wLU[2][v], c1, c2, c3, m3LU[v], m2LU[v], m1LU[v], p1LU[v], p2LU[v], eps);
+        weno_weights(p->weno_type, &wRU[0][v], &wRU[1][v], &wRU[2][v], c1, c2, c3, m3RU[v], m2RU[v], m1RU[v], p1RU[v], p2RU[v], eps);
       }
     }
     /* four characteristic interpolations; the averaged state takes

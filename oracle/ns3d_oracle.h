@@ -17,6 +17,8 @@ extern "C" {
 
 enum { ORC_INTERP_COMPONENTS = 0, ORC_INTERP_CHARACTERISTIC = 1 };
 enum { ORC_UPWIND_ROE = 0, ORC_UPWIND_RUSANOV = 1 };
+/* WENO weight formulations (weno.inp: mapped / borges / yc; default JS) */
+enum { ORC_WENO_JS = 0, ORC_WENO_MAPPED = 1, ORC_WENO_Z = 2, ORC_WENO_YC = 3 };
 
 typedef struct {
   int dim[3];          /* interior grid size                                  */
@@ -29,6 +31,7 @@ typedef struct {
   int upwind;          /* ORC_UPWIND_*                                        */
   double eps;          /* WENO epsilon                                        */
   const double *dxinv; /* (dim0+2g)+(dim1+2g)+(dim2+2g) inverse spacings      */
+  int weno_type;       /* ORC_WENO_*                                          */
 } orc_params;
 
 /* Number of doubles in one ghosted state array. */
@@ -56,6 +59,11 @@ double orc_cfl(const orc_params *p, const double *u, double dt);
  * _WENOWeights_v_JS_), exposed for unit tests. */
 void orc_weno_js_weights(double m3, double m2, double m1, double p1, double p2,
                          double eps, double w[3]);
+
+/* Weights of one 5-point stencil for a weight formulation ORC_WENO_*
+ * (macros _WENOWeights_v_JS_/_M_/_Z_/_YC_), exposed for unit tests. */
+void orc_weno_weights(int type, double m3, double m2, double m1, double p1, double p2, double eps,
+                      double w[3]);
 
 /* Component-wise WENO5-JS interpolant at the interface right of f[2] from
  * the stencil f[0..4] (left-biased). */

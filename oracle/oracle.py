@@ -22,7 +22,9 @@ class OrcParams(ctypes.Structure):
                 ("gamma", ctypes.c_double), ("Re", ctypes.c_double),
                 ("Pr", ctypes.c_double), ("interp", ctypes.c_int),
                 ("upwind", ctypes.c_int), ("eps", ctypes.c_double),
-                ("dxinv", ctypes.POINTER(ctypes.c_double))]
+                ("dxinv", ctypes.POINTER(ctypes.c_double)), ("weno_type", ctypes.c_int)]
+
+WENO_TYPES = {"js": 0, "mapped": 1, "borges": 2, "yc": 3}
 
 
 def lib():
@@ -42,6 +44,7 @@ def lib():
         L.orc_cfl.argtypes = [pp, dp, ctypes.c_double]; L.orc_cfl.restype = ctypes.c_double
         L.orc_periodic_bc.argtypes = [pp, dp]; L.orc_periodic_bc.restype = None
         L.orc_weno_js_weights.argtypes = [ctypes.c_double] * 6 + [dp]
+        L.orc_weno_weights.argtypes = [ctypes.c_int] + [ctypes.c_double] * 6 + [dp]
         L.orc_roe_average.argtypes = [dp, dp, dp, ctypes.c_double]
         L.orc_left_eigenvectors.argtypes = [dp, dp, ctypes.c_double, ctypes.c_int]
         L.orc_right_eigenvectors.argtypes = [dp, dp, ctypes.c_double, ctypes.c_int]
@@ -60,7 +63,7 @@ class Oracle:
     """Oracle bound to one workload (grid, physics, scheme)."""
 
     def __init__(self, dims, interp="components", upwind="rusanov", Re=-1.0, Minf=1.0,
-                 Pr=0.72, gamma=1.4, eps=1e-6, ghosts=3):
+                 Pr=0.72, gamma=1.4, eps=1e-6, ghosts=3, weno_type="js"):
         from hypar_amd import decks
         self.dims = tuple(int(d) for d in dims)
         self.g = ghosts
@@ -75,10 +78,12 @@ class Oracle:
         self.p.upwind = UPWIND[upwind]
         self.p.eps = eps
         self.p.dxinv = _dp(self.dxinv)
+        self.p.weno_type = WENO_TYPES[weno_type]
 
     @classmethod
     def for_workload(cls, w):
-        return cls(w.dims, w.interp, w.upwind, w.Re, w.Minf, w.Pr, w.gamma, w.weno_eps)
+        return cls(w.dims, w.interp, w.upwind, w.Re, w.Minf, w.Pr, w.gamma, w.weno_eps,
+                   weno_type=w.weno_type)
 
     # layout helpers ------------------------------------------------------
     def ghosted(self, u):

@@ -39,4 +39,5 @@ def workload_from_meta(meta):
                           upwind=meta["upwind"], Re=meta["Re"], Minf=meta["Minf"], Pr=meta["Pr"],
                           gamma=meta["gamma"], weno_eps=meta["eps"], n_iter=meta["steps"],
                           seed=meta["seed"], field=meta["field"], dt=meta["dt"],
-                          time_scheme_type=meta.get("time_scheme_type", "44"))
+                          time_scheme_type=meta.get("time_scheme_type", "44"),
+                          weno_type=meta.get("weno_type", "js"))

@@ -41,6 +41,10 @@ REF_BIN = os.path.join(ROOT, "oracle", "_ref", "HyPar")
 
 
 def _workload(kind: str, dims, n_iter: int) -> decks.Workload:
+    # "<scheme>+<weights>" selects a WENO weight formulation other than JS
+    kind, _, weno = kind.partition("+")
+    if weno:
+        return dataclasses.replace(_workload(kind, dims, n_iter), weno_type=weno)
     if kind == "c0_ssprk3":
         return dataclasses.replace(decks.config0(dims[0], n_iter), dims=tuple(dims), n_iter=n_iter,
                                    dt=None, time_scheme_type="ssprk3")
@@ -65,6 +69,13 @@ FIXTURES = {
     "c1_32_s20": ("c1", (32, 32, 32), 20, False),
     "c1_64_s200": ("c1", (64, 64, 64), 200, False),
     "c0_ssprk3_16_s4": ("c0_ssprk3", (16, 16, 16), 4, True),
+    # mapped (Henrick et al.), Z (Borges et al.) and Yamaleev-Carpenter weights
+    "c0_mapped_16_s4": ("c0+mapped", (16, 16, 16), 4, True),
+    "c0_borges_16_s4": ("c0+borges", (16, 16, 16), 4, True),
+    "c0_yc_16_s4": ("c0+yc", (16, 16, 16), 4, True),
+    "c1_mapped_16_s4": ("c1+mapped", (16, 16, 16), 4, True),
+    "c1_borges_16_s4": ("c1+borges", (16, 16, 16), 4, True),
+    "c1_yc_16_s4": ("c1+yc", (16, 16, 16), 4, True),
 }
 
 ENSEMBLE = {
@@ -101,7 +112,7 @@ def make_single(name: str):
     meta = {"name": name, "scheme": kind, "dims": list(dims), "steps": steps, "dt": w.dt,
             "seed": w.seed, "interp": w.interp, "upwind": w.upwind, "Re": w.Re,
             "Minf": w.Minf, "Pr": w.Pr, "gamma": w.gamma, "eps": w.weno_eps,
-            "field": w.field, "time_scheme_type": w.time_scheme_type,
+            "field": w.field, "time_scheme_type": w.time_scheme_type, "weno_type": w.weno_type,
             "reference_wall_s": wall, "reference_threads": 1,
             "u0_summary": summary(u0), "uN_summary": summary(uN)}
     arrays = {"meta": np.array(json.dumps(meta))}

@@ -33,7 +33,8 @@ def _solver(w):
     return NS3D.from_workload(w)
 
 
-FULL = ["c0_16_s1", "c0_16_s10", "c1_16_s1", "c1_16_s10", "c0_ragged_s3", "c1_ragged_s3"]
+WEIGHTS = [f"{c}_{t}_16_s4" for c in ("c0", "c1") for t in ("mapped", "borges", "yc")]
+FULL = ["c0_16_s1", "c0_16_s10", "c1_16_s1", "c1_16_s10", "c0_ragged_s3", "c1_ragged_s3"] + WEIGHTS
 
 
 @pytest.mark.parametrize("name", FULL)

@@ -93,6 +93,42 @@ def test_weno_fifth_order_on_smooth_data():
     assert min(rates) > 4.0, (errs, rates)
 
 
+# ------------------------------------------- mapped / Z / Yamaleev-Carpenter
+WTYPES = {"js": 0, "mapped": 1, "borges": 2, "yc": 3}
+
+
+def _weights(kind, f, eps=1e-6):
+    out = (ctypes.c_double * 3)()
+    lib().orc_weno_weights(WTYPES[kind], *[ctypes.c_double(v) for v in f], ctypes.c_double(eps), out)
+    return np.array(out[:])
+
+
+@pytest.mark.parametrize("kind", ["js", "mapped", "borges", "yc"])
+def test_weights_convex_and_optimal_on_constant_data(kind):
+    c = np.array([0.1, 0.6, 0.3])
+    assert np.allclose(_weights(kind, [2.5] * 5), c, rtol=0, atol=4e-16)
+    rng = np.random.default_rng(11)
+    for _ in range(100):
+        w = _weights(kind, rng.normal(size=5) * rng.choice([1e-2, 1.0, 1e2]))
+        assert abs(w.sum() - 1.0) < 1e-14 and (w >= 0).all()
+
+
+def test_modified_weights_approach_optimal_faster_on_smooth_data():
+    """On smooth data the Jiang-Shu weights deviate from the optimal ones by
+    O(h^2); the mapping of Henrick et al. (g(c) = c, g'(c) = 0), the WENO-Z
+    indicator tau = |b1-b3| = O(h^5) of Borges et al. and the squared
+    4th difference of Yamaleev & Carpenter all cut that deviation by orders
+    of magnitude."""
+    c = np.array([0.1, 0.6, 0.3])
+    h = 0.05
+    x = (np.arange(5) - 2) * h
+    avg = (np.cos(x - h / 2 + 1) - np.cos(x + h / 2 + 1)) / h
+    dev = {k: np.abs(_weights(k, avg) - c).max() for k in WTYPES}
+    assert dev["js"] > 1e-4
+    for k in ("mapped", "borges", "yc"):
+        assert dev[k] < 1e-2 * dev["js"], dev
+
+
 # ------------------------------------------------------------- eigen system
 def _euler_flux(u, d):
     rho = u[0]
