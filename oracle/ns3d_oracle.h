@@ -1,6 +1,6 @@
 /* ns3d_oracle.h -- CPU restatement of HyPar's explicit NS3D WENO5 + RK4 path.
  *
- * TEST INFRASTRUCTURE ONLY: this is the checker that the HIP path is compared
+ * TEST INFRASTRUCTURE ONLY: this is the CPU reference that the HIP path is compared
  * against.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
  * leg may load it.  The product (libhypar_mi355x.so) never links or calls it.
  *

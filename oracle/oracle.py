@@ -1,7 +1,8 @@
 """ctypes loader for the C restatement oracle (libns3d_oracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py as the checker, never by the product path.
+cpu_baseline leg of bench.py as the reference to compare against, never by the
+product path.
 """
 from __future__ import annotations
 
