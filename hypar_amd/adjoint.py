@@ -40,6 +40,31 @@ def trajectory(u0: torch.Tensor, solver: NS3D, dt: float, nsteps: int, every: in
     return RK4Trajectory.apply(u0, solver, dt, nsteps, every)
 
 
+class EnsembleTrajectory(torch.autograd.Function):
+    """Batched u_N = Phi^N(u_0) of all members of an Ensemble (each with its
+    own dt), differentiable in the batched u_0."""
+
+    @staticmethod
+    def forward(ctx, u0: torch.Tensor, ens, nsteps: int, every: int):
+        u = u0.detach().clone().contiguous()
+        ens.adjoint_forward(u, nsteps, every)
+        ctx.ens = ens
+        ctx.tape = ens.tape_id
+        return u
+
+    @staticmethod
+    def backward(ctx, gN: torch.Tensor):
+        if ctx.ens.tape_id != ctx.tape:
+            raise RuntimeError("the ensemble's adjoint tape was overwritten by a later forward sweep")
+        lam = gN.detach().clone().contiguous()
+        ctx.ens.adjoint_reverse(lam)
+        return lam, None, None, None
+
+
+def ensemble_trajectory(u0: torch.Tensor, ens, nsteps: int, every: int = 10) -> torch.Tensor:
+    return EnsembleTrajectory.apply(u0, ens, nsteps, every)
+
+
 def cell_volumes(dims, device="cuda") -> torch.Tensor:
     """dV = 1/(dxinv_x dxinv_y dxinv_z) per cell (the reference's volume
     element, src/HyParFunctions/VolumeIntegral.c:38-39), shaped (nz, ny, nx)."""

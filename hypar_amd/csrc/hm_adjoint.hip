@@ -1,11 +1,11 @@
 // hm_adjoint.hip -- discrete adjoint of the WENO5 + RK4 step (gfx950, FP64).
 //
 // The reference has no adjoint; this differentiates this library's own
-// discretisation of the inviscid NavierStokes3D right-hand side with
-// component-wise WENO5-JS reconstruction and Rusanov upwinding
-// (the operator of src/HyParFunctions/HyperbolicFunction.c:31-222 with
-// NavierStokes3DUpwindRusanov, NavierStokes3DUpwind.c:349-420), by hand, in
-// reverse mode.
+// discretisation of the NavierStokes3D right-hand side by hand, in reverse
+// mode: component-wise WENO5-JS with Rusanov upwinding, characteristic-wise
+// WENO5-JS with Roe or Rusanov upwinding (the operator of
+// src/HyParFunctions/HyperbolicFunction.c:31-222, NavierStokes3DUpwind.c:40-124,
+// 349-420), and the viscous term (NavierStokes3DParabolicFunction.c:50-325).
 //
 // rhs_i = -sum_d dxinv_i (F_{i+1/2} - F_{i-1/2}) along each direction, so the
 // adjoint of interface flux F_{i-1/2} is  g = dxinv_i lam_i - dxinv_{i-1} lam_{i-1}.
@@ -106,10 +106,10 @@ __device__ __forceinline__ void speed_vjp(const double *u, double gamma, double 
   ub[1 + DIR] += ab * (sg * ir);
 }
 
-// gradient of c + |v_d| of the Roe average of (uL, uR)
-template <int DIR>
-__device__ __forceinline__ void roe_speed_vjp(const double *uL, const double *uR, double gamma, double ab,
-                                              double ubL[NV], double ubR[NV]) {
+// adjoint of the Roe-averaged velocity (gv) and total enthalpy (gH) of
+// (uL, uR) onto the two states (navierstokes3d.h:144-169)
+__device__ __forceinline__ void roe_avg_vjp(const double *uL, const double *uR, double gamma, const double gv[3],
+                                            double gH, double ubL[NV], double ubR[NV]) {
   const double gm1 = gamma - 1.0;
   const double *U[2] = {uL, uR};
   double t[2], v[2][3], H[2], rho[2], q2[2];
@@ -125,12 +125,6 @@ __device__ __forceinline__ void roe_speed_vjp(const double *uL, const double *uR
   double va[3];
   for (int j = 0; j < 3; j++) va[j] = (t[0] * v[0][j] + t[1] * v[1][j]) * den;
   const double Ha = (t[0] * H[0] + t[1] * H[1]) * den;
-  const double ca = sqrt(gm1 * (Ha - 0.5 * (va[0] * va[0] + va[1] * va[1] + va[2] * va[2])));
-  const double gH = ab * gm1 / (2.0 * ca);
-  double gv[3];
-  for (int j = 0; j < 3; j++) gv[j] = -ab * gm1 * va[j] / (2.0 * ca);
-  const double sg = va[DIR] > 0 ? 1.0 : (va[DIR] < 0 ? -1.0 : 0.0);
-  gv[DIR] += ab * sg;
   double *UB[2] = {ubL, ubR};
   for (int s = 0; s < 2; s++) {
     const double w = t[s] * den, ir = 1.0 / rho[s];
@@ -146,6 +140,34 @@ __device__ __forceinline__ void roe_speed_vjp(const double *uL, const double *uR
     }
     UB[s][4] += gHs * gamma * ir;
   }
+}
+
+// gradient of c + |v_d| of the Roe average of (uL, uR)
+template <int DIR>
+__device__ __forceinline__ void roe_speed_vjp(const double *uL, const double *uR, double gamma, double ab,
+                                              double ubL[NV], double ubR[NV]) {
+  const double gm1 = gamma - 1.0;
+  double va[3], Ha;
+  {
+    double t[2], v[2][3], H[2];
+    const double *U[2] = {uL, uR};
+    for (int s = 0; s < 2; s++) {
+      const double ir = 1.0 / U[s][0];
+      for (int j = 0; j < 3; j++) v[s][j] = U[s][1 + j] * ir;
+      H[s] = gamma * U[s][4] * ir - gm1 * 0.5 * (v[s][0] * v[s][0] + v[s][1] * v[s][1] + v[s][2] * v[s][2]);
+      t[s] = sqrt(U[s][0]);
+    }
+    const double den = 1.0 / (t[0] + t[1]);
+    for (int j = 0; j < 3; j++) va[j] = (t[0] * v[0][j] + t[1] * v[1][j]) * den;
+    Ha = (t[0] * H[0] + t[1] * H[1]) * den;
+  }
+  const double ca = sqrt(gm1 * (Ha - 0.5 * (va[0] * va[0] + va[1] * va[1] + va[2] * va[2])));
+  const double gH = ab * gm1 / (2.0 * ca);
+  double gv[3];
+  for (int j = 0; j < 3; j++) gv[j] = -ab * gm1 * va[j] / (2.0 * ca);
+  const double sg = va[DIR] > 0 ? 1.0 : (va[DIR] < 0 ? -1.0 : 0.0);
+  gv[DIR] += ab * sg;
+  roe_avg_vjp(uL, uR, gamma, gv, gH, ubL, ubR);
 }
 
 // ub += A(u)^T fb, A = d(Euler flux along DIR)/d(conserved variables)
@@ -176,16 +198,13 @@ __device__ __forceinline__ void flux_jac_T(const double *u, double gamma, const 
 enum { VJP_WRITE = 0, VJP_ADD = 1, VJP_ADD_L = 2, VJP_ADD_LSET = 3 };
 
 template <int DIR>
-__global__ void __launch_bounds__(VJP_LINE_MAX) k_rhs_vjp(const double *__restrict__ u, const double *__restrict__ lam,
-                                                      const double *__restrict__ nu_in, double ca, double cb,
-                                                      double *__restrict__ nu, double *__restrict__ L, int mode,
-                                                      int nx, int ny, int nz, const double *__restrict__ dxg,
-                                                      double gamma, double eps) {
-  extern __shared__ double smem[];
+__device__ __forceinline__ void rhs_vjp_line(const double *__restrict__ u, const double *__restrict__ lam,
+                                             const double *__restrict__ nu_in, double ca, double cb,
+                                             double *__restrict__ nu, double *__restrict__ L, int mode, int nx,
+                                             int ny, int nz, const double *__restrict__ dxg, double gamma,
+                                             double eps, size_t line, double *smem) {
   const size_t ncell = (size_t)nx * ny * nz;
   const int n = DIR == 0 ? nx : (DIR == 1 ? ny : nz);
-  // neighbouring lines share cache lines along x: keep them on one XCD
-  const size_t line = xcd_block();
   size_t base, stride;
   if (DIR == 0) { base = line * nx; stride = 1; }
   else if (DIR == 1) { base = (line % nx) + (line / nx) * (size_t)nx * ny; stride = nx; }
@@ -339,6 +358,724 @@ __global__ void __launch_bounds__(VJP_LINE_MAX) k_rhs_vjp(const double *__restri
   }
 }
 
+// ---------------------------------------------------------------------------
+// characteristic-wise WENO5-JS with Roe or Rusanov upwinding
+// ---------------------------------------------------------------------------
+// The forward flux at an interface is F = R g with, for every characteristic
+// field k (rows of L at the Roe average of the two cells next to it),
+//   g_k = (fL_k + fR_k)/2 - l_k (uR_k - uL_k)/2,
+// fL/fR (uL/uR) the WENO5 values of L f (L u) over the stencil and l_k = |lambda_k|
+// with Harten's fix (Roe) or the Rusanov speed; this is the reference's
+// R (WENO of L f) - R |Lambda| L R (WENO of L u) with L R = I
+// (Interp1PrimFifthOrderWENOChar.c:86-198, NavierStokes3DUpwind.c:40-124).
+// The reverse pass runs field by field: WENO adjoints of the projected
+// stencils, then back through L onto the cells' fluxes and states, while the
+// derivatives of L, R and l_k with respect to the Roe state (v, a) are summed
+// and finally taken back through the Roe average onto cells I-1 and I.  The
+// interpolated values use the modified solution, which equals u exactly in
+// exact arithmetic, so both adjoints land on u.
+constexpr int VJP_CHAR_LINE_MAX = 256;
+
+struct RoeAdj {
+  double v[3], vn, a, a2, ek, h0;
+};
+
+template <int DIR>
+__device__ __forceinline__ RoeAdj roe_adj_state(const double uL[NV], const double uR[NV], double gamma) {
+  const double gm1 = gamma - 1.0;
+  RoeAdj s;
+  const double iL = 1.0 / uL[0], iR = 1.0 / uR[0];
+  double vL[3], vR[3];
+  for (int j = 0; j < 3; j++) { vL[j] = uL[1 + j] * iL; vR[j] = uR[1 + j] * iR; }
+  const double qL = vL[0] * vL[0] + vL[1] * vL[1] + vL[2] * vL[2];
+  const double qR = vR[0] * vR[0] + vR[1] * vR[1] + vR[2] * vR[2];
+  const double HL = gamma * uL[4] * iL - gm1 * 0.5 * qL, HR = gamma * uR[4] * iR - gm1 * 0.5 * qR;
+  const double tL = sqrt(uL[0]), tR = sqrt(uR[0]), den = 1.0 / (tL + tR);
+  for (int j = 0; j < 3; j++) s.v[j] = (tL * vL[j] + tR * vR[j]) * den;
+  const double H = (tL * HL + tR * HR) * den;
+  s.ek = 0.5 * (s.v[0] * s.v[0] + s.v[1] * s.v[1] + s.v[2] * s.v[2]);
+  s.a2 = (H - s.ek) * gm1;
+  s.a = sqrt(s.a2);
+  s.vn = s.v[DIR];
+  s.h0 = s.a2 / gm1 + s.ek;
+  return s;
+}
+
+// row k of L at the Roe state: k = 0 entropy, 1 u-a, 2 u+a, 3 and 4 the two
+// shear rows in row order (navierstokes3d.h:288-374)
+template <int DIR>
+__device__ __forceinline__ void l_row(const RoeAdj &s, double gm1, int k, double r[NV]) {
+  if (k <= 2) {
+    const double ia2 = 1.0 / s.a2;
+    if (k == 0) {
+      r[0] = (s.a2 - gm1 * s.ek) * ia2;
+      for (int j = 0; j < 3; j++) r[1 + j] = gm1 * s.v[j] * ia2;
+      r[4] = -gm1 * ia2;
+    } else {
+      const double h = 0.5 * ia2, sg = k == 1 ? 1.0 : -1.0;
+      r[0] = (gm1 * s.ek + sg * s.a * s.vn) * h;
+      for (int j = 0; j < 3; j++) r[1 + j] = -gm1 * s.v[j] * h;
+      r[1 + DIR] -= sg * s.a * h;
+      r[4] = gm1 * h;
+    }
+    return;
+  }
+  for (int j = 0; j < NV; j++) r[j] = 0.0;
+  if (DIR == 0) {
+    if (k == 3) { r[0] = s.v[1]; r[2] = -1.0; } else { r[0] = -s.v[2]; r[3] = 1.0; }
+  } else if (DIR == 1) {
+    if (k == 3) { r[0] = -s.v[0]; r[1] = 1.0; } else { r[0] = s.v[2]; r[3] = -1.0; }
+  } else {
+    if (k == 3) { r[0] = s.v[0]; r[1] = -1.0; } else { r[0] = -s.v[1]; r[2] = 1.0; }
+  }
+}
+
+// adds (d row_k / d(v, a))^T X to (gv, ab): the Roe-state derivative of
+// sum_p (adjoint of the projection at p) * (row_k . x_p), X = the adjoint-
+// weighted sum of the projected vectors
+template <int DIR>
+__device__ __forceinline__ void l_row_vjp(const RoeAdj &s, double gm1, int k, const double X[NV], double gv[3],
+                                          double &ab) {
+  const double a = s.a, ia2 = 1.0 / s.a2, ia3 = ia2 / a;
+  const double vX = s.v[0] * X[1] + s.v[1] * X[2] + s.v[2] * X[3];
+  if (k == 0) {
+    for (int j = 0; j < 3; j++) gv[j] += gm1 * ia2 * (X[1 + j] - X[0] * s.v[j]);
+    ab += 2.0 * gm1 * ia3 * (X[0] * s.ek - vX + X[4]);
+  } else if (k <= 2) {
+    const double sg = k == 1 ? 1.0 : -1.0, h = 0.5 * ia2;
+    for (int j = 0; j < 3; j++) gv[j] += gm1 * h * (X[0] * s.v[j] - X[1 + j]);
+    gv[DIR] += sg * a * h * X[0];
+    ab += -X[0] * (2.0 * gm1 * s.ek + sg * a * s.vn) * h / a + gm1 * ia3 * (vX - X[4]) + sg * h * X[1 + DIR];
+  } else if (DIR == 0) {
+    if (k == 3) gv[1] += X[0]; else gv[2] -= X[0];
+  } else if (DIR == 1) {
+    if (k == 3) gv[0] -= X[0]; else gv[2] += X[0];
+  } else {
+    if (k == 3) gv[0] += X[0]; else gv[1] -= X[0];
+  }
+}
+
+__device__ __forceinline__ double harten_d(double l) {
+  const double delta = 0.000001;
+  return fabs(l) < delta ? l / delta : (l > 0 ? 1.0 : (l < 0 ? -1.0 : 0.0));
+}
+
+// nu (+)= (d hyp_DIR-part of rhs / d u)^T mu along one direction for the
+// characteristic scheme; one workgroup per line, one thread per interface,
+// modes as k_rhs_vjp.  LDS (dynamic, 70 doubles per cell of the line): state
+// [5][n], cell flux [5][n], and the flux and state adjoint contributions of
+// every interface to its six stencil cells, [6][5][n] each; every cell then
+// sums its six slots in a fixed order.
+template <int DIR, int UPW>
+__device__ __forceinline__ void rhs_vjp_char_line(
+    const double *__restrict__ u, const double *__restrict__ lam, const double *__restrict__ nu_in, double ca,
+    double cb, double *__restrict__ nu, double *__restrict__ L, int mode, int nx, int ny, int nz,
+    const double *__restrict__ dxg, double gamma, double eps, size_t line, double *smem) {
+  const size_t ncell = (size_t)nx * ny * nz;
+  const int n = DIR == 0 ? nx : (DIR == 1 ? ny : nz);
+  size_t base, stride;
+  if (DIR == 0) { base = line * nx; stride = 1; }
+  else if (DIR == 1) { base = (line % nx) + (line / nx) * (size_t)nx * ny; stride = nx; }
+  else { base = line; stride = (size_t)nx * ny; }
+  const double *dxd = dxg + 3 + (DIR >= 1 ? nx + 6 : 0) + (DIR == 2 ? ny + 6 : 0);
+  double *sU = smem;           // [5][n]
+  double *sF = sU + 5 * n;     // [5][n]
+  double *cF = sF + 5 * n;     // [6][5][n]
+  double *cU = cF + 30 * n;    // [6][5][n]
+  const int t = threadIdx.x;
+  const bool active = t < n;
+  const double gm1 = gamma - 1.0;
+  double lF[NV] = {0, 0, 0, 0, 0};  // adjoint of the interface flux
+  if (active) {
+    for (int k = 0; k < 30; k++) cF[k * n + t] = cU[k * n + t] = 0.0;
+    const int tm = wrapn(t - 1, n);
+    const size_t c = base + (size_t)t * stride, cm = base + (size_t)tm * stride;
+    double us[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      us[v] = u[(size_t)v * ncell + c];
+      sU[v * n + t] = us[v];
+      const size_t o = (size_t)v * ncell;
+      const double mu = cb * lam[o + c] + (nu_in ? ca * nu_in[o + c] : 0.0);
+      const double mum = cb * lam[o + cm] + (nu_in ? ca * nu_in[o + cm] : 0.0);
+      lF[v] = dxd[t] * mu - dxd[tm] * mum;
+    }
+    const Prim q = flowvar(us, gamma);
+    double fl[NV];
+    euler_flux<DIR>(fl, q);
+#pragma unroll
+    for (int v = 0; v < NV; v++) sF[v * n + t] = fl[v];
+  }
+  __syncthreads();
+  if (active) {
+    int cs[6];
+#pragma unroll
+    for (int p = 0; p < 6; p++) cs[p] = wrapn(t + p - 3, n);
+    double u2[NV], u3[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) { u2[v] = sU[v * n + cs[2]]; u3[v] = sU[v * n + cs[3]]; }
+    const RoeAdj s = roe_adj_state<DIR>(u2, u3, gamma);
+    // Rusanov speed: max(max(aL, aR), max(aR, aA)), the branch the forward selects
+    double alpha = 0.0;
+    int sel = 0;
+    if (UPW == UPWIND_RUSANOV) {
+      const Prim q2 = flowvar(u2, gamma), q3 = flowvar(u3, gamma);
+      const double aL = sqrt(gamma * q2.P / q2.rho) + fabs(vel<DIR>(q2));
+      const double aR = sqrt(gamma * q3.P / q3.rho) + fabs(vel<DIR>(q3));
+      const double aA = s.a + fabs(s.vn);
+      const double m01 = fmax(aL, aR), m12 = fmax(aR, aA);
+      alpha = fmax(m01, m12);
+      sel = m01 >= m12 ? (aL >= aR ? 0 : 1) : (aR >= aA ? 1 : 2);
+    }
+    // lambda_g = R^T lambda_F (columns of R, navierstokes3d.h:384-470)
+    const double eb = lF[4];
+    const double sumb = lF[0] + s.v[0] * lF[1] + s.v[1] * lF[2] + s.v[2] * lF[3];
+    const double difb = lF[1 + DIR] + s.vn * eb;
+    double gbk[5];
+    gbk[0] = sumb + s.ek * eb;
+    gbk[1] = sumb + s.h0 * eb - s.a * difb;
+    gbk[2] = sumb + s.h0 * eb + s.a * difb;
+    if (DIR == 0) { gbk[3] = -lF[2] - s.v[1] * eb; gbk[4] = lF[3] + s.v[2] * eb; }
+    else if (DIR == 1) { gbk[3] = lF[1] + s.v[0] * eb; gbk[4] = -lF[3] - s.v[2] * eb; }
+    else { gbk[3] = -lF[1] - s.v[0] * eb; gbk[4] = lF[2] + s.v[1] * eb; }
+    const double ls = UPW == UPWIND_ROE ? harten_abs(s.vn) : alpha;
+    const double lm = UPW == UPWIND_ROE ? harten_abs(s.vn - s.a) : alpha;
+    const double lp = UPW == UPWIND_ROE ? harten_abs(s.vn + s.a) : alpha;
+    double gv[3] = {0, 0, 0}, ab = 0.0;
+    double g[5], lb[5];
+#pragma unroll 1
+    for (int k = 0; k < 5; k++) {
+      double row[NV];
+      l_row<DIR>(s, gm1, k, row);
+      // projected stencils
+      double pF[6], pU[6];
+#pragma unroll
+      for (int p = 0; p < 6; p++) {
+        double f = 0.0, x = 0.0;
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          f += row[v] * sF[v * n + cs[p]];
+          x += row[v] * sU[v * n + cs[p]];
+        }
+        pF[p] = f;
+        pU[p] = x;
+      }
+      const double gbv = k == 0 ? gbk[0] : (k == 1 ? gbk[1] : (k == 2 ? gbk[2] : (k == 3 ? gbk[3] : gbk[4])));
+      const double lk = k == 1 ? lm : (k == 2 ? lp : ls);
+      double bF[6] = {0, 0, 0, 0, 0, 0}, bU[6] = {0, 0, 0, 0, 0, 0};
+      double m[5], mb[5];
+#pragma unroll
+      for (int j = 0; j < 5; j++) { m[j] = pF[j]; mb[j] = 0.0; }
+      const double fl = weno5_vjp(m, m, eps, 0.5 * gbv, mb, mb);
+#pragma unroll
+      for (int j = 0; j < 5; j++) { bF[j] += mb[j]; m[j] = pF[5 - j]; mb[j] = 0.0; }
+      const double fr = weno5_vjp(m, m, eps, 0.5 * gbv, mb, mb);
+#pragma unroll
+      for (int j = 0; j < 5; j++) { bF[5 - j] += mb[j]; m[j] = pU[j]; mb[j] = 0.0; }
+      const double ul = weno5_vjp(m, m, eps, 0.5 * lk * gbv, mb, mb);
+#pragma unroll
+      for (int j = 0; j < 5; j++) { bU[j] += mb[j]; m[j] = pU[5 - j]; mb[j] = 0.0; }
+      const double ur = weno5_vjp(m, m, eps, -0.5 * lk * gbv, mb, mb);
+#pragma unroll
+      for (int j = 0; j < 5; j++) bU[5 - j] += mb[j];
+      const double gk = 0.5 * (fl + fr) - lk * (0.5 * (ur - ul));
+      const double lbk = -0.5 * (ur - ul) * gbv;
+      if (k == 0) { g[0] = gk; lb[0] = lbk; }
+      else if (k == 1) { g[1] = gk; lb[1] = lbk; }
+      else if (k == 2) { g[2] = gk; lb[2] = lbk; }
+      else if (k == 3) { g[3] = gk; lb[3] = lbk; }
+      else { g[4] = gk; lb[4] = lbk; }
+      // back through the row onto the stencil cells; X for the row's derivatives
+      double X[NV] = {0, 0, 0, 0, 0};
+#pragma unroll
+      for (int p = 0; p < 6; p++) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          X[v] += bF[p] * sF[v * n + cs[p]] + bU[p] * sU[v * n + cs[p]];
+          cF[(p * NV + v) * n + t] += bF[p] * row[v];
+          cU[(p * NV + v) * n + t] += bU[p] * row[v];
+        }
+      }
+      l_row_vjp<DIR>(s, gm1, k, X, gv, ab);
+    }
+    // R (columns of navierstokes3d.h:384-470) at the Roe state
+    const double sum = g[0] + g[1] + g[2];
+    for (int j = 0; j < 3; j++) gv[j] += lF[1 + j] * sum;
+    const double h0b = eb * (g[1] + g[2]);
+    const double ekb = eb * g[0] + h0b;
+    gv[DIR] += eb * s.a * (g[2] - g[1]);
+    ab += difb * (g[2] - g[1]) + h0b * 2.0 * s.a / gm1;
+    if (DIR == 0) { gv[1] -= eb * g[3]; gv[2] += eb * g[4]; }
+    else if (DIR == 1) { gv[0] += eb * g[3]; gv[2] -= eb * g[4]; }
+    else { gv[1] += eb * g[4]; gv[0] -= eb * g[3]; }
+    for (int j = 0; j < 3; j++) gv[j] += ekb * s.v[j];
+    // eigenvalues
+    double ub2[NV] = {0, 0, 0, 0, 0}, ub3[NV] = {0, 0, 0, 0, 0};
+    if (UPW == UPWIND_ROE) {
+      const double hm = harten_d(s.vn - s.a), hp = harten_d(s.vn + s.a);
+      gv[DIR] += harten_d(s.vn) * (lb[0] + lb[3] + lb[4]) + hm * lb[1] + hp * lb[2];
+      ab += hp * lb[2] - hm * lb[1];
+    } else {
+      const double alb = lb[0] + lb[1] + lb[2] + lb[3] + lb[4];
+      if (sel == 0) speed_vjp<DIR>(u2, gamma, alb, ub2);
+      else if (sel == 1) speed_vjp<DIR>(u3, gamma, alb, ub3);
+      else roe_speed_vjp<DIR>(u2, u3, gamma, alb, ub2, ub3);
+    }
+    // a = sqrt((H - |v|^2/2)(gamma-1)) -> (v, H) of the Roe average -> the two cells
+    const double ka = ab * gm1 / (2.0 * s.a);
+    for (int j = 0; j < 3; j++) gv[j] -= ka * s.v[j];
+    roe_avg_vjp(u2, u3, gamma, gv, ka, ub2, ub3);
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      cU[(2 * NV + v) * n + t] += ub2[v];
+      cU[(3 * NV + v) * n + t] += ub3[v];
+    }
+  }
+  __syncthreads();
+  // slot p of interface i belongs to cell i-3+p: cell t sums slot p of
+  // interface t+3-p, p = 0..5, in that order
+  double accF[NV] = {0, 0, 0, 0, 0}, accU[NV] = {0, 0, 0, 0, 0};
+  if (active) {
+#pragma unroll
+    for (int p = 0; p < 6; p++) {
+      const int i = wrapn(t + 3 - p, n);
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        accF[v] += cF[(p * NV + v) * n + i];
+        accU[v] += cU[(p * NV + v) * n + i];
+      }
+    }
+  }
+  if (active) {
+    double uc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) uc[v] = sU[v * n + t];
+    flux_jac_T<DIR>(uc, gamma, accF, accU);
+    const size_t c = base + (size_t)t * stride;
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      const size_t o = (size_t)v * ncell + c;
+      if (mode == VJP_WRITE) {
+        nu[o] = accU[v];
+      } else {
+        const double r = nu[o] + accU[v];
+        nu[o] = r;
+        if (mode == VJP_ADD_L) L[o] += r;
+        else if (mode == VJP_ADD_LSET) L[o] = lam[o] + r;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// viscous term
+// ---------------------------------------------------------------------------
+// par_i = sum_d dxinv_d(i) D_d[FV_d](i), D the 4th-order central difference
+// over FV_d at i-2..i+2 along d, where FV_d is the viscous flux of the
+// primitive variables Q = (u, v, w, T) and their gradients (4th-order central,
+// periodic) inside the grid, and the ghost-layer flux (biased stencil in the
+// second layer, zero derivatives across d) at the two positions on each side
+// of a line (NavierStokes3DParabolicFunction.c:50-325,
+// FirstDerivativeFourthOrder.c:36-129).  The reverse pass:
+//  k_visc_vjp_cell   flux adjoints of every cell from its interior readers,
+//                    back through the flux onto Q (stored in QB) and onto the
+//                    gradients (stored in GB, already times dxinv);
+//  k_visc_vjp_ghost  the same for the ghost-layer fluxes of each line, whose
+//                    gradient adjoints are spread over their stencils into QB;
+//  k_visc_vjp_gather QB plus the transposed gradient stencils over GB, then
+//                    back through the primitive variables onto u.
+struct ViscC {
+  double gamma, inv_Re, inv_gm1, inv_Pr;
+};
+
+// weight of F(r+s) in D at reader r, s = -2, -1, +1, +2
+__device__ __forceinline__ double d4w(int s) {
+  return s == -2 ? 1.0 / 12.0 : (s == -1 ? -8.0 / 12.0 : (s == 1 ? 8.0 / 12.0 : -1.0 / 12.0));
+}
+
+// adjoint of the viscous flux along DIR at a point: fb = adjoint of
+// (tau_d1, tau_d2, tau_d3, energy flux); adds to qb (u, v, w, T values) and
+// grb (gradient entries, grb[c][e] = d/d(dq_c/dx_e)); returns the adjoint of mu
+template <int DIR>
+__device__ __forceinline__ double visc_flux_vjp(const double q[4], const double (&gr)[4][3], double mu,
+                                                const ViscC &vc, const double fb[4], double qb[4],
+                                                double (&grb)[4][3]) {
+  const double tt = 2.0 / 3.0;
+  const double mr = mu * vc.inv_Re, kq = mr * vc.inv_gm1 * vc.inv_Pr;
+  const double ux = gr[0][0], uy = gr[0][1], uz = gr[0][2];
+  const double vx = gr[1][0], vy = gr[1][1], vz = gr[1][2];
+  const double wx = gr[2][0], wy = gr[2][1], wz = gr[2][2];
+  double b1, b2, b3;
+  if (DIR == 0) { b1 = tt * (2 * ux - vy - wz); b2 = uy + vx; b3 = uz + wx; }
+  else if (DIR == 1) { b1 = uy + vx; b2 = tt * (-ux + 2 * vy - wz); b3 = vz + wy; }
+  else { b1 = uz + wx; b2 = vz + wy; b3 = tt * (-ux - vy + 2 * wz); }
+  const double gT = gr[3][DIR];
+  const double tb1 = fb[0] + q[0] * fb[3], tb2 = fb[1] + q[1] * fb[3], tb3 = fb[2] + q[2] * fb[3];
+  qb[0] += fb[3] * (mr * b1);
+  qb[1] += fb[3] * (mr * b2);
+  qb[2] += fb[3] * (mr * b3);
+  const double mub = (tb1 * b1 + tb2 * b2 + tb3 * b3) * vc.inv_Re + fb[3] * gT * (vc.inv_Re * vc.inv_gm1 * vc.inv_Pr);
+  const double c1 = mr * tb1, c2 = mr * tb2, c3 = mr * tb3;
+  if (DIR == 0) {
+    grb[0][0] += 2 * tt * c1; grb[1][1] -= tt * c1; grb[2][2] -= tt * c1;
+    grb[0][1] += c2; grb[1][0] += c2;
+    grb[0][2] += c3; grb[2][0] += c3;
+  } else if (DIR == 1) {
+    grb[0][1] += c1; grb[1][0] += c1;
+    grb[0][0] -= tt * c2; grb[1][1] += 2 * tt * c2; grb[2][2] -= tt * c2;
+    grb[1][2] += c3; grb[2][1] += c3;
+  } else {
+    grb[0][2] += c1; grb[2][0] += c1;
+    grb[1][2] += c2; grb[2][1] += c2;
+    grb[0][0] -= tt * c3; grb[1][1] -= tt * c3; grb[2][2] += 2 * tt * c3;
+  }
+  grb[3][DIR] += kq * fb[3];
+  return mub;
+}
+
+// mu_c at a cell: momentum and energy components of cb*lam + ca*nu_in
+__device__ __forceinline__ void visc_mu(const double *__restrict__ lam, const double *__restrict__ nu_in, double ca,
+                                        double cb, size_t ncell, size_t c, double m[4]) {
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const size_t o = (size_t)(1 + k) * ncell + c;
+    m[k] = cb * lam[o] + (nu_in ? ca * nu_in[o] : 0.0);
+  }
+}
+
+__device__ __forceinline__ void visc_vjp_cell(const double *__restrict__ Q, const double *__restrict__ lam,
+                                              const double *__restrict__ nu_in, double ca, double cb,
+                                              double *__restrict__ QB, double *__restrict__ GB, int nx, int ny,
+                                              int nz, const double *__restrict__ dxg, const ViscC &vc, size_t idx) {
+  const size_t ncell = (size_t)nx * ny * nz;
+  if (idx >= ncell) return;
+  const int n[3] = {nx, ny, nz};
+  const int pos[3] = {(int)(idx % nx), (int)((idx / nx) % ny), (int)(idx / ((size_t)nx * ny))};
+  const size_t st[3] = {1, (size_t)nx, (size_t)nx * ny};
+  const double *dx[3] = {dxg + 3, dxg + (nx + 6) + 3, dxg + (nx + 6) + (ny + 6) + 3};
+  double q[4], gr[4][3];
+#pragma unroll
+  for (int c = 0; c < 4; c++) q[c] = Q[(size_t)c * ncell + idx];
+  double fb[3][4];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const size_t line0 = idx - (size_t)pos[d] * st[d];
+    double f[4][4];  // Q at offsets -2, -1, +1, +2
+    const int offs[4] = {-2, -1, 1, 2};
+#pragma unroll
+    for (int k = 0; k < 4; k++) fb[d][k] = 0.0;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      const size_t cw = line0 + (size_t)wrapn(pos[d] + offs[s], n[d]) * st[d];
+#pragma unroll
+      for (int c = 0; c < 4; c++) f[s][c] = Q[(size_t)c * ncell + cw];
+      // reader r = pos - s reads this cell with weight d4w(s), inside the grid only
+      const int r = pos[d] - offs[s];
+      if (r >= 0 && r < n[d]) {
+        double m[4];
+        visc_mu(lam, nu_in, ca, cb, ncell, line0 + (size_t)r * st[d], m);
+        const double w = d4w(offs[s]) * dx[d][r];
+#pragma unroll
+        for (int k = 0; k < 4; k++) fb[d][k] += w * m[k];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) gr[c][d] = d4c(f[0][c], f[1][c], f[2][c], f[3][c]) * dx[d][pos[d]];
+  }
+  const double mu = exp(0.76 * log(q[3]));
+  double qb[4] = {0, 0, 0, 0}, grb[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  double mub = visc_flux_vjp<0>(q, gr, mu, vc, fb[0], qb, grb);
+  mub += visc_flux_vjp<1>(q, gr, mu, vc, fb[1], qb, grb);
+  mub += visc_flux_vjp<2>(q, gr, mu, vc, fb[2], qb, grb);
+  qb[3] += mub * (0.76 * mu / q[3]);
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    QB[(size_t)c * ncell + idx] = qb[c];
+#pragma unroll
+    for (int e = 0; e < 3; e++) GB[(size_t)(c * 3 + e) * ncell + idx] = grb[c][e] * dx[e][pos[e]];
+  }
+}
+
+// one thread per line along DIR: the four ghost-layer fluxes -2, -1, n, n+1
+template <int DIR>
+__device__ __forceinline__ void visc_vjp_ghost_line(const double *__restrict__ Q, const double *__restrict__ lam,
+                                                    const double *__restrict__ nu_in, double ca, double cb,
+                                                    double *__restrict__ QB, int nx, int ny, int nz,
+                                                    const double *__restrict__ dxg, const ViscC &vc, size_t line) {
+  const size_t ncell = (size_t)nx * ny * nz;
+  const int n = DIR == 0 ? nx : (DIR == 1 ? ny : nz);
+  const size_t nplane = ncell / n;
+  if (line >= nplane) return;
+  size_t base, stride;
+  if (DIR == 0) { base = line * nx; stride = 1; }
+  else if (DIR == 1) { base = (line % nx) + (line / nx) * (size_t)nx * ny; stride = nx; }
+  else { base = line; stride = (size_t)nx * ny; }
+  const double *dxd = dxg + (DIR >= 1 ? nx + 6 : 0) + (DIR == 2 ? ny + 6 : 0);  // entry 0: position -3
+  // readers 0, 1, n-2, n-1, times their dxinv
+  double a[4][4];
+  const int rd[4] = {0, 1, n - 2, n - 1};
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    visc_mu(lam, nu_in, ca, cb, ncell, base + (size_t)rd[r] * stride, a[r]);
+#pragma unroll
+    for (int k = 0; k < 4; k++) a[r][k] *= dxd[3 + rd[r]];
+  }
+  for (int slot = 0; slot < 4; slot++) {
+    const int gp = slot < 2 ? slot - 2 : n + slot - 2;
+    double fb[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      if (slot == 0) fb[k] = d4w(-2) * a[0][k];
+      else if (slot == 1) fb[k] = d4w(-1) * a[0][k] + d4w(-2) * a[1][k];
+      else if (slot == 2) fb[k] = d4w(1) * a[3][k] + d4w(2) * a[2][k];
+      else fb[k] = d4w(2) * a[3][k];
+    }
+    // the forward evaluation at this ghost position (k_visc_ghost)
+    double f[7][4];
+    size_t cw[7];
+#pragma unroll
+    for (int s = 0; s < 7; s++) {
+      cw[s] = base + (size_t)wrapn(gp - 3 + s, n) * stride;  // n >= 5: one wrap suffices
+#pragma unroll
+      for (int c = 0; c < 4; c++) f[s][c] = Q[(size_t)c * ncell + cw[s]];
+    }
+    // stencil weights over positions gp-3 .. gp+3
+    double w7[7] = {0, 0, 0, 0, 0, 0, 0};
+    const double r12 = 1.0 / 12.0;
+    if (gp == -2) { w7[2] = -3 * r12; w7[3] = -10 * r12; w7[4] = 18 * r12; w7[5] = -6 * r12; w7[6] = r12; }
+    else if (gp == n + 1) { w7[0] = -r12; w7[1] = 6 * r12; w7[2] = -18 * r12; w7[3] = 10 * r12; w7[4] = 3 * r12; }
+    else { w7[1] = r12; w7[2] = -8 * r12; w7[4] = 8 * r12; w7[5] = -r12; }
+    const double dxgh = dxd[3 + gp];
+    double gr[4][3];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      double d = 0.0;
+      if (gp == -2) d = (-3 * f[2][c] - 10 * f[3][c] + 18 * f[4][c] - 6 * f[5][c] + f[6][c]) * r12;
+      else if (gp == n + 1) d = (-f[0][c] + 6 * f[1][c] - 18 * f[2][c] + 10 * f[3][c] + 3 * f[4][c]) * r12;
+      else d = d4c(f[1][c], f[2][c], f[4][c], f[5][c]);
+#pragma unroll
+      for (int e = 0; e < 3; e++) gr[c][e] = e == DIR ? d * dxgh : 0.0;
+    }
+    const double q[4] = {f[3][0], f[3][1], f[3][2], f[3][3]};
+    const double mu = exp(0.76 * log(q[3]));
+    double qb[4] = {0, 0, 0, 0}, grb[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    const double mub = visc_flux_vjp<DIR>(q, gr, mu, vc, fb, qb, grb);
+    qb[3] += mub * (0.76 * mu / q[3]);
+#pragma unroll
+    for (int c = 0; c < 4; c++) QB[(size_t)c * ncell + cw[3]] += qb[c];
+#pragma unroll
+    for (int s = 0; s < 7; s++) {
+      if (w7[s] == 0.0) continue;
+#pragma unroll
+      for (int c = 0; c < 4; c++) QB[(size_t)c * ncell + cw[s]] += w7[s] * dxgh * grb[c][DIR];
+    }
+  }
+}
+
+// nu = (d par/d u)^T mu: QB plus the transposed gradient stencils over GB,
+// then back through (u, v, w, T)(u) (NavierStokes3DParabolicFunction.c:85-108)
+__device__ __forceinline__ void visc_vjp_gather(const double *__restrict__ U, const double *__restrict__ QB,
+                                                const double *__restrict__ GB, double *__restrict__ nu, int nx,
+                                                int ny, int nz, double gamma, size_t idx) {
+  const size_t ncell = (size_t)nx * ny * nz;
+  if (idx >= ncell) return;
+  const int n[3] = {nx, ny, nz};
+  const int pos[3] = {(int)(idx % nx), (int)((idx / nx) % ny), (int)(idx / ((size_t)nx * ny))};
+  const size_t st[3] = {1, (size_t)nx, (size_t)nx * ny};
+  double qb[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) qb[c] = QB[(size_t)c * ncell + idx];
+  const int offs[4] = {-2, -1, 1, 2};
+#pragma unroll
+  for (int e = 0; e < 3; e++) {
+    const size_t line0 = idx - (size_t)pos[e] * st[e];
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      // reader j = m - s reads this cell with weight d4w(s) (periodic)
+      const size_t j = line0 + (size_t)wrapn(pos[e] - offs[s], n[e]) * st[e];
+      const double w = d4w(offs[s]);
+#pragma unroll
+      for (int c = 0; c < 4; c++) qb[c] += w * GB[(size_t)(c * 3 + e) * ncell + j];
+    }
+  }
+  double u[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) u[v] = U[(size_t)v * ncell + idx];
+  const double gm1 = gamma - 1.0;
+  const double ir = 1.0 / u[0];
+  const double vx = u[1] * ir, vy = u[2] * ir, vz = u[3] * ir;
+  const double q2 = vx * vx + vy * vy + vz * vz;
+  const double k = gamma * gm1 * qb[3];  // T = gamma (gamma-1) (e/rho - |m|^2/(2 rho^2))
+  double ub[NV];
+  ub[0] = -(qb[0] * vx + qb[1] * vy + qb[2] * vz) * ir + k * (-u[4] * ir * ir + q2 * ir);
+  ub[1] = qb[0] * ir - k * vx * ir;
+  ub[2] = qb[1] * ir - k * vy * ir;
+  ub[3] = qb[2] * ir - k * vz * ir;
+  ub[4] = k * ir;
+#pragma unroll
+  for (int v = 0; v < NV; v++) nu[(size_t)v * ncell + idx] = ub[v];
+}
+
+// ---------------------------------------------------------------------------
+// kernels: one grid
+// ---------------------------------------------------------------------------
+// neighbouring lines share cache lines along x: keep them on one XCD
+template <int DIR>
+__global__ void __launch_bounds__(VJP_LINE_MAX) k_rhs_vjp(const double *__restrict__ u, const double *__restrict__ lam,
+                                                      const double *__restrict__ nu_in, double ca, double cb,
+                                                      double *__restrict__ nu, double *__restrict__ L, int mode,
+                                                      int nx, int ny, int nz, const double *__restrict__ dxg,
+                                                      double gamma, double eps) {
+  extern __shared__ double smem[];
+  rhs_vjp_line<DIR>(u, lam, nu_in, ca, cb, nu, L, mode, nx, ny, nz, dxg, gamma, eps, xcd_block(), smem);
+}
+
+template <int DIR, int UPW>
+__global__ void __launch_bounds__(VJP_CHAR_LINE_MAX) k_rhs_vjp_char(
+    const double *__restrict__ u, const double *__restrict__ lam, const double *__restrict__ nu_in, double ca,
+    double cb, double *__restrict__ nu, double *__restrict__ L, int mode, int nx, int ny, int nz,
+    const double *__restrict__ dxg, double gamma, double eps) {
+  extern __shared__ double smem[];
+  rhs_vjp_char_line<DIR, UPW>(u, lam, nu_in, ca, cb, nu, L, mode, nx, ny, nz, dxg, gamma, eps, xcd_block(), smem);
+}
+
+__global__ void __launch_bounds__(256) k_visc_vjp_cell(const double *__restrict__ Q, const double *__restrict__ lam,
+                                                       const double *__restrict__ nu_in, double ca, double cb,
+                                                       double *__restrict__ QB, double *__restrict__ GB, int nx,
+                                                       int ny, int nz, const double *__restrict__ dxg, ViscC vc) {
+  visc_vjp_cell(Q, lam, nu_in, ca, cb, QB, GB, nx, ny, nz, dxg, vc, (size_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+template <int DIR>
+__global__ void __launch_bounds__(256) k_visc_vjp_ghost(const double *__restrict__ Q, const double *__restrict__ lam,
+                                                        const double *__restrict__ nu_in, double ca, double cb,
+                                                        double *__restrict__ QB, int nx, int ny, int nz,
+                                                        const double *__restrict__ dxg, ViscC vc) {
+  visc_vjp_ghost_line<DIR>(Q, lam, nu_in, ca, cb, QB, nx, ny, nz, dxg, vc,
+                           (size_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ void __launch_bounds__(256) k_visc_vjp_gather(const double *__restrict__ U, const double *__restrict__ QB,
+                                                         const double *__restrict__ GB, double *__restrict__ nu,
+                                                         int nx, int ny, int nz, double gamma) {
+  visc_vjp_gather(U, QB, GB, nu, nx, ny, nz, gamma, (size_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// kernels: all members of an ensemble
+// ---------------------------------------------------------------------------
+// Member m's state, adjoints and work space sit at 5*off, 5*off and 20*off;
+// its stage coefficients are dt_m * (A, B).  Line work lists hold each
+// member's first line along a direction (nmem + 1 entries).
+struct EnsVjp {
+  const EnsMember *mem;
+  int nmem;
+  const double *u, *lam, *nu_in;
+  double A, B;
+  double *nu, *L, *work;
+  int mode;
+  double gamma, eps;
+  ViscC vc;
+};
+
+__device__ __forceinline__ int ens_find(const unsigned *first, int nmem, unsigned item) {
+  int lo = 0, hi = nmem - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (first[mid] <= item) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ int ens_find_cell(const EnsMember *mem, int nmem, size_t idx) {
+  int lo = 0, hi = nmem - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (mem[mid].off <= idx) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+#define ENS_MEMBER_ARGS(a, M)                                                                              \
+  a.u + 5 * M.off, a.lam + 5 * M.off, a.nu_in ? a.nu_in + 5 * M.off : nullptr, M.dt * a.A, M.dt * a.B,     \
+      a.nu + 5 * M.off, a.L ? a.L + 5 * M.off : nullptr, a.mode, M.n[0], M.n[1], M.n[2], M.dxg, a.gamma, a.eps
+
+template <int DIR>
+__global__ void __launch_bounds__(VJP_LINE_MAX) k_ens_rhs_vjp(EnsVjp a, const unsigned *__restrict__ first,
+                                                          unsigned nitems) {
+  extern __shared__ double smem[];
+  for (unsigned item = blockIdx.x; item < nitems; item += gridDim.x) {
+    const int m = ens_find(first, a.nmem, item);
+    const EnsMember M = a.mem[m];
+    rhs_vjp_line<DIR>(ENS_MEMBER_ARGS(a, M), item - first[m], smem);
+    __syncthreads();
+  }
+}
+
+template <int DIR, int UPW>
+__global__ void __launch_bounds__(VJP_CHAR_LINE_MAX) k_ens_rhs_vjp_char(EnsVjp a, const unsigned *__restrict__ first,
+                                                                    unsigned nitems) {
+  extern __shared__ double smem[];
+  for (unsigned item = blockIdx.x; item < nitems; item += gridDim.x) {
+    const int m = ens_find(first, a.nmem, item);
+    const EnsMember M = a.mem[m];
+    rhs_vjp_char_line<DIR, UPW>(ENS_MEMBER_ARGS(a, M), item - first[m], smem);
+    __syncthreads();
+  }
+}
+
+#undef ENS_MEMBER_ARGS
+
+// primitive variables of every member's state into its work space
+__global__ void __launch_bounds__(256) k_ens_vjp_prims(EnsVjp a, size_t ncell) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ncell) return;
+  const EnsMember M = a.mem[ens_find_cell(a.mem, a.nmem, idx)];
+  const size_t c = idx - M.off;
+  double s[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) s[v] = a.u[5 * M.off + (size_t)v * M.ncell + c];
+  const Prim q = flowvar(s, a.gamma);
+  double *Q = a.work + 20 * M.off;
+  Q[c] = q.vx;
+  Q[M.ncell + c] = q.vy;
+  Q[2 * M.ncell + c] = q.vz;
+  Q[3 * M.ncell + c] = a.gamma * q.P / q.rho;
+}
+
+__global__ void __launch_bounds__(256) k_ens_visc_vjp_cell(EnsVjp a, size_t ncell) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ncell) return;
+  const EnsMember M = a.mem[ens_find_cell(a.mem, a.nmem, idx)];
+  const double *Q = a.work + 20 * M.off;
+  double *QB = a.work + 20 * M.off + 4 * M.ncell, *GB = a.work + 20 * M.off + 8 * M.ncell;
+  visc_vjp_cell(Q, a.lam + 5 * M.off, a.nu_in ? a.nu_in + 5 * M.off : nullptr, M.dt * a.A, M.dt * a.B, QB, GB,
+                M.n[0], M.n[1], M.n[2], M.dxg, a.vc, idx - M.off);
+}
+
+template <int DIR>
+__global__ void __launch_bounds__(256) k_ens_visc_vjp_ghost(EnsVjp a, const unsigned *__restrict__ first,
+                                                            unsigned nitems) {
+  const unsigned item = blockIdx.x * blockDim.x + threadIdx.x;
+  if (item >= nitems) return;
+  const int m = ens_find(first, a.nmem, item);
+  const EnsMember M = a.mem[m];
+  const double *Q = a.work + 20 * M.off;
+  double *QB = a.work + 20 * M.off + 4 * M.ncell;
+  visc_vjp_ghost_line<DIR>(Q, a.lam + 5 * M.off, a.nu_in ? a.nu_in + 5 * M.off : nullptr, M.dt * a.A,
+                           M.dt * a.B, QB, M.n[0], M.n[1], M.n[2], M.dxg, a.vc, item - first[m]);
+}
+
+__global__ void __launch_bounds__(256) k_ens_visc_vjp_gather(EnsVjp a, size_t ncell) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ncell) return;
+  const EnsMember M = a.mem[ens_find_cell(a.mem, a.nmem, idx)];
+  const double *QB = a.work + 20 * M.off + 4 * M.ncell, *GB = a.work + 20 * M.off + 8 * M.ncell;
+  visc_vjp_gather(a.u + 5 * M.off, QB, GB, a.nu + 5 * M.off, M.n[0], M.n[1], M.n[2], a.gamma, idx - M.off);
+}
+
 // y = a*x + b*y
 __global__ void k_axpby(const double *__restrict__ x, double a, double *__restrict__ y, double b, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -347,28 +1084,140 @@ __global__ void k_axpby(const double *__restrict__ x, double a, double *__restri
 
 // ---------------------------------------------------------------------------
 bool vjp_supported(const Grid &g, const Scheme &sc) {
-  return sc.interp == INTERP_COMP && sc.upwind == UPWIND_RUSANOV && sc.weno == WENO_JS && !(sc.Re > 0) &&
-         g.n[0] <= VJP_LINE_MAX &&
-         g.n[1] <= VJP_LINE_MAX && g.n[2] <= VJP_LINE_MAX;
+  if (sc.weno != WENO_JS) return false;
+  if (sc.interp == INTERP_COMP && sc.upwind != UPWIND_RUSANOV) return false;
+  const int lmax = sc.interp == INTERP_CHAR ? VJP_CHAR_LINE_MAX : VJP_LINE_MAX;
+  return g.n[0] <= lmax && g.n[1] <= lmax && g.n[2] <= lmax;
+}
+
+size_t vjp_work_size(const Grid &g, const Scheme &sc) { return sc.Re > 0 ? 20 * g.ncell : 0; }
+
+// dynamic LDS beyond 64 KiB needs the kernel's opt-in (160 KiB per CU on gfx950)
+static void allow_lds(const void *kernel, size_t bytes) {
+  if (bytes > 65536) (void)hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+template <int DIR>
+static void launch_vjp_dir(const Grid &g, const Scheme &sc, const double *u, const double *lam, const double *nu_in,
+                           double ca, double cb, double *nu, double *L, int mode, const double *dxg, hipStream_t st) {
+  const unsigned lines = (unsigned)(g.ncell / g.n[DIR]);
+  const unsigned threads = (unsigned)((g.n[DIR] + 63) / 64 * 64);
+  if (sc.interp == INTERP_COMP) {
+    const size_t lds = (size_t)23 * g.n[DIR] * sizeof(double);
+    allow_lds((const void *)k_rhs_vjp<DIR>, lds);
+    hipLaunchKernelGGL(k_rhs_vjp<DIR>, dim3(lines), dim3(threads), lds, st, u, lam, nu_in, ca, cb, nu, L, mode,
+                       g.n[0], g.n[1], g.n[2], dxg, sc.gamma, sc.eps);
+    return;
+  }
+  const size_t lds = (size_t)70 * g.n[DIR] * sizeof(double);
+  if (sc.upwind == UPWIND_ROE) {
+    allow_lds((const void *)k_rhs_vjp_char<DIR, UPWIND_ROE>, lds);
+    hipLaunchKernelGGL((k_rhs_vjp_char<DIR, UPWIND_ROE>), dim3(lines), dim3(threads), lds, st, u, lam, nu_in, ca,
+                       cb, nu, L, mode, g.n[0], g.n[1], g.n[2], dxg, sc.gamma, sc.eps);
+  } else {
+    allow_lds((const void *)k_rhs_vjp_char<DIR, UPWIND_RUSANOV>, lds);
+    hipLaunchKernelGGL((k_rhs_vjp_char<DIR, UPWIND_RUSANOV>), dim3(lines), dim3(threads), lds, st, u, lam, nu_in,
+                       ca, cb, nu, L, mode, g.n[0], g.n[1], g.n[2], dxg, sc.gamma, sc.eps);
+  }
 }
 
 void launch_rhs_vjp(const Grid &g, const Scheme &sc, const double *u, const double *lam, const double *nu_in,
-                    double ca, double cb, double *nu, double *L, bool set_L, const double *dxg, hipStream_t st) {
-  for (int d = 0; d < 3; d++) {
-    const unsigned lines = (unsigned)(g.ncell / g.n[d]);
-    const unsigned threads = (unsigned)((g.n[d] + 63) / 64 * 64);
-    const size_t lds = (size_t)23 * g.n[d] * sizeof(double);
-    const int mode = d == 0 ? VJP_WRITE : (d == 2 && L ? (set_L ? VJP_ADD_LSET : VJP_ADD_L) : VJP_ADD);
-    if (d == 0)
-      hipLaunchKernelGGL(k_rhs_vjp<0>, dim3(lines), dim3(threads), lds, st, u, lam, nu_in, ca, cb, nu, L, mode,
-                         g.n[0], g.n[1], g.n[2], dxg, sc.gamma, sc.eps);
-    else if (d == 1)
-      hipLaunchKernelGGL(k_rhs_vjp<1>, dim3(lines), dim3(threads), lds, st, u, lam, nu_in, ca, cb, nu, L, mode,
-                         g.n[0], g.n[1], g.n[2], dxg, sc.gamma, sc.eps);
-    else
-      hipLaunchKernelGGL(k_rhs_vjp<2>, dim3(lines), dim3(threads), lds, st, u, lam, nu_in, ca, cb, nu, L, mode,
-                         g.n[0], g.n[1], g.n[2], dxg, sc.gamma, sc.eps);
+                    double ca, double cb, double *nu, double *L, bool set_L, const double *dxg, double *work,
+                    hipStream_t st) {
+  const bool visc = sc.Re > 0;
+  if (visc) {
+    // work: Q [4], QB [4], GB [12] (times ncell)
+    double *Q = work, *QB = work + 4 * g.ncell, *GB = work + 8 * g.ncell;
+    const ViscC vc = {sc.gamma, 1.0 / sc.Re, 1.0 / (sc.gamma - 1.0), 1.0 / sc.Pr};
+    const unsigned nb = (unsigned)((g.ncell + 255) / 256);
+    launch_prims(g, sc, u, Q, st);
+    hipLaunchKernelGGL(k_visc_vjp_cell, dim3(nb), dim3(256), 0, st, Q, lam, nu_in, ca, cb, QB, GB, g.n[0], g.n[1],
+                       g.n[2], dxg, vc);
+    const unsigned gx = (unsigned)((g.ncell / g.n[0] + 255) / 256), gy = (unsigned)((g.ncell / g.n[1] + 255) / 256),
+                   gz = (unsigned)((g.ncell / g.n[2] + 255) / 256);
+    hipLaunchKernelGGL(k_visc_vjp_ghost<0>, dim3(gx), dim3(256), 0, st, Q, lam, nu_in, ca, cb, QB, g.n[0], g.n[1],
+                       g.n[2], dxg, vc);
+    hipLaunchKernelGGL(k_visc_vjp_ghost<1>, dim3(gy), dim3(256), 0, st, Q, lam, nu_in, ca, cb, QB, g.n[0], g.n[1],
+                       g.n[2], dxg, vc);
+    hipLaunchKernelGGL(k_visc_vjp_ghost<2>, dim3(gz), dim3(256), 0, st, Q, lam, nu_in, ca, cb, QB, g.n[0], g.n[1],
+                       g.n[2], dxg, vc);
+    hipLaunchKernelGGL(k_visc_vjp_gather, dim3(nb), dim3(256), 0, st, u, QB, GB, nu, g.n[0], g.n[1], g.n[2],
+                       sc.gamma);
   }
+  const int last = L ? (set_L ? VJP_ADD_LSET : VJP_ADD_L) : VJP_ADD;
+  launch_vjp_dir<0>(g, sc, u, lam, nu_in, ca, cb, nu, L, visc ? VJP_ADD : VJP_WRITE, dxg, st);
+  launch_vjp_dir<1>(g, sc, u, lam, nu_in, ca, cb, nu, L, VJP_ADD, dxg, st);
+  launch_vjp_dir<2>(g, sc, u, lam, nu_in, ca, cb, nu, L, last, dxg, st);
+}
+
+bool ens_vjp_supported(const Scheme &sc, const int nmax[3]) {
+  Grid g;
+  for (int d = 0; d < 3; d++) g.n[d] = nmax[d];
+  g.ncell = 1;
+  return vjp_supported(g, sc);
+}
+
+size_t ens_vjp_work_size(const Scheme &sc, size_t ncell) { return sc.Re > 0 ? 20 * ncell : 0; }
+
+template <int DIR>
+static void launch_ens_vjp_dir(const Scheme &sc, const EnsVjpPlan &p, const EnsVjp &a, hipStream_t st) {
+  const unsigned items = p.nlines[DIR];
+  const unsigned grid = items < 4096 ? items : 4096;  // persistent blocks stride over the lines
+  const unsigned threads = (unsigned)((p.nmax[DIR] + 63) / 64 * 64);
+  if (sc.interp == INTERP_COMP) {
+    const size_t lds = (size_t)23 * p.nmax[DIR] * sizeof(double);
+    allow_lds((const void *)k_ens_rhs_vjp<DIR>, lds);
+    hipLaunchKernelGGL(k_ens_rhs_vjp<DIR>, dim3(grid), dim3(threads), lds, st, a, p.lines0[DIR], items);
+    return;
+  }
+  const size_t lds = (size_t)70 * p.nmax[DIR] * sizeof(double);
+  if (sc.upwind == UPWIND_ROE) {
+    allow_lds((const void *)k_ens_rhs_vjp_char<DIR, UPWIND_ROE>, lds);
+    hipLaunchKernelGGL((k_ens_rhs_vjp_char<DIR, UPWIND_ROE>), dim3(grid), dim3(threads), lds, st, a,
+                       p.lines0[DIR], items);
+  } else {
+    allow_lds((const void *)k_ens_rhs_vjp_char<DIR, UPWIND_RUSANOV>, lds);
+    hipLaunchKernelGGL((k_ens_rhs_vjp_char<DIR, UPWIND_RUSANOV>), dim3(grid), dim3(threads), lds, st, a,
+                       p.lines0[DIR], items);
+  }
+}
+
+void launch_ens_rhs_vjp(const Scheme &sc, const EnsVjpPlan &p, const double *u, const double *lam,
+                        const double *nu_in, double A, double B, double *nu, double *L, bool set_L, double *work,
+                        hipStream_t st) {
+  EnsVjp a;
+  a.mem = p.mem;
+  a.nmem = p.nmem;
+  a.u = u;
+  a.lam = lam;
+  a.nu_in = nu_in;
+  a.A = A;
+  a.B = B;
+  a.nu = nu;
+  a.L = L;
+  a.work = work;
+  a.gamma = sc.gamma;
+  a.eps = sc.eps;
+  a.vc = {sc.gamma, sc.Re > 0 ? 1.0 / sc.Re : 0.0, 1.0 / (sc.gamma - 1.0), 1.0 / sc.Pr};
+  const bool visc = sc.Re > 0;
+  if (visc) {
+    const unsigned nb = (unsigned)((p.ncell + 255) / 256);
+    hipLaunchKernelGGL(k_ens_vjp_prims, dim3(nb), dim3(256), 0, st, a, p.ncell);
+    hipLaunchKernelGGL(k_ens_visc_vjp_cell, dim3(nb), dim3(256), 0, st, a, p.ncell);
+    hipLaunchKernelGGL(k_ens_visc_vjp_ghost<0>, dim3((p.nlines[0] + 255) / 256), dim3(256), 0, st, a, p.lines0[0],
+                       p.nlines[0]);
+    hipLaunchKernelGGL(k_ens_visc_vjp_ghost<1>, dim3((p.nlines[1] + 255) / 256), dim3(256), 0, st, a, p.lines0[1],
+                       p.nlines[1]);
+    hipLaunchKernelGGL(k_ens_visc_vjp_ghost<2>, dim3((p.nlines[2] + 255) / 256), dim3(256), 0, st, a, p.lines0[2],
+                       p.nlines[2]);
+    hipLaunchKernelGGL(k_ens_visc_vjp_gather, dim3(nb), dim3(256), 0, st, a, p.ncell);
+  }
+  a.mode = visc ? VJP_ADD : VJP_WRITE;
+  launch_ens_vjp_dir<0>(sc, p, a, st);
+  a.mode = VJP_ADD;
+  launch_ens_vjp_dir<1>(sc, p, a, st);
+  a.mode = L ? (set_L ? VJP_ADD_LSET : VJP_ADD_L) : VJP_ADD;
+  launch_ens_vjp_dir<2>(sc, p, a, st);
 }
 
 void launch_axpby(const double *x, double a, double *y, double b, size_t n, hipStream_t st) {

@@ -7,6 +7,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -37,6 +38,7 @@ struct hm_solver {
   size_t tmp_elems;
   // adjoint tape: checkpoints, segment stage states, scratch
   std::vector<double *> ckpt, stages, vscratch;
+  std::vector<double *> vwork;  // viscous adjoint work space (one buffer)
   int tape_nsteps, tape_every;
   double tape_dt;
   // reverse sweep: a second stream recomputes the next segment while the
@@ -175,7 +177,7 @@ void hm_destroy(hm_solver *s) {
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   if (s->part_host) (void)hipHostFree(s->part_host);
-  for (auto *v : {&s->ckpt, &s->stages, &s->vscratch})
+  for (auto *v : {&s->ckpt, &s->stages, &s->vscratch, &s->vwork})
     for (double *b : *v) (void)hipFree(b);
   for (hipEvent_t e : {s->ev_ready[0], s->ev_ready[1], s->ev_done[0], s->ev_done[1], s->ev_start})
     if (e) (void)hipEventDestroy(e);
@@ -522,16 +524,28 @@ static hm_status rk4_step_store(hm_solver *s, double *u, double dt, double *cons
 // lam_out = lam + sum_s nu_s, nu_s = J(U_s)^T mu_s,
 // mu_s = dt b_s lam + dt a_{s+1,s} nu_{s+1} (formed inside the VJP kernels);
 // lam_out must differ from lam
+static hm_status vjp_work(hm_solver *s, double **w) {
+  const size_t n = hm::vjp_work_size(s->g, s->sc);
+  *w = nullptr;
+  if (n == 0) return HM_OK;
+  hm_status st = ensure_buffers(s->vwork, 1, n);
+  if (st == HM_OK) *w = s->vwork[0];
+  return st;
+}
+
 static hm_status rk4_step_vjp(hm_solver *s, double *const U[4], double dt, const double *lam, double *lam_out) {
   const size_t n5 = 5 * s->g.ncell;
   hm_status st = ensure_buffers(s->vscratch, 3, n5);
   if (st != HM_OK) return st;
+  double *work;
+  if ((st = vjp_work(s, &work)) != HM_OK) return st;
   double *nuA = s->vscratch[0], *nuB = s->vscratch[1];
   const double *nu_in = nullptr;
   double *nu_out = nuA;
   for (int stage = 3; stage >= 0; stage--) {
     const double ca = stage < 3 ? dt * RK_A_SUB[stage] : 0.0, cb = dt * RK_B[stage];
-    hm::launch_rhs_vjp(s->g, s->sc, U[stage], lam, nu_in, ca, cb, nu_out, lam_out, stage == 3, s->dxg, s->stream);
+    hm::launch_rhs_vjp(s->g, s->sc, U[stage], lam, nu_in, ca, cb, nu_out, lam_out, stage == 3, s->dxg, work,
+                       s->stream);
     if ((st = check_launch("rk4_step_vjp")) != HM_OK) return st;
     nu_in = nu_out;
     nu_out = nu_out == nuA ? nuB : nuA;
@@ -543,8 +557,9 @@ static hm_status check_vjp(hm_solver *s, const char *name) {
   if (!s) return fail(HM_ERR_ARG, std::string(name) + ": solver is NULL");
   if (!hm::vjp_supported(s->g, s->sc))
     return fail(HM_ERR_UNSUPPORTED, std::string(name) +
-                                        ": the adjoint is built for the inviscid component-wise WENO5-JS + "
-                                        "Rusanov scheme with grid sides <= 512");
+                                        ": the adjoint is built for WENO5-JS weights with component-wise "
+                                        "Rusanov (grid sides <= 512) or characteristic-wise Roe/Rusanov "
+                                        "(grid sides <= 256) schemes");
   return HM_OK;
 }
 
@@ -555,7 +570,9 @@ hm_status hm_rhs_vjp(hm_solver *s, double *out, const double *u, const double *l
   if ((st = check_state(s, u, "hm_rhs_vjp: u")) != HM_OK) return st;
   if ((st = check_state(s, lam, "hm_rhs_vjp: lam")) != HM_OK) return st;
   if (out == u || out == lam) return fail(HM_ERR_ARG, "hm_rhs_vjp: out must not alias u or lam");
-  hm::launch_rhs_vjp(s->g, s->sc, u, lam, nullptr, 0.0, 1.0, out, nullptr, false, s->dxg, s->stream);
+  double *work;
+  if ((st = vjp_work(s, &work)) != HM_OK) return st;
+  hm::launch_rhs_vjp(s->g, s->sc, u, lam, nullptr, 0.0, 1.0, out, nullptr, false, s->dxg, work, s->stream);
   return check_launch("hm_rhs_vjp");
 }
 
@@ -750,8 +767,20 @@ struct hm_ensemble {
   double *dxg = nullptr;
   size_t ncell = 0;
   double *R = nullptr, *S[2] = {nullptr, nullptr}, *acc = nullptr;
+  // viscous members: fluxes, ghost-layer fluxes, primitive variables of all
+  // members, and the work list of the viscous-flux launch
+  double *FV = nullptr, *FVg[3] = {nullptr, nullptr, nullptr}, *Q = nullptr;
+  unsigned *item0v = nullptr;
+  unsigned nitems_v = 0;
   unsigned *item0[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   unsigned nitems[2][3] = {{0, 0, 0}, {0, 0, 0}};
+  // adjoint: first line of every member along each direction, longest
+  // lines, and the tape (checkpoints, stage states of one segment, scratch)
+  unsigned *lines0[3] = {nullptr, nullptr, nullptr};
+  unsigned nlines[3] = {0, 0, 0};
+  int nmax[3] = {0, 0, 0};
+  std::vector<double *> ckpt, stages, vscratch, vwork;
+  int tape_nsteps = 0, tape_every = 0;
   int shape = 0;
   unsigned grid[3] = {0, 0, 0};
   bool dt_set = false;
@@ -772,12 +801,15 @@ void ens_drop_graph(hm_ensemble *e) {
   e->graph_u = nullptr;
 }
 
-// the 12 launches of one RK4 step of all members (stage s reads S[(s-1)%2],
-// writes S[s%2]; the last stage writes u)
+// the launches of one RK4 step of all members (stage s reads S[(s-1)%2],
+// writes S[s%2]; the last stage writes u): per stage the viscous fluxes of
+// the stage state (viscous members; the previous stage's z pass left its
+// primitive variables in Q) and the three direction passes
 void ens_step_launches(hm_ensemble *e, double *u, hipStream_t st) {
   for (int stage = 0; stage < 4; stage++) {
     const double *U = stage == 0 ? u : e->S[(stage - 1) % 2];
     double *Sn = stage < 3 ? e->S[stage % 2] : nullptr;
+    if (e->sc.Re > 0) hm::launch_ens_visc(e->sc, e->mem_dev, e->nmem, e->item0v, e->nitems_v, st);
     for (int d = 0; d < 3; d++)
       hm::launch_ens_pass(e->sc, d, e->shape, e->mem_dev, e->nmem, e->item0[e->shape][d], e->nitems[e->shape][d],
                           e->grid[d], U, e->R, u, Sn, e->acc, stage, st);
@@ -797,6 +829,22 @@ hm_status ens_capture(hm_ensemble *e, double *u) {
   return HM_OK;
 }
 
+// one RK4 step of all members in place on u, keeping the stage states
+// U1, U2, U3 (eager launches; the adjoint's forward and recomputation)
+hm_status ens_step_store(hm_ensemble *e, double *u, double *const U[3]) {
+  hipStream_t st = e->stream;
+  if (e->sc.Re > 0) hm::launch_ens_prims(e->sc, e->mem_dev, e->nmem, u, e->ncell, st);
+  for (int stage = 0; stage < 4; stage++) {
+    const double *Uin = stage == 0 ? u : U[stage - 1];
+    double *Sn = stage < 3 ? U[stage] : nullptr;
+    if (e->sc.Re > 0) hm::launch_ens_visc(e->sc, e->mem_dev, e->nmem, e->item0v, e->nitems_v, st);
+    for (int d = 0; d < 3; d++)
+      hm::launch_ens_pass(e->sc, d, e->shape, e->mem_dev, e->nmem, e->item0[e->shape][d], e->nitems[e->shape][d],
+                          e->grid[d], Uin, e->R, u, Sn, e->acc, stage, st);
+  }
+  return check_launch("hm_ensemble: step");
+}
+
 hm_status ens_check(const hm_ensemble *e, const void *u, const char *name) {
   if (!e) return fail(HM_ERR_ARG, std::string(name) + ": ensemble is NULL");
   if (!u) return fail(HM_ERR_ARG, std::string(name) + ": state is NULL");
@@ -812,13 +860,18 @@ void hm_ensemble_destroy(hm_ensemble *e) {
   if (!e) return;
   ens_drop_graph(e);
   if (e->cap) (void)hipStreamDestroy(e->cap);
-  double *bufs[] = {e->dxg, e->R, e->S[0], e->S[1], e->acc};
+  double *bufs[] = {e->dxg, e->R, e->S[0], e->S[1], e->acc, e->FV, e->FVg[0], e->FVg[1], e->FVg[2], e->Q};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   if (e->mem_dev) (void)hipFree(e->mem_dev);
+  if (e->item0v) (void)hipFree(e->item0v);
   for (int sh = 0; sh < 2; sh++)
     for (int d = 0; d < 3; d++)
       if (e->item0[sh][d]) (void)hipFree(e->item0[sh][d]);
+  for (int d = 0; d < 3; d++)
+    if (e->lines0[d]) (void)hipFree(e->lines0[d]);
+  for (auto *v : {&e->ckpt, &e->stages, &e->vscratch, &e->vwork})
+    for (double *b : *v) (void)hipFree(b);
   delete e;
 }
 
@@ -832,8 +885,10 @@ hm_status hm_ensemble_create(const hm_config *members, int nmembers, hm_ensemble
       if (c.dim[d] < 5) return fail(HM_ERR_ARG, "hm_ensemble_create: every grid dimension must be >= 5");
     if (c.interp != c0.interp || c.upwind != c0.upwind || c.gamma != c0.gamma || c.weno_eps != c0.weno_eps)
       return fail(HM_ERR_ARG, "hm_ensemble_create: members must share interpolation, upwinding, gamma and eps");
-    if (c.Re > 0)
-      return fail(HM_ERR_UNSUPPORTED, "hm_ensemble_create: batched members are inviscid (Re <= 0)");
+    if ((c.Re > 0) != (c0.Re > 0) || (c0.Re > 0 && (c.Re != c0.Re || c.Minf != c0.Minf || c.Pr != c0.Pr)))
+      return fail(HM_ERR_ARG, "hm_ensemble_create: members must share Re, Minf and Pr");
+    if (c.Re > 0 && !(c.Minf > 0 && c.Pr > 0))
+      return fail(HM_ERR_ARG, "hm_ensemble_create: viscous members need Minf > 0 and Pr > 0");
     if (c.weno_type != HM_WENO_JS)
       return fail(HM_ERR_UNSUPPORTED, "hm_ensemble_create: batched members use Jiang-Shu weights");
   }
@@ -848,7 +903,7 @@ hm_status hm_ensemble_create(const hm_config *members, int nmembers, hm_ensemble
   e->sc.interp = c0.interp == HM_INTERP_CHARACTERISTIC ? hm::INTERP_CHAR : hm::INTERP_COMP;
   e->sc.upwind = c0.upwind == HM_UPWIND_ROE ? hm::UPWIND_ROE : hm::UPWIND_RUSANOV;
   e->sc.gamma = c0.gamma;
-  e->sc.Re = -1.0;
+  e->sc.Re = c0.Re > 0 ? c0.Re / c0.Minf : -1.0;
   e->sc.Pr = c0.Pr;
   e->sc.eps = c0.weno_eps;
   e->sc.weno = hm::WENO_JS;
@@ -864,6 +919,8 @@ hm_status hm_ensemble_create(const hm_config *members, int nmembers, hm_ensemble
     M.ncell = (size_t)c.dim[0] * c.dim[1] * c.dim[2];
     M.off = e->ncell;
     M.dt = 0.0;
+    M.fv = M.q = nullptr;
+    M.fvg[0] = M.fvg[1] = M.fvg[2] = nullptr;
     e->ncell += M.ncell;
     const size_t ndx = (size_t)(c.dim[0] + 6) + (c.dim[1] + 6) + (c.dim[2] + 6);
     dxoff[m] = dxh.size();
@@ -893,6 +950,33 @@ hm_status hm_ensemble_create(const hm_config *members, int nmembers, hm_ensemble
     goto bad;
   }
   for (int m = 0; m < nmembers; m++) e->mem[m].dxg = e->dxg + dxoff[m];
+  if (e->sc.Re > 0) {
+    size_t ng[3] = {0, 0, 0};
+    for (int m = 0; m < nmembers; m++)
+      for (int d = 0; d < 3; d++) ng[d] += 16 * (e->mem[m].ncell / e->mem[m].n[d]);
+    if ((st = dmalloc(&e->FV, 12 * e->ncell)) != HM_OK) goto bad;
+    if ((st = dmalloc(&e->Q, 4 * e->ncell)) != HM_OK) goto bad;
+    for (int d = 0; d < 3; d++)
+      if ((st = dmalloc(&e->FVg[d], ng[d])) != HM_OK) goto bad;
+    size_t og[3] = {0, 0, 0};
+    std::vector<unsigned> pre(nmembers + 1, 0);
+    for (int m = 0; m < nmembers; m++) {
+      hm::EnsMember &M = e->mem[m];
+      M.fv = e->FV + 12 * M.off;
+      M.q = e->Q + 4 * M.off;
+      for (int d = 0; d < 3; d++) {
+        M.fvg[d] = e->FVg[d] + og[d];
+        og[d] += 16 * (M.ncell / M.n[d]);
+      }
+      pre[m + 1] = pre[m] + hm::ens_visc_items_per_member(M.n);
+    }
+    e->nitems_v = pre[nmembers];
+    if (hipMalloc((void **)&e->item0v, pre.size() * sizeof(unsigned)) != hipSuccess ||
+        hipMemcpy(e->item0v, pre.data(), pre.size() * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess) {
+      st = fail(HM_ERR_NOMEM, "hm_ensemble_create: viscous work list");
+      goto bad;
+    }
+  }
   if (hipMalloc((void **)&e->mem_dev, nmembers * sizeof(hm::EnsMember)) != hipSuccess) {
     st = fail(HM_ERR_NOMEM, "hm_ensemble_create: member table");
     goto bad;
@@ -913,6 +997,19 @@ hm_status hm_ensemble_create(const hm_config *members, int nmembers, hm_ensemble
         goto bad;
       }
     }
+  for (int d = 0; d < 3; d++) {
+    std::vector<unsigned> pre(nmembers + 1, 0);
+    for (int m = 0; m < nmembers; m++) {
+      pre[m + 1] = pre[m] + (unsigned)(e->mem[m].ncell / e->mem[m].n[d]);
+      e->nmax[d] = std::max(e->nmax[d], e->mem[m].n[d]);
+    }
+    e->nlines[d] = pre[nmembers];
+    if (hipMalloc((void **)&e->lines0[d], pre.size() * sizeof(unsigned)) != hipSuccess ||
+        hipMemcpy(e->lines0[d], pre.data(), pre.size() * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess) {
+      st = fail(HM_ERR_NOMEM, "hm_ensemble_create: line lists");
+      goto bad;
+    }
+  }
   if (hipStreamCreate(&e->cap) != hipSuccess) {
     st = fail(HM_ERR_HIP, "hm_ensemble_create: stream creation failed");
     goto bad;
@@ -977,6 +1074,9 @@ hm_status hm_ensemble_rk4(hm_ensemble *e, double *u, int nsteps) {
   bool stale = !e->exec || e->graph_u != u || e->graph_shape != e->shape;
   for (int d = 0; d < 3; d++) stale = stale || e->graph_grid[d] != e->grid[d];
   if (stale && (st = ens_capture(e, u)) != HM_OK) return st;
+  // primitive variables of the current state for the first stage's viscous
+  // fluxes; every later stage finds them written by the z pass before it
+  if (e->sc.Re > 0) hm::launch_ens_prims(e->sc, e->mem_dev, e->nmem, u, e->ncell, e->stream);
   for (int it = 0; it < nsteps; it++) HM_HIP(hipGraphLaunch(e->exec, e->stream));
   return HM_OK;
 }
@@ -1036,6 +1136,113 @@ hm_status hm_ensemble_tune(hm_ensemble *e, const double *u, int reps, int *shape
   if (shape) *shape = best_shape;
   if (grid)
     for (int d = 0; d < 3; d++) grid[d] = best_grid[d];
+  return HM_OK;
+}
+
+// ---- batched discrete adjoint ----------------------------------------
+static hm_status ens_check_vjp(hm_ensemble *e, const void *u, const char *name) {
+  hm_status st = ens_check(e, u, name);
+  if (st != HM_OK) return st;
+  if (!hm::ens_vjp_supported(e->sc, e->nmax))
+    return fail(HM_ERR_UNSUPPORTED, std::string(name) +
+                                        ": the adjoint is built for WENO5-JS weights with component-wise "
+                                        "Rusanov (sides <= 512) or characteristic-wise Roe/Rusanov (sides <= 256)");
+  if (!e->dt_set) return fail(HM_ERR_ARG, std::string(name) + ": set the members' dt first (hm_ensemble_set_dt)");
+  return HM_OK;
+}
+
+static hm_status ens_buffers(std::vector<double *> &v, size_t count, size_t n) {
+  while (v.size() < count) {
+    double *p = nullptr;
+    hm_status st = dmalloc(&p, n);
+    if (st != HM_OK) return st;
+    v.push_back(p);
+  }
+  return HM_OK;
+}
+
+hm_status hm_ensemble_adjoint_forward(hm_ensemble *e, double *u, int nsteps, int every) {
+  hm_status st = ens_check_vjp(e, u, "hm_ensemble_adjoint_forward");
+  if (st != HM_OK) return st;
+  if (nsteps < 1 || every < 1)
+    return fail(HM_ERR_ARG, "hm_ensemble_adjoint_forward: nsteps and checkpoint_every must be >= 1");
+  const size_t n5 = 5 * e->ncell;
+  const int nck = (nsteps + every - 1) / every;
+  if ((st = ens_buffers(e->ckpt, (size_t)nck, n5)) != HM_OK) return st;
+  if ((st = ens_buffers(e->stages, (size_t)4 * every, n5)) != HM_OK) return st;
+  const int nlast = (nck - 1) * every;
+  for (int n = 0; n < nsteps; n++) {
+    if (n % every == 0)
+      HM_HIP(hipMemcpyAsync(e->ckpt[n / every], u, n5 * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+    if (n < nlast) {
+      if ((st = hm_ensemble_rk4(e, u, 1)) != HM_OK) return st;
+    } else {
+      // the last segment keeps its stage states for the reverse sweep
+      double **U = &e->stages[(size_t)4 * (n - nlast)];
+      HM_HIP(hipMemcpyAsync(U[0], u, n5 * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+      double *const Us[3] = {U[1], U[2], U[3]};
+      if ((st = ens_step_store(e, u, Us)) != HM_OK) return st;
+    }
+  }
+  e->tape_nsteps = nsteps;
+  e->tape_every = every;
+  return HM_OK;
+}
+
+hm_status hm_ensemble_adjoint_reverse(hm_ensemble *e, double *lam) {
+  hm_status st = ens_check_vjp(e, lam, "hm_ensemble_adjoint_reverse");
+  if (st != HM_OK) return st;
+  if (e->tape_nsteps < 1) return fail(HM_ERR_ARG, "hm_ensemble_adjoint_reverse: no forward sweep recorded");
+  const int N = e->tape_nsteps, K = e->tape_every;
+  const size_t n5 = 5 * e->ncell;
+  if ((st = ens_buffers(e->vscratch, 4, n5)) != HM_OK) return st;
+  double *work = nullptr;
+  const size_t nw = hm::ens_vjp_work_size(e->sc, e->ncell);
+  if (nw) {
+    if ((st = ens_buffers(e->vwork, 1, nw)) != HM_OK) return st;
+    work = e->vwork[0];
+  }
+  hm::EnsVjpPlan plan;
+  plan.mem = e->mem_dev;
+  plan.nmem = e->nmem;
+  plan.ncell = e->ncell;
+  for (int d = 0; d < 3; d++) {
+    plan.lines0[d] = e->lines0[d];
+    plan.nlines[d] = e->nlines[d];
+    plan.nmax[d] = e->nmax[d];
+  }
+  const int nck = (N + K - 1) / K;
+  double *cur = lam;
+  for (int seg = nck - 1; seg >= 0; seg--) {
+    const int n0 = seg * K, n1 = (n0 + K < N) ? n0 + K : N;
+    if (seg + 1 < nck) {
+      // recompute the segment from its checkpoint, keeping the stage states
+      for (int n = n0; n < n1; n++) {
+        double **U = &e->stages[(size_t)4 * (n - n0)];
+        if (n == n0) HM_HIP(hipMemcpyAsync(U[0], e->ckpt[seg], n5 * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+        double *next = n + 1 < n1 ? e->stages[(size_t)4 * (n - n0 + 1)] : e->vscratch[3];
+        HM_HIP(hipMemcpyAsync(next, U[0], n5 * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+        double *const Us[3] = {U[1], U[2], U[3]};
+        if ((st = ens_step_store(e, next, Us)) != HM_OK) return st;
+      }
+    }
+    for (int n = n1 - 1; n >= n0; n--) {
+      double **U = &e->stages[(size_t)4 * (n - n0)];
+      double *out = cur == lam ? e->vscratch[2] : lam;
+      // step adjoint: out = cur + sum_s nu_s (stage coefficients formed per member)
+      const double *nu_in = nullptr;
+      double *nu_out = e->vscratch[0];
+      for (int stage = 3; stage >= 0; stage--) {
+        const double A = stage < 3 ? RK_A_SUB[stage] : 0.0, B = RK_B[stage];
+        hm::launch_ens_rhs_vjp(e->sc, plan, U[stage], cur, nu_in, A, B, nu_out, out, stage == 3, work, e->stream);
+        if ((st = check_launch("hm_ensemble_adjoint_reverse")) != HM_OK) return st;
+        nu_in = nu_out;
+        nu_out = nu_out == e->vscratch[0] ? e->vscratch[1] : e->vscratch[0];
+      }
+      cur = out;
+    }
+  }
+  if (cur != lam) HM_HIP(hipMemcpyAsync(lam, cur, n5 * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
   return HM_OK;
 }
 

@@ -80,13 +80,16 @@ void launch_pass(const Grid &g, const Scheme &sc, int dir, const PassBuffers &b,
 // primitive variables (u,v,w,T) of u into Q
 void launch_prims(const Grid &g, const Scheme &sc, const double *u, double *Q, hipStream_t st);
 
-// batched ensemble passes (hm_pass.hip): inviscid members of one scheme
+// batched ensemble passes (hm_pass.hip): members of one scheme
 struct EnsMember {
   int n[3];
   size_t ncell;
   size_t off;         // first cell of the member; its state is u[5*off .. 5*(off+ncell))
   const double *dxg;  // ghosted dxinv x|y|z of the member (device)
   double dt;          // the member's time step
+  // viscous members: viscous fluxes (12*ncell), ghost-layer fluxes per
+  // direction, primitive variables (4*ncell); null when inviscid
+  double *fv, *fvg[3], *q;
 };
 // work items of one member in pass dir with tile shape `shape` (0: 4
 // wavefronts per block, 1: 2 wavefronts per block)
@@ -97,13 +100,41 @@ void launch_ens_pass(const Scheme &sc, int dir, int shape, const EnsMember *mem,
                      unsigned nitems, unsigned grid, const double *U, double *R, double *u, double *S, double *acc,
                      int stage, hipStream_t st);
 
+// viscous fluxes of all members (hm_kernels.hip): work items per member and
+// one launch over all of them; primitive variables of all members
+unsigned ens_visc_items_per_member(const int n[3]);
+void launch_ens_visc(const Scheme &sc, const EnsMember *mem, int nmem, const unsigned *item0v, unsigned nitems,
+                     hipStream_t st);
+void launch_ens_prims(const Scheme &sc, const EnsMember *mem, int nmem, const double *u, size_t ncell,
+                      hipStream_t st);
+
 // discrete adjoint (hm_adjoint.hip)
 bool vjp_supported(const Grid &g, const Scheme &sc);
+// doubles of device work space launch_rhs_vjp needs (viscous schemes)
+size_t vjp_work_size(const Grid &g, const Scheme &sc);
 // nu = (d rhs/d u)^T mu at u with mu = cb*lam + ca*nu_in (nu_in may be
 // null); when L is not null the finished nu is also added into L (set_L:
-// L = lam + nu)
+// L = lam + nu); work holds vjp_work_size doubles
 void launch_rhs_vjp(const Grid &g, const Scheme &sc, const double *u, const double *lam, const double *nu_in,
-                    double ca, double cb, double *nu, double *L, bool set_L, const double *dxg, hipStream_t st);
+                    double ca, double cb, double *nu, double *L, bool set_L, const double *dxg, double *work,
+                    hipStream_t st);
 void launch_axpby(const double *x, double a, double *y, double b, size_t n, hipStream_t st);
+
+// the same for all members of an ensemble at once: member m's stage
+// coefficients are dt_m*A and dt_m*B; lines0[d] lists each member's first
+// line along d (nmem + 1 entries), nmax[d] the longest line along d
+struct EnsVjpPlan {
+  const EnsMember *mem;  // device member table
+  int nmem;
+  size_t ncell;
+  const unsigned *lines0[3];
+  unsigned nlines[3];
+  int nmax[3];
+};
+bool ens_vjp_supported(const Scheme &sc, const int nmax[3]);
+size_t ens_vjp_work_size(const Scheme &sc, size_t ncell);
+void launch_ens_rhs_vjp(const Scheme &sc, const EnsVjpPlan &p, const double *u, const double *lam,
+                        const double *nu_in, double A, double B, double *nu, double *L, bool set_L, double *work,
+                        hipStream_t st);
 
 }  // namespace hm

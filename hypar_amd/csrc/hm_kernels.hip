@@ -227,15 +227,19 @@ __device__ __forceinline__ void visc_flux(double fv[4], const QT &q, const doubl
 constexpr int VTX = 64, VTY = 4, VKS = 16;
 constexpr int VHX = VTX + 4, VHY = VTY + 4;
 
-__global__ void __launch_bounds__(VTX * VTY) k_visc_flux(const double *__restrict__ Q, double *__restrict__ FV,
-                                                         int nx, int ny, int nz,
-                                                         const double *__restrict__ dxg,  // ghosted dxinv x|y|z
-                                                         ViscParams vp) {
-  __shared__ double sQ[4][VHY][VHX];
+// tiles (x, y, z-segment) of a grid
+__host__ __device__ __forceinline__ unsigned visc_tiles(int nx, int ny, int nz) {
+  return (unsigned)(((nx + VTX - 1) / VTX) * ((ny + VTY - 1) / VTY) * ((nz + VKS - 1) / VKS));
+}
+
+// tile b of the grid; every thread of the block takes part
+__device__ __forceinline__ void visc_flux_tile(const double *__restrict__ Q, double *__restrict__ FV, int nx,
+                                               int ny, int nz,
+                                               const double *__restrict__ dxg,  // ghosted dxinv x|y|z
+                                               const ViscParams &vp, int b, double (&sQ)[4][VHY][VHX]) {
   const size_t ncell = (size_t)nx * ny * nz;
   const size_t plane = (size_t)nx * ny;
   const int ntx = (nx + VTX - 1) / VTX, nty = (ny + VTY - 1) / VTY;
-  const int b = blockIdx.x;
   const int tx = b % ntx, ty = (b / ntx) % nty, kz = b / (ntx * nty);
   const int x0 = tx * VTX, y0 = ty * VTY, k0 = kz * VKS;
   const int lx = threadIdx.x % VTX, ly = threadIdx.x / VTX;
@@ -295,6 +299,13 @@ __global__ void __launch_bounds__(VTX * VTY) k_visc_flux(const double *__restric
   }
 }
 
+__global__ void __launch_bounds__(VTX * VTY) k_visc_flux(const double *__restrict__ Q, double *__restrict__ FV,
+                                                         int nx, int ny, int nz, const double *__restrict__ dxg,
+                                                         ViscParams vp) {
+  __shared__ double sQ[4][VHY][VHX];
+  visc_flux_tile(Q, FV, nx, ny, nz, dxg, vp, blockIdx.x, sQ);
+}
+
 // Viscous flux along DIR at the ghost positions gp in {-2,-1,n,n+1} of every
 // line along DIR (other indices interior), as the reference evaluates it
 // from its ghost layers: primitive values are the periodic copies, the
@@ -306,13 +317,12 @@ __global__ void __launch_bounds__(VTX * VTY) k_visc_flux(const double *__restric
 // other indices are interior.
 // Output FVg[(slot*4 + c)*nplane + line], slot 0..3 for gp = -2,-1,n,n+1.
 template <int DIR>
-__global__ void __launch_bounds__(256) k_visc_ghost(const double *__restrict__ Q, double *__restrict__ FVg,
-                                                    int nx, int ny, int nz, const double *__restrict__ dxg,
-                                                    ViscParams vp) {
+__device__ __forceinline__ void visc_ghost_one(const double *__restrict__ Q, double *__restrict__ FVg, int nx,
+                                               int ny, int nz, const double *__restrict__ dxg, const ViscParams &vp,
+                                               size_t t) {
   const size_t ncell = (size_t)nx * ny * nz;
   const int n = DIR == 0 ? nx : (DIR == 1 ? ny : nz);
   const size_t nplane = ncell / n;
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 4 * nplane) return;
   const int slot = (int)(t / nplane);
   const size_t line = t % nplane;
@@ -357,6 +367,78 @@ __global__ void __launch_bounds__(256) k_visc_ghost(const double *__restrict__ Q
   visc_flux<DIR>(fv, q, gr, mu, vp);
 #pragma unroll
   for (int c = 0; c < 4; c++) FVg[(size_t)(slot * 4 + c) * nplane + line] = fv[c];
+}
+
+template <int DIR>
+__global__ void __launch_bounds__(256) k_visc_ghost(const double *__restrict__ Q, double *__restrict__ FVg,
+                                                    int nx, int ny, int nz, const double *__restrict__ dxg,
+                                                    ViscParams vp) {
+  visc_ghost_one<DIR>(Q, FVg, nx, ny, nz, dxg, vp, (size_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Viscous fluxes of all members of an ensemble: one persistent launch over
+// work items (member, tile) and (member, 256 ghost positions of one
+// direction); item0v holds each member's first item.
+struct EnsViscArgs {
+  const EnsMember *mem;
+  int nmem;
+  const unsigned *item0v;
+  unsigned nitems;
+  ViscParams vp;
+};
+
+__host__ __device__ __forceinline__ unsigned ghost_blocks(size_t ncell, int n) {
+  return (unsigned)((4 * (ncell / n) + 255) / 256);
+}
+
+__global__ void __launch_bounds__(VTX * VTY) k_ens_visc(EnsViscArgs a) {
+  __shared__ double sQ[4][VHY][VHX];
+  for (unsigned item = blockIdx.x; item < a.nitems; item += gridDim.x) {
+    int lo = 0, hi = a.nmem - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (a.item0v[mid] <= item) lo = mid;
+      else hi = mid - 1;
+    }
+    const EnsMember &M = a.mem[lo];
+    const int nx = M.n[0], ny = M.n[1], nz = M.n[2];
+    unsigned r = item - a.item0v[lo];
+    const unsigned tiles = visc_tiles(nx, ny, nz);
+    if (r < tiles) {
+      visc_flux_tile(M.q, M.fv, nx, ny, nz, M.dxg, a.vp, (int)r, sQ);
+      continue;
+    }
+    r -= tiles;
+    const unsigned gx = ghost_blocks(M.ncell, nx), gy = ghost_blocks(M.ncell, ny);
+    const size_t t = threadIdx.x;
+    if (r < gx) visc_ghost_one<0>(M.q, M.fvg[0], nx, ny, nz, M.dxg, a.vp, (size_t)r * 256 + t);
+    else if (r < gx + gy) visc_ghost_one<1>(M.q, M.fvg[1], nx, ny, nz, M.dxg, a.vp, (size_t)(r - gx) * 256 + t);
+    else visc_ghost_one<2>(M.q, M.fvg[2], nx, ny, nz, M.dxg, a.vp, (size_t)(r - gx - gy) * 256 + t);
+  }
+}
+
+// primitive variables of every member's state
+__global__ void __launch_bounds__(256) k_ens_prims(const EnsMember *__restrict__ mem, int nmem,
+                                                   const double *__restrict__ u, size_t ncell, double gamma) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ncell) return;
+  int lo = 0, hi = nmem - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (mem[mid].off <= idx) lo = mid;
+    else hi = mid - 1;
+  }
+  const EnsMember &M = mem[lo];
+  const size_t c = idx - M.off;
+  const double *um = u + 5 * M.off;
+  double s[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) s[v] = um[(size_t)v * M.ncell + c];
+  const Prim q = flowvar(s, gamma);
+  M.q[c] = q.vx;
+  M.q[M.ncell + c] = q.vy;
+  M.q[2 * M.ncell + c] = q.vz;
+  M.q[3 * M.ncell + c] = gamma * q.P / q.rho;
 }
 
 // ---------------------------------------------------------------------------
@@ -557,7 +639,7 @@ size_t visc_ghost_size(const Grid &g, int dir) { return 16 * (g.ncell / g.n[dir]
 void launch_visc_flux(const Grid &g, const Scheme &sc, const double *Q, double *FV, double *const FVg[3],
                       const double *dxg, hipStream_t st) {
   const ViscParams vp = visc_params(sc);
-  const unsigned nb = (unsigned)(((g.n[0] + VTX - 1) / VTX) * ((g.n[1] + VTY - 1) / VTY) * ((g.n[2] + VKS - 1) / VKS));
+  const unsigned nb = visc_tiles(g.n[0], g.n[1], g.n[2]);
   hipLaunchKernelGGL(k_visc_flux, dim3(nb), dim3(VTX * VTY), 0, st, Q, FV, g.n[0], g.n[1], g.n[2], dxg, vp);
   hipLaunchKernelGGL(k_visc_ghost<0>, dim3(nblocks(4 * (g.ncell / g.n[0]), 256)), dim3(256), 0, st, Q,
                      FVg[0], g.n[0], g.n[1], g.n[2], dxg, vp);
@@ -565,6 +647,28 @@ void launch_visc_flux(const Grid &g, const Scheme &sc, const double *Q, double *
                      FVg[1], g.n[0], g.n[1], g.n[2], dxg, vp);
   hipLaunchKernelGGL(k_visc_ghost<2>, dim3(nblocks(4 * (g.ncell / g.n[2]), 256)), dim3(256), 0, st, Q,
                      FVg[2], g.n[0], g.n[1], g.n[2], dxg, vp);
+}
+
+unsigned ens_visc_items_per_member(const int n[3]) {
+  const size_t ncell = (size_t)n[0] * n[1] * n[2];
+  return visc_tiles(n[0], n[1], n[2]) + ghost_blocks(ncell, n[0]) + ghost_blocks(ncell, n[1]) +
+         ghost_blocks(ncell, n[2]);
+}
+
+void launch_ens_visc(const Scheme &sc, const EnsMember *mem, int nmem, const unsigned *item0v, unsigned nitems,
+                     hipStream_t st) {
+  EnsViscArgs a;
+  a.mem = mem;
+  a.nmem = nmem;
+  a.item0v = item0v;
+  a.nitems = nitems;
+  a.vp = visc_params(sc);
+  hipLaunchKernelGGL(k_ens_visc, dim3(nitems), dim3(VTX * VTY), 0, st, a);
+}
+
+void launch_ens_prims(const Scheme &sc, const EnsMember *mem, int nmem, const double *u, size_t ncell,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(k_ens_prims, dim3(nblocks(ncell, 256)), dim3(256), 0, st, mem, nmem, u, ncell, sc.gamma);
 }
 
 void launch_stage(const Grid &g, const Scheme &sc, const double *fIx, const double *fIy,

@@ -635,7 +635,7 @@ struct EnsArgs {
   double gamma, gm1, inv_gm1, eps4;
   const double *U;         // stage states of all members (member m at 5*off)
   double *R, *u, *S, *acc;
-  int mode, stage;
+  int mode, stage, visc;
 };
 
 // the member of a work item and the item's index within the member
@@ -649,7 +649,7 @@ __device__ __forceinline__ int ens_member(const EnsArgs &a, unsigned item) {
   return lo;
 }
 
-__device__ __forceinline__ void ens_setup(const EnsArgs &a, int m, PassConst &k, PassIO &io) {
+__device__ __forceinline__ void ens_setup(const EnsArgs &a, int m, int dir, PassConst &k, PassIO &io) {
   const EnsMember M = a.mem[m];
   for (int d = 0; d < 3; d++) k.n[d] = M.n[d];
   k.ncell = M.ncell;
@@ -661,14 +661,14 @@ __device__ __forceinline__ void ens_setup(const EnsArgs &a, int m, PassConst &k,
   const size_t o = 5 * M.off;
   io.U = a.U + o;
   io.R = a.R + o;
-  io.FV = nullptr;
-  io.FVg = nullptr;
+  io.FV = M.fv;
+  io.FVg = M.fvg[dir];
   io.u = a.u + o;
   io.S = a.S ? a.S + o : nullptr;
   io.acc = a.acc + o;
   io.out = nullptr;
-  io.Q = nullptr;
-  io.visc = 0;
+  io.Q = M.q;
+  io.visc = a.visc;
   io.mode = a.mode;
   io.stage = a.stage;
   const double A_sub[3] = {0.5, 0.5, 1.0}, b[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
@@ -683,7 +683,7 @@ __global__ void __launch_bounds__(64 * LPB) k_ens_x(EnsArgs a) {
     const int m = ens_member(a, item);
     PassConst k;
     PassIO io;
-    ens_setup(a, m, k, io);
+    ens_setup(a, m, 0, k, io);
     pass_x_body<INTERP, UPW, LPB, WENO_JS>(k, io, item - a.item0[m], sh);
     __syncthreads();
   }
@@ -696,7 +696,7 @@ __global__ void __launch_bounds__(64 * MR) k_ens_m(EnsArgs a) {
     const int m = ens_member(a, item);
     PassConst k;
     PassIO io;
-    ens_setup(a, m, k, io);
+    ens_setup(a, m, DIR, k, io);
     pass_m_body<DIR, INTERP, UPW, MR, WENO_JS>(k, io, item - a.item0[m], sh);
     __syncthreads();
   }
@@ -824,6 +824,7 @@ void launch_ens_pass(const Scheme &sc, int dir, int shape, const EnsMember *mem,
   a.acc = acc;
   a.mode = OUT_STAGE;
   a.stage = stage;
+  a.visc = sc.Re > 0;
   if (grid == 0 || grid > nitems) grid = nitems;
   if (sc.interp == INTERP_CHAR) {
     if (sc.upwind == UPWIND_ROE) ens_launch_t<INTERP_CHAR, UPWIND_ROE>(dir, shape, a, grid, st);

@@ -1,7 +1,7 @@
 """Batched ensembles of NS3D grids through the C-ABI (hm_ensemble_*).
 
 Members may have uneven grid sizes and each has its own dt; they share the
-scheme and are inviscid.  The batched device state is one float64 tensor
+scheme, inviscid or viscous.  The batched device state is one float64 tensor
 holding member m's hm-layout state at [5*offset_m, 5*(offset_m + ncell_m)).
 Work runs on torch's current stream.
 """
@@ -18,7 +18,7 @@ from . import lib as _lib
 
 class Ensemble:
     def __init__(self, dims: Sequence[Sequence[int]], interp: str = "components", upwind: str = "rusanov",
-                 gamma: float = 1.4, eps: float = 1e-6):
+                 gamma: float = 1.4, eps: float = 1e-6, Re: float = -1.0, Minf: float = 1.0, Pr: float = 0.72):
         if not torch.cuda.is_available():
             raise RuntimeError("Ensemble needs a GPU (torch.cuda.is_available() is False)")
         self.L = _lib.load()
@@ -27,7 +27,7 @@ class Ensemble:
         cfgs = (_lib.HMConfig * n)()
         for c, d in zip(cfgs, self.dims):
             c.dim[:] = list(d)
-            c.gamma, c.Re, c.Minf, c.Pr = gamma, -1.0, 1.0, 0.72
+            c.gamma, c.Re, c.Minf, c.Pr = gamma, Re, Minf, Pr
             c.interp = _lib.INTERP[interp]
             c.upwind = _lib.UPWIND[upwind]
             c.weno_eps = eps
@@ -88,6 +88,22 @@ class Ensemble:
         self._stream()
         _lib.check(self.L.hm_ensemble_rk4(self.h, ctypes.c_void_p(u.data_ptr()), int(nsteps)))
         return u
+
+    # -- batched discrete adjoint ---------------------------------------
+    def adjoint_forward(self, u: torch.Tensor, nsteps: int, every: int = 10) -> torch.Tensor:
+        """Advance u nsteps RK4 steps in place, checkpointing every `every` steps."""
+        self._check(u)
+        self._stream()
+        _lib.check(self.L.hm_ensemble_adjoint_forward(self.h, ctypes.c_void_p(u.data_ptr()), int(nsteps), int(every)))
+        self.tape_id = getattr(self, "tape_id", 0) + 1
+        return u
+
+    def adjoint_reverse(self, lam: torch.Tensor) -> torch.Tensor:
+        """lam: dJ/du_N of all members in, dJ/du_0 out (in place)."""
+        self._check(lam)
+        self._stream()
+        _lib.check(self.L.hm_ensemble_adjoint_reverse(self.h, ctypes.c_void_p(lam.data_ptr())))
+        return lam
 
     def set_tiles(self, shape: int, grid=(0, 0, 0)):
         g = (ctypes.c_uint * 3)(*grid)

@@ -80,6 +80,8 @@ SIGNATURES = [
     ("hm_ensemble_set_stream", [_VP, _VP], ctypes.c_int),
     ("hm_ensemble_set_dt", [_VP, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     ("hm_ensemble_rk4", [_VP, _DP, ctypes.c_int], ctypes.c_int),
+    ("hm_ensemble_adjoint_forward", [_VP, _DP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    ("hm_ensemble_adjoint_reverse", [_VP, _DP], ctypes.c_int),
     ("hm_ensemble_set_tiles", [_VP, ctypes.c_int, ctypes.POINTER(ctypes.c_uint)], ctypes.c_int),
     ("hm_ensemble_get_tiles", [_VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint)], ctypes.c_int),
     ("hm_ensemble_tune", [_VP, _DP, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint)],

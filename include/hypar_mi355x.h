@@ -163,8 +163,9 @@ hm_status hm_fill(hm_solver *s, double *x, double value, size_t n);
 
 /* ---- discrete adjoint ------------------------------------------------
  * No counterpart in the reference (it has no adjoint).  Reverse-mode
- * derivatives of this library's own discretisation; built for the inviscid
- * component-wise WENO5-JS + Rusanov scheme with grid sides <= 512
+ * derivatives of this library's own discretisation, inviscid or viscous;
+ * built for WENO5-JS weights with component-wise Rusanov (grid sides <= 512)
+ * or characteristic-wise Roe or Rusanov upwinding (grid sides <= 256)
  * (HM_ERR_UNSUPPORTED otherwise).  lam, out: hm-layout device arrays. */
 /* out = (d rhs / d u)^T lam at state u */
 hm_status hm_rhs_vjp(hm_solver *s, double *out, const double *u, const double *lam);
@@ -199,9 +200,11 @@ hm_status hm_profile_kernels(hm_solver *s, const double *u, int reps, double *ms
  * sizes and each with its own dt, advance together: every direction pass
  * of an RK stage is one persistent launch over the (member, tile) work
  * items of all members, and one RK4 step is a captured HIP graph replayed
- * per step.  Members share the scheme (interpolation, upwinding, gamma,
- * eps) and are inviscid.  The batched state holds member m's hm-layout
- * state at u[5*offset_m .. 5*(offset_m + ncell_m)).
+ * per step; viscous members add one launch per stage over the (member,
+ * tile) items of the viscous fluxes.  Members share the scheme
+ * (interpolation, upwinding, gamma, eps, Re, Minf, Pr; WENO5-JS weights).
+ * The batched state holds member m's hm-layout state at
+ * u[5*offset_m .. 5*(offset_m + ncell_m)).
  * ------------------------------------------------------------------------ */
 typedef struct hm_ensemble hm_ensemble;
 
@@ -225,6 +228,15 @@ hm_status hm_ensemble_get_tiles(const hm_ensemble *e, int *shape, unsigned *grid
  * shape and grid size on a scratch copy of u (u is not modified), keeps
  * the fastest and returns it. */
 hm_status hm_ensemble_tune(hm_ensemble *e, const double *u, int reps, int *shape, unsigned *grid);
+/* Batched discrete adjoint (no counterpart in the reference): the adjoint
+ * of hm_adjoint_forward / hm_adjoint_reverse for all members at once, each
+ * with its own dt.  The forward sweep advances the batched state nsteps RK4
+ * steps in place with a checkpoint every `checkpoint_every` steps; the
+ * reverse sweep takes lam = dJ/du_N of all members and returns dJ/du_0.
+ * Every VJP launch of a stage covers the (member, line) work items of all
+ * members.  Built for the schemes hm_rhs_vjp supports. */
+hm_status hm_ensemble_adjoint_forward(hm_ensemble *e, double *u, int nsteps, int checkpoint_every);
+hm_status hm_ensemble_adjoint_reverse(hm_ensemble *e, double *lam);
 
 #ifdef __cplusplus
 }

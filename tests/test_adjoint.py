@@ -1,4 +1,5 @@
-"""Discrete adjoint of the inviscid WENO5-JS + Rusanov RK4 step.
+"""Discrete adjoint of the WENO5-JS RK4 step: component-wise Rusanov and
+characteristic-wise Roe/Rusanov, inviscid and viscous.
 
 The reference has no adjoint, so the checks are (a) the FP64 CPU reference
 gradient: torch.autograd through oracle/torch_ref.py, whose forward is pinned
@@ -18,27 +19,37 @@ from oracle.oracle import Oracle
 DIMS = (12, 10, 14)
 
 
-def _small():
-    w = dataclasses.replace(decks.config0(12, 1), dims=DIMS, dt=None)
-    u0, dt = decks.prepare(w)
-    dx = [torch.tensor(decks.dxinv_1d(decks.grid_1d(n))[3:-3]) for n in DIMS]
-    return w, u0, dt, dx
-
-
 # ------------------------------------------------------------------ CPU
-def test_torch_reference_matches_oracle():
-    w, u0, dt, dx = _small()
+SCHEMES = [("components", "rusanov", -1.0), ("characteristic", "roe", -1.0), ("characteristic", "rusanov", -1.0),
+           ("components", "roe", -1.0), ("characteristic", "roe", 1000.0), ("components", "rusanov", 1000.0)]
+
+
+def _scheme_case(interp, upwind, Re, dims=DIMS):
+    w = dataclasses.replace(decks.config1(12, 1), dims=dims, dt=None, interp=interp, upwind=upwind, Re=Re)
+    u0, dt = decks.prepare(w)
+    dx = [torch.tensor(decks.dxinv_1d(decks.grid_1d(n))[3:-3]) for n in dims]
+    scheme = dict(interp=interp, upwind=upwind, Re=Re / w.Minf if Re > 0 else -1.0, Pr=w.Pr)
+    return w, u0, dt, dx, scheme
+
+
+@pytest.mark.parametrize("interp,upwind,Re", SCHEMES)
+def test_torch_reference_matches_oracle(interp, upwind, Re):
+    w, u0, dt, dx, sch = _scheme_case(interp, upwind, Re)
     o = Oracle.for_workload(w)
-    r = T.to_aos(T.rhs(T.to_soa(u0), dx))
+    r = T.to_aos(T.rhs(T.to_soa(u0), dx, **sch))
     assert decks.rel_l2(r, o.rhs(u0)) < 1e-13
-    u2 = T.to_aos(T.rk4_step(T.rk4_step(T.to_soa(u0), dt, dx), dt, dx))
+    if Re > 0:
+        par = T.to_aos(T.parabolic(T.to_soa(u0), dx, sch["Re"], sch["Pr"]))
+        assert decks.rel_l2(par, o.parabolic(u0)) < 1e-14
+    u2 = T.to_aos(T.rk4_step(T.rk4_step(T.to_soa(u0), dt, dx, **sch), dt, dx, **sch))
     assert decks.rel_l2(u2, o.rk4(u0, dt, 2)) < 1e-14
 
 
-def test_torch_reference_gradient_matches_finite_differences():
-    w, u0, dt, dx = _small()
+@pytest.mark.parametrize("interp,upwind,Re", [("components", "rusanov", -1.0), ("characteristic", "roe", 1000.0)])
+def test_torch_reference_gradient_matches_finite_differences(interp, upwind, Re):
+    w, u0, dt, dx, sch = _scheme_case(interp, upwind, Re)
     x = T.to_soa(u0).requires_grad_(True)
-    J = T.kinetic_energy(T.rk4_step(x, dt, dx), dx)
+    J = T.kinetic_energy(T.rk4_step(x, dt, dx, **sch), dx)
     (g,) = torch.autograd.grad(J, x)
     rng = np.random.default_rng(4)
     for _ in range(3):
@@ -46,17 +57,20 @@ def test_torch_reference_gradient_matches_finite_differences():
         d /= d.norm()
         h = 1e-5
         with torch.no_grad():
-            jp = T.kinetic_energy(T.rk4_step(x + h * d, dt, dx), dx)
-            jm = T.kinetic_energy(T.rk4_step(x - h * d, dt, dx), dx)
+            jp = T.kinetic_energy(T.rk4_step(x + h * d, dt, dx, **sch), dx)
+            jm = T.kinetic_energy(T.rk4_step(x - h * d, dt, dx, **sch), dx)
         fd = float((jp - jm) / (2 * h))
         ad = float((g * d).sum())
         assert abs(fd - ad) <= 1e-6 * abs(ad)
 
 
 # ------------------------------------------------------------------ GPU
-def _gpu_solver(dims):
+VJP_SCHEMES = [c for c in SCHEMES if not (c[0] == "components" and c[1] == "roe")]
+
+
+def _gpu_solver(w):
     from hypar_amd.solver import NS3D
-    return NS3D(dims, "components", "rusanov")
+    return NS3D.from_workload(w)
 
 
 def _hm(x):
@@ -64,52 +78,90 @@ def _hm(x):
     return x.detach().reshape(-1).contiguous().cuda()
 
 
+def _rel(got, ref):
+    return float((got - ref).norm() / ref.norm())
+
+
 @pytest.mark.gpu
-def test_rhs_vjp_matches_autograd():
-    w, u0, dt, dx = _small()
+@pytest.mark.parametrize("interp,upwind,Re", VJP_SCHEMES)
+def test_rhs_vjp_matches_autograd(interp, upwind, Re):
+    w, u0, dt, dx, sch = _scheme_case(interp, upwind, Re)
     x = T.to_soa(u0).requires_grad_(True)
     lam = torch.tensor(np.random.default_rng(1).normal(size=x.shape))
-    (ref,) = torch.autograd.grad(T.rhs(x, dx), x, lam)
-    s = _gpu_solver(DIMS)
+    (ref,) = torch.autograd.grad(T.rhs(x, dx, **sch), x, lam)
+    print(f"[gpu] rhs vjp {interp}/{upwind} Re={Re} dims={DIMS}", flush=True)
+    s = _gpu_solver(w)
     got = s.rhs_vjp(_hm(x), _hm(lam)).cpu().view_as(ref)
-    err = float((got - ref).norm() / ref.norm())
-    print(f"[gpu] rhs vjp rel err {err:.2e}", flush=True)
+    err = _rel(got, ref)
+    print(f"[gpu]   rel err {err:.2e}", flush=True)
     assert err <= 1e-12
 
 
 @pytest.mark.gpu
-def test_rk4_step_vjp_matches_autograd():
-    w, u0, dt, dx = _small()
+@pytest.mark.parametrize("interp,upwind,Re", [("characteristic", "roe", 1000.0), ("characteristic", "rusanov", -1.0)])
+def test_rhs_vjp_on_ragged_lines(interp, upwind, Re):
+    """Sides of 5 and 7 (stencils wrap more than once around a line) and a
+    side above one wavefront."""
+    for dims in [(5, 7, 6), (70, 5, 9)]:
+        w, u0, dt, dx, sch = _scheme_case(interp, upwind, Re, dims)
+        x = T.to_soa(u0).requires_grad_(True)
+        lam = torch.tensor(np.random.default_rng(7).normal(size=x.shape))
+        (ref,) = torch.autograd.grad(T.rhs(x, dx, **sch), x, lam)
+        print(f"[gpu] rhs vjp {interp}/{upwind} Re={Re} dims={dims}", flush=True)
+        s = _gpu_solver(w)
+        err = _rel(s.rhs_vjp(_hm(x), _hm(lam)).cpu().view_as(ref), ref)
+        print(f"[gpu]   rel err {err:.2e}", flush=True)
+        assert err <= 1e-12
+
+
+@pytest.mark.gpu
+def test_vjp_rejects_unsupported_schemes():
+    from hypar_amd import lib
+    from hypar_amd.solver import NS3D
+    for kw in [dict(interp="components", upwind="roe"), dict(weno_type="mapped")]:
+        s = NS3D(DIMS, **kw)
+        u = torch.ones(5 * s.ncell, dtype=torch.float64, device="cuda")
+        with pytest.raises(lib.HMError) as ex:
+            s.rhs_vjp(u, u.clone())
+        assert ex.value.code == lib.HM_ERR_UNSUPPORTED
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("interp,upwind,Re", VJP_SCHEMES)
+def test_rk4_step_vjp_matches_autograd(interp, upwind, Re):
+    w, u0, dt, dx, sch = _scheme_case(interp, upwind, Re)
     x = T.to_soa(u0).requires_grad_(True)
     lam = torch.tensor(np.random.default_rng(2).normal(size=x.shape))
-    (ref,) = torch.autograd.grad(T.rk4_step(x, dt, dx), x, lam)
-    s = _gpu_solver(DIMS)
+    (ref,) = torch.autograd.grad(T.rk4_step(x, dt, dx, **sch), x, lam)
+    print(f"[gpu] rk4 step vjp {interp}/{upwind} Re={Re}", flush=True)
+    s = _gpu_solver(w)
     g = _hm(lam)
     s.rk4_vjp(_hm(x), dt, g)
-    err = float((g.cpu().view_as(ref) - ref).norm() / ref.norm())
-    print(f"[gpu] rk4 step vjp rel err {err:.2e}", flush=True)
+    err = _rel(g.cpu().view_as(ref), ref)
+    print(f"[gpu]   rel err {err:.2e}", flush=True)
     assert err <= 1e-12
 
 
 @pytest.mark.gpu
-def test_checkpointed_trajectory_gradient_matches_autograd():
+@pytest.mark.parametrize("interp,upwind,Re", [("components", "rusanov", -1.0), ("characteristic", "roe", 1000.0)])
+def test_checkpointed_trajectory_gradient_matches_autograd(interp, upwind, Re):
     """7 steps with a checkpoint every 3 (segments 3, 3, 1) against torch
     autograd through the unrolled CPU trajectory."""
-    from hypar_amd.adjoint import trajectory
-    w, u0, dt, dx = _small()
+    from hypar_amd.adjoint import kinetic_energy, trajectory
+    w, u0, dt, dx, sch = _scheme_case(interp, upwind, Re)
     x = T.to_soa(u0).requires_grad_(True)
     y = x
     for _ in range(7):
-        y = T.rk4_step(y, dt, dx)
+        y = T.rk4_step(y, dt, dx, **sch)
     (ref,) = torch.autograd.grad(T.kinetic_energy(y, dx), x)
-    s = _gpu_solver(DIMS)
+    print(f"[gpu] trajectory gradient {interp}/{upwind} Re={Re}", flush=True)
+    s = _gpu_solver(w)
     xg = _hm(x).requires_grad_(True)
     yN = trajectory(xg, s, dt, 7, 3)
     assert decks.rel_l2(yN.detach().cpu().numpy(), y.detach().reshape(-1).numpy()) < 1e-12
-    from hypar_amd.adjoint import kinetic_energy
-    (got,) = torch.autograd.grad(kinetic_energy(yN, DIMS), xg)
-    err = float((got.cpu().view_as(ref) - ref).norm() / ref.norm())
-    print(f"[gpu] trajectory gradient rel err {err:.2e}", flush=True)
+    (got,) = torch.autograd.grad(kinetic_energy(yN, w.dims), xg)
+    err = _rel(got.cpu().view_as(ref), ref)
+    print(f"[gpu]   rel err {err:.2e}", flush=True)
     assert err <= 1e-11
 
 

@@ -1,5 +1,6 @@
 """Batched ensembles (hm_ensemble_*): members of uneven sizes, each with its
-own dt, advanced together by persistent launches replayed from a HIP graph.
+own dt, inviscid or viscous, advanced together by persistent launches
+replayed from a HIP graph.
 
 Checks: every member against the single-grid path (the same device
 arithmetic, so agreement is at rounding level or exact), against the
@@ -21,12 +22,12 @@ TOL = 1e-12
 DIMS = [(17, 13, 11), (32, 48, 64), (5, 5, 5), (64, 32, 48), (70, 7, 9), (6, 66, 5)]
 
 
-def _members(dims, interp="components", upwind="rusanov"):
+def _members(dims, interp="components", upwind="rusanov", Re=-1.0):
     from hypar_amd import decks
     ws, u0s, dts = [], [], []
     for m, d in enumerate(dims):
         w = dataclasses.replace(decks.config0(8, 1), dims=tuple(d), interp=interp, upwind=upwind,
-                                seed=20262 * 1000 + m, dt=None)
+                                seed=20262 * 1000 + m, dt=None, Re=Re, Minf=0.1 if Re > 0 else 1.0)
         u0, dt = decks.prepare(w)
         ws.append(w)
         u0s.append(u0)
@@ -44,13 +45,14 @@ def _single(w, u0, dt, nsteps):
     return out
 
 
-@pytest.mark.parametrize("interp,upwind", [("components", "rusanov"), ("characteristic", "roe")])
-def test_members_match_single_grid(interp, upwind):
+@pytest.mark.parametrize("interp,upwind,Re", [("components", "rusanov", -1.0), ("characteristic", "roe", -1.0),
+                                               ("characteristic", "roe", 1000.0)])
+def test_members_match_single_grid(interp, upwind, Re):
     from hypar_amd import decks
     from hypar_amd.ensemble import Ensemble
-    ws, u0s, dts = _members(DIMS, interp, upwind)
-    print(f"[gpu] ensemble {interp}/{upwind} dims={DIMS}", flush=True)
-    e = Ensemble(DIMS, interp, upwind)
+    ws, u0s, dts = _members(DIMS, interp, upwind, Re)
+    print(f"[gpu] ensemble {interp}/{upwind} Re={Re} dims={DIMS}", flush=True)
+    e = Ensemble(DIMS, interp, upwind, Re=Re, Minf=0.1 if Re > 0 else 1.0)
     u = e.to_device(u0s)
     e.set_dt(dts)
     e.rk4(u, 3)
@@ -128,11 +130,64 @@ def test_graph_replays_between_eager_work():
             assert err <= 1e-14, (rep, m, err)
 
 
+def test_viscous_reference_fixtures():
+    """Viscous characteristic/Roe members, each with its own dt, against the
+    reference's output for the same decks: member 0 after one step, member 1
+    (a ragged grid) after three."""
+    from hypar_amd import decks
+    from hypar_amd.ensemble import Ensemble
+    fx = [load_fixture("c1_16_s1"), load_fixture("c1_ragged_s3")]
+    dims = [tuple(meta["dims"]) for meta, _ in fx]
+    m0 = fx[0][0]
+    e = Ensemble(dims, "characteristic", "roe", Re=m0["Re"], Minf=m0["Minf"], Pr=m0["Pr"])
+    u = e.to_device([z["u0"] for _, z in fx])
+    e.set_dt([meta["dt"] for meta, _ in fx])
+    e.rk4(u, 1)
+    err = decks.rel_l2(e.to_host(u, 0), fx[0][1]["uN"])
+    print(f"[gpu] viscous ensemble member 0 {dims[0]} after 1 step: rel L2 vs reference {err:.2e}", flush=True)
+    assert err <= TOL
+    e.rk4(u, 2)
+    err = decks.rel_l2(e.to_host(u, 1), fx[1][1]["uN"])
+    print(f"[gpu] viscous ensemble member 1 {dims[1]} after 3 steps: rel L2 vs reference {err:.2e}", flush=True)
+    assert err <= TOL
+
+
+@pytest.mark.parametrize("interp,upwind,Re", [("components", "rusanov", -1.0), ("characteristic", "roe", 1000.0)])
+def test_ensemble_adjoint_matches_single_grid(interp, upwind, Re):
+    """Batched trajectory gradient (5 steps, checkpoint every 2, each member
+    with its own dt) against each member's single-grid adjoint, which
+    test_adjoint.py checks against torch autograd."""
+    from hypar_amd import decks
+    from hypar_amd.adjoint import ensemble_trajectory, kinetic_energy, trajectory
+    from hypar_amd.ensemble import Ensemble
+    from hypar_amd.solver import NS3D
+    dims = [(12, 10, 14), (16, 12, 10), (10, 14, 12)]
+    ws, u0s, dts = _members(dims, interp, upwind, Re)
+    print(f"[gpu] ensemble adjoint {interp}/{upwind} Re={Re} dims={dims}", flush=True)
+    e = Ensemble(dims, interp, upwind, Re=Re, Minf=0.1 if Re > 0 else 1.0)
+    e.set_dt(dts)
+    u0 = e.to_device(u0s).requires_grad_(True)
+    uN = ensemble_trajectory(u0, e, 5, 2)
+    J = sum(kinetic_energy(e.member(uN, m), dims[m]) for m in range(len(dims)))
+    (g,) = torch.autograd.grad(J, u0)
+    for m, w in enumerate(ws):
+        s = NS3D.from_workload(w)
+        x = s.to_device(u0s[m]).requires_grad_(True)
+        y = trajectory(x, s, dts[m], 5, 2)
+        assert decks.rel_l2(e.member(uN, m).detach().cpu().numpy(), y.detach().cpu().numpy()) <= 1e-14
+        (gs,) = torch.autograd.grad(kinetic_energy(y, w.dims), x)
+        err = float((e.member(g, m) - gs).norm() / gs.norm())
+        print(f"[gpu]   member {m} {w.dims}: gradient rel err vs single grid {err:.2e}", flush=True)
+        assert err <= 1e-12
+        s.close()
+
+
 def test_rejections():
     from hypar_amd import lib
     from hypar_amd.ensemble import Ensemble
     with pytest.raises(lib.HMError):
         Ensemble([(4, 8, 8)])
+
     e = Ensemble([(8, 8, 8)])
     u = torch.zeros(5 * 512, dtype=torch.float64, device="cuda")
     with pytest.raises(lib.HMError) as ex:
