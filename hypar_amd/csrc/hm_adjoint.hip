@@ -377,7 +377,7 @@ __device__ __forceinline__ void rhs_vjp_line(const double *__restrict__ u, const
 constexpr int VJP_CHAR_LINE_MAX = 256;
 
 struct RoeAdj {
-  double v[3], vn, a, a2, ek, h0;
+  double v[3], vn, a, a2, ia2, ek, h0;
 };
 
 template <int DIR>
@@ -396,6 +396,7 @@ __device__ __forceinline__ RoeAdj roe_adj_state(const double uL[NV], const doubl
   s.ek = 0.5 * (s.v[0] * s.v[0] + s.v[1] * s.v[1] + s.v[2] * s.v[2]);
   s.a2 = (H - s.ek) * gm1;
   s.a = sqrt(s.a2);
+  s.ia2 = 1.0 / s.a2;
   s.vn = s.v[DIR];
   s.h0 = s.a2 / gm1 + s.ek;
   return s;
@@ -406,7 +407,7 @@ __device__ __forceinline__ RoeAdj roe_adj_state(const double uL[NV], const doubl
 template <int DIR>
 __device__ __forceinline__ void l_row(const RoeAdj &s, double gm1, int k, double r[NV]) {
   if (k <= 2) {
-    const double ia2 = 1.0 / s.a2;
+    const double ia2 = s.ia2;
     if (k == 0) {
       r[0] = (s.a2 - gm1 * s.ek) * ia2;
       for (int j = 0; j < 3; j++) r[1 + j] = gm1 * s.v[j] * ia2;
@@ -436,7 +437,7 @@ __device__ __forceinline__ void l_row(const RoeAdj &s, double gm1, int k, double
 template <int DIR>
 __device__ __forceinline__ void l_row_vjp(const RoeAdj &s, double gm1, int k, const double X[NV], double gv[3],
                                           double &ab) {
-  const double a = s.a, ia2 = 1.0 / s.a2, ia3 = ia2 / a;
+  const double a = s.a, ia2 = s.ia2, ia = a * ia2, ia3 = ia2 * ia;
   const double vX = s.v[0] * X[1] + s.v[1] * X[2] + s.v[2] * X[3];
   if (k == 0) {
     for (int j = 0; j < 3; j++) gv[j] += gm1 * ia2 * (X[1 + j] - X[0] * s.v[j]);
@@ -445,7 +446,7 @@ __device__ __forceinline__ void l_row_vjp(const RoeAdj &s, double gm1, int k, co
     const double sg = k == 1 ? 1.0 : -1.0, h = 0.5 * ia2;
     for (int j = 0; j < 3; j++) gv[j] += gm1 * h * (X[0] * s.v[j] - X[1 + j]);
     gv[DIR] += sg * a * h * X[0];
-    ab += -X[0] * (2.0 * gm1 * s.ek + sg * a * s.vn) * h / a + gm1 * ia3 * (vX - X[4]) + sg * h * X[1 + DIR];
+    ab += -X[0] * (2.0 * gm1 * s.ek + sg * a * s.vn) * h * ia + gm1 * ia3 * (vX - X[4]) + sg * h * X[1 + DIR];
   } else if (DIR == 0) {
     if (k == 3) gv[1] += X[0]; else gv[2] -= X[0];
   } else if (DIR == 1) {
@@ -1006,7 +1007,8 @@ template <int DIR>
 __global__ void __launch_bounds__(VJP_LINE_MAX) k_ens_rhs_vjp(EnsVjp a, const unsigned *__restrict__ first,
                                                           unsigned nitems) {
   extern __shared__ double smem[];
-  for (unsigned item = blockIdx.x; item < nitems; item += gridDim.x) {
+  const XcdItems r = xcd_items(nitems);
+  for (unsigned item = r.first; item < r.end; item += r.step) {
     const int m = ens_find(first, a.nmem, item);
     const EnsMember M = a.mem[m];
     rhs_vjp_line<DIR>(ENS_MEMBER_ARGS(a, M), item - first[m], smem);
@@ -1018,7 +1020,8 @@ template <int DIR, int UPW>
 __global__ void __launch_bounds__(VJP_CHAR_LINE_MAX) k_ens_rhs_vjp_char(EnsVjp a, const unsigned *__restrict__ first,
                                                                     unsigned nitems) {
   extern __shared__ double smem[];
-  for (unsigned item = blockIdx.x; item < nitems; item += gridDim.x) {
+  const XcdItems r = xcd_items(nitems);
+  for (unsigned item = r.first; item < r.end; item += r.step) {
     const int m = ens_find(first, a.nmem, item);
     const EnsMember M = a.mem[m];
     rhs_vjp_char_line<DIR, UPW>(ENS_MEMBER_ARGS(a, M), item - first[m], smem);

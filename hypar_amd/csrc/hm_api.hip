@@ -378,6 +378,25 @@ hm_status hm_cfl(hm_solver *s, const double *u, double dt, double *cfl) {
   return HM_OK;
 }
 
+hm_status hm_volume_integral(hm_solver *s, const double *u, double *out) {
+  hm_status st = check_state(s, u, "hm_volume_integral: u");
+  if (st != HM_OK) return st;
+  if (!out) return fail(HM_ERR_ARG, "hm_volume_integral: out is NULL");
+  unsigned nb = hm::reduction_blocks(s->g.ncell);
+  if (5 * nb > s->npart) nb = s->npart / 5 > 0 ? s->npart / 5 : 1;
+  hm::launch_volint(s->g, u, s->part, nb, s->dxg, s->stream);
+  if ((st = check_launch("hm_volume_integral")) != HM_OK) return st;
+  HM_HIP(hipMemcpyAsync(s->part_host, s->part, 5 * nb * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HM_HIP(hipStreamSynchronize(s->stream));
+  for (int v = 0; v < 5; v++) {
+    double sum = 0.0;
+    for (unsigned b = 0; b < nb; b++) sum += s->part_host[(size_t)v * nb + b];
+    out[v] = sum;
+    if (!std::isfinite(sum)) return fail(HM_ERR_NONFINITE, "hm_volume_integral: state holds NaN/Inf");
+  }
+  return HM_OK;
+}
+
 hm_status hm_step_norm(hm_solver *s, const double *a, const double *b, double *norm) {
   hm_status st = check_state(s, a, "hm_step_norm: a");
   if (st != HM_OK) return st;

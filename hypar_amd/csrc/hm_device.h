@@ -226,6 +226,22 @@ __device__ __forceinline__ size_t xcd_block() {
   return (size_t)((x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k);
 }
 
+// Persistent blocks striding over nitems work items: XCD x of the 8 takes a
+// contiguous range of the items, which its blocks stride through, so items
+// that share cache lines (neighbouring grid lines) meet in one L2.  Needs
+// gridDim.x >= min(nitems, 8).
+struct XcdItems {
+  unsigned first, end, step;
+};
+__device__ __forceinline__ XcdItems xcd_items(unsigned nitems) {
+  const unsigned b = blockIdx.x, nb = gridDim.x, x = b & 7u, k = b >> 3;
+  const unsigned qi = nitems >> 3, ri = nitems & 7u;
+  const unsigned first = x < ri ? x * (qi + 1) : ri * (qi + 1) + (x - ri) * qi;
+  const unsigned len = qi + (x < ri ? 1u : 0u);
+  const unsigned qb = nb >> 3, rb = nb & 7u;
+  return {first + k, first + len, qb + (x < rb ? 1u : 0u)};
+}
+
 // 4th-order central difference (FirstDerivativeFourthOrder.c:103)
 __device__ __forceinline__ double d4c(double fm2, double fm1, double fp1, double fp2) {
   return (fm2 - 8 * fm1 + 8 * fp1 - fp2) * (1.0 / 12.0);

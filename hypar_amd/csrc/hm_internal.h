@@ -42,6 +42,8 @@ void launch_stage(const Grid &g, const Scheme &sc, const double *fIx, const doub
                   double *acc, double *out, int mode, const double *dxg, int stage, double ca,
                   double cb, hipStream_t st);
 unsigned reduction_blocks(size_t n);
+// volume integral partials: part[v*nb + block], 5*nb entries
+void launch_volint(const Grid &g, const double *u, double *part, unsigned nb, const double *dxg, hipStream_t st);
 void launch_cfl(const Grid &g, const Scheme &sc, const double *u, double *part, unsigned nb,
                 const double *dxg, double dt, hipStream_t st);
 void launch_sqdiff(const double *a, const double *b, double *part, unsigned nb, size_t n,

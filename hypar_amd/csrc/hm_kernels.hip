@@ -680,9 +680,48 @@ void launch_stage(const Grid &g, const Scheme &sc, const double *fIx, const doub
                      out, mode, g.n[0], g.n[1], g.n[2], dxg, sc.Re > 0 ? 1 : 0, stage, ca, cb);
 }
 
+// Volume integral of every conserved variable: sum of u dV over the
+// interior cells with dV = 1/dxinv_x 1/dxinv_y 1/dxinv_z
+// (src/HyParFunctions/VolumeIntegral.c:14-50); one partial per block and
+// variable, part[v*gridDim.x + block].
+__global__ void __launch_bounds__(256) k_volint(const double *__restrict__ u, double *__restrict__ part, int nx,
+                                                int ny, int nz, const double *__restrict__ dxg) {
+  const size_t ncell = (size_t)nx * ny * nz;
+  __shared__ double red[NV][4];
+  double acc[NV] = {0, 0, 0, 0, 0};
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < ncell;
+       idx += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(idx % nx);
+    const int j = (int)((idx / nx) % ny);
+    const int k = (int)(idx / ((size_t)nx * ny));
+    double dV = 1.0;
+    dV *= 1.0 / dxg[3 + i];
+    dV *= 1.0 / dxg[(nx + 6) + 3 + j];
+    dV *= 1.0 / dxg[(nx + 6) + (ny + 6) + 3 + k];
+#pragma unroll
+    for (int v = 0; v < NV; v++) acc[v] += u[(size_t)v * ncell + idx] * dV;
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    const double t = wave_sum(acc[v]);
+    if (lane == 0) red[v][w] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double r = 0.0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); q++) r += red[threadIdx.x][q];
+    part[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = r;
+  }
+}
+
 unsigned reduction_blocks(size_t n) {
   unsigned b = nblocks(n, 256);
   return b > 2048 ? 2048 : (b == 0 ? 1 : b);
+}
+
+void launch_volint(const Grid &g, const double *u, double *part, unsigned nb, const double *dxg, hipStream_t st) {
+  hipLaunchKernelGGL(k_volint, dim3(nb), dim3(256), 0, st, u, part, g.n[0], g.n[1], g.n[2], dxg);
 }
 
 void launch_cfl(const Grid &g, const Scheme &sc, const double *u, double *part, unsigned nb,

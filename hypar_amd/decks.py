@@ -275,7 +275,7 @@ def _weno_flags(weno_type: str):
 
 def write_solver_inp(path: str, dims_list, n_iter: int, dt: float, interp: str,
                      screen_op_iter: int = 1, file_op_iter: Optional[int] = None,
-                     time_scheme_type: str = "44"):
+                     time_scheme_type: str = "44", conservation_check: str = "no"):
     size = " ".join(" ".join(str(n) for n in d) for d in dims_list)
     with open(os.path.join(path, "solver.inp"), "w") as f:
         f.write("begin\n")
@@ -293,7 +293,7 @@ def write_solver_inp(path: str, dims_list, n_iter: int, dt: float, interp: str,
         f.write("  par_space_type      nonconservative-2stage\n")
         f.write("  par_space_scheme    4\n")
         f.write(f"  dt                  {dt:.17g}\n")
-        f.write("  conservation_check  no\n")
+        f.write(f"  conservation_check  {conservation_check}\n")
         f.write(f"  screen_op_iter      {screen_op_iter}\n")
         f.write(f"  file_op_iter        {file_op_iter if file_op_iter else n_iter}\n")
         f.write("  ip_file_type        binary\n")
@@ -351,7 +351,7 @@ def write_initial(path: str, dims, u: np.ndarray, fname: str = "initial.inp"):
 
 
 def write_deck(path: str, w: Workload, u0: Optional[np.ndarray] = None,
-               screen_op_iter: int = 1) -> np.ndarray:
+               screen_op_iter: int = 1, conservation_check: str = "no") -> np.ndarray:
     """Write a complete single-simulation deck; returns the initial state."""
     os.makedirs(path, exist_ok=True)
     if u0 is None:
@@ -359,7 +359,7 @@ def write_deck(path: str, w: Workload, u0: Optional[np.ndarray] = None,
     elif w.dt is None:
         w.dt = cfl_dt(u0, w.dims, w.gamma, w.cfl)
     write_solver_inp(path, [w.dims], w.n_iter, w.dt, w.interp, screen_op_iter,
-                     time_scheme_type=w.time_scheme_type)
+                     time_scheme_type=w.time_scheme_type, conservation_check=conservation_check)
     write_physics_inp(path, w)
     write_weno_inp(path, w)
     write_boundary_inp(path)

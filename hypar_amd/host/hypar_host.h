@@ -60,6 +60,10 @@ typedef struct HyPar {
   NavierStokes3D *physics;
   WENOParameters *interp;
   int nBoundaryZones, all_periodic;
+  /* conservation check (include/hypar.h:376-392) and call counts */
+  double VolumeIntegral[5], VolumeIntegralInitial[5], TotalBoundaryIntegral[5];
+  double StepBoundaryIntegral[30], ConservationError[5];
+  int count_hyp, count_par, count_sou;
   /* slots (reference signatures) */
   int (*HyperbolicFunction)(double *, double *, void *, void *, double, int, hy_flux_fn, hy_upwind_fn);
   int (*ParabolicFunction)(double *, double *, void *, void *, double);
@@ -126,6 +130,13 @@ int WENOFifthOrderCalculateWeights(double *fC, double *uC, double *x, int dir, v
 int HyperbolicFunction(double *hyp, double *u, void *s, void *m, double t, int LimFlag,
                        hy_flux_fn FFunction, hy_upwind_fn UpwindFunction);
 int SourceFunction(double *source, double *u, void *s, void *m, double t);
+
+/* ---- conservation check and error files ---- */
+int VolumeIntegral(double *VolumeIntegral, double *u, void *s, void *m);
+int BoundaryIntegral(void *s, void *m);
+int CalculateConservationError(void *s, void *m);
+int SimulationWriteErrors(const char *dir, HyPar *sims, int nsims, double solver_runtime,
+                          double main_runtime);
 int ApplyBoundaryConditions(void *s, void *m, double *x, double *xref, double waqt);
 
 /* ---- time integration (TimeInitialize.c, TimeRK.c, TimeRHSFunctionExplicit.c) ---- */

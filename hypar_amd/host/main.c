@@ -62,6 +62,7 @@ static int output_solution(const char *dir, HyPar *sims, int nsims) {
 }
 
 int main(int argc, char **argv) {
+  const double t_main = now();
   const char *dir = ".";
   int slots = 0;
   for (int i = 1; i < argc; i++) {
@@ -95,6 +96,15 @@ int main(int argc, char **argv) {
     printf("Domain %d: %d x %d x %d, %s WENO5, %s upwinding, %s\n", n, s->dim_global[0],
            s->dim_global[1], s->dim_global[2], s->interp_type, s->physics->upw_choice,
            s->physics->Re > 0 ? "viscous" : "inviscid");
+    /* InitialSolution.c:117-135 and Initialize.c:304-307 */
+    if (VolumeIntegral(s->VolumeIntegralInitial, s->u, s, &mpi[n])) return 1;
+    if (nsims > 1) printf("Volume integral of the initial solution on domain %d:\n", n);
+    else printf("Volume integral of the initial solution:\n");
+    for (int v = 0; v < s->nvars; v++) printf("%2d:  %1.16E\n", v, s->VolumeIntegralInitial[v]);
+    for (int v = 0; v < s->nvars; v++) {
+      s->TotalBoundaryIntegral[v] = 0.0;
+      s->ConservationError[v] = -1.0;
+    }
   }
   TimeIntegration TS;
   printf("Setting up time integration.\n");
@@ -124,6 +134,7 @@ int main(int argc, char **argv) {
   }
   printf("Completed time integration (Final time: %f), total wctime: %f (seconds).\n", TS.waqt, total);
   if (output_solution(dir, sims, nsims)) return 1;
+  if (SimulationWriteErrors(dir, sims, nsims, total, now() - t_main)) return 1;
   TimeCleanup(&TS);
   for (int n = 0; n < nsims; n++) hy_cleanup(&sims[n]);
   free(sims);

@@ -273,6 +273,7 @@ int SourceFunction(double *source, double *u, void *s, void *m, double t) {
   HyPar *h = (HyPar *)s;
   if (hm_check(hm_fill(h->dev, source, 0.0, n5(h)), "SourceFunction")) return 1;
   if (h->SFunction) {
+    h->count_sou++;
     /* the physics source accumulates into a scratch array */
     if (h->SFunction(h->uC, u, s, m, t)) return 1;
     if (hm_check(hm_axpy(h->dev, h->uC, 1.0, source, n5(h)), "SourceFunction")) return 1;
@@ -431,8 +432,10 @@ int TimeRK(void *ts) {
 /* one fused 3N RK4 step per member (same operation order as TimeRK) */
 int TimeRKFused(void *ts) {
   TimeIntegration *TS = (TimeIntegration *)ts;
-  for (int n = 0; n < TS->nsims; n++)
+  for (int n = 0; n < TS->nsims; n++) {
     if (hm_check(hm_rk4(TS->sims[n].dev, TS->sims[n].u, TS->dt, 1), "hm_rk4")) return 1;
+    TS->sims[n].count_sou += 4;  /* the (zero) source term of each of the four stages */
+  }
   return 0;
 }
 
@@ -469,6 +472,110 @@ int TimePostStep(TimeIntegration *TS) {
       npts += (double)s->npoints;
     }
     TS->norm = sqrt(sum / npts);
+  }
+  /* TimePostStep.c:78-93 */
+  for (int n = 0; n < TS->nsims; n++) {
+    HyPar *s = &TS->sims[n];
+    if (!strcmp(s->ConservationCheck, "yes")) {
+      if (VolumeIntegral(s->VolumeIntegral, s->u, s, &TS->mpi[n])) return 1;
+      if (BoundaryIntegral(s, &TS->mpi[n])) return 1;
+      if (CalculateConservationError(s, &TS->mpi[n])) return 1;
+    }
+  }
+  return 0;
+}
+
+/* ---- conservation check -------------------------------------------------- */
+
+/* VolumeIntegral.c:14-50 (the sum over cells runs on the device) */
+int VolumeIntegral(double *VolumeIntegral, double *u, void *s, void *m) {
+  (void)m;
+  HyPar *h = (HyPar *)s;
+  return hm_check(hm_volume_integral(h->dev, u, VolumeIntegral), "VolumeIntegral");
+}
+
+/* BoundaryIntegral.c:17-58: adds the step's boundary flux integral, times
+ * the face areas, to the running total.  The step integral is the dt b_s
+ * weighted sum of the stage integrals (TimeRK.c:182-195), each of which
+ * HyperbolicFunction.c:103-106 forms as -F at the low face plus F at the
+ * high face of every line.  Every boundary of this path is periodic: both
+ * faces of a line are the same interface and carry the same flux value, so
+ * each stage integral is exactly zero and StepBoundaryIntegral stays zero. */
+int BoundaryIntegral(void *s, void *m) {
+  (void)m;
+  HyPar *h = (HyPar *)s;
+  const int nv = h->nvars;
+  double local[5] = {0, 0, 0, 0, 0};
+  for (int d = 0; d < h->ndims; d++) {
+    for (int v = 0; v < nv; v++) {
+      double dS = 1.0;
+      for (int k = 0; k < h->ndims; k++) {
+        if (k == d) continue;
+        /* uniform grid assumed: dxinv at the middle of the line */
+        int off = 0;
+        for (int q = 0; q < k; q++) off += h->dim_local[q] + 2 * h->ghosts;
+        dS *= 1.0 / h->dxinv[off + h->ghosts + h->dim_local[k] / 2];
+      }
+      local[v] += h->StepBoundaryIntegral[(2 * d + 0) * nv + v] * dS;
+      local[v] += h->StepBoundaryIntegral[(2 * d + 1) * nv + v] * dS;
+    }
+  }
+  for (int v = 0; v < nv; v++) h->TotalBoundaryIntegral[v] += local[v];
+  return 0;
+}
+
+/* CalculateConservationError.c:16-38 */
+int CalculateConservationError(void *s, void *m) {
+  (void)m;
+  HyPar *h = (HyPar *)s;
+  for (int v = 0; v < h->nvars; v++) {
+    const double a = fabs(h->VolumeIntegralInitial[v]);
+    const double base = a > 1.0 ? a : 1.0;
+    const double e = h->VolumeIntegral[v] + h->TotalBoundaryIntegral[v] - h->VolumeIntegralInitial[v];
+    h->ConservationError[v] = sqrt(e * e) / base;
+  }
+  return 0;
+}
+
+/* SimulationWriteErrors.c:17-134: errors, conservation errors and call
+ * counts per member (no exact solution is read, so the solution errors are
+ * -1 as CalculateError.c:61 leaves them) */
+int SimulationWriteErrors(const char *dir, HyPar *sims, int nsims, double solver_runtime,
+                          double main_runtime) {
+  if (nsims > 1) printf("\n");
+  for (int n = 0; n < nsims; n++) {
+    HyPar *h = &sims[n];
+    char idx[HY_STR] = "", name[3 * HY_STR];
+    if (nsims > 1) {
+      char tmp[HY_STR];
+      hy_member_index(n, nsims, tmp);
+      snprintf(idx, sizeof idx, "_%s", tmp);
+    }
+    snprintf(name, sizeof name, "%s/errors%s.dat", dir, idx);
+    FILE *out = fopen(name, "w");
+    if (!out) return hy_error("cannot write %s", name);
+    for (int d = 0; d < h->ndims; d++) fprintf(out, "%4d ", h->dim_global[d]);
+    for (int d = 0; d < h->ndims; d++) fprintf(out, "%4d ", 1);
+    fprintf(out, "%1.16E  ", h->dt);
+    fprintf(out, "%1.16E %1.16E %1.16E   ", -1.0, -1.0, -1.0);
+    fprintf(out, "%1.16E %1.16E\n", solver_runtime, main_runtime);
+    fclose(out);
+    snprintf(name, sizeof name, "%s/conservation%s.dat", dir, idx);
+    if (!(out = fopen(name, "w"))) return hy_error("cannot write %s", name);
+    for (int d = 0; d < h->ndims; d++) fprintf(out, "%4d ", h->dim_global[d]);
+    for (int d = 0; d < h->ndims; d++) fprintf(out, "%4d ", 1);
+    fprintf(out, "%1.16E  ", h->dt);
+    for (int v = 0; v < h->nvars; v++) fprintf(out, "%1.16E ", h->ConservationError[v]);
+    fprintf(out, "\n");
+    fclose(out);
+    snprintf(name, sizeof name, "%s/function_counts%s.dat", dir, idx);
+    if (!(out = fopen(name, "w"))) return hy_error("cannot write %s", name);
+    fprintf(out, "%d\n%d\n%d\n%d\n", h->n_iter, h->count_hyp, h->count_par, h->count_sou);
+    fclose(out);
+    if (!strcmp(h->ConservationCheck, "yes")) {
+      printf("Conservation Errors:\n");
+      for (int v = 0; v < h->nvars; v++) printf("\t%1.16E\n", h->ConservationError[v]);
+    }
   }
   return 0;
 }

@@ -139,6 +139,14 @@ class NS3D:
 
     # ------------------------------------------------------------------
     # discrete adjoint (inviscid component-wise WENO5 + Rusanov)
+    def volume_integral(self, u: torch.Tensor) -> np.ndarray:
+        """Sum over cells of u dV for each conserved variable (synchronises)."""
+        self._check_state(u)
+        self._stream()
+        out = (ctypes.c_double * 5)()
+        _lib.check(self.L.hm_volume_integral(self.h, ctypes.c_void_p(u.data_ptr()), out))
+        return np.array(out[:])
+
     def rhs_vjp(self, u: torch.Tensor, lam: torch.Tensor) -> torch.Tensor:
         """(d rhs/d u)^T lam at state u."""
         self._check_state(u)

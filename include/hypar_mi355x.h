@@ -28,6 +28,8 @@
  *   hm_cfl ............. HyPar::ComputeCFL = NavierStokes3DComputeCFL
  *                        (include/hypar.h:269, NavierStokes3DComputeCFL.c:16-55)
  *   hm_step_norm ....... TimePostStep norm (src/TimeIntegration/TimePostStep.c:43-63)
+ *   hm_volume_integral . HyPar::VolumeIntegralFunction = VolumeIntegral
+ *                        (include/hypar.h:388, src/HyParFunctions/VolumeIntegral.c:14-50)
  *   hm_flux ... hm_fill  the per-slot operators (see below)
  */
 #ifndef HYPAR_MI355X_H
@@ -118,6 +120,10 @@ hm_status hm_parabolic(hm_solver *s, double *par, const double *u);
 hm_status hm_cfl(hm_solver *s, const double *u, double dt, double *cfl);
 /* sqrt(sum((a-b)^2) / ncell) over all 5 variables (TimePostStep norm). */
 hm_status hm_step_norm(hm_solver *s, const double *a, const double *b, double *norm);
+/* Volume integral of each conserved variable, out[v] = sum over interior
+ * cells of u_v dV, dV = 1/dxinv_x 1/dxinv_y 1/dxinv_z (the conservation
+ * check of TimePostStep.c:81-93).  Synchronises; HM_ERR_NONFINITE on NaN/Inf. */
+hm_status hm_volume_integral(hm_solver *s, const double *u, double *out);
 
 /* ---- device memory (so that host code needs no HIP headers) ---------- */
 /* Allocate / free n doubles of device memory on the current device. */

@@ -807,3 +807,21 @@ double orc_cfl(const orc_params *p, const double *u, double dt) {
       }
   return max_cfl;
 }
+
+/* VolumeIntegral.c:14-50: sum of u dV over the interior, x fastest, with
+ * dV = 1/dxinv_x 1/dxinv_y 1/dxinv_z multiplied in that order */
+void orc_volume_integral(const orc_params *p, const double *u, double *out) {
+  grid_t G = mkgrid(p);
+  const double *dxi[3] = {dxinv_dir(p, 0), dxinv_dir(p, 1), dxinv_dir(p, 2)};
+  for (int v = 0; v < NV; v++) out[v] = 0.0;
+  for (int k = 0; k < G.n[2]; k++)
+    for (int j = 0; j < G.n[1]; j++)
+      for (int i = 0; i < G.n[0]; i++) {
+        const double *q = u + NV * pidx(&G, i, j, k);
+        double dV = 1.0;
+        dV *= (1.0 / dxi[0][i]);
+        dV *= (1.0 / dxi[1][j]);
+        dV *= (1.0 / dxi[2][k]);
+        for (int v = 0; v < NV; v++) out[v] += (q[v] * dV);
+      }
+}

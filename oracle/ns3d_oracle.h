@@ -55,6 +55,9 @@ int orc_rk4(const orc_params *p, double *u, double dt, int nsteps);
 /* Max CFL over the interior (NavierStokes3DComputeCFL). */
 double orc_cfl(const orc_params *p, const double *u, double dt);
 
+/* Volume integral of each variable over the interior (VolumeIntegral). */
+void orc_volume_integral(const orc_params *p, const double *u, double *out);
+
 /* Component-wise WENO5-JS weights of one 5-point stencil (macro
  * _WENOWeights_v_JS_), exposed for unit tests. */
 void orc_weno_js_weights(double m3, double m2, double m1, double p1, double p2,

@@ -43,6 +43,7 @@ def lib():
         L.orc_parabolic.argtypes = [pp, dp, dp]; L.orc_parabolic.restype = ctypes.c_int
         L.orc_rk4.argtypes = [pp, dp, ctypes.c_double, ctypes.c_int]; L.orc_rk4.restype = ctypes.c_int
         L.orc_cfl.argtypes = [pp, dp, ctypes.c_double]; L.orc_cfl.restype = ctypes.c_double
+        L.orc_volume_integral.argtypes = [pp, dp, dp]; L.orc_volume_integral.restype = None
         L.orc_periodic_bc.argtypes = [pp, dp]; L.orc_periodic_bc.restype = None
         L.orc_weno_js_weights.argtypes = [ctypes.c_double] * 6 + [dp]
         L.orc_weno_weights.argtypes = [ctypes.c_int] + [ctypes.c_double] * 6 + [dp]
@@ -132,3 +133,9 @@ class Oracle:
     def cfl(self, u, dt):
         G = self.ghosted(u)
         return float(lib().orc_cfl(ctypes.byref(self.p), _dp(G), float(dt)))
+
+    def volume_integral(self, u):
+        G = self.ghosted(u)
+        out = np.zeros(5)
+        lib().orc_volume_integral(ctypes.byref(self.p), _dp(G), _dp(out))
+        return out

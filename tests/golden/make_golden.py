@@ -78,19 +78,40 @@ FIXTURES = {
     "c1_yc_16_s4": ("c1+yc", (16, 16, 16), 4, True),
 }
 
+# fixtures whose decks turn on the conservation check; they also record the
+# reference's printed initial volume integral and its conservation.dat
+CONSERVATION = {
+    "c1_16_s4_cons": ("c1", (16, 16, 16), 4, True),
+    "c0_ragged_s3_cons": ("c0", (20, 12, 16), 3, True),
+}
+FIXTURES.update(CONSERVATION)
+
 ENSEMBLE = {
     # three members of uneven sizes, shared dt and n_iter
     "ens3_s4": ([(16, 12, 20), (12, 20, 16), (20, 16, 12)], 4),
 }
 
 
-def run_reference(deck_dir: str) -> float:
+def run_reference(deck_dir: str, want_stdout: bool = False):
     env = dict(os.environ, OMP_NUM_THREADS="1")
     t = time.time()
     r = subprocess.run([REF_BIN], cwd=deck_dir, capture_output=True, text=True, env=env)
     if r.returncode != 0:
         raise RuntimeError(f"reference failed in {deck_dir}:\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
-    return time.time() - t
+    wall = time.time() - t
+    return (wall, r.stdout) if want_stdout else wall
+
+
+def conservation_outputs(deck_dir: str, stdout: str) -> dict:
+    """The initial volume integral the reference prints
+    (src/Simulation/InitialSolution.c:127-132) and the conservation errors of
+    its conservation.dat (src/Simulation/SimulationWriteErrors.c:83-89)."""
+    lines = stdout.splitlines()
+    i = next(k for k, l in enumerate(lines) if l.startswith("Volume integral of the initial solution"))
+    vi = [float(lines[i + 1 + v].split()[1]) for v in range(5)]
+    with open(os.path.join(deck_dir, "conservation.dat")) as f:
+        fields = f.read().split()
+    return {"volume_integral_initial": vi, "conservation_dat": fields}
 
 
 def summary(u: np.ndarray) -> dict:
@@ -105,16 +126,20 @@ def subsample(u: np.ndarray, step: int = 4) -> np.ndarray:
 def make_single(name: str):
     kind, dims, steps, full = FIXTURES[name]
     w = _workload(kind, dims, steps)
+    cons = name in CONSERVATION
+    extra = {}
     with tempfile.TemporaryDirectory() as d:
-        u0 = decks.write_deck(d, w, screen_op_iter=max(1, steps))
-        wall = run_reference(d)
+        u0 = decks.write_deck(d, w, screen_op_iter=max(1, steps), conservation_check="yes" if cons else "no")
+        wall, out = run_reference(d, want_stdout=True)
         _, _, uN = decks.read_op_bin(os.path.join(d, "op.bin"))
+        if cons:
+            extra = conservation_outputs(d, out)
     meta = {"name": name, "scheme": kind, "dims": list(dims), "steps": steps, "dt": w.dt,
             "seed": w.seed, "interp": w.interp, "upwind": w.upwind, "Re": w.Re,
             "Minf": w.Minf, "Pr": w.Pr, "gamma": w.gamma, "eps": w.weno_eps,
             "field": w.field, "time_scheme_type": w.time_scheme_type, "weno_type": w.weno_type,
             "reference_wall_s": wall, "reference_threads": 1,
-            "u0_summary": summary(u0), "uN_summary": summary(uN)}
+            "u0_summary": summary(u0), "uN_summary": summary(uN), **extra}
     arrays = {"meta": np.array(json.dumps(meta))}
     if full:
         arrays["u0"] = u0

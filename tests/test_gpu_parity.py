@@ -246,3 +246,19 @@ def test_full_size_invariants_256():
     rs = s.rhs(us).view(5, 256, 256, 256)
     rr = torch.roll(r.view(5, 256, 256, 256), 8, dims=2)
     assert float((rs - rr).abs().max()) <= 1e-11 * float(r.abs().max())
+
+
+@pytest.mark.gpu
+def test_volume_integral_matches_oracle():
+    """hm_volume_integral against the oracle's restatement of
+    VolumeIntegral.c on a ragged grid (sums over cells in a different order,
+    so agreement is to rounding)."""
+    from hypar_amd.solver import NS3D
+    from oracle.oracle import Oracle
+    meta, z = load_fixture("c0_ragged_s3")
+    w = workload_from_meta(meta)
+    s = NS3D.from_workload(w)
+    got = s.volume_integral(s.to_device(z["uN"]))
+    ref = Oracle.for_workload(w).volume_integral(z["uN"])
+    print(f"[gpu] volume integral {got} oracle {ref}", flush=True)
+    assert np.all(np.abs(got - ref) <= 1e-13 * np.maximum(np.abs(ref), 1.0))

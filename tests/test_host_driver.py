@@ -24,11 +24,11 @@ def run_driver(d, *args, timeout=300):
     return subprocess.run([DRIVER, *args, d], capture_output=True, text=True, timeout=timeout)
 
 
-def _deck(tmp_path, name="c0_16_s1"):
+def _deck(tmp_path, name="c0_16_s1", conservation_check="no"):
     meta, z = load_fixture(name)
     w = workload_from_meta(meta)
     d = str(tmp_path / name)
-    decks.write_deck(d, w, u0=z["u0"], screen_op_iter=1)
+    decks.write_deck(d, w, u0=z["u0"], screen_op_iter=1, conservation_check=conservation_check)
     return d, w, meta, z
 
 
@@ -105,3 +105,34 @@ def test_driver_ensemble_matches_reference(tmp_path):
     for i in range(len(members)):
         _, _, u = decks.read_op_bin(os.path.join(d, f"op_{i}.bin"))
         assert decks.rel_l2(u, z[f"uN_{i}"]) <= 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,args", [("c1_16_s4_cons", []), ("c1_16_s4_cons", ["--slots"]),
+                                       ("c0_ragged_s3_cons", [])])
+def test_driver_conservation_check_matches_reference(tmp_path, name, args):
+    """conservation_check yes: the printed initial volume integral, and
+    conservation.dat, against the reference's for the same deck.  The viscous
+    deck's momentum and energy errors are O(1e-8): the reference's ghost-
+    layer viscous fluxes do not telescope, and this path evaluates them the
+    same way.  The inviscid deck's errors are rounding noise (~1e-15)."""
+    d, w, meta, z = _deck(tmp_path, name, conservation_check="yes")
+    print(f"[gpu] driver conservation {name} {args}", flush=True)
+    r = run_driver(d, *args)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    lines = r.stdout.splitlines()
+    i = next(k for k, l in enumerate(lines) if l.startswith("Volume integral of the initial solution"))
+    vi = np.array([float(lines[i + 1 + v].split()[1]) for v in range(5)])
+    ref = np.array(meta["volume_integral_initial"])
+    scale = np.maximum(np.abs(ref), 1.0)
+    assert np.all(np.abs(vi - ref) <= 1e-13 * scale), (vi, ref)
+    with open(os.path.join(d, "conservation.dat")) as f:
+        fields = f.read().split()
+    rf = meta["conservation_dat"]
+    assert fields[:7] == rf[:7]  # sizes, processes, dt
+    got, want = np.array([float(x) for x in fields[7:]]), np.array([float(x) for x in rf[7:]])
+    print(f"[gpu]   conservation errors {got} reference {want}", flush=True)
+    assert np.all(np.abs(got - want) <= 1e-12 + 1e-4 * np.abs(want)), (got, want)
+    assert "Conservation Errors:" in r.stdout
+    _, _, u = decks.read_op_bin(os.path.join(d, "op.bin"))
+    assert decks.rel_l2(u, z["uN"]) <= 1e-12

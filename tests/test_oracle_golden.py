@@ -64,3 +64,13 @@ def test_weight_formulation_changes_the_result(name):
     w = dataclasses.replace(workload_from_meta(meta), weno_type="js")
     js = Oracle.for_workload(w).rk4(z["u0"], w.dt, meta["steps"])
     assert decks.rel_l2(js, z["uN"]) > 1e-9
+
+
+def test_oracle_volume_integral_matches_reference_print():
+    """The oracle's VolumeIntegral against the value the reference printed
+    for the same initial state (%1.16E, so 17 significant digits)."""
+    for name in ["c1_16_s4_cons", "c0_ragged_s3_cons"]:
+        meta, z = load_fixture(name)
+        got = Oracle.for_workload(workload_from_meta(meta)).volume_integral(z["u0"])
+        ref = np.array(meta["volume_integral_initial"])
+        assert np.all(np.abs(got - ref) <= 1e-15 * np.maximum(np.abs(ref), 1e-3)), (name, got, ref)
