@@ -629,16 +629,17 @@ hm_status hm_adjoint_forward(hm_solver *s, double *u, double dt, int nsteps, int
   const int nck = (nsteps + every - 1) / every;
   if ((st = ensure_buffers(s->ckpt, (size_t)nck, n5)) != HM_OK) return st;
   if ((st = ensure_buffers(s->stages, (size_t)8 * every, n5)) != HM_OK) return st;
-  const int nlast = (nck - 1) * every;
+  // the last two segments keep their stage states in the two stage sets:
+  // the reverse sweep starts on them without recomputation
+  const int nkeep = (nck >= 2 ? nck - 2 : 0) * every;
   for (int n = 0; n < nsteps; n++) {
     if (n % every == 0)
       HM_HIP(hipMemcpyAsync(s->ckpt[n / every], u, n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
-    if (n < nlast) {
+    if (n < nkeep) {
       if ((st = hm_rk4(s, u, dt, 1)) != HM_OK) return st;
     } else {
-      // the last segment keeps its stage states: the reverse sweep starts on
-      // them without recomputation
-      double **U = segment_stages(s, nck - 1, every) + (size_t)4 * (n - nlast);
+      const int seg = n / every;
+      double **U = segment_stages(s, seg, every) + (size_t)4 * (n - seg * every);
       HM_HIP(hipMemcpyAsync(U[0], u, n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
       double *const Us[3] = {U[1], U[2], U[3]};
       if ((st = rk4_step_store(s, u, dt, Us)) != HM_OK) return st;
@@ -689,11 +690,13 @@ hm_status hm_adjoint_reverse(hm_solver *s, double *lam) {
   // the recomputation stream starts after everything queued so far
   HM_HIP(hipEventRecord(s->ev_start, main));
   HM_HIP(hipStreamWaitEvent(s->aux, s->ev_start, 0));
+  // segments nck-2 and nck-1 kept their stage states in the forward sweep
+  const int kept = nck >= 2 ? nck - 2 : 0;
   for (int seg = nck - 1; seg >= 0; seg--) {
     const int n0 = seg * K, n1 = (n0 + K < N) ? n0 + K : N;
-    if (seg > 0) {
+    if (seg > 0 && seg - 1 < kept) {
       // recompute the previous segment into the other set on the second
-      // stream, once the adjoint that last read that set is done
+      // stream, once the adjoint that last read that set (segment seg+1) is done
       const int o = (seg - 1) % 2;
       if (seg + 1 < nck) HM_HIP(hipStreamWaitEvent(s->aux, s->ev_done[o], 0));
       s->stream = s->aux;
@@ -702,9 +705,9 @@ hm_status hm_adjoint_reverse(hm_solver *s, double *lam) {
       if (st != HM_OK) return st;
       HM_HIP(hipEventRecord(s->ev_ready[o], s->aux));
     }
-    // the last segment's states come from the forward sweep; the others
-    // from the recomputation queued in the previous iteration
-    if (seg + 1 < nck) HM_HIP(hipStreamWaitEvent(main, s->ev_ready[seg % 2], 0));
+    // kept segments come from the forward sweep; the others from the
+    // recomputation queued in the previous iteration
+    if (seg < kept) HM_HIP(hipStreamWaitEvent(main, s->ev_ready[seg % 2], 0));
     double **S = segment_stages(s, seg, K);
     for (int n = n1 - 1; n >= n0; n--) {
       // the step adjoint alternates between the caller's array and scratch
@@ -1188,16 +1191,17 @@ hm_status hm_ensemble_adjoint_forward(hm_ensemble *e, double *u, int nsteps, int
   const size_t n5 = 5 * e->ncell;
   const int nck = (nsteps + every - 1) / every;
   if ((st = ens_buffers(e->ckpt, (size_t)nck, n5)) != HM_OK) return st;
-  if ((st = ens_buffers(e->stages, (size_t)4 * every, n5)) != HM_OK) return st;
-  const int nlast = (nck - 1) * every;
+  if ((st = ens_buffers(e->stages, (size_t)8 * every, n5)) != HM_OK) return st;
+  // the last two segments keep their stage states (segment seg in set seg % 2)
+  const int nkeep = (nck >= 2 ? nck - 2 : 0) * every;
   for (int n = 0; n < nsteps; n++) {
     if (n % every == 0)
       HM_HIP(hipMemcpyAsync(e->ckpt[n / every], u, n5 * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
-    if (n < nlast) {
+    if (n < nkeep) {
       if ((st = hm_ensemble_rk4(e, u, 1)) != HM_OK) return st;
     } else {
-      // the last segment keeps its stage states for the reverse sweep
-      double **U = &e->stages[(size_t)4 * (n - nlast)];
+      const int seg = n / every;
+      double **U = &e->stages[(size_t)4 * every * (seg % 2) + (size_t)4 * (n - seg * every)];
       HM_HIP(hipMemcpyAsync(U[0], u, n5 * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
       double *const Us[3] = {U[1], U[2], U[3]};
       if ((st = ens_step_store(e, u, Us)) != HM_OK) return st;
@@ -1231,22 +1235,24 @@ hm_status hm_ensemble_adjoint_reverse(hm_ensemble *e, double *lam) {
     plan.nmax[d] = e->nmax[d];
   }
   const int nck = (N + K - 1) / K;
+  const int kept = nck >= 2 ? nck - 2 : 0;  // segments the forward sweep kept
   double *cur = lam;
   for (int seg = nck - 1; seg >= 0; seg--) {
     const int n0 = seg * K, n1 = (n0 + K < N) ? n0 + K : N;
-    if (seg + 1 < nck) {
+    double **S = &e->stages[(size_t)4 * K * (seg % 2)];
+    if (seg < kept) {
       // recompute the segment from its checkpoint, keeping the stage states
       for (int n = n0; n < n1; n++) {
-        double **U = &e->stages[(size_t)4 * (n - n0)];
+        double **U = S + (size_t)4 * (n - n0);
         if (n == n0) HM_HIP(hipMemcpyAsync(U[0], e->ckpt[seg], n5 * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
-        double *next = n + 1 < n1 ? e->stages[(size_t)4 * (n - n0 + 1)] : e->vscratch[3];
+        double *next = n + 1 < n1 ? S[(size_t)4 * (n - n0 + 1)] : e->vscratch[3];
         HM_HIP(hipMemcpyAsync(next, U[0], n5 * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
         double *const Us[3] = {U[1], U[2], U[3]};
         if ((st = ens_step_store(e, next, Us)) != HM_OK) return st;
       }
     }
     for (int n = n1 - 1; n >= n0; n--) {
-      double **U = &e->stages[(size_t)4 * (n - n0)];
+      double **U = S + (size_t)4 * (n - n0);
       double *out = cur == lam ? e->vscratch[2] : lam;
       // step adjoint: out = cur + sum_s nu_s (stage coefficients formed per member)
       const double *nu_in = nullptr;
