@@ -189,193 +189,8 @@ __device__ __forceinline__ void flux_jac_T(const double *u, double gamma, const 
   ub[4] += fb[1 + DIR] * gm1 + fb[4] * gamma * vn;
 }
 
-// nu (+)= (d rhs_DIR / d u)^T mu along one direction, one workgroup per line,
-// with mu = cb*lam + ca*nu_in (nu_in may be null).  mode VJP_WRITE stores,
-// VJP_ADD accumulates into nu, VJP_ADD_L accumulates into nu and adds the
-// finished nu into L, VJP_ADD_LSET sets L = lam + the finished nu.  LDS (dynamic, 23 doubles per cell of the line):
-// state [5][n], cell flux [5][n], modified energy [n], stencil
-// contributions of the flux and solution adjoints [6][n] each.
-enum { VJP_WRITE = 0, VJP_ADD = 1, VJP_ADD_L = 2, VJP_ADD_LSET = 3 };
-
-template <int DIR>
-__device__ __forceinline__ void rhs_vjp_line(const double *__restrict__ u, const double *__restrict__ lam,
-                                             const double *__restrict__ nu_in, double ca, double cb,
-                                             double *__restrict__ nu, double *__restrict__ L, int mode, int nx,
-                                             int ny, int nz, const double *__restrict__ dxg, double gamma,
-                                             double eps, size_t line, double *smem) {
-  const size_t ncell = (size_t)nx * ny * nz;
-  const int n = DIR == 0 ? nx : (DIR == 1 ? ny : nz);
-  size_t base, stride;
-  if (DIR == 0) { base = line * nx; stride = 1; }
-  else if (DIR == 1) { base = (line % nx) + (line / nx) * (size_t)nx * ny; stride = nx; }
-  else { base = line; stride = (size_t)nx * ny; }
-  const double *dxd = dxg + 3 + (DIR >= 1 ? nx + 6 : 0) + (DIR == 2 ? ny + 6 : 0);
-  double *sU = smem;            // [5][n]
-  double *sFl = sU + 5 * n;     // [5][n]
-  double *sC4 = sFl + 5 * n;    // [n]
-  double *cF = sC4 + n;         // [6][n]
-  double *cU = cF + 6 * n;      // [6][n]
-  const int t = threadIdx.x;
-  const bool active = t < n;
-  const double inv_gm1 = 1.0 / (gamma - 1.0);
-  const int tm = active ? wrapn(t - 1, n) : 0;
-  double g[NV] = {0, 0, 0, 0, 0};
-  if (active) {
-    const size_t c = base + (size_t)t * stride, cm = base + (size_t)tm * stride;
-    double us[NV];
-#pragma unroll
-    for (int v = 0; v < NV; v++) {
-      us[v] = u[(size_t)v * ncell + c];
-      sU[v * n + t] = us[v];
-      const size_t o = (size_t)v * ncell;
-      const double mu = cb * lam[o + c] + (nu_in ? ca * nu_in[o + c] : 0.0);
-      const double mum = cb * lam[o + cm] + (nu_in ? ca * nu_in[o + cm] : 0.0);
-      g[v] = dxd[t] * mu - dxd[tm] * mum;
-    }
-    const Prim q = flowvar(us, gamma);
-    double fl[NV];
-    euler_flux<DIR>(fl, q);
-#pragma unroll
-    for (int v = 0; v < NV; v++) sFl[v * n + t] = fl[v];
-    sC4[t] = modified_energy(q, inv_gm1);
-  }
-  __syncthreads();
-  int cs[6];
-  double alpha = 0.0, abar = 0.0;
-  int sel = 0;
-  if (active) {
-#pragma unroll
-    for (int s = 0; s < 6; s++) cs[s] = wrapn(t + s - 3, n);
-    double u2[NV], u3[NV];
-#pragma unroll
-    for (int v = 0; v < NV; v++) { u2[v] = sU[v * n + cs[2]]; u3[v] = sU[v * n + cs[3]]; }
-    const Prim q2 = flowvar(u2, gamma), q3 = flowvar(u3, gamma);
-    const Prim a = roe_average(u2, u3, gamma);
-    const double aL = sqrt(gamma * q2.P / q2.rho) + fabs(vel<DIR>(q2));
-    const double aR = sqrt(gamma * q3.P / q3.rho) + fabs(vel<DIR>(q3));
-    const double aA = sqrt(gamma * a.P / a.rho) + fabs(vel<DIR>(a));
-    const double m01 = fmax(aL, aR), m12 = fmax(aR, aA);
-    alpha = fmax(m01, m12);
-    // alpha = max(max(aL,aR), max(aR,aA)): derivative of the selected term
-    sel = m01 >= m12 ? (aL >= aR ? 0 : 1) : (aR >= aA ? 1 : 2);
-  }
-  double accF[NV], accU[NV];
-#pragma unroll
-  for (int v = 0; v < NV; v++) {
-    if (active) {
-      double fb[6] = {0, 0, 0, 0, 0, 0}, ub[6] = {0, 0, 0, 0, 0, 0};
-      double F[6], U[6], C[6];
-#pragma unroll
-      for (int s = 0; s < 6; s++) {
-        F[s] = sFl[v * n + cs[s]];
-        U[s] = sU[v * n + cs[s]];
-        C[s] = v == 4 ? sC4[cs[s]] : U[s];
-      }
-      double mL[5], gL[5], mR[5], gR[5];
-      double mbL[5] = {0, 0, 0, 0, 0}, gbL[5] = {0, 0, 0, 0, 0};
-      double mbR[5] = {0, 0, 0, 0, 0}, gbR[5] = {0, 0, 0, 0, 0};
-      // fluxes: smoothness and values are the same stencil
-#pragma unroll
-      for (int k = 0; k < 5; k++) { mL[k] = F[k]; mR[k] = F[5 - k]; }
-      weno5_vjp(mL, mL, eps, 0.5 * g[v], mbL, gbL);
-      weno5_vjp(mR, mR, eps, 0.5 * g[v], mbR, gbR);
-#pragma unroll
-      for (int k = 0; k < 5; k++) {
-        fb[k] += mbL[k] + gbL[k];
-        fb[5 - k] += mbR[k] + gbR[k];
-        mbL[k] = gbL[k] = mbR[k] = gbR[k] = 0.0;
-      }
-      // solution: weights from u, values from the modified solution (the
-      // same function of u, so both adjoints land on u)
-#pragma unroll
-      for (int k = 0; k < 5; k++) { mL[k] = U[k]; gL[k] = C[k]; mR[k] = U[5 - k]; gR[k] = C[5 - k]; }
-      const double uLv = weno5_vjp(mL, gL, eps, 0.5 * alpha * g[v], mbL, gbL);
-      const double uRv = weno5_vjp(mR, gR, eps, -0.5 * alpha * g[v], mbR, gbR);
-      abar += -0.5 * (uRv - uLv) * g[v];
-#pragma unroll
-      for (int k = 0; k < 5; k++) {
-        ub[k] += mbL[k] + gbL[k];
-        ub[5 - k] += mbR[k] + gbR[k];
-      }
-      // slot s of interface t belongs to cell t-3+s
-#pragma unroll
-      for (int s = 0; s < 6; s++) {
-        cF[s * n + t] = fb[s];
-        cU[s * n + t] = ub[s];
-      }
-    }
-    __syncthreads();
-    // cell t sums slot s of interface t+3-s, s = 0..5, in that order
-    accF[v] = accU[v] = 0.0;
-    if (active) {
-#pragma unroll
-      for (int s = 0; s < 6; s++) {
-        const int i = wrapn(t + 3 - s, n);
-        accF[v] += cF[s * n + i];
-        accU[v] += cU[s * n + i];
-      }
-    }
-    __syncthreads();
-  }
-  // Rusanov speed adjoint onto the two cells next to interface t: cell t-1
-  // (slot in cF) and cell t (slot in cU)
-  if (active) {
-    double u2[NV], u3[NV], ub2[NV] = {0, 0, 0, 0, 0}, ub3[NV] = {0, 0, 0, 0, 0};
-#pragma unroll
-    for (int v = 0; v < NV; v++) { u2[v] = sU[v * n + cs[2]]; u3[v] = sU[v * n + cs[3]]; }
-    if (sel == 0) speed_vjp<DIR>(u2, gamma, abar, ub2);
-    else if (sel == 1) speed_vjp<DIR>(u3, gamma, abar, ub3);
-    else roe_speed_vjp<DIR>(u2, u3, gamma, abar, ub2, ub3);
-#pragma unroll
-    for (int v = 0; v < NV; v++) {
-      cF[v * n + t] = ub2[v];
-      cU[v * n + t] = ub3[v];
-    }
-  }
-  __syncthreads();
-  if (active) {
-    const int tp = wrapn(t + 1, n);
-    double uc[NV];
-#pragma unroll
-    for (int v = 0; v < NV; v++) {
-      uc[v] = sU[v * n + t];
-      accU[v] += cF[v * n + tp] + cU[v * n + t];
-    }
-    flux_jac_T<DIR>(uc, gamma, accF, accU);
-    const size_t c = base + (size_t)t * stride;
-#pragma unroll
-    for (int v = 0; v < NV; v++) {
-      const size_t o = (size_t)v * ncell + c;
-      if (mode == VJP_WRITE) {
-        nu[o] = accU[v];
-      } else {
-        const double r = nu[o] + accU[v];
-        nu[o] = r;
-        if (mode == VJP_ADD_L) L[o] += r;
-        else if (mode == VJP_ADD_LSET) L[o] = lam[o] + r;
-      }
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// characteristic-wise WENO5-JS with Roe or Rusanov upwinding
-// ---------------------------------------------------------------------------
-// The forward flux at an interface is F = R g with, for every characteristic
-// field k (rows of L at the Roe average of the two cells next to it),
-//   g_k = (fL_k + fR_k)/2 - l_k (uR_k - uL_k)/2,
-// fL/fR (uL/uR) the WENO5 values of L f (L u) over the stencil and l_k = |lambda_k|
-// with Harten's fix (Roe) or the Rusanov speed; this is the reference's
-// R (WENO of L f) - R |Lambda| L R (WENO of L u) with L R = I
-// (Interp1PrimFifthOrderWENOChar.c:86-198, NavierStokes3DUpwind.c:40-124).
-// The reverse pass runs field by field: WENO adjoints of the projected
-// stencils, then back through L onto the cells' fluxes and states, while the
-// derivatives of L, R and l_k with respect to the Roe state (v, a) are summed
-// and finally taken back through the Roe average onto cells I-1 and I.  The
-// interpolated values use the modified solution, which equals u exactly in
-// exact arithmetic, so both adjoints land on u.
-constexpr int VJP_CHAR_LINE_MAX = 256;
-
+// Roe state of an interface for the adjoints of Roe upwinding and of the
+// characteristic scheme, and the rows of L with their derivatives
 struct RoeAdj {
   double v[3], vn, a, a2, ia2, ek, h0;
 };
@@ -460,6 +275,283 @@ __device__ __forceinline__ double harten_d(double l) {
   const double delta = 0.000001;
   return fabs(l) < delta ? l / delta : (l > 0 ? 1.0 : (l < 0 ? -1.0 : 0.0));
 }
+
+// nu (+)= (d rhs_DIR / d u)^T mu along one direction, one workgroup per line,
+// with mu = cb*lam + ca*nu_in (nu_in may be null).  mode VJP_WRITE stores,
+// VJP_ADD accumulates into nu, VJP_ADD_L accumulates into nu and adds the
+// finished nu into L, VJP_ADD_LSET sets L = lam + the finished nu.  LDS (dynamic, 23 doubles per cell of the line):
+// state [5][n], cell flux [5][n], modified energy [n], stencil
+// contributions of the flux and solution adjoints [6][n] each.
+enum { VJP_WRITE = 0, VJP_ADD = 1, VJP_ADD_L = 2, VJP_ADD_LSET = 3 };
+
+template <int DIR, int UPW>
+__device__ __forceinline__ void rhs_vjp_line(const double *__restrict__ u, const double *__restrict__ lam,
+                                             const double *__restrict__ nu_in, double ca, double cb,
+                                             double *__restrict__ nu, double *__restrict__ L, int mode, int nx,
+                                             int ny, int nz, const double *__restrict__ dxg, double gamma,
+                                             double eps, size_t line, double *smem) {
+  const size_t ncell = (size_t)nx * ny * nz;
+  const int n = DIR == 0 ? nx : (DIR == 1 ? ny : nz);
+  size_t base, stride;
+  if (DIR == 0) { base = line * nx; stride = 1; }
+  else if (DIR == 1) { base = (line % nx) + (line / nx) * (size_t)nx * ny; stride = nx; }
+  else { base = line; stride = (size_t)nx * ny; }
+  const double *dxd = dxg + 3 + (DIR >= 1 ? nx + 6 : 0) + (DIR == 2 ? ny + 6 : 0);
+  double *sU = smem;            // [5][n]
+  double *sFl = sU + 5 * n;     // [5][n]
+  double *sC4 = sFl + 5 * n;    // [n]
+  double *cF = sC4 + n;         // [6][n]
+  double *cU = cF + 6 * n;      // [6][n]
+  const int t = threadIdx.x;
+  const bool active = t < n;
+  const double inv_gm1 = 1.0 / (gamma - 1.0);
+  const int tm = active ? wrapn(t - 1, n) : 0;
+  double g[NV] = {0, 0, 0, 0, 0};
+  if (active) {
+    const size_t c = base + (size_t)t * stride, cm = base + (size_t)tm * stride;
+    double us[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      us[v] = u[(size_t)v * ncell + c];
+      sU[v * n + t] = us[v];
+      const size_t o = (size_t)v * ncell;
+      const double mu = cb * lam[o + c] + (nu_in ? ca * nu_in[o + c] : 0.0);
+      const double mum = cb * lam[o + cm] + (nu_in ? ca * nu_in[o + cm] : 0.0);
+      g[v] = dxd[t] * mu - dxd[tm] * mum;
+    }
+    const Prim q = flowvar(us, gamma);
+    double fl[NV];
+    euler_flux<DIR>(fl, q);
+#pragma unroll
+    for (int v = 0; v < NV; v++) sFl[v * n + t] = fl[v];
+    sC4[t] = modified_energy(q, inv_gm1);
+  }
+  __syncthreads();
+  int cs[6];
+  double alpha = 0.0, abar = 0.0;
+  int sel = 0;
+  // Roe: d = R |Lambda| L du with du = (uR - uL)/2 and fI = fh - d
+  // (NavierStokes3DUpwind.c:40-124); wR = A^T lambda_F, A = R |Lambda| L,
+  // so that the solution WENO adjoints are -/+ wR/2
+  double wR[NV] = {0, 0, 0, 0, 0}, du[NV] = {0, 0, 0, 0, 0};
+  if (active && UPW == UPWIND_ROE) {
+#pragma unroll
+    for (int s = 0; s < 6; s++) cs[s] = wrapn(t + s - 3, n);
+    double u2[NV], u3[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) { u2[v] = sU[v * n + cs[2]]; u3[v] = sU[v * n + cs[3]]; }
+    const RoeAdj s = roe_adj_state<DIR>(u2, u3, gamma);
+    const double gm1 = gamma - 1.0;
+    const double eb = g[4];
+    const double sumb = g[0] + s.v[0] * g[1] + s.v[1] * g[2] + s.v[2] * g[3];
+    const double difb = g[1 + DIR] + s.vn * eb;
+    double lg[5];  // (R^T lambda_F)_k |lambda_k|
+    lg[0] = (sumb + s.ek * eb) * harten_abs(s.vn);
+    lg[1] = (sumb + s.h0 * eb - s.a * difb) * harten_abs(s.vn - s.a);
+    lg[2] = (sumb + s.h0 * eb + s.a * difb) * harten_abs(s.vn + s.a);
+    if (DIR == 0) { lg[3] = -g[2] - s.v[1] * eb; lg[4] = g[3] + s.v[2] * eb; }
+    else if (DIR == 1) { lg[3] = g[1] + s.v[0] * eb; lg[4] = -g[3] - s.v[2] * eb; }
+    else { lg[3] = -g[1] - s.v[0] * eb; lg[4] = g[2] + s.v[1] * eb; }
+    lg[3] *= harten_abs(s.vn);
+    lg[4] *= harten_abs(s.vn);
+#pragma unroll 1
+    for (int k = 0; k < 5; k++) {
+      double row[NV];
+      l_row<DIR>(s, gm1, k, row);
+      const double c = k == 0 ? lg[0] : (k == 1 ? lg[1] : (k == 2 ? lg[2] : (k == 3 ? lg[3] : lg[4])));
+#pragma unroll
+      for (int v = 0; v < NV; v++) wR[v] += c * row[v];
+    }
+  }
+  if (active && UPW == UPWIND_RUSANOV) {
+#pragma unroll
+    for (int s = 0; s < 6; s++) cs[s] = wrapn(t + s - 3, n);
+    double u2[NV], u3[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) { u2[v] = sU[v * n + cs[2]]; u3[v] = sU[v * n + cs[3]]; }
+    const Prim q2 = flowvar(u2, gamma), q3 = flowvar(u3, gamma);
+    const Prim a = roe_average(u2, u3, gamma);
+    const double aL = sqrt(gamma * q2.P / q2.rho) + fabs(vel<DIR>(q2));
+    const double aR = sqrt(gamma * q3.P / q3.rho) + fabs(vel<DIR>(q3));
+    const double aA = sqrt(gamma * a.P / a.rho) + fabs(vel<DIR>(a));
+    const double m01 = fmax(aL, aR), m12 = fmax(aR, aA);
+    alpha = fmax(m01, m12);
+    // alpha = max(max(aL,aR), max(aR,aA)): derivative of the selected term
+    sel = m01 >= m12 ? (aL >= aR ? 0 : 1) : (aR >= aA ? 1 : 2);
+  }
+  double accF[NV], accU[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    if (active) {
+      double fb[6] = {0, 0, 0, 0, 0, 0}, ub[6] = {0, 0, 0, 0, 0, 0};
+      double F[6], U[6], C[6];
+#pragma unroll
+      for (int s = 0; s < 6; s++) {
+        F[s] = sFl[v * n + cs[s]];
+        U[s] = sU[v * n + cs[s]];
+        C[s] = v == 4 ? sC4[cs[s]] : U[s];
+      }
+      double mL[5], gL[5], mR[5], gR[5];
+      double mbL[5] = {0, 0, 0, 0, 0}, gbL[5] = {0, 0, 0, 0, 0};
+      double mbR[5] = {0, 0, 0, 0, 0}, gbR[5] = {0, 0, 0, 0, 0};
+      // fluxes: smoothness and values are the same stencil
+#pragma unroll
+      for (int k = 0; k < 5; k++) { mL[k] = F[k]; mR[k] = F[5 - k]; }
+      weno5_vjp(mL, mL, eps, 0.5 * g[v], mbL, gbL);
+      weno5_vjp(mR, mR, eps, 0.5 * g[v], mbR, gbR);
+#pragma unroll
+      for (int k = 0; k < 5; k++) {
+        fb[k] += mbL[k] + gbL[k];
+        fb[5 - k] += mbR[k] + gbR[k];
+        mbL[k] = gbL[k] = mbR[k] = gbR[k] = 0.0;
+      }
+      // solution: weights from u, values from the modified solution (the
+      // same function of u, so both adjoints land on u)
+#pragma unroll
+      for (int k = 0; k < 5; k++) { mL[k] = U[k]; gL[k] = C[k]; mR[k] = U[5 - k]; gR[k] = C[5 - k]; }
+      const double uLv = weno5_vjp(mL, gL, eps, UPW == UPWIND_ROE ? 0.5 * wR[v] : 0.5 * alpha * g[v], mbL, gbL);
+      const double uRv = weno5_vjp(mR, gR, eps, UPW == UPWIND_ROE ? -0.5 * wR[v] : -0.5 * alpha * g[v], mbR, gbR);
+      if (UPW == UPWIND_ROE) du[v] = 0.5 * (uRv - uLv);
+      else abar += -0.5 * (uRv - uLv) * g[v];
+#pragma unroll
+      for (int k = 0; k < 5; k++) {
+        ub[k] += mbL[k] + gbL[k];
+        ub[5 - k] += mbR[k] + gbR[k];
+      }
+      // slot s of interface t belongs to cell t-3+s
+#pragma unroll
+      for (int s = 0; s < 6; s++) {
+        cF[s * n + t] = fb[s];
+        cU[s * n + t] = ub[s];
+      }
+    }
+    __syncthreads();
+    // cell t sums slot s of interface t+3-s, s = 0..5, in that order
+    accF[v] = accU[v] = 0.0;
+    if (active) {
+#pragma unroll
+      for (int s = 0; s < 6; s++) {
+        const int i = wrapn(t + 3 - s, n);
+        accF[v] += cF[s * n + i];
+        accU[v] += cU[s * n + i];
+      }
+    }
+    __syncthreads();
+  }
+  // adjoint of the upwinding's dependence on the two cells next to interface
+  // t, cell t-1 (slot in cF) and cell t (slot in cU): the Rusanov speed, or
+  // the Roe state inside R |Lambda| L
+  if (active) {
+    double u2[NV], u3[NV], ub2[NV] = {0, 0, 0, 0, 0}, ub3[NV] = {0, 0, 0, 0, 0};
+#pragma unroll
+    for (int v = 0; v < NV; v++) { u2[v] = sU[v * n + cs[2]]; u3[v] = sU[v * n + cs[3]]; }
+    if (UPW == UPWIND_ROE) {
+      const RoeAdj s = roe_adj_state<DIR>(u2, u3, gamma);
+      const double gm1 = gamma - 1.0;
+      // adjoint of d is -lambda_F; lambda_g = R^T (-lambda_F)
+      double ld[NV];
+#pragma unroll
+      for (int v = 0; v < NV; v++) ld[v] = -g[v];
+      const double eb = ld[4];
+      const double sumb = ld[0] + s.v[0] * ld[1] + s.v[1] * ld[2] + s.v[2] * ld[3];
+      const double difb = ld[1 + DIR] + s.vn * eb;
+      double lgk[5], lam_k[5], gk[5], pk[5];
+      lgk[0] = sumb + s.ek * eb;
+      lgk[1] = sumb + s.h0 * eb - s.a * difb;
+      lgk[2] = sumb + s.h0 * eb + s.a * difb;
+      if (DIR == 0) { lgk[3] = -ld[2] - s.v[1] * eb; lgk[4] = ld[3] + s.v[2] * eb; }
+      else if (DIR == 1) { lgk[3] = ld[1] + s.v[0] * eb; lgk[4] = -ld[3] - s.v[2] * eb; }
+      else { lgk[3] = -ld[1] - s.v[0] * eb; lgk[4] = ld[2] + s.v[1] * eb; }
+      lam_k[0] = lam_k[3] = lam_k[4] = harten_abs(s.vn);
+      lam_k[1] = harten_abs(s.vn - s.a);
+      lam_k[2] = harten_abs(s.vn + s.a);
+      double gv[3] = {0, 0, 0}, ab = 0.0;
+      // rows of L: p_k = l_k . du, g_k = |lambda_k| p_k
+#pragma unroll
+      for (int k = 0; k < 5; k++) {
+        double row[NV];
+        l_row<DIR>(s, gm1, k, row);
+        pk[k] = row[0] * du[0] + row[1] * du[1] + row[2] * du[2] + row[3] * du[3] + row[4] * du[4];
+        gk[k] = lam_k[k] * pk[k];
+        double X[NV];
+        const double c = lgk[k] * lam_k[k];
+#pragma unroll
+        for (int v = 0; v < NV; v++) X[v] = c * du[v];
+        l_row_vjp<DIR>(s, gm1, k, X, gv, ab);
+      }
+      // R at the Roe state, with g = |Lambda| L du
+      const double sum = gk[0] + gk[1] + gk[2];
+      for (int j = 0; j < 3; j++) gv[j] += ld[1 + j] * sum;
+      const double h0b = eb * (gk[1] + gk[2]);
+      const double ekb = eb * gk[0] + h0b;
+      gv[DIR] += eb * s.a * (gk[2] - gk[1]);
+      ab += difb * (gk[2] - gk[1]) + h0b * 2.0 * s.a / gm1;
+      if (DIR == 0) { gv[1] -= eb * gk[3]; gv[2] += eb * gk[4]; }
+      else if (DIR == 1) { gv[0] += eb * gk[3]; gv[2] -= eb * gk[4]; }
+      else { gv[1] += eb * gk[4]; gv[0] -= eb * gk[3]; }
+      for (int j = 0; j < 3; j++) gv[j] += ekb * s.v[j];
+      // |lambda_k| with Harten's fix
+      const double hm = harten_d(s.vn - s.a), hp = harten_d(s.vn + s.a);
+      gv[DIR] += harten_d(s.vn) * (lgk[0] * pk[0] + lgk[3] * pk[3] + lgk[4] * pk[4]) + hm * lgk[1] * pk[1] +
+                 hp * lgk[2] * pk[2];
+      ab += hp * lgk[2] * pk[2] - hm * lgk[1] * pk[1];
+      const double ka = ab * gm1 / (2.0 * s.a);
+      for (int j = 0; j < 3; j++) gv[j] -= ka * s.v[j];
+      roe_avg_vjp(u2, u3, gamma, gv, ka, ub2, ub3);
+    } else {
+      if (sel == 0) speed_vjp<DIR>(u2, gamma, abar, ub2);
+      else if (sel == 1) speed_vjp<DIR>(u3, gamma, abar, ub3);
+      else roe_speed_vjp<DIR>(u2, u3, gamma, abar, ub2, ub3);
+    }
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      cF[v * n + t] = ub2[v];
+      cU[v * n + t] = ub3[v];
+    }
+  }
+  __syncthreads();
+  if (active) {
+    const int tp = wrapn(t + 1, n);
+    double uc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      uc[v] = sU[v * n + t];
+      accU[v] += cF[v * n + tp] + cU[v * n + t];
+    }
+    flux_jac_T<DIR>(uc, gamma, accF, accU);
+    const size_t c = base + (size_t)t * stride;
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      const size_t o = (size_t)v * ncell + c;
+      if (mode == VJP_WRITE) {
+        nu[o] = accU[v];
+      } else {
+        const double r = nu[o] + accU[v];
+        nu[o] = r;
+        if (mode == VJP_ADD_L) L[o] += r;
+        else if (mode == VJP_ADD_LSET) L[o] = lam[o] + r;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// characteristic-wise WENO5-JS with Roe or Rusanov upwinding
+// ---------------------------------------------------------------------------
+// The forward flux at an interface is F = R g with, for every characteristic
+// field k (rows of L at the Roe average of the two cells next to it),
+//   g_k = (fL_k + fR_k)/2 - l_k (uR_k - uL_k)/2,
+// fL/fR (uL/uR) the WENO5 values of L f (L u) over the stencil and l_k = |lambda_k|
+// with Harten's fix (Roe) or the Rusanov speed; this is the reference's
+// R (WENO of L f) - R |Lambda| L R (WENO of L u) with L R = I
+// (Interp1PrimFifthOrderWENOChar.c:86-198, NavierStokes3DUpwind.c:40-124).
+// The reverse pass runs field by field: WENO adjoints of the projected
+// stencils, then back through L onto the cells' fluxes and states, while the
+// derivatives of L, R and l_k with respect to the Roe state (v, a) are summed
+// and finally taken back through the Roe average onto cells I-1 and I.  The
+// interpolated values use the modified solution, which equals u exactly in
+// exact arithmetic, so both adjoints land on u.
+constexpr int VJP_CHAR_LINE_MAX = 256;
 
 // nu (+)= (d hyp_DIR-part of rhs / d u)^T mu along one direction for the
 // characteristic scheme; one workgroup per line, one thread per interface,
@@ -921,14 +1013,14 @@ __device__ __forceinline__ void visc_vjp_gather(const double *__restrict__ U, co
 // kernels: one grid
 // ---------------------------------------------------------------------------
 // neighbouring lines share cache lines along x: keep them on one XCD
-template <int DIR>
+template <int DIR, int UPW>
 __global__ void __launch_bounds__(VJP_LINE_MAX) k_rhs_vjp(const double *__restrict__ u, const double *__restrict__ lam,
                                                       const double *__restrict__ nu_in, double ca, double cb,
                                                       double *__restrict__ nu, double *__restrict__ L, int mode,
                                                       int nx, int ny, int nz, const double *__restrict__ dxg,
                                                       double gamma, double eps) {
   extern __shared__ double smem[];
-  rhs_vjp_line<DIR>(u, lam, nu_in, ca, cb, nu, L, mode, nx, ny, nz, dxg, gamma, eps, xcd_block(), smem);
+  rhs_vjp_line<DIR, UPW>(u, lam, nu_in, ca, cb, nu, L, mode, nx, ny, nz, dxg, gamma, eps, xcd_block(), smem);
 }
 
 template <int DIR, int UPW>
@@ -1003,7 +1095,7 @@ __device__ __forceinline__ int ens_find_cell(const EnsMember *mem, int nmem, siz
   a.u + 5 * M.off, a.lam + 5 * M.off, a.nu_in ? a.nu_in + 5 * M.off : nullptr, M.dt * a.A, M.dt * a.B,     \
       a.nu + 5 * M.off, a.L ? a.L + 5 * M.off : nullptr, a.mode, M.n[0], M.n[1], M.n[2], M.dxg, a.gamma, a.eps
 
-template <int DIR>
+template <int DIR, int UPW>
 __global__ void __launch_bounds__(VJP_LINE_MAX) k_ens_rhs_vjp(EnsVjp a, const unsigned *__restrict__ first,
                                                           unsigned nitems) {
   extern __shared__ double smem[];
@@ -1011,7 +1103,7 @@ __global__ void __launch_bounds__(VJP_LINE_MAX) k_ens_rhs_vjp(EnsVjp a, const un
   for (unsigned item = r.first; item < r.end; item += r.step) {
     const int m = ens_find(first, a.nmem, item);
     const EnsMember M = a.mem[m];
-    rhs_vjp_line<DIR>(ENS_MEMBER_ARGS(a, M), item - first[m], smem);
+    rhs_vjp_line<DIR, UPW>(ENS_MEMBER_ARGS(a, M), item - first[m], smem);
     __syncthreads();
   }
 }
@@ -1088,7 +1180,6 @@ __global__ void k_axpby(const double *__restrict__ x, double a, double *__restri
 // ---------------------------------------------------------------------------
 bool vjp_supported(const Grid &g, const Scheme &sc) {
   if (sc.weno != WENO_JS) return false;
-  if (sc.interp == INTERP_COMP && sc.upwind != UPWIND_RUSANOV) return false;
   const int lmax = sc.interp == INTERP_CHAR ? VJP_CHAR_LINE_MAX : VJP_LINE_MAX;
   return g.n[0] <= lmax && g.n[1] <= lmax && g.n[2] <= lmax;
 }
@@ -1107,9 +1198,15 @@ static void launch_vjp_dir(const Grid &g, const Scheme &sc, const double *u, con
   const unsigned threads = (unsigned)((g.n[DIR] + 63) / 64 * 64);
   if (sc.interp == INTERP_COMP) {
     const size_t lds = (size_t)23 * g.n[DIR] * sizeof(double);
-    allow_lds((const void *)k_rhs_vjp<DIR>, lds);
-    hipLaunchKernelGGL(k_rhs_vjp<DIR>, dim3(lines), dim3(threads), lds, st, u, lam, nu_in, ca, cb, nu, L, mode,
-                       g.n[0], g.n[1], g.n[2], dxg, sc.gamma, sc.eps);
+    if (sc.upwind == UPWIND_ROE) {
+      allow_lds((const void *)k_rhs_vjp<DIR, UPWIND_ROE>, lds);
+      hipLaunchKernelGGL((k_rhs_vjp<DIR, UPWIND_ROE>), dim3(lines), dim3(threads), lds, st, u, lam, nu_in, ca, cb,
+                         nu, L, mode, g.n[0], g.n[1], g.n[2], dxg, sc.gamma, sc.eps);
+    } else {
+      allow_lds((const void *)k_rhs_vjp<DIR, UPWIND_RUSANOV>, lds);
+      hipLaunchKernelGGL((k_rhs_vjp<DIR, UPWIND_RUSANOV>), dim3(lines), dim3(threads), lds, st, u, lam, nu_in, ca,
+                         cb, nu, L, mode, g.n[0], g.n[1], g.n[2], dxg, sc.gamma, sc.eps);
+    }
     return;
   }
   const size_t lds = (size_t)70 * g.n[DIR] * sizeof(double);
@@ -1169,8 +1266,15 @@ static void launch_ens_vjp_dir(const Scheme &sc, const EnsVjpPlan &p, const EnsV
   const unsigned threads = (unsigned)((p.nmax[DIR] + 63) / 64 * 64);
   if (sc.interp == INTERP_COMP) {
     const size_t lds = (size_t)23 * p.nmax[DIR] * sizeof(double);
-    allow_lds((const void *)k_ens_rhs_vjp<DIR>, lds);
-    hipLaunchKernelGGL(k_ens_rhs_vjp<DIR>, dim3(grid), dim3(threads), lds, st, a, p.lines0[DIR], items);
+    if (sc.upwind == UPWIND_ROE) {
+      allow_lds((const void *)k_ens_rhs_vjp<DIR, UPWIND_ROE>, lds);
+      hipLaunchKernelGGL((k_ens_rhs_vjp<DIR, UPWIND_ROE>), dim3(grid), dim3(threads), lds, st, a, p.lines0[DIR],
+                         items);
+    } else {
+      allow_lds((const void *)k_ens_rhs_vjp<DIR, UPWIND_RUSANOV>, lds);
+      hipLaunchKernelGGL((k_ens_rhs_vjp<DIR, UPWIND_RUSANOV>), dim3(grid), dim3(threads), lds, st, a,
+                         p.lines0[DIR], items);
+    }
     return;
   }
   const size_t lds = (size_t)70 * p.nmax[DIR] * sizeof(double);

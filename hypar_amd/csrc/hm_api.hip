@@ -576,9 +576,8 @@ static hm_status check_vjp(hm_solver *s, const char *name) {
   if (!s) return fail(HM_ERR_ARG, std::string(name) + ": solver is NULL");
   if (!hm::vjp_supported(s->g, s->sc))
     return fail(HM_ERR_UNSUPPORTED, std::string(name) +
-                                        ": the adjoint is built for WENO5-JS weights with component-wise "
-                                        "Rusanov (grid sides <= 512) or characteristic-wise Roe/Rusanov "
-                                        "(grid sides <= 256) schemes");
+                                        ": the adjoint is built for WENO5-JS weights, component-wise (grid "
+                                        "sides <= 512) or characteristic-wise (grid sides <= 256)");
   return HM_OK;
 }
 
@@ -1167,8 +1166,8 @@ static hm_status ens_check_vjp(hm_ensemble *e, const void *u, const char *name) 
   if (st != HM_OK) return st;
   if (!hm::ens_vjp_supported(e->sc, e->nmax))
     return fail(HM_ERR_UNSUPPORTED, std::string(name) +
-                                        ": the adjoint is built for WENO5-JS weights with component-wise "
-                                        "Rusanov (sides <= 512) or characteristic-wise Roe/Rusanov (sides <= 256)");
+                                        ": the adjoint is built for WENO5-JS weights, component-wise (sides "
+                                        "<= 512) or characteristic-wise (sides <= 256)");
   if (!e->dt_set) return fail(HM_ERR_ARG, std::string(name) + ": set the members' dt first (hm_ensemble_set_dt)");
   return HM_OK;
 }

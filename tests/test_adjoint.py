@@ -1,5 +1,5 @@
-"""Discrete adjoint of the WENO5-JS RK4 step: component-wise Rusanov and
-characteristic-wise Roe/Rusanov, inviscid and viscous.
+"""Discrete adjoint of the WENO5-JS RK4 step: component-wise and
+characteristic-wise, Roe and Rusanov upwinding, inviscid and viscous.
 
 The reference has no adjoint, so the checks are (a) the FP64 CPU reference
 gradient: torch.autograd through oracle/torch_ref.py, whose forward is pinned
@@ -65,7 +65,7 @@ def test_torch_reference_gradient_matches_finite_differences(interp, upwind, Re)
 
 
 # ------------------------------------------------------------------ GPU
-VJP_SCHEMES = [c for c in SCHEMES if not (c[0] == "components" and c[1] == "roe")]
+VJP_SCHEMES = SCHEMES + [("components", "roe", 1000.0)]
 
 
 def _gpu_solver(w):
@@ -118,7 +118,7 @@ def test_rhs_vjp_on_ragged_lines(interp, upwind, Re):
 def test_vjp_rejects_unsupported_schemes():
     from hypar_amd import lib
     from hypar_amd.solver import NS3D
-    for kw in [dict(interp="components", upwind="roe"), dict(weno_type="mapped")]:
+    for kw in [dict(weno_type="mapped"), dict(weno_type="yc", interp="characteristic", upwind="roe")]:
         s = NS3D(DIMS, **kw)
         u = torch.ones(5 * s.ncell, dtype=torch.float64, device="cuda")
         with pytest.raises(lib.HMError) as ex:

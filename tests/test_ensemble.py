@@ -152,7 +152,8 @@ def test_viscous_reference_fixtures():
     assert err <= TOL
 
 
-@pytest.mark.parametrize("interp,upwind,Re", [("components", "rusanov", -1.0), ("characteristic", "roe", 1000.0)])
+@pytest.mark.parametrize("interp,upwind,Re", [("components", "rusanov", -1.0), ("characteristic", "roe", 1000.0),
+                                               ("components", "roe", -1.0)])
 def test_ensemble_adjoint_matches_single_grid(interp, upwind, Re):
     """Batched trajectory gradient (5 steps, checkpoint every 2, each member
     with its own dt) against each member's single-grid adjoint, which
