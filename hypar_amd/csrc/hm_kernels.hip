@@ -224,7 +224,8 @@ __device__ __forceinline__ void visc_flux(double fv[4], const QT &q, const doubl
 // tile's plane k with a 2-cell halo in x and y is staged in LDS for the x-
 // and y-derivatives.  Every plane of Q is thus read from HBM about once per
 // tile, instead of once per stencil point.
-constexpr int VTX = 64, VTY = 4, VKS = 16;
+constexpr int VTX = 64, VTY = 8, VKS = 16;
+constexpr int VT = VTX * VTY;  // threads of a viscous-flux block
 constexpr int VHX = VTX + 4, VHY = VTY + 4;
 
 // tiles (x, y, z-segment) of a grid
@@ -262,8 +263,9 @@ __device__ __forceinline__ void visc_flux_tile(const double *__restrict__ Q, dou
     const size_t pk = (size_t)k * plane;
     for (int c = threadIdx.x; c < VHX * VHY; c += VTX * VTY) {
       const int hx = c % VHX, hy = c / VHX;
-      // x0 - 2 + hx spans [x0 - 2, x0 + VTX + 1], which can exceed 2 nx
-      const int gi = (x0 - 2 + hx + 2 * nx) % nx, gj = wrap(y0 - 2 + hy, ny);
+      // x0 - 2 + hx spans [x0 - 2, x0 + VTX + 1] and y0 - 2 + hy spans
+      // [y0 - 2, y0 + VTY + 1], which can exceed 2 nx and 2 ny
+      const int gi = (x0 - 2 + hx + 2 * nx) % nx, gj = (y0 - 2 + hy + 2 * ny) % ny;
       const QT h = load_q(Q, ncell, pk + (size_t)gj * nx + gi);
       sQ[0][hy][hx] = h.u;
       sQ[1][hy][hx] = h.v;
@@ -377,7 +379,7 @@ __global__ void __launch_bounds__(256) k_visc_ghost(const double *__restrict__ Q
 }
 
 // Viscous fluxes of all members of an ensemble: one persistent launch over
-// work items (member, tile) and (member, 256 ghost positions of one
+// work items (member, tile) and (member, VT ghost positions of one
 // direction); item0v holds each member's first item.
 struct EnsViscArgs {
   const EnsMember *mem;
@@ -388,7 +390,7 @@ struct EnsViscArgs {
 };
 
 __host__ __device__ __forceinline__ unsigned ghost_blocks(size_t ncell, int n) {
-  return (unsigned)((4 * (ncell / n) + 255) / 256);
+  return (unsigned)((4 * (ncell / n) + VT - 1) / VT);
 }
 
 __global__ void __launch_bounds__(VTX * VTY) k_ens_visc(EnsViscArgs a) {
@@ -411,9 +413,9 @@ __global__ void __launch_bounds__(VTX * VTY) k_ens_visc(EnsViscArgs a) {
     r -= tiles;
     const unsigned gx = ghost_blocks(M.ncell, nx), gy = ghost_blocks(M.ncell, ny);
     const size_t t = threadIdx.x;
-    if (r < gx) visc_ghost_one<0>(M.q, M.fvg[0], nx, ny, nz, M.dxg, a.vp, (size_t)r * 256 + t);
-    else if (r < gx + gy) visc_ghost_one<1>(M.q, M.fvg[1], nx, ny, nz, M.dxg, a.vp, (size_t)(r - gx) * 256 + t);
-    else visc_ghost_one<2>(M.q, M.fvg[2], nx, ny, nz, M.dxg, a.vp, (size_t)(r - gx - gy) * 256 + t);
+    if (r < gx) visc_ghost_one<0>(M.q, M.fvg[0], nx, ny, nz, M.dxg, a.vp, (size_t)r * VT + t);
+    else if (r < gx + gy) visc_ghost_one<1>(M.q, M.fvg[1], nx, ny, nz, M.dxg, a.vp, (size_t)(r - gx) * VT + t);
+    else visc_ghost_one<2>(M.q, M.fvg[2], nx, ny, nz, M.dxg, a.vp, (size_t)(r - gx - gy) * VT + t);
   }
 }
 
