@@ -308,16 +308,15 @@ __device__ __forceinline__ void rhs_vjp_line(const double *__restrict__ u, const
   const int tm = active ? wrapn(t - 1, n) : 0;
   double g[NV] = {0, 0, 0, 0, 0};
   if (active) {
-    const size_t c = base + (size_t)t * stride, cm = base + (size_t)tm * stride;
+    const size_t c = base + (size_t)t * stride;
     double us[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) {
       us[v] = u[(size_t)v * ncell + c];
       sU[v * n + t] = us[v];
       const size_t o = (size_t)v * ncell;
-      const double mu = cb * lam[o + c] + (nu_in ? ca * nu_in[o + c] : 0.0);
-      const double mum = cb * lam[o + cm] + (nu_in ? ca * nu_in[o + cm] : 0.0);
-      g[v] = dxd[t] * mu - dxd[tm] * mum;
+      // dxinv mu of this cell, staged in the (not yet used) slot buffer
+      cF[v * n + t] = dxd[t] * (cb * lam[o + c] + (nu_in ? ca * nu_in[o + c] : 0.0));
     }
     const Prim q = flowvar(us, gamma);
     double fl[NV];
@@ -326,6 +325,10 @@ __device__ __forceinline__ void rhs_vjp_line(const double *__restrict__ u, const
     for (int v = 0; v < NV; v++) sFl[v * n + t] = fl[v];
     sC4[t] = modified_energy(q, inv_gm1);
   }
+  __syncthreads();
+  if (active)
+#pragma unroll
+    for (int v = 0; v < NV; v++) g[v] = cF[v * n + t] - cF[v * n + tm];
   __syncthreads();
   int cs[6];
   double alpha = 0.0, abar = 0.0;
@@ -580,18 +583,15 @@ __device__ __forceinline__ void rhs_vjp_char_line(
   const double gm1 = gamma - 1.0;
   double lF[NV] = {0, 0, 0, 0, 0};  // adjoint of the interface flux
   if (active) {
-    for (int k = 0; k < 30; k++) cF[k * n + t] = cU[k * n + t] = 0.0;
-    const int tm = wrapn(t - 1, n);
-    const size_t c = base + (size_t)t * stride, cm = base + (size_t)tm * stride;
+    const size_t c = base + (size_t)t * stride;
     double us[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) {
       us[v] = u[(size_t)v * ncell + c];
       sU[v * n + t] = us[v];
       const size_t o = (size_t)v * ncell;
-      const double mu = cb * lam[o + c] + (nu_in ? ca * nu_in[o + c] : 0.0);
-      const double mum = cb * lam[o + cm] + (nu_in ? ca * nu_in[o + cm] : 0.0);
-      lF[v] = dxd[t] * mu - dxd[tm] * mum;
+      // dxinv mu of this cell, staged in the (not yet used) slot buffer
+      cU[v * n + t] = dxd[t] * (cb * lam[o + c] + (nu_in ? ca * nu_in[o + c] : 0.0));
     }
     const Prim q = flowvar(us, gamma);
     double fl[NV];
@@ -600,6 +600,14 @@ __device__ __forceinline__ void rhs_vjp_char_line(
     for (int v = 0; v < NV; v++) sF[v * n + t] = fl[v];
   }
   __syncthreads();
+  if (active) {
+    const int tm = wrapn(t - 1, n);
+#pragma unroll
+    for (int v = 0; v < NV; v++) lF[v] = cU[v * n + t] - cU[v * n + tm];
+  }
+  __syncthreads();
+  if (active)
+    for (int k = 0; k < 30; k++) cF[k * n + t] = cU[k * n + t] = 0.0;
   if (active) {
     int cs[6];
 #pragma unroll
