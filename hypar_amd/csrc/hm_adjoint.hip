@@ -1252,10 +1252,12 @@ void launch_rhs_vjp(const Grid &g, const Scheme &sc, const double *u, const doub
     hipLaunchKernelGGL(k_visc_vjp_gather, dim3(nb), dim3(256), 0, st, u, QB, GB, nu, g.n[0], g.n[1], g.n[2],
                        sc.gamma);
   }
+  // z first and x last: the strided z lines take the plain store, the
+  // contiguous x lines the read-modify-write of nu and L
   const int last = L ? (set_L ? VJP_ADD_LSET : VJP_ADD_L) : VJP_ADD;
-  launch_vjp_dir<0>(g, sc, u, lam, nu_in, ca, cb, nu, L, visc ? VJP_ADD : VJP_WRITE, dxg, st);
+  launch_vjp_dir<2>(g, sc, u, lam, nu_in, ca, cb, nu, L, visc ? VJP_ADD : VJP_WRITE, dxg, st);
   launch_vjp_dir<1>(g, sc, u, lam, nu_in, ca, cb, nu, L, VJP_ADD, dxg, st);
-  launch_vjp_dir<2>(g, sc, u, lam, nu_in, ca, cb, nu, L, last, dxg, st);
+  launch_vjp_dir<0>(g, sc, u, lam, nu_in, ca, cb, nu, L, last, dxg, st);
 }
 
 bool ens_vjp_supported(const Scheme &sc, const int nmax[3]) {
@@ -1327,12 +1329,12 @@ void launch_ens_rhs_vjp(const Scheme &sc, const EnsVjpPlan &p, const double *u, 
                        p.nlines[2]);
     hipLaunchKernelGGL(k_ens_visc_vjp_gather, dim3(nb), dim3(256), 0, st, a, p.ncell);
   }
-  a.mode = visc ? VJP_ADD : VJP_WRITE;
-  launch_ens_vjp_dir<0>(sc, p, a, st);
+  a.mode = visc ? VJP_ADD : VJP_WRITE;  // z first, x last (as for one grid)
+  launch_ens_vjp_dir<2>(sc, p, a, st);
   a.mode = VJP_ADD;
   launch_ens_vjp_dir<1>(sc, p, a, st);
   a.mode = L ? (set_L ? VJP_ADD_LSET : VJP_ADD_L) : VJP_ADD;
-  launch_ens_vjp_dir<2>(sc, p, a, st);
+  launch_ens_vjp_dir<0>(sc, p, a, st);
 }
 
 void launch_axpby(const double *x, double a, double *y, double b, size_t n, hipStream_t st) {
