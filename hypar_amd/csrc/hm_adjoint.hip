@@ -555,13 +555,23 @@ __device__ __forceinline__ void rhs_vjp_line(const double *__restrict__ u, const
 // interpolated values use the modified solution, which equals u exactly in
 // exact arithmetic, so both adjoints land on u.
 constexpr int VJP_CHAR_LINE_MAX = 256;
+constexpr int VJP_CHAR_CS = 11;   // doubles per cell record: state [5], flux [5], pad
+constexpr int VJP_CHAR_ES = 11;   // doubles per exchange record
+constexpr int VJP_CHAR_LDS = VJP_CHAR_CS + VJP_CHAR_ES;  // doubles of LDS per cell of the line
 
 // nu (+)= (d hyp_DIR-part of rhs / d u)^T mu along one direction for the
 // characteristic scheme; one workgroup per line, one thread per interface,
-// modes as k_rhs_vjp.  LDS (dynamic, 70 doubles per cell of the line): state
-// [5][n], cell flux [5][n], and the flux and state adjoint contributions of
-// every interface to its six stencil cells, [6][5][n] each; every cell then
-// sums its six slots in a fixed order.
+// modes as k_rhs_vjp.  LDS (dynamic, VJP_CHAR_LDS doubles per cell of the
+// line), cell-major so that one address per stencil cell serves all its
+// variables through immediate offsets, with odd record lengths so that the
+// 64-bit accesses of a wave spread over all banks: a record per cell of its
+// state and flux [n][11] and an exchange record per interface [n][11].
+// The WENO5 of L f and the WENO5 of L u are taken back in two parts of five
+// fields each.  After each field k every interface posts the adjoints of its
+// six projected values and its row of L; every cell then takes the six posts
+// that touch it back through those rows, in a fixed order, so the sums are
+// deterministic.  The Roe-state derivatives of R, L and |lambda_k| are summed
+// field by field in registers.
 template <int DIR, int UPW>
 __device__ __forceinline__ void rhs_vjp_char_line(
     const double *__restrict__ u, const double *__restrict__ lam, const double *__restrict__ nu_in, double ca,
@@ -574,10 +584,9 @@ __device__ __forceinline__ void rhs_vjp_char_line(
   else if (DIR == 1) { base = (line % nx) + (line / nx) * (size_t)nx * ny; stride = nx; }
   else { base = line; stride = (size_t)nx * ny; }
   const double *dxd = dxg + 3 + (DIR >= 1 ? nx + 6 : 0) + (DIR == 2 ? ny + 6 : 0);
-  double *sU = smem;           // [5][n]
-  double *sF = sU + 5 * n;     // [5][n]
-  double *cF = sF + 5 * n;     // [6][5][n]
-  double *cU = cF + 30 * n;    // [6][5][n]
+  constexpr int CS = VJP_CHAR_CS, ES = VJP_CHAR_ES;
+  double *sC = smem;        // [n][CS]: state [5], flux [5]
+  double *E = sC + CS * n;  // [n][ES]: six stencil adjoints [6], row of L [5]
   const int t = threadIdx.x;
   const bool active = t < n;
   const double gm1 = gamma - 1.0;
@@ -588,136 +597,163 @@ __device__ __forceinline__ void rhs_vjp_char_line(
 #pragma unroll
     for (int v = 0; v < NV; v++) {
       us[v] = u[(size_t)v * ncell + c];
-      sU[v * n + t] = us[v];
+      sC[t * CS + v] = us[v];
       const size_t o = (size_t)v * ncell;
-      // dxinv mu of this cell, staged in the (not yet used) slot buffer
-      cU[v * n + t] = dxd[t] * (cb * lam[o + c] + (nu_in ? ca * nu_in[o + c] : 0.0));
+      // dxinv mu of this cell, staged in the (not yet used) exchange buffer
+      E[t * ES + v] = dxd[t] * (cb * lam[o + c] + (nu_in ? ca * nu_in[o + c] : 0.0));
     }
     const Prim q = flowvar(us, gamma);
     double fl[NV];
     euler_flux<DIR>(fl, q);
 #pragma unroll
-    for (int v = 0; v < NV; v++) sF[v * n + t] = fl[v];
+    for (int v = 0; v < NV; v++) sC[t * CS + NV + v] = fl[v];
   }
   __syncthreads();
   if (active) {
     const int tm = wrapn(t - 1, n);
 #pragma unroll
-    for (int v = 0; v < NV; v++) lF[v] = cU[v * n + t] - cU[v * n + tm];
+    for (int v = 0; v < NV; v++) lF[v] = E[t * ES + v] - E[tm * ES + v];
   }
   __syncthreads();
-  if (active)
-    for (int k = 0; k < 30; k++) cF[k * n + t] = cU[k * n + t] = 0.0;
-  if (active) {
-    int cs[6];
+  int cs[6];
 #pragma unroll
-    for (int p = 0; p < 6; p++) cs[p] = wrapn(t + p - 3, n);
+  for (int p = 0; p < 6; p++) cs[p] = active ? wrapn(t + p - 3, n) : 0;
+  RoeAdj s = {};
+  double ls = 0.0, lm = 0.0, lp = 0.0, sumb = 0.0, difb = 0.0;
+  int sel = 0;
+  const double eb = lF[4];
+  if (active) {
     double u2[NV], u3[NV];
 #pragma unroll
-    for (int v = 0; v < NV; v++) { u2[v] = sU[v * n + cs[2]]; u3[v] = sU[v * n + cs[3]]; }
-    const RoeAdj s = roe_adj_state<DIR>(u2, u3, gamma);
-    // Rusanov speed: max(max(aL, aR), max(aR, aA)), the branch the forward selects
-    double alpha = 0.0;
-    int sel = 0;
+    for (int v = 0; v < NV; v++) { u2[v] = sC[cs[2] * CS + v]; u3[v] = sC[cs[3] * CS + v]; }
+    s = roe_adj_state<DIR>(u2, u3, gamma);
     if (UPW == UPWIND_RUSANOV) {
+      // max(max(aL, aR), max(aR, aA)), the branch the forward selects
       const Prim q2 = flowvar(u2, gamma), q3 = flowvar(u3, gamma);
       const double aL = sqrt(gamma * q2.P / q2.rho) + fabs(vel<DIR>(q2));
       const double aR = sqrt(gamma * q3.P / q3.rho) + fabs(vel<DIR>(q3));
       const double aA = s.a + fabs(s.vn);
       const double m01 = fmax(aL, aR), m12 = fmax(aR, aA);
-      alpha = fmax(m01, m12);
+      ls = lm = lp = fmax(m01, m12);
       sel = m01 >= m12 ? (aL >= aR ? 0 : 1) : (aR >= aA ? 1 : 2);
+    } else {
+      ls = harten_abs(s.vn);
+      lm = harten_abs(s.vn - s.a);
+      lp = harten_abs(s.vn + s.a);
     }
     // lambda_g = R^T lambda_F (columns of R, navierstokes3d.h:384-470)
-    const double eb = lF[4];
-    const double sumb = lF[0] + s.v[0] * lF[1] + s.v[1] * lF[2] + s.v[2] * lF[3];
-    const double difb = lF[1 + DIR] + s.vn * eb;
-    double gbk[5];
-    gbk[0] = sumb + s.ek * eb;
-    gbk[1] = sumb + s.h0 * eb - s.a * difb;
-    gbk[2] = sumb + s.h0 * eb + s.a * difb;
-    if (DIR == 0) { gbk[3] = -lF[2] - s.v[1] * eb; gbk[4] = lF[3] + s.v[2] * eb; }
-    else if (DIR == 1) { gbk[3] = lF[1] + s.v[0] * eb; gbk[4] = -lF[3] - s.v[2] * eb; }
-    else { gbk[3] = -lF[1] - s.v[0] * eb; gbk[4] = lF[2] + s.v[1] * eb; }
-    const double ls = UPW == UPWIND_ROE ? harten_abs(s.vn) : alpha;
-    const double lm = UPW == UPWIND_ROE ? harten_abs(s.vn - s.a) : alpha;
-    const double lp = UPW == UPWIND_ROE ? harten_abs(s.vn + s.a) : alpha;
-    double gv[3] = {0, 0, 0}, ab = 0.0;
-    double g[5], lb[5];
+    sumb = lF[0] + s.v[0] * lF[1] + s.v[1] * lF[2] + s.v[2] * lF[3];
+    difb = lF[1 + DIR] + s.vn * eb;
+  }
+  double gv[3] = {0, 0, 0}, ab = 0.0, alb = 0.0;
+  // part 0 takes the WENO5 of L f (left and right biased) back onto the cell
+  // fluxes, part 1 the WENO5 of L u onto the cell states; in between, the
+  // flux adjoint of every cell goes through its flux Jacobian
+  double acc[NV] = {0, 0, 0, 0, 0};
+#pragma unroll 1
+  for (int part = 0; part < 2; part++) {
+    const int S = part == 0 ? NV : 0;  // flux or state within the cell record
 #pragma unroll 1
     for (int k = 0; k < 5; k++) {
-      double row[NV];
-      l_row<DIR>(s, gm1, k, row);
-      // projected stencils
-      double pF[6], pU[6];
+      if (active) {
+        double row[NV];
+        l_row<DIR>(s, gm1, k, row);
+        double gb;  // (R^T lambda_F)_k
+        if (k == 0) gb = sumb + s.ek * eb;
+        else if (k == 1) gb = sumb + s.h0 * eb - s.a * difb;
+        else if (k == 2) gb = sumb + s.h0 * eb + s.a * difb;
+        else if (DIR == 0) gb = k == 3 ? -lF[2] - s.v[1] * eb : lF[3] + s.v[2] * eb;
+        else if (DIR == 1) gb = k == 3 ? lF[1] + s.v[0] * eb : -lF[3] - s.v[2] * eb;
+        else gb = k == 3 ? -lF[1] - s.v[0] * eb : lF[2] + s.v[1] * eb;
+        const double lk = k == 1 ? lm : (k == 2 ? lp : ls);
+        double pr[6], b[6], m[5], mb[5];
 #pragma unroll
-      for (int p = 0; p < 6; p++) {
-        double f = 0.0, x = 0.0;
+        for (int p = 0; p < 6; p++) {
+          double x = 0.0;
 #pragma unroll
-        for (int v = 0; v < NV; v++) {
-          f += row[v] * sF[v * n + cs[p]];
-          x += row[v] * sU[v * n + cs[p]];
+          for (int v = 0; v < NV; v++) x += row[v] * sC[cs[p] * CS + S + v];
+          pr[p] = x;
         }
-        pF[p] = f;
-        pU[p] = x;
+        const double obL = part == 0 ? 0.5 * gb : 0.5 * lk * gb;
+#pragma unroll
+        for (int j = 0; j < 5; j++) { m[j] = pr[j]; mb[j] = 0.0; }
+        const double wl = weno5_vjp(m, m, eps, obL, mb, mb);
+#pragma unroll
+        for (int j = 0; j < 5; j++) { b[j] = mb[j]; m[j] = pr[5 - j]; mb[j] = 0.0; }
+        b[5] = 0.0;
+        const double wr = weno5_vjp(m, m, eps, part == 0 ? obL : -obL, mb, mb);
+#pragma unroll
+        for (int j = 0; j < 5; j++) b[5 - j] += mb[j];
+        // post the six adjoints and the row; X for the row's derivatives
+        double X[NV] = {0, 0, 0, 0, 0};
+#pragma unroll
+        for (int p = 0; p < 6; p++) {
+          E[t * ES + p] = b[p];
+#pragma unroll
+          for (int v = 0; v < NV; v++) X[v] += b[p] * sC[cs[p] * CS + S + v];
+        }
+#pragma unroll
+        for (int v = 0; v < NV; v++) E[t * ES + 6 + v] = row[v];
+        // this part's share of the forward g_k = (fL + fR)/2 - l_k (uR - uL)/2
+        const double gk = part == 0 ? 0.5 * (wl + wr) : -0.5 * lk * (wr - wl);
+        // term k of R g at the Roe state
+        if (k < 3) {
+#pragma unroll
+          for (int j = 0; j < 3; j++) gv[j] += (lF[1 + j] + eb * s.v[j]) * gk;
+          if (k > 0) {
+            const double sg = k == 2 ? 1.0 : -1.0;
+            gv[DIR] += sg * eb * s.a * gk;
+            ab += sg * difb * gk + eb * gk * 2.0 * s.a / gm1;
+          }
+        } else if (DIR == 0) {
+          if (k == 3) gv[1] -= eb * gk; else gv[2] += eb * gk;
+        } else if (DIR == 1) {
+          if (k == 3) gv[0] += eb * gk; else gv[2] -= eb * gk;
+        } else {
+          if (k == 3) gv[0] -= eb * gk; else gv[1] += eb * gk;
+        }
+        // l_k: Harten's |lambda| (Roe) or the common Rusanov speed
+        if (part == 1) {
+          const double lbk = -0.5 * (wr - wl) * gb;  // adjoint of l_k
+          if (UPW == UPWIND_ROE) {
+            if (k == 1) { const double h = harten_d(s.vn - s.a) * lbk; gv[DIR] += h; ab -= h; }
+            else if (k == 2) { const double h = harten_d(s.vn + s.a) * lbk; gv[DIR] += h; ab += h; }
+            else gv[DIR] += harten_d(s.vn) * lbk;
+          } else {
+            alb += lbk;
+          }
+        }
+        l_row_vjp<DIR>(s, gm1, k, X, gv, ab);
       }
-      const double gbv = k == 0 ? gbk[0] : (k == 1 ? gbk[1] : (k == 2 ? gbk[2] : (k == 3 ? gbk[3] : gbk[4])));
-      const double lk = k == 1 ? lm : (k == 2 ? lp : ls);
-      double bF[6] = {0, 0, 0, 0, 0, 0}, bU[6] = {0, 0, 0, 0, 0, 0};
-      double m[5], mb[5];
+      __syncthreads();
+      // cell t takes post p of interface t+3-p, p = 0..5, back through that
+      // interface's row
+      if (active) {
 #pragma unroll
-      for (int j = 0; j < 5; j++) { m[j] = pF[j]; mb[j] = 0.0; }
-      const double fl = weno5_vjp(m, m, eps, 0.5 * gbv, mb, mb);
+        for (int p = 0; p < 6; p++) {
+          const double *e = E + wrapn(t + 3 - p, n) * ES;
+          const double bp = e[p];
 #pragma unroll
-      for (int j = 0; j < 5; j++) { bF[j] += mb[j]; m[j] = pF[5 - j]; mb[j] = 0.0; }
-      const double fr = weno5_vjp(m, m, eps, 0.5 * gbv, mb, mb);
-#pragma unroll
-      for (int j = 0; j < 5; j++) { bF[5 - j] += mb[j]; m[j] = pU[j]; mb[j] = 0.0; }
-      const double ul = weno5_vjp(m, m, eps, 0.5 * lk * gbv, mb, mb);
-#pragma unroll
-      for (int j = 0; j < 5; j++) { bU[j] += mb[j]; m[j] = pU[5 - j]; mb[j] = 0.0; }
-      const double ur = weno5_vjp(m, m, eps, -0.5 * lk * gbv, mb, mb);
-#pragma unroll
-      for (int j = 0; j < 5; j++) bU[5 - j] += mb[j];
-      const double gk = 0.5 * (fl + fr) - lk * (0.5 * (ur - ul));
-      const double lbk = -0.5 * (ur - ul) * gbv;
-      if (k == 0) { g[0] = gk; lb[0] = lbk; }
-      else if (k == 1) { g[1] = gk; lb[1] = lbk; }
-      else if (k == 2) { g[2] = gk; lb[2] = lbk; }
-      else if (k == 3) { g[3] = gk; lb[3] = lbk; }
-      else { g[4] = gk; lb[4] = lbk; }
-      // back through the row onto the stencil cells; X for the row's derivatives
-      double X[NV] = {0, 0, 0, 0, 0};
-#pragma unroll
-      for (int p = 0; p < 6; p++) {
-#pragma unroll
-        for (int v = 0; v < NV; v++) {
-          X[v] += bF[p] * sF[v * n + cs[p]] + bU[p] * sU[v * n + cs[p]];
-          cF[(p * NV + v) * n + t] += bF[p] * row[v];
-          cU[(p * NV + v) * n + t] += bU[p] * row[v];
+          for (int v = 0; v < NV; v++) acc[v] += bp * e[6 + v];
         }
       }
-      l_row_vjp<DIR>(s, gm1, k, X, gv, ab);
+      __syncthreads();
     }
-    // R (columns of navierstokes3d.h:384-470) at the Roe state
-    const double sum = g[0] + g[1] + g[2];
-    for (int j = 0; j < 3; j++) gv[j] += lF[1 + j] * sum;
-    const double h0b = eb * (g[1] + g[2]);
-    const double ekb = eb * g[0] + h0b;
-    gv[DIR] += eb * s.a * (g[2] - g[1]);
-    ab += difb * (g[2] - g[1]) + h0b * 2.0 * s.a / gm1;
-    if (DIR == 0) { gv[1] -= eb * g[3]; gv[2] += eb * g[4]; }
-    else if (DIR == 1) { gv[0] += eb * g[3]; gv[2] -= eb * g[4]; }
-    else { gv[1] += eb * g[4]; gv[0] -= eb * g[3]; }
-    for (int j = 0; j < 3; j++) gv[j] += ekb * s.v[j];
-    // eigenvalues
-    double ub2[NV] = {0, 0, 0, 0, 0}, ub3[NV] = {0, 0, 0, 0, 0};
-    if (UPW == UPWIND_ROE) {
-      const double hm = harten_d(s.vn - s.a), hp = harten_d(s.vn + s.a);
-      gv[DIR] += harten_d(s.vn) * (lb[0] + lb[3] + lb[4]) + hm * lb[1] + hp * lb[2];
-      ab += hp * lb[2] - hm * lb[1];
-    } else {
-      const double alb = lb[0] + lb[1] + lb[2] + lb[3] + lb[4];
+    if (part == 0 && active) {
+      double uc[NV], ub[NV] = {0, 0, 0, 0, 0};
+#pragma unroll
+      for (int v = 0; v < NV; v++) uc[v] = sC[t * CS + v];
+      flux_jac_T<DIR>(uc, gamma, acc, ub);
+#pragma unroll
+      for (int v = 0; v < NV; v++) acc[v] = ub[v];
+    }
+  }
+  // the Roe state's (and the Rusanov speed's) dependence on cells t-1 and t
+  if (active) {
+    double u2[NV], u3[NV], ub2[NV] = {0, 0, 0, 0, 0}, ub3[NV] = {0, 0, 0, 0, 0};
+#pragma unroll
+    for (int v = 0; v < NV; v++) { u2[v] = sC[cs[2] * CS + v]; u3[v] = sC[cs[3] * CS + v]; }
+    if (UPW == UPWIND_RUSANOV) {
       if (sel == 0) speed_vjp<DIR>(u2, gamma, alb, ub2);
       else if (sel == 1) speed_vjp<DIR>(u3, gamma, alb, ub3);
       else roe_speed_vjp<DIR>(u2, u3, gamma, alb, ub2, ub3);
@@ -728,38 +764,23 @@ __device__ __forceinline__ void rhs_vjp_char_line(
     roe_avg_vjp(u2, u3, gamma, gv, ka, ub2, ub3);
 #pragma unroll
     for (int v = 0; v < NV; v++) {
-      cU[(2 * NV + v) * n + t] += ub2[v];
-      cU[(3 * NV + v) * n + t] += ub3[v];
+      E[t * ES + v] = ub2[v];
+      E[t * ES + NV + v] = ub3[v];
     }
   }
   __syncthreads();
-  // slot p of interface i belongs to cell i-3+p: cell t sums slot p of
-  // interface t+3-p, p = 0..5, in that order
-  double accF[NV] = {0, 0, 0, 0, 0}, accU[NV] = {0, 0, 0, 0, 0};
   if (active) {
+    const int tp = wrapn(t + 1, n);
 #pragma unroll
-    for (int p = 0; p < 6; p++) {
-      const int i = wrapn(t + 3 - p, n);
-#pragma unroll
-      for (int v = 0; v < NV; v++) {
-        accF[v] += cF[(p * NV + v) * n + i];
-        accU[v] += cU[(p * NV + v) * n + i];
-      }
-    }
-  }
-  if (active) {
-    double uc[NV];
-#pragma unroll
-    for (int v = 0; v < NV; v++) uc[v] = sU[v * n + t];
-    flux_jac_T<DIR>(uc, gamma, accF, accU);
+    for (int v = 0; v < NV; v++) acc[v] += E[tp * ES + v] + E[t * ES + NV + v];
     const size_t c = base + (size_t)t * stride;
 #pragma unroll
     for (int v = 0; v < NV; v++) {
       const size_t o = (size_t)v * ncell + c;
       if (mode == VJP_WRITE) {
-        nu[o] = accU[v];
+        nu[o] = acc[v];
       } else {
-        const double r = nu[o] + accU[v];
+        const double r = nu[o] + acc[v];
         nu[o] = r;
         if (mode == VJP_ADD_L) L[o] += r;
         else if (mode == VJP_ADD_LSET) L[o] = lam[o] + r;
@@ -1217,7 +1238,7 @@ static void launch_vjp_dir(const Grid &g, const Scheme &sc, const double *u, con
     }
     return;
   }
-  const size_t lds = (size_t)70 * g.n[DIR] * sizeof(double);
+  const size_t lds = (size_t)VJP_CHAR_LDS * g.n[DIR] * sizeof(double);
   if (sc.upwind == UPWIND_ROE) {
     allow_lds((const void *)k_rhs_vjp_char<DIR, UPWIND_ROE>, lds);
     hipLaunchKernelGGL((k_rhs_vjp_char<DIR, UPWIND_ROE>), dim3(lines), dim3(threads), lds, st, u, lam, nu_in, ca,
@@ -1287,7 +1308,7 @@ static void launch_ens_vjp_dir(const Scheme &sc, const EnsVjpPlan &p, const EnsV
     }
     return;
   }
-  const size_t lds = (size_t)70 * p.nmax[DIR] * sizeof(double);
+  const size_t lds = (size_t)VJP_CHAR_LDS * p.nmax[DIR] * sizeof(double);
   if (sc.upwind == UPWIND_ROE) {
     allow_lds((const void *)k_ens_rhs_vjp_char<DIR, UPWIND_ROE>, lds);
     hipLaunchKernelGGL((k_ens_rhs_vjp_char<DIR, UPWIND_ROE>), dim3(grid), dim3(threads), lds, st, a,
