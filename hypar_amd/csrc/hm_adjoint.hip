@@ -87,6 +87,76 @@ __device__ __forceinline__ double weno5_vjp(const double m[5], const double g[5]
   return out;
 }
 
+// one side of a WENO5-JS interpolation from its smoothness indicators
+// beta_k and candidate values f_k: the interpolated value, the weights, and
+// ob times d out/d beta_k = -2 ob w_k (f_k - out)/(beta_k + eps).  One
+// reciprocal, of (sum a_k)(prod (beta_k + eps)), serves both 1/sum a_k and
+// the 1/(beta_k + eps).
+struct W5Side {
+  double out, w1, w2, w3, g1, g2, g3;
+};
+
+__device__ __forceinline__ W5Side w5_side(double b1, double b2, double b3, double f1, double f2, double f3,
+                                          double eps, double ob) {
+  const double e1 = b1 + eps, e2 = b2 + eps, e3 = b3 + eps;
+  const double s1 = e1 * e1, s2 = e2 * e2, s3 = e3 * e3;
+  const double a1 = 0.1 * (s2 * s3), a2 = 0.6 * (s1 * s3), a3 = 0.3 * (s1 * s2);
+  const double A = a1 + a2 + a3, e23 = e2 * e3, P = e1 * e23;
+  const double q = vjp_rcp(A * P);
+  const double inv = q * P, k = -2.0 * ob * (q * A);
+  W5Side r;
+  r.w1 = a1 * inv;
+  r.w2 = a2 * inv;
+  r.w3 = a3 * inv;
+  r.out = r.w1 * f1 + r.w2 * f2 + r.w3 * f3;
+  r.g1 = k * r.w1 * (f1 - r.out) * e23;
+  r.g2 = k * r.w2 * (f2 - r.out) * (e1 * e3);
+  r.g3 = k * r.w3 * (f3 - r.out) * (e1 * e2);
+  return r;
+}
+
+// reverse of the left- and right-biased WENO5-JS of one interface over one
+// stencil b[0..5] (cells I-3..I+2) that supplies both the smoothness
+// indicators and the interpolated values: adds obL d vl/d b + obR d vr/d b
+// to bb and returns vl, vr.  The second differences the two sides share are
+// formed once and their adjoints summed.
+__device__ __forceinline__ void weno5_lr_vjp(const double b[6], double eps, double obL, double obR, double bb[6],
+                                             double &vl, double &vr) {
+  const double c1312 = 13.0 / 12.0, c14 = 0.25, s6 = 1.0 / 6.0;
+  const double d1 = b[0] - 2 * b[1] + b[2], d2 = b[1] - 2 * b[2] + b[3];
+  const double d3 = b[2] - 2 * b[3] + b[4], d4 = b[3] - 2 * b[4] + b[5];
+  const double q2 = c1312 * (d2 * d2), q3 = c1312 * (d3 * d3);
+  const double rl1 = b[0] - 4 * b[1] + 3 * b[2], rl2 = b[1] - b[3], rl3 = 3 * b[2] - 4 * b[3] + b[4];
+  const double rr1 = b[5] - 4 * b[4] + 3 * b[3], rr2 = b[4] - b[2], rr3 = 3 * b[3] - 4 * b[2] + b[1];
+  // left: second differences d1, d2, d3; right (mirrored): d4, d3, d2
+  const W5Side L = w5_side(c1312 * (d1 * d1) + c14 * (rl1 * rl1), q2 + c14 * (rl2 * rl2), q3 + c14 * (rl3 * rl3),
+                           (2 * s6) * b[0] + (-7 * s6) * b[1] + (11 * s6) * b[2],
+                           (-s6) * b[1] + (5 * s6) * b[2] + (2 * s6) * b[3],
+                           (2 * s6) * b[2] + (5 * s6) * b[3] + (-s6) * b[4], eps, obL);
+  const W5Side R = w5_side(c1312 * (d4 * d4) + c14 * (rr1 * rr1), q3 + c14 * (rr2 * rr2), q2 + c14 * (rr3 * rr3),
+                           (2 * s6) * b[5] + (-7 * s6) * b[4] + (11 * s6) * b[3],
+                           (-s6) * b[4] + (5 * s6) * b[3] + (2 * s6) * b[2],
+                           (2 * s6) * b[3] + (5 * s6) * b[2] + (-s6) * b[1], eps, obR);
+  vl = L.out;
+  vr = R.out;
+  // interpolated values
+  const double wl1 = obL * L.w1, wl2 = obL * L.w2, wl3 = obL * L.w3;
+  const double wr1 = obR * R.w1, wr2 = obR * R.w2, wr3 = obR * R.w3;
+  // smoothness indicators: second differences and the one-sided differences
+  const double D1 = (2 * c1312) * d1 * L.g1, D2 = (2 * c1312) * d2 * (L.g2 + R.g3);
+  const double D3 = (2 * c1312) * d3 * (L.g3 + R.g2), D4 = (2 * c1312) * d4 * R.g1;
+  const double L1 = (2 * c14) * rl1 * L.g1, L2 = (2 * c14) * rl2 * L.g2, L3 = (2 * c14) * rl3 * L.g3;
+  const double R1 = (2 * c14) * rr1 * R.g1, R2 = (2 * c14) * rr2 * R.g2, R3 = (2 * c14) * rr3 * R.g3;
+  bb[0] += (2 * s6) * wl1 + D1 + L1;
+  bb[1] += (-7 * s6) * wl1 + (-s6) * wl2 + (-s6) * wr3 - 2 * D1 + D2 - 4 * L1 + L2 + R3;
+  bb[2] += (11 * s6) * wl1 + (5 * s6) * wl2 + (2 * s6) * wl3 + (2 * s6) * wr2 + (5 * s6) * wr3 + D1 - 2 * D2 + D3 +
+           3 * L1 + 3 * L3 - R2 - 4 * R3;
+  bb[3] += (2 * s6) * wl2 + (5 * s6) * wl3 + (11 * s6) * wr1 + (5 * s6) * wr2 + (2 * s6) * wr3 + D2 - 2 * D3 + D4 -
+           L2 - 4 * L3 + 3 * R1 + 3 * R3;
+  bb[4] += (-s6) * wl3 + (-7 * s6) * wr1 + (-s6) * wr2 + D3 - 2 * D4 + L3 - 4 * R1 + R2;
+  bb[5] += (2 * s6) * wr1 + D4 + R1;
+}
+
 // gradient of c + |v_d| of one state with respect to its conserved variables
 template <int DIR>
 __device__ __forceinline__ void speed_vjp(const double *u, double gamma, double ab, double ub[NV]) {
@@ -655,9 +725,13 @@ __device__ __forceinline__ void rhs_vjp_char_line(
     const int S = part == 0 ? NV : 0;  // flux or state within the cell record
 #pragma unroll 1
     for (int k = 0; k < 5; k++) {
+      // the shear rows k = 3, 4 of L have two entries: r0 at 0 and +-1 at J
+      const bool shear = k >= 3;
+      constexpr int JA = DIR == 0 ? 2 : 1, JB = DIR == 2 ? 2 : 3;
+      constexpr double CA = DIR == 1 ? 1.0 : -1.0, CB = DIR == 1 ? -1.0 : 1.0;
+      const int J = k == 3 ? JA : JB;
+      const double cJ = k == 3 ? CA : CB;
       if (active) {
-        double row[NV];
-        l_row<DIR>(s, gm1, k, row);
         double gb;  // (R^T lambda_F)_k
         if (k == 0) gb = sumb + s.ek * eb;
         else if (k == 1) gb = sumb + s.h0 * eb - s.a * difb;
@@ -666,34 +740,45 @@ __device__ __forceinline__ void rhs_vjp_char_line(
         else if (DIR == 1) gb = k == 3 ? lF[1] + s.v[0] * eb : -lF[3] - s.v[2] * eb;
         else gb = k == 3 ? -lF[1] - s.v[0] * eb : lF[2] + s.v[1] * eb;
         const double lk = k == 1 ? lm : (k == 2 ? lp : ls);
-        double pr[6], b[6], m[5], mb[5];
+        double row[NV], pr[6], b[6];
+        if (!shear) {
+          l_row<DIR>(s, gm1, k, row);
 #pragma unroll
-        for (int p = 0; p < 6; p++) {
-          double x = 0.0;
+          for (int p = 0; p < 6; p++) {
+            double x = 0.0;
 #pragma unroll
-          for (int v = 0; v < NV; v++) x += row[v] * sC[cs[p] * CS + S + v];
-          pr[p] = x;
+            for (int v = 0; v < NV; v++) x += row[v] * sC[cs[p] * CS + S + v];
+            pr[p] = x;
+          }
+        } else {
+          if (DIR == 0) row[0] = k == 3 ? s.v[1] : -s.v[2];
+          else if (DIR == 1) row[0] = k == 3 ? -s.v[0] : s.v[2];
+          else row[0] = k == 3 ? s.v[0] : -s.v[1];
+#pragma unroll
+          for (int p = 0; p < 6; p++) pr[p] = row[0] * sC[cs[p] * CS + S] + cJ * sC[cs[p] * CS + S + J];
         }
         const double obL = part == 0 ? 0.5 * gb : 0.5 * lk * gb;
 #pragma unroll
-        for (int j = 0; j < 5; j++) { m[j] = pr[j]; mb[j] = 0.0; }
-        const double wl = weno5_vjp(m, m, eps, obL, mb, mb);
-#pragma unroll
-        for (int j = 0; j < 5; j++) { b[j] = mb[j]; m[j] = pr[5 - j]; mb[j] = 0.0; }
-        b[5] = 0.0;
-        const double wr = weno5_vjp(m, m, eps, part == 0 ? obL : -obL, mb, mb);
-#pragma unroll
-        for (int j = 0; j < 5; j++) b[5 - j] += mb[j];
+        for (int p = 0; p < 6; p++) b[p] = 0.0;
+        double wl, wr;
+        weno5_lr_vjp(pr, eps, obL, part == 0 ? obL : -obL, b, wl, wr);
         // post the six adjoints and the row; X for the row's derivatives
         double X[NV] = {0, 0, 0, 0, 0};
 #pragma unroll
-        for (int p = 0; p < 6; p++) {
-          E[t * ES + p] = b[p];
+        for (int p = 0; p < 6; p++) E[t * ES + p] = b[p];
+        if (!shear) {
 #pragma unroll
-          for (int v = 0; v < NV; v++) X[v] += b[p] * sC[cs[p] * CS + S + v];
+          for (int p = 0; p < 6; p++)
+#pragma unroll
+            for (int v = 0; v < NV; v++) X[v] += b[p] * sC[cs[p] * CS + S + v];
+#pragma unroll
+          for (int v = 0; v < NV; v++) E[t * ES + 6 + v] = row[v];
+        } else {
+          // only r0 depends on the Roe state
+#pragma unroll
+          for (int p = 0; p < 6; p++) X[0] += b[p] * sC[cs[p] * CS + S];
+          E[t * ES + 6] = row[0];
         }
-#pragma unroll
-        for (int v = 0; v < NV; v++) E[t * ES + 6 + v] = row[v];
         // this part's share of the forward g_k = (fL + fR)/2 - l_k (uR - uL)/2
         const double gk = part == 0 ? 0.5 * (wl + wr) : -0.5 * lk * (wr - wl);
         // term k of R g at the Roe state
@@ -729,12 +814,25 @@ __device__ __forceinline__ void rhs_vjp_char_line(
       // cell t takes post p of interface t+3-p, p = 0..5, back through that
       // interface's row
       if (active) {
+        if (!shear) {
 #pragma unroll
-        for (int p = 0; p < 6; p++) {
-          const double *e = E + wrapn(t + 3 - p, n) * ES;
-          const double bp = e[p];
+          for (int p = 0; p < 6; p++) {
+            const double *e = E + wrapn(t + 3 - p, n) * ES;
+            const double bp = e[p];
 #pragma unroll
-          for (int v = 0; v < NV; v++) acc[v] += bp * e[6 + v];
+            for (int v = 0; v < NV; v++) acc[v] += bp * e[6 + v];
+          }
+        } else {
+          double bsum = 0.0;
+#pragma unroll
+          for (int p = 0; p < 6; p++) {
+            const double *e = E + wrapn(t + 3 - p, n) * ES;
+            const double bp = e[p];
+            acc[0] += bp * e[6];
+            bsum += bp;
+          }
+          acc[JA] += (k == 3 ? CA : 0.0) * bsum;
+          acc[JB] += (k == 4 ? CB : 0.0) * bsum;
         }
       }
       __syncthreads();
