@@ -897,9 +897,13 @@ __device__ __forceinline__ void rhs_vjp_char_line(
 // second layer, zero derivatives across d) at the two positions on each side
 // of a line (NavierStokes3DParabolicFunction.c:50-325,
 // FirstDerivativeFourthOrder.c:36-129).  The reverse pass:
-//  k_visc_vjp_cell   flux adjoints of every cell from its interior readers,
+//  k_visc_vjp_tile   flux adjoints of every cell from its interior readers,
 //                    back through the flux onto Q (stored in QB) and onto the
-//                    gradients (stored in GB, already times dxinv);
+//                    gradients (stored in GB, already times dxinv), by tiles
+//                    of VTX x VTY columns marching through VKS z-planes: Q
+//                    and the readers' adjoint weights of a plane sit in LDS
+//                    with a 2-cell halo in x and y, the z stencil in
+//                    registers, so each is read about once per tile;
 //  k_visc_vjp_ghost  the same for the ghost-layer fluxes of each line, whose
 //                    gradient adjoints are spread over their stencils into QB;
 //  k_visc_vjp_gather QB plus the transposed gradient stencils over GB, then
@@ -963,56 +967,104 @@ __device__ __forceinline__ void visc_mu(const double *__restrict__ lam, const do
   }
 }
 
-__device__ __forceinline__ void visc_vjp_cell(const double *__restrict__ Q, const double *__restrict__ lam,
+__device__ __forceinline__ void visc_vjp_tile(const double *__restrict__ Q, const double *__restrict__ lam,
                                               const double *__restrict__ nu_in, double ca, double cb,
                                               double *__restrict__ QB, double *__restrict__ GB, int nx, int ny,
-                                              int nz, const double *__restrict__ dxg, const ViscC &vc, size_t idx) {
+                                              int nz, const double *__restrict__ dxg, const ViscC &vc, int b,
+                                              double (&sQ)[4][VHY][VHX], double (&sM)[4][VHY][VHX]) {
   const size_t ncell = (size_t)nx * ny * nz;
-  if (idx >= ncell) return;
-  const int n[3] = {nx, ny, nz};
-  const int pos[3] = {(int)(idx % nx), (int)((idx / nx) % ny), (int)(idx / ((size_t)nx * ny))};
-  const size_t st[3] = {1, (size_t)nx, (size_t)nx * ny};
-  const double *dx[3] = {dxg + 3, dxg + (nx + 6) + 3, dxg + (nx + 6) + (ny + 6) + 3};
-  double q[4], gr[4][3];
+  const size_t plane = (size_t)nx * ny;
+  const int ntx = (nx + VTX - 1) / VTX, nty = (ny + VTY - 1) / VTY;
+  const int tx = b % ntx, ty = (b / ntx) % nty, kz = b / (ntx * nty);
+  const int x0 = tx * VTX, y0 = ty * VTY, k0 = kz * VKS;
+  const int lx = threadIdx.x % VTX, ly = threadIdx.x / VTX;
+  const int i = x0 + lx, j = y0 + ly;
+  const bool own = i < nx && j < ny;
+  const int ic = own ? i : 0, jc = own ? j : 0;
+  const size_t col = (size_t)jc * nx + ic;
+  const double *dx = dxg + 3, *dy = dxg + (nx + 6) + 3, *dz = dxg + (nx + 6) + (ny + 6) + 3;
+  const int k1 = min(k0 + VKS, nz);
+  const int offs[4] = {-2, -1, 1, 2};
+  // window: Q and the weights m of this column at planes k-2..k+2
+  double q[5][4], m[5][4];
 #pragma unroll
-  for (int c = 0; c < 4; c++) q[c] = Q[(size_t)c * ncell + idx];
-  double fb[3][4];
+  for (int s = 0; s < 4; s++) {
+    const size_t c = col + (size_t)wrapn(k0 - 2 + s, nz) * plane;
 #pragma unroll
-  for (int d = 0; d < 3; d++) {
-    const size_t line0 = idx - (size_t)pos[d] * st[d];
-    double f[4][4];  // Q at offsets -2, -1, +1, +2
-    const int offs[4] = {-2, -1, 1, 2};
+    for (int v = 0; v < 4; v++) q[s + 1][v] = Q[(size_t)v * ncell + c];
+    visc_mu(lam, nu_in, ca, cb, ncell, c, m[s + 1]);
+  }
+  for (int k = k0; k < k1; k++) {
 #pragma unroll
-    for (int k = 0; k < 4; k++) fb[d][k] = 0.0;
+    for (int s = 0; s < 4; s++)
 #pragma unroll
-    for (int s = 0; s < 4; s++) {
-      const size_t cw = line0 + (size_t)wrapn(pos[d] + offs[s], n[d]) * st[d];
+      for (int v = 0; v < 4; v++) { q[s][v] = q[s + 1][v]; m[s][v] = m[s + 1][v]; }
+    {
+      const size_t c = col + (size_t)wrapn(k + 2, nz) * plane;
 #pragma unroll
-      for (int c = 0; c < 4; c++) f[s][c] = Q[(size_t)c * ncell + cw];
-      // reader r = pos - s reads this cell with weight d4w(s), inside the grid only
-      const int r = pos[d] - offs[s];
-      if (r >= 0 && r < n[d]) {
-        double m[4];
-        visc_mu(lam, nu_in, ca, cb, ncell, line0 + (size_t)r * st[d], m);
-        const double w = d4w(offs[s]) * dx[d][r];
+      for (int v = 0; v < 4; v++) q[4][v] = Q[(size_t)v * ncell + c];
+      visc_mu(lam, nu_in, ca, cb, ncell, c, m[4]);
+    }
+    // tile of plane k with halo
+    const size_t pk = (size_t)k * plane;
+    for (int c = threadIdx.x; c < VHX * VHY; c += VTX * VTY) {
+      const int hx = c % VHX, hy = c / VHX;
+      // x0 - 2 + hx spans [x0 - 2, x0 + VTX + 1] and y0 - 2 + hy spans
+      // [y0 - 2, y0 + VTY + 1], which can exceed 2 nx and 2 ny
+      const int gi = (x0 - 2 + hx + 2 * nx) % nx, gj = (y0 - 2 + hy + 2 * ny) % ny;
+      const size_t h = pk + (size_t)gj * nx + gi;
+      double mh[4];
+      visc_mu(lam, nu_in, ca, cb, ncell, h, mh);
 #pragma unroll
-        for (int k = 0; k < 4; k++) fb[d][k] += w * m[k];
+      for (int v = 0; v < 4; v++) {
+        sQ[v][hy][hx] = Q[(size_t)v * ncell + h];
+        sM[v][hy][hx] = mh[v];
       }
     }
+    __syncthreads();
+    if (own) {
+      double gr[4][3];
 #pragma unroll
-    for (int c = 0; c < 4; c++) gr[c][d] = d4c(f[0][c], f[1][c], f[2][c], f[3][c]) * dx[d][pos[d]];
-  }
-  const double mu = exp(0.76 * log(q[3]));
-  double qb[4] = {0, 0, 0, 0}, grb[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-  double mub = visc_flux_vjp<0>(q, gr, mu, vc, fb[0], qb, grb);
-  mub += visc_flux_vjp<1>(q, gr, mu, vc, fb[1], qb, grb);
-  mub += visc_flux_vjp<2>(q, gr, mu, vc, fb[2], qb, grb);
-  qb[3] += mub * (0.76 * mu / q[3]);
+      for (int c = 0; c < 4; c++) {
+        const double *r = &sQ[c][ly + 2][0];
+        gr[c][0] = d4c(r[lx], r[lx + 1], r[lx + 3], r[lx + 4]) * dx[i];
+        gr[c][1] = d4c(sQ[c][ly][lx + 2], sQ[c][ly + 1][lx + 2], sQ[c][ly + 3][lx + 2], sQ[c][ly + 4][lx + 2]) * dy[j];
+        gr[c][2] = d4c(q[0][c], q[1][c], q[3][c], q[4][c]) * dz[k];
+      }
+      const double qc[4] = {q[2][0], q[2][1], q[2][2], q[2][3]};
+      const double mu = exp(0.76 * log(qc[3]));
+      double qb[4] = {0, 0, 0, 0}, grb[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, mub = 0.0;
+      // direction by direction: reader r = pos - s reads this cell's flux
+      // with weight d4w(s) dxinv(r), inside the grid only
 #pragma unroll
-  for (int c = 0; c < 4; c++) {
-    QB[(size_t)c * ncell + idx] = qb[c];
+      for (int d = 0; d < 3; d++) {
+        double fb[4] = {0, 0, 0, 0};
 #pragma unroll
-    for (int e = 0; e < 3; e++) GB[(size_t)(c * 3 + e) * ncell + idx] = grb[c][e] * dx[e][pos[e]];
+        for (int s = 0; s < 4; s++) {
+          const int o = offs[s];
+          const int r = (d == 0 ? i : (d == 1 ? j : k)) - o;
+          if (r >= 0 && r < (d == 0 ? nx : (d == 1 ? ny : nz))) {
+            const double w = d4w(o) * (d == 0 ? dx[r] : (d == 1 ? dy[r] : dz[r]));
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+              fb[c] += w * (d == 0 ? sM[c][ly + 2][lx + 2 - o] : (d == 1 ? sM[c][ly + 2 - o][lx + 2] : m[2 - o][c]));
+          }
+        }
+        if (d == 0) mub += visc_flux_vjp<0>(qc, gr, mu, vc, fb, qb, grb);
+        else if (d == 1) mub += visc_flux_vjp<1>(qc, gr, mu, vc, fb, qb, grb);
+        else mub += visc_flux_vjp<2>(qc, gr, mu, vc, fb, qb, grb);
+      }
+      qb[3] += mub * (0.76 * mu / qc[3]);
+      const size_t idx = pk + col;
+      const double dxe[3] = {dx[i], dy[j], dz[k]};
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        QB[(size_t)c * ncell + idx] = qb[c];
+#pragma unroll
+        for (int e = 0; e < 3; e++) GB[(size_t)(c * 3 + e) * ncell + idx] = grb[c][e] * dxe[e];
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -1159,11 +1211,12 @@ __global__ void __launch_bounds__(VJP_CHAR_LINE_MAX) k_rhs_vjp_char(
   rhs_vjp_char_line<DIR, UPW>(u, lam, nu_in, ca, cb, nu, L, mode, nx, ny, nz, dxg, gamma, eps, xcd_block(), smem);
 }
 
-__global__ void __launch_bounds__(256) k_visc_vjp_cell(const double *__restrict__ Q, const double *__restrict__ lam,
-                                                       const double *__restrict__ nu_in, double ca, double cb,
-                                                       double *__restrict__ QB, double *__restrict__ GB, int nx,
-                                                       int ny, int nz, const double *__restrict__ dxg, ViscC vc) {
-  visc_vjp_cell(Q, lam, nu_in, ca, cb, QB, GB, nx, ny, nz, dxg, vc, (size_t)blockIdx.x * blockDim.x + threadIdx.x);
+__global__ void __launch_bounds__(VT) k_visc_vjp_tile(const double *__restrict__ Q, const double *__restrict__ lam,
+                                                     const double *__restrict__ nu_in, double ca, double cb,
+                                                     double *__restrict__ QB, double *__restrict__ GB, int nx, int ny,
+                                                     int nz, const double *__restrict__ dxg, ViscC vc) {
+  __shared__ double sQ[4][VHY][VHX], sM[4][VHY][VHX];
+  visc_vjp_tile(Q, lam, nu_in, ca, cb, QB, GB, nx, ny, nz, dxg, vc, blockIdx.x, sQ, sM);
 }
 
 template <int DIR>
@@ -1267,14 +1320,18 @@ __global__ void __launch_bounds__(256) k_ens_vjp_prims(EnsVjp a, size_t ncell) {
   Q[3 * M.ncell + c] = a.gamma * q.P / q.rho;
 }
 
-__global__ void __launch_bounds__(256) k_ens_visc_vjp_cell(EnsVjp a, size_t ncell) {
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= ncell) return;
-  const EnsMember M = a.mem[ens_find_cell(a.mem, a.nmem, idx)];
-  const double *Q = a.work + 20 * M.off;
-  double *QB = a.work + 20 * M.off + 4 * M.ncell, *GB = a.work + 20 * M.off + 8 * M.ncell;
-  visc_vjp_cell(Q, a.lam + 5 * M.off, a.nu_in ? a.nu_in + 5 * M.off : nullptr, M.dt * a.A, M.dt * a.B, QB, GB,
-                M.n[0], M.n[1], M.n[2], M.dxg, a.vc, idx - M.off);
+// tiles of all members; tiles0 holds each member's first tile
+__global__ void __launch_bounds__(VT) k_ens_visc_vjp_tile(EnsVjp a, const unsigned *__restrict__ tiles0,
+                                                         unsigned ntiles) {
+  __shared__ double sQ[4][VHY][VHX], sM[4][VHY][VHX];
+  for (unsigned item = blockIdx.x; item < ntiles; item += gridDim.x) {
+    const int m = ens_find(tiles0, a.nmem, item);
+    const EnsMember M = a.mem[m];
+    const double *Q = a.work + 20 * M.off;
+    double *QB = a.work + 20 * M.off + 4 * M.ncell, *GB = a.work + 20 * M.off + 8 * M.ncell;
+    visc_vjp_tile(Q, a.lam + 5 * M.off, a.nu_in ? a.nu_in + 5 * M.off : nullptr, M.dt * a.A, M.dt * a.B, QB, GB,
+                  M.n[0], M.n[1], M.n[2], M.dxg, a.vc, (int)(item - tiles0[m]), sQ, sM);
+  }
 }
 
 template <int DIR>
@@ -1358,8 +1415,8 @@ void launch_rhs_vjp(const Grid &g, const Scheme &sc, const double *u, const doub
     const ViscC vc = {sc.gamma, 1.0 / sc.Re, 1.0 / (sc.gamma - 1.0), 1.0 / sc.Pr};
     const unsigned nb = (unsigned)((g.ncell + 255) / 256);
     launch_prims(g, sc, u, Q, st);
-    hipLaunchKernelGGL(k_visc_vjp_cell, dim3(nb), dim3(256), 0, st, Q, lam, nu_in, ca, cb, QB, GB, g.n[0], g.n[1],
-                       g.n[2], dxg, vc);
+    hipLaunchKernelGGL(k_visc_vjp_tile, dim3(visc_tiles(g.n[0], g.n[1], g.n[2])), dim3(VT), 0, st, Q, lam, nu_in, ca,
+                       cb, QB, GB, g.n[0], g.n[1], g.n[2], dxg, vc);
     const unsigned gx = (unsigned)((g.ncell / g.n[0] + 255) / 256), gy = (unsigned)((g.ncell / g.n[1] + 255) / 256),
                    gz = (unsigned)((g.ncell / g.n[2] + 255) / 256);
     hipLaunchKernelGGL(k_visc_vjp_ghost<0>, dim3(gx), dim3(256), 0, st, Q, lam, nu_in, ca, cb, QB, g.n[0], g.n[1],
@@ -1439,7 +1496,7 @@ void launch_ens_rhs_vjp(const Scheme &sc, const EnsVjpPlan &p, const double *u, 
   if (visc) {
     const unsigned nb = (unsigned)((p.ncell + 255) / 256);
     hipLaunchKernelGGL(k_ens_vjp_prims, dim3(nb), dim3(256), 0, st, a, p.ncell);
-    hipLaunchKernelGGL(k_ens_visc_vjp_cell, dim3(nb), dim3(256), 0, st, a, p.ncell);
+    hipLaunchKernelGGL(k_ens_visc_vjp_tile, dim3(p.ntiles), dim3(VT), 0, st, a, p.tiles0, p.ntiles);
     hipLaunchKernelGGL(k_ens_visc_vjp_ghost<0>, dim3((p.nlines[0] + 255) / 256), dim3(256), 0, st, a, p.lines0[0],
                        p.nlines[0]);
     hipLaunchKernelGGL(k_ens_visc_vjp_ghost<1>, dim3((p.nlines[1] + 255) / 256), dim3(256), 0, st, a, p.lines0[1],

@@ -793,6 +793,8 @@ struct hm_ensemble {
   double *FV = nullptr, *FVg[3] = {nullptr, nullptr, nullptr}, *Q = nullptr;
   unsigned *item0v = nullptr;
   unsigned nitems_v = 0;
+  unsigned *tiles0 = nullptr;  // adjoint: each member's first viscous tile
+  unsigned ntiles = 0;
   unsigned *item0[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   unsigned nitems[2][3] = {{0, 0, 0}, {0, 0, 0}};
   // adjoint: first line of every member along each direction, longest
@@ -886,6 +888,7 @@ void hm_ensemble_destroy(hm_ensemble *e) {
     if (b) (void)hipFree(b);
   if (e->mem_dev) (void)hipFree(e->mem_dev);
   if (e->item0v) (void)hipFree(e->item0v);
+  if (e->tiles0) (void)hipFree(e->tiles0);
   for (int sh = 0; sh < 2; sh++)
     for (int d = 0; d < 3; d++)
       if (e->item0[sh][d]) (void)hipFree(e->item0[sh][d]);
@@ -995,6 +998,16 @@ hm_status hm_ensemble_create(const hm_config *members, int nmembers, hm_ensemble
     if (hipMalloc((void **)&e->item0v, pre.size() * sizeof(unsigned)) != hipSuccess ||
         hipMemcpy(e->item0v, pre.data(), pre.size() * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess) {
       st = fail(HM_ERR_NOMEM, "hm_ensemble_create: viscous work list");
+      goto bad;
+    }
+    for (int m = 0; m < nmembers; m++) {
+      const int *n = e->mem[m].n;
+      pre[m + 1] = pre[m] + hm::visc_tiles(n[0], n[1], n[2]);
+    }
+    e->ntiles = pre[nmembers];
+    if (hipMalloc((void **)&e->tiles0, pre.size() * sizeof(unsigned)) != hipSuccess ||
+        hipMemcpy(e->tiles0, pre.data(), pre.size() * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess) {
+      st = fail(HM_ERR_NOMEM, "hm_ensemble_create: viscous tile list");
       goto bad;
     }
   }
@@ -1233,6 +1246,8 @@ hm_status hm_ensemble_adjoint_reverse(hm_ensemble *e, double *lam) {
     plan.nlines[d] = e->nlines[d];
     plan.nmax[d] = e->nmax[d];
   }
+  plan.tiles0 = e->tiles0;
+  plan.ntiles = e->ntiles;
   const int nck = (N + K - 1) / K;
   const int kept = nck >= 2 ? nck - 2 : 0;  // segments the forward sweep kept
   double *cur = lam;

@@ -82,6 +82,18 @@ void launch_pass(const Grid &g, const Scheme &sc, int dir, const PassBuffers &b,
 // primitive variables (u,v,w,T) of u into Q
 void launch_prims(const Grid &g, const Scheme &sc, const double *u, double *Q, hipStream_t st);
 
+// tiles of the viscous kernels (hm_kernels.hip, hm_adjoint.hip): a block
+// owns VTX x VTY (x, y) columns and marches through VKS z-planes with a
+// 2-cell halo in x and y
+constexpr int VTX = 64, VTY = 8, VKS = 16;
+constexpr int VT = VTX * VTY;  // threads of a viscous block
+constexpr int VHX = VTX + 4, VHY = VTY + 4;
+
+// tiles (x, y, z-segment) of a grid
+__host__ __device__ inline unsigned visc_tiles(int nx, int ny, int nz) {
+  return (unsigned)(((nx + VTX - 1) / VTX) * ((ny + VTY - 1) / VTY) * ((nz + VKS - 1) / VKS));
+}
+
 // batched ensemble passes (hm_pass.hip): members of one scheme
 struct EnsMember {
   int n[3];
@@ -132,6 +144,8 @@ struct EnsVjpPlan {
   const unsigned *lines0[3];
   unsigned nlines[3];
   int nmax[3];
+  const unsigned *tiles0;  // viscous members: each member's first viscous tile (nmem + 1 entries)
+  unsigned ntiles;
 };
 bool ens_vjp_supported(const Scheme &sc, const int nmax[3]);
 size_t ens_vjp_work_size(const Scheme &sc, size_t ncell);

@@ -224,14 +224,7 @@ __device__ __forceinline__ void visc_flux(double fv[4], const QT &q, const doubl
 // tile's plane k with a 2-cell halo in x and y is staged in LDS for the x-
 // and y-derivatives.  Every plane of Q is thus read from HBM about once per
 // tile, instead of once per stencil point.
-constexpr int VTX = 64, VTY = 8, VKS = 16;
-constexpr int VT = VTX * VTY;  // threads of a viscous-flux block
-constexpr int VHX = VTX + 4, VHY = VTY + 4;
-
-// tiles (x, y, z-segment) of a grid
-__host__ __device__ __forceinline__ unsigned visc_tiles(int nx, int ny, int nz) {
-  return (unsigned)(((nx + VTX - 1) / VTX) * ((ny + VTY - 1) / VTY) * ((nz + VKS - 1) / VKS));
-}
+// (tile constants VTX, VTY, VKS and visc_tiles: hm_internal.h)
 
 // tile b of the grid; every thread of the block takes part
 __device__ __forceinline__ void visc_flux_tile(const double *__restrict__ Q, double *__restrict__ FV, int nx,
