@@ -44,49 +44,6 @@ __device__ __forceinline__ double vjp_rcp(double d) {
   return __builtin_fma(r, e, r);
 }
 
-// reverse of weno5(m3..p2; g3..q2): adds d out/d m (smoothness stencil) to mb
-// and d out/d g (interpolated values) to gb, scaled by ob; returns out
-__device__ __forceinline__ double weno5_vjp(const double m[5], const double g[5], double eps, double ob,
-                                            double mb[5], double gb[5]) {
-  const double c1312 = 13.0 / 12.0, c14 = 0.25, s6 = 1.0 / 6.0;
-  const double t1 = m[0] - 2 * m[1] + m[2], r1 = m[0] - 4 * m[1] + 3 * m[2];
-  const double t2 = m[1] - 2 * m[2] + m[3], r2 = m[1] - m[3];
-  const double t3 = m[2] - 2 * m[3] + m[4], r3 = 3 * m[2] - 4 * m[3] + m[4];
-  const double b1 = c1312 * t1 * t1 + c14 * r1 * r1;
-  const double b2 = c1312 * t2 * t2 + c14 * r2 * r2;
-  const double b3 = c1312 * t3 * t3 + c14 * r3 * r3;
-  const double e1 = b1 + eps, e2 = b2 + eps, e3 = b3 + eps;
-  const double s1 = e1 * e1, s2 = e2 * e2, s3 = e3 * e3;
-  const double a1 = 0.1 * (s2 * s3), a2 = 0.6 * (s1 * s3), a3 = 0.3 * (s1 * s2);
-  const double inv = vjp_rcp(a1 + a2 + a3);
-  const double w1 = a1 * inv, w2 = a2 * inv, w3 = a3 * inv;
-  const double f1 = (2 * s6) * g[0] + (-7 * s6) * g[1] + (11 * s6) * g[2];
-  const double f2 = (-s6) * g[1] + (5 * s6) * g[2] + (2 * s6) * g[3];
-  const double f3 = (2 * s6) * g[2] + (5 * s6) * g[3] + (-s6) * g[4];
-  const double out = w1 * f1 + w2 * f2 + w3 * f3;
-  // values
-  gb[0] += ob * (w1 * (2 * s6));
-  gb[1] += ob * (w1 * (-7 * s6) + w2 * (-s6));
-  gb[2] += ob * (w1 * (11 * s6) + w2 * (5 * s6) + w3 * (2 * s6));
-  gb[3] += ob * (w2 * (2 * s6) + w3 * (5 * s6));
-  gb[4] += ob * (w3 * (-s6));
-  // weights: d out/d b_k = -2 w_k (f_k - out) / (b_k + eps), with
-  // 1/e_k from one reciprocal of e1 e2 e3
-  const double re = vjp_rcp(e1 * e2 * e3);
-  const double bb1 = -2.0 * ob * w1 * (f1 - out) * (re * (e2 * e3));
-  const double bb2 = -2.0 * ob * w2 * (f2 - out) * (re * (e1 * e3));
-  const double bb3 = -2.0 * ob * w3 * (f3 - out) * (re * (e1 * e2));
-  const double tt1 = bb1 * (2 * c1312) * t1, rr1 = bb1 * (2 * c14) * r1;
-  const double tt2 = bb2 * (2 * c1312) * t2, rr2 = bb2 * (2 * c14) * r2;
-  const double tt3 = bb3 * (2 * c1312) * t3, rr3 = bb3 * (2 * c14) * r3;
-  mb[0] += tt1 + rr1;
-  mb[1] += -2 * tt1 - 4 * rr1 + tt2 + rr2;
-  mb[2] += tt1 + 3 * rr1 - 2 * tt2 + tt3 + 3 * rr3;
-  mb[3] += tt2 - rr2 - 2 * tt3 - 4 * rr3;
-  mb[4] += tt3 + rr3;
-  return out;
-}
-
 // one side of a WENO5-JS interpolation from its smoothness indicators
 // beta_k and candidate values f_k: the interpolated value, the weights, and
 // ob times d out/d beta_k = -2 ob w_k (f_k - out)/(beta_k + eps).  One
@@ -115,13 +72,14 @@ __device__ __forceinline__ W5Side w5_side(double b1, double b2, double b3, doubl
   return r;
 }
 
-// reverse of the left- and right-biased WENO5-JS of one interface over one
-// stencil b[0..5] (cells I-3..I+2) that supplies both the smoothness
-// indicators and the interpolated values: adds obL d vl/d b + obR d vr/d b
-// to bb and returns vl, vr.  The second differences the two sides share are
-// formed once and their adjoints summed.
-__device__ __forceinline__ void weno5_lr_vjp(const double b[6], double eps, double obL, double obR, double bb[6],
-                                             double &vl, double &vr) {
+// reverse of the left- and right-biased WENO5-JS of one interface, with
+// smoothness indicators from b[0..5] and interpolated values from g[0..5]
+// (cells I-3..I+2; g = b except for the modified energy, a function of the
+// same state): adds obL d vl/d(b, g) + obR d vr/d(b, g) to bb and returns
+// vl, vr.  The second differences the two sides share are formed once and
+// their adjoints summed.
+__device__ __forceinline__ void weno5_lr_vjp(const double b[6], const double g[6], double eps, double obL,
+                                             double obR, double bb[6], double &vl, double &vr) {
   const double c1312 = 13.0 / 12.0, c14 = 0.25, s6 = 1.0 / 6.0;
   const double d1 = b[0] - 2 * b[1] + b[2], d2 = b[1] - 2 * b[2] + b[3];
   const double d3 = b[2] - 2 * b[3] + b[4], d4 = b[3] - 2 * b[4] + b[5];
@@ -130,13 +88,13 @@ __device__ __forceinline__ void weno5_lr_vjp(const double b[6], double eps, doub
   const double rr1 = b[5] - 4 * b[4] + 3 * b[3], rr2 = b[4] - b[2], rr3 = 3 * b[3] - 4 * b[2] + b[1];
   // left: second differences d1, d2, d3; right (mirrored): d4, d3, d2
   const W5Side L = w5_side(c1312 * (d1 * d1) + c14 * (rl1 * rl1), q2 + c14 * (rl2 * rl2), q3 + c14 * (rl3 * rl3),
-                           (2 * s6) * b[0] + (-7 * s6) * b[1] + (11 * s6) * b[2],
-                           (-s6) * b[1] + (5 * s6) * b[2] + (2 * s6) * b[3],
-                           (2 * s6) * b[2] + (5 * s6) * b[3] + (-s6) * b[4], eps, obL);
+                           (2 * s6) * g[0] + (-7 * s6) * g[1] + (11 * s6) * g[2],
+                           (-s6) * g[1] + (5 * s6) * g[2] + (2 * s6) * g[3],
+                           (2 * s6) * g[2] + (5 * s6) * g[3] + (-s6) * g[4], eps, obL);
   const W5Side R = w5_side(c1312 * (d4 * d4) + c14 * (rr1 * rr1), q3 + c14 * (rr2 * rr2), q2 + c14 * (rr3 * rr3),
-                           (2 * s6) * b[5] + (-7 * s6) * b[4] + (11 * s6) * b[3],
-                           (-s6) * b[4] + (5 * s6) * b[3] + (2 * s6) * b[2],
-                           (2 * s6) * b[3] + (5 * s6) * b[2] + (-s6) * b[1], eps, obR);
+                           (2 * s6) * g[5] + (-7 * s6) * g[4] + (11 * s6) * g[3],
+                           (-s6) * g[4] + (5 * s6) * g[3] + (2 * s6) * g[2],
+                           (2 * s6) * g[3] + (5 * s6) * g[2] + (-s6) * g[1], eps, obR);
   vl = L.out;
   vr = R.out;
   // interpolated values
@@ -464,33 +422,15 @@ __device__ __forceinline__ void rhs_vjp_line(const double *__restrict__ u, const
         U[s] = sU[v * n + cs[s]];
         C[s] = v == 4 ? sC4[cs[s]] : U[s];
       }
-      double mL[5], gL[5], mR[5], gR[5];
-      double mbL[5] = {0, 0, 0, 0, 0}, gbL[5] = {0, 0, 0, 0, 0};
-      double mbR[5] = {0, 0, 0, 0, 0}, gbR[5] = {0, 0, 0, 0, 0};
+      double fl, fr, uLv, uRv;
       // fluxes: smoothness and values are the same stencil
-#pragma unroll
-      for (int k = 0; k < 5; k++) { mL[k] = F[k]; mR[k] = F[5 - k]; }
-      weno5_vjp(mL, mL, eps, 0.5 * g[v], mbL, gbL);
-      weno5_vjp(mR, mR, eps, 0.5 * g[v], mbR, gbR);
-#pragma unroll
-      for (int k = 0; k < 5; k++) {
-        fb[k] += mbL[k] + gbL[k];
-        fb[5 - k] += mbR[k] + gbR[k];
-        mbL[k] = gbL[k] = mbR[k] = gbR[k] = 0.0;
-      }
+      weno5_lr_vjp(F, F, eps, 0.5 * g[v], 0.5 * g[v], fb, fl, fr);
       // solution: weights from u, values from the modified solution (the
       // same function of u, so both adjoints land on u)
-#pragma unroll
-      for (int k = 0; k < 5; k++) { mL[k] = U[k]; gL[k] = C[k]; mR[k] = U[5 - k]; gR[k] = C[5 - k]; }
-      const double uLv = weno5_vjp(mL, gL, eps, UPW == UPWIND_ROE ? 0.5 * wR[v] : 0.5 * alpha * g[v], mbL, gbL);
-      const double uRv = weno5_vjp(mR, gR, eps, UPW == UPWIND_ROE ? -0.5 * wR[v] : -0.5 * alpha * g[v], mbR, gbR);
+      const double ob = UPW == UPWIND_ROE ? 0.5 * wR[v] : 0.5 * alpha * g[v];
+      weno5_lr_vjp(U, C, eps, ob, -ob, ub, uLv, uRv);
       if (UPW == UPWIND_ROE) du[v] = 0.5 * (uRv - uLv);
       else abar += -0.5 * (uRv - uLv) * g[v];
-#pragma unroll
-      for (int k = 0; k < 5; k++) {
-        ub[k] += mbL[k] + gbL[k];
-        ub[5 - k] += mbR[k] + gbR[k];
-      }
       // slot s of interface t belongs to cell t-3+s
 #pragma unroll
       for (int s = 0; s < 6; s++) {
@@ -761,7 +701,7 @@ __device__ __forceinline__ void rhs_vjp_char_line(
 #pragma unroll
         for (int p = 0; p < 6; p++) b[p] = 0.0;
         double wl, wr;
-        weno5_lr_vjp(pr, eps, obL, part == 0 ? obL : -obL, b, wl, wr);
+        weno5_lr_vjp(pr, pr, eps, obL, part == 0 ? obL : -obL, b, wl, wr);
         // post the six adjoints and the row; X for the row's derivatives
         double X[NV] = {0, 0, 0, 0, 0};
 #pragma unroll
