@@ -618,8 +618,10 @@ __global__ void __launch_bounds__(256) k_pass_x(PassConst k, PassIO io) {
   pass_x_body<INTERP, UPW, 4, W>(k, io, blockIdx.x, sh);
 }
 
+// at most 256 registers: two waves per SIMD (the characteristic passes sit
+// at the limit, and one register more would halve the occupancy)
 template <int DIR, int INTERP, int UPW, int W>
-__global__ void __launch_bounds__(256) k_pass_m(PassConst k, PassIO io) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_pass_m(PassConst k, PassIO io) {
   __shared__ MShared<4> sh;
   pass_m_body<DIR, INTERP, UPW, 4, W>(k, io, blockIdx.x, sh);
 }
@@ -690,7 +692,7 @@ __global__ void __launch_bounds__(64 * LPB) k_ens_x(EnsArgs a) {
 }
 
 template <int DIR, int INTERP, int UPW, int MR>
-__global__ void __launch_bounds__(64 * MR) k_ens_m(EnsArgs a) {
+__global__ void __launch_bounds__(64 * MR) __attribute__((amdgpu_waves_per_eu(2))) k_ens_m(EnsArgs a) {
   __shared__ MShared<MR> sh;
   for (unsigned item = blockIdx.x; item < a.nitems; item += gridDim.x) {
     const int m = ens_member(a, item);
