@@ -100,9 +100,11 @@ def test_rhs_vjp_matches_autograd(interp, upwind, Re):
 @pytest.mark.gpu
 @pytest.mark.parametrize("interp,upwind,Re", [("characteristic", "roe", 1000.0), ("characteristic", "rusanov", -1.0)])
 def test_rhs_vjp_on_ragged_lines(interp, upwind, Re):
-    """Sides of 5 and 7 (stencils wrap more than once around a line) and
-    sides above one and two wavefronts."""
-    for dims in [(5, 7, 6), (70, 5, 9), (130, 6, 5)]:
+    """Sides of 5 and 7 (stencils wrap more than once around a line), sides
+    above one and two wavefronts, and for the viscous tiles (64 x 8 columns,
+    16 planes per segment) partial tiles in x, three tiles in y and three
+    z segments."""
+    for dims in [(5, 7, 6), (70, 5, 9), (130, 6, 5), (9, 19, 37)]:
         w, u0, dt, dx, sch = _scheme_case(interp, upwind, Re, dims)
         x = T.to_soa(u0).requires_grad_(True)
         lam = torch.tensor(np.random.default_rng(7).normal(size=x.shape))

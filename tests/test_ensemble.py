@@ -157,12 +157,13 @@ def test_viscous_reference_fixtures():
 def test_ensemble_adjoint_matches_single_grid(interp, upwind, Re):
     """Batched trajectory gradient (5 steps, checkpoint every 2, each member
     with its own dt) against each member's single-grid adjoint, which
-    test_adjoint.py checks against torch autograd."""
+    test_adjoint.py checks against torch autograd.  The last member spans two
+    x tiles and two z segments of the viscous kernels."""
     from hypar_amd import decks
     from hypar_amd.adjoint import ensemble_trajectory, kinetic_energy, trajectory
     from hypar_amd.ensemble import Ensemble
     from hypar_amd.solver import NS3D
-    dims = [(12, 10, 14), (16, 12, 10), (10, 14, 12)]
+    dims = [(12, 10, 14), (16, 12, 10), (10, 14, 12), (70, 9, 18)]
     ws, u0s, dts = _members(dims, interp, upwind, Re)
     print(f"[gpu] ensemble adjoint {interp}/{upwind} Re={Re} dims={dims}", flush=True)
     e = Ensemble(dims, interp, upwind, Re=Re, Minf=0.1 if Re > 0 else 1.0)
