@@ -660,10 +660,10 @@ __device__ __forceinline__ void rhs_vjp_char_line(
   // fluxes, part 1 the WENO5 of L u onto the cell states; in between, the
   // flux adjoint of every cell goes through its flux Jacobian
   double acc[NV] = {0, 0, 0, 0, 0};
-#pragma unroll 1
+#pragma unroll
   for (int part = 0; part < 2; part++) {
     const int S = part == 0 ? NV : 0;  // flux or state within the cell record
-#pragma unroll 1
+#pragma unroll
     for (int k = 0; k < 5; k++) {
       // the shear rows k = 3, 4 of L have two entries: r0 at 0 and +-1 at J
       const bool shear = k >= 3;
