@@ -44,40 +44,111 @@ __device__ __forceinline__ double vjp_rcp(double d) {
   return __builtin_fma(r, e, r);
 }
 
-// one side of a WENO5-JS interpolation from its smoothness indicators
-// beta_k and candidate values f_k: the interpolated value, the weights, and
-// ob times d out/d beta_k = -2 ob w_k (f_k - out)/(beta_k + eps).  One
-// reciprocal, of (sum a_k)(prod (beta_k + eps)), serves both 1/sum a_k and
-// the 1/(beta_k + eps).
+// one side of a WENO5 interpolation from its smoothness indicators beta_k
+// and candidate values f_k: the interpolated value, the weights, ob times
+// d out/d beta_k (g1..g3) and, for the YC weights, ob times d out/d tau (gt)
+// with tau the YC indicator passed in.  W selects the weights
+// (include/interpolation.h:265-285, 314-341, 426-447, 479-500):
+//  * JS:     a_k = c_k/(beta_k + eps)^2;
+//            d out/d beta_k = -2 w_k (f_k - out)/(beta_k + eps)
+//  * Z, YC:  a_k = c_k (1 + (t/(beta_k + eps))^2), t = |beta_3 - beta_1| (Z)
+//            or tau (YC); t's own derivative is folded into g1, g3 (Z) or
+//            returned in gt (YC)
+//  * mapped: the JS weights u_k mapped by u (c + c^2 - 3 c u + u^2)/
+//            (c^2 + u (1 - 2c)) and renormalised; the adjoint goes back
+//            through the mapping and the JS normalisation
+// The a_k are formed multiplied through by their common denominator; one
+// reciprocal of (sum a_k)(prod (beta_k + eps)) gives both 1/sum a_k and the
+// 1/(beta_k + eps) (the mapped weights take a second one for the mapping).
 struct W5Side {
-  double out, w1, w2, w3, g1, g2, g3;
+  double out, w1, w2, w3, g1, g2, g3, gt;
 };
 
+template <int W>
 __device__ __forceinline__ W5Side w5_side(double b1, double b2, double b3, double f1, double f2, double f3,
-                                          double eps, double ob) {
+                                          double eps, double ob, double tau) {
   const double e1 = b1 + eps, e2 = b2 + eps, e3 = b3 + eps;
   const double s1 = e1 * e1, s2 = e2 * e2, s3 = e3 * e3;
-  const double a1 = 0.1 * (s2 * s3), a2 = 0.6 * (s1 * s3), a3 = 0.3 * (s1 * s2);
-  const double A = a1 + a2 + a3, e23 = e2 * e3, P = e1 * e23;
-  const double q = vjp_rcp(A * P);
-  const double inv = q * P, k = -2.0 * ob * (q * A);
   W5Side r;
-  r.w1 = a1 * inv;
-  r.w2 = a2 * inv;
-  r.w3 = a3 * inv;
-  r.out = r.w1 * f1 + r.w2 * f2 + r.w3 * f3;
-  r.g1 = k * r.w1 * (f1 - r.out) * e23;
-  r.g2 = k * r.w2 * (f2 - r.out) * (e1 * e3);
-  r.g3 = k * r.w3 * (f3 - r.out) * (e1 * e2);
+  r.gt = 0.0;
+  if (W == WENO_JS) {
+    const double a1 = 0.1 * (s2 * s3), a2 = 0.6 * (s1 * s3), a3 = 0.3 * (s1 * s2);
+    const double A = a1 + a2 + a3, e23 = e2 * e3, P = e1 * e23;
+    const double q = vjp_rcp(A * P);
+    const double inv = q * P, k = -2.0 * ob * (q * A);
+    r.w1 = a1 * inv;
+    r.w2 = a2 * inv;
+    r.w3 = a3 * inv;
+    r.out = r.w1 * f1 + r.w2 * f2 + r.w3 * f3;
+    r.g1 = k * r.w1 * (f1 - r.out) * e23;
+    r.g2 = k * r.w2 * (f2 - r.out) * (e1 * e3);
+    r.g3 = k * r.w3 * (f3 - r.out) * (e1 * e2);
+  } else if (W == WENO_Z || W == WENO_YC) {
+    const double t = W == WENO_Z ? fabs(b3 - b1) : tau, t2 = t * t;
+    const double p1 = s2 * s3, p2 = s1 * s3, p3 = s1 * s2;  // products of the other two s_k
+    const double a1 = 0.1 * (s1 + t2) * p1, a2 = 0.6 * (s2 + t2) * p2, a3 = 0.3 * (s3 + t2) * p3;
+    const double A = a1 + a2 + a3, e23 = e2 * e3, P = e1 * e23;
+    const double q = vjp_rcp(A * P);
+    const double inv = q * P;
+    r.w1 = a1 * inv;
+    r.w2 = a2 * inv;
+    r.w3 = a3 * inv;
+    r.out = r.w1 * f1 + r.w2 * f2 + r.w3 * f3;
+    // d out/d a_k (unscaled a_k) = s1 s2 s3 (f_k - out)/A, d a_k/d beta_k =
+    // -2 c_k t^2/(beta_k + eps)^3 and d a_k/d t = 2 c_k t/(beta_k + eps)^2
+    const double h1 = ob * (f1 - r.out), h2 = ob * (f2 - r.out), h3 = ob * (f3 - r.out);
+    r.g1 = (-2.0 * 0.1) * t2 * h1 * p1 * e23 * q;
+    r.g2 = (-2.0 * 0.6) * t2 * h2 * p2 * (e1 * e3) * q;
+    r.g3 = (-2.0 * 0.3) * t2 * h3 * p3 * (e1 * e2) * q;
+    const double gt = 2.0 * t * inv * (0.1 * h1 * p1 + 0.6 * h2 * p2 + 0.3 * h3 * p3);
+    if (W == WENO_Z) {
+      // t = |beta_3 - beta_1|; at beta_3 = beta_1 the derivative is taken as 0
+      const double sg = b3 > b1 ? 1.0 : (b3 < b1 ? -1.0 : 0.0);
+      r.g3 += sg * gt;
+      r.g1 -= sg * gt;
+    } else {
+      r.gt = gt;
+    }
+  } else {
+    const double c1 = 0.1, c2 = 0.6, c3 = 0.3;
+    const double at1 = c1 * (s2 * s3), at2 = c2 * (s1 * s3), at3 = c3 * (s1 * s2);
+    const double At = at1 + at2 + at3, e23 = e2 * e3, P = e1 * e23;
+    const double q = vjp_rcp(At * P);
+    const double it = q * P;
+    const double u1 = at1 * it, u2 = at2 * it, u3 = at3 * it;  // JS weights
+    const double N1 = c1 + c1 * c1 - 3 * c1 * u1 + u1 * u1, D1 = c1 * c1 + u1 * (1.0 - 2.0 * c1);
+    const double N2 = c2 + c2 * c2 - 3 * c2 * u2 + u2 * u2, D2 = c2 * c2 + u2 * (1.0 - 2.0 * c2);
+    const double N3 = c3 + c3 * c3 - 3 * c3 * u3 + u3 * u3, D3 = c3 * c3 + u3 * (1.0 - 2.0 * c3);
+    const double o1 = D2 * D3, o2 = D1 * D3, o3 = D1 * D2, DD = D1 * o1;
+    const double B1 = u1 * N1 * o1, B2 = u2 * N2 * o2, B3 = u3 * N3 * o3, B = B1 + B2 + B3;
+    const double qm = vjp_rcp(B * DD);
+    const double im = qm * DD;
+    r.w1 = B1 * im;
+    r.w2 = B2 * im;
+    r.w3 = B3 * im;
+    r.out = r.w1 * f1 + r.w2 * f2 + r.w3 * f3;
+    // d out/d a_k = DD (f_k - out)/B with a_k = u_k N_k/D_k, whose derivative
+    // is ((N_k + u_k N_k') D_k - u_k N_k D_k')/D_k^2; DD/(B D_k^2) = o_k^2 qm
+    const double h1 = ob * (f1 - r.out), h2 = ob * (f2 - r.out), h3 = ob * (f3 - r.out);
+    const double du1 = h1 * ((N1 + u1 * (2 * u1 - 3 * c1)) * D1 - u1 * N1 * (1.0 - 2.0 * c1)) * (o1 * o1) * qm;
+    const double du2 = h2 * ((N2 + u2 * (2 * u2 - 3 * c2)) * D2 - u2 * N2 * (1.0 - 2.0 * c2)) * (o2 * o2) * qm;
+    const double du3 = h3 * ((N3 + u3 * (2 * u3 - 3 * c3)) * D3 - u3 * N3 * (1.0 - 2.0 * c3)) * (o3 * o3) * qm;
+    // through u_k = at_k/sum at, at_k = c_k/(beta_k + eps)^2
+    const double U = du1 * u1 + du2 * u2 + du3 * u3, iP = q * At;
+    r.g1 = -2.0 * (du1 - U) * u1 * e23 * iP;
+    r.g2 = -2.0 * (du2 - U) * u2 * (e1 * e3) * iP;
+    r.g3 = -2.0 * (du3 - U) * u3 * (e1 * e2) * iP;
+  }
   return r;
 }
 
-// reverse of the left- and right-biased WENO5-JS of one interface, with
+// reverse of the left- and right-biased WENO5 of one interface, with
 // smoothness indicators from b[0..5] and interpolated values from g[0..5]
 // (cells I-3..I+2; g = b except for the modified energy, a function of the
 // same state): adds obL d vl/d(b, g) + obR d vr/d(b, g) to bb and returns
 // vl, vr.  The second differences the two sides share are formed once and
 // their adjoints summed.
+template <int W>
 __device__ __forceinline__ void weno5_lr_vjp(const double b[6], const double g[6], double eps, double obL,
                                              double obR, double bb[6], double &vl, double &vr) {
   const double c1312 = 13.0 / 12.0, c14 = 0.25, s6 = 1.0 / 6.0;
@@ -86,15 +157,18 @@ __device__ __forceinline__ void weno5_lr_vjp(const double b[6], const double g[6
   const double q2 = c1312 * (d2 * d2), q3 = c1312 * (d3 * d3);
   const double rl1 = b[0] - 4 * b[1] + 3 * b[2], rl2 = b[1] - b[3], rl3 = 3 * b[2] - 4 * b[3] + b[4];
   const double rr1 = b[5] - 4 * b[4] + 3 * b[3], rr2 = b[4] - b[2], rr3 = 3 * b[3] - 4 * b[2] + b[1];
+  // YC: undivided 4th differences of the two stencils, tau = their squares
+  const double tl = W == WENO_YC ? b[0] - 4 * b[1] + 6 * b[2] - 4 * b[3] + b[4] : 0.0;
+  const double tr = W == WENO_YC ? b[5] - 4 * b[4] + 6 * b[3] - 4 * b[2] + b[1] : 0.0;
   // left: second differences d1, d2, d3; right (mirrored): d4, d3, d2
-  const W5Side L = w5_side(c1312 * (d1 * d1) + c14 * (rl1 * rl1), q2 + c14 * (rl2 * rl2), q3 + c14 * (rl3 * rl3),
-                           (2 * s6) * g[0] + (-7 * s6) * g[1] + (11 * s6) * g[2],
-                           (-s6) * g[1] + (5 * s6) * g[2] + (2 * s6) * g[3],
-                           (2 * s6) * g[2] + (5 * s6) * g[3] + (-s6) * g[4], eps, obL);
-  const W5Side R = w5_side(c1312 * (d4 * d4) + c14 * (rr1 * rr1), q3 + c14 * (rr2 * rr2), q2 + c14 * (rr3 * rr3),
-                           (2 * s6) * g[5] + (-7 * s6) * g[4] + (11 * s6) * g[3],
-                           (-s6) * g[4] + (5 * s6) * g[3] + (2 * s6) * g[2],
-                           (2 * s6) * g[3] + (5 * s6) * g[2] + (-s6) * g[1], eps, obR);
+  const W5Side L = w5_side<W>(c1312 * (d1 * d1) + c14 * (rl1 * rl1), q2 + c14 * (rl2 * rl2),
+                              q3 + c14 * (rl3 * rl3), (2 * s6) * g[0] + (-7 * s6) * g[1] + (11 * s6) * g[2],
+                              (-s6) * g[1] + (5 * s6) * g[2] + (2 * s6) * g[3],
+                              (2 * s6) * g[2] + (5 * s6) * g[3] + (-s6) * g[4], eps, obL, tl * tl);
+  const W5Side R = w5_side<W>(c1312 * (d4 * d4) + c14 * (rr1 * rr1), q3 + c14 * (rr2 * rr2),
+                              q2 + c14 * (rr3 * rr3), (2 * s6) * g[5] + (-7 * s6) * g[4] + (11 * s6) * g[3],
+                              (-s6) * g[4] + (5 * s6) * g[3] + (2 * s6) * g[2],
+                              (2 * s6) * g[3] + (5 * s6) * g[2] + (-s6) * g[1], eps, obR, tr * tr);
   vl = L.out;
   vr = R.out;
   // interpolated values
@@ -113,6 +187,15 @@ __device__ __forceinline__ void weno5_lr_vjp(const double b[6], const double g[6
            L2 - 4 * L3 + 3 * R1 + 3 * R3;
   bb[4] += (-s6) * wl3 + (-7 * s6) * wr1 + (-s6) * wr2 + D3 - 2 * D4 + L3 - 4 * R1 + R2;
   bb[5] += (2 * s6) * wr1 + D4 + R1;
+  if (W == WENO_YC) {
+    const double TL = 2.0 * tl * L.gt, TR = 2.0 * tr * R.gt;
+    bb[0] += TL;
+    bb[1] += -4 * TL + TR;
+    bb[2] += 6 * TL - 4 * TR;
+    bb[3] += -4 * TL + 6 * TR;
+    bb[4] += TL - 4 * TR;
+    bb[5] += TR;
+  }
 }
 
 // gradient of c + |v_d| of one state with respect to its conserved variables
@@ -312,7 +395,7 @@ __device__ __forceinline__ double harten_d(double l) {
 // contributions of the flux and solution adjoints [6][n] each.
 enum { VJP_WRITE = 0, VJP_ADD = 1, VJP_ADD_L = 2, VJP_ADD_LSET = 3 };
 
-template <int DIR, int UPW>
+template <int DIR, int UPW, int W>
 __device__ __forceinline__ void rhs_vjp_line(const double *__restrict__ u, const double *__restrict__ lam,
                                              const double *__restrict__ nu_in, double ca, double cb,
                                              double *__restrict__ nu, double *__restrict__ L, int mode, int nx,
@@ -424,11 +507,11 @@ __device__ __forceinline__ void rhs_vjp_line(const double *__restrict__ u, const
       }
       double fl, fr, uLv, uRv;
       // fluxes: smoothness and values are the same stencil
-      weno5_lr_vjp(F, F, eps, 0.5 * g[v], 0.5 * g[v], fb, fl, fr);
+      weno5_lr_vjp<W>(F, F, eps, 0.5 * g[v], 0.5 * g[v], fb, fl, fr);
       // solution: weights from u, values from the modified solution (the
       // same function of u, so both adjoints land on u)
       const double ob = UPW == UPWIND_ROE ? 0.5 * wR[v] : 0.5 * alpha * g[v];
-      weno5_lr_vjp(U, C, eps, ob, -ob, ub, uLv, uRv);
+      weno5_lr_vjp<W>(U, C, eps, ob, -ob, ub, uLv, uRv);
       if (UPW == UPWIND_ROE) du[v] = 0.5 * (uRv - uLv);
       else abar += -0.5 * (uRv - uLv) * g[v];
       // slot s of interface t belongs to cell t-3+s
@@ -582,7 +665,7 @@ constexpr int VJP_CHAR_LDS = VJP_CHAR_CS + VJP_CHAR_ES;  // doubles of LDS per c
 // that touch it back through those rows, in a fixed order, so the sums are
 // deterministic.  The Roe-state derivatives of R, L and |lambda_k| are summed
 // field by field in registers.
-template <int DIR, int UPW>
+template <int DIR, int UPW, int W>
 __device__ __forceinline__ void rhs_vjp_char_line(
     const double *__restrict__ u, const double *__restrict__ lam, const double *__restrict__ nu_in, double ca,
     double cb, double *__restrict__ nu, double *__restrict__ L, int mode, int nx, int ny, int nz,
@@ -701,7 +784,7 @@ __device__ __forceinline__ void rhs_vjp_char_line(
 #pragma unroll
         for (int p = 0; p < 6; p++) b[p] = 0.0;
         double wl, wr;
-        weno5_lr_vjp(pr, pr, eps, obL, part == 0 ? obL : -obL, b, wl, wr);
+        weno5_lr_vjp<W>(pr, pr, eps, obL, part == 0 ? obL : -obL, b, wl, wr);
         // post the six adjoints and the row; X for the row's derivatives
         double X[NV] = {0, 0, 0, 0, 0};
 #pragma unroll
@@ -1132,23 +1215,23 @@ __device__ __forceinline__ void visc_vjp_gather(const double *__restrict__ U, co
 // kernels: one grid
 // ---------------------------------------------------------------------------
 // neighbouring lines share cache lines along x: keep them on one XCD
-template <int DIR, int UPW>
+template <int DIR, int UPW, int W>
 __global__ void __launch_bounds__(VJP_LINE_MAX) k_rhs_vjp(const double *__restrict__ u, const double *__restrict__ lam,
                                                       const double *__restrict__ nu_in, double ca, double cb,
                                                       double *__restrict__ nu, double *__restrict__ L, int mode,
                                                       int nx, int ny, int nz, const double *__restrict__ dxg,
                                                       double gamma, double eps) {
   extern __shared__ double smem[];
-  rhs_vjp_line<DIR, UPW>(u, lam, nu_in, ca, cb, nu, L, mode, nx, ny, nz, dxg, gamma, eps, xcd_block(), smem);
+  rhs_vjp_line<DIR, UPW, W>(u, lam, nu_in, ca, cb, nu, L, mode, nx, ny, nz, dxg, gamma, eps, xcd_block(), smem);
 }
 
-template <int DIR, int UPW>
-__global__ void __launch_bounds__(VJP_CHAR_LINE_MAX) k_rhs_vjp_char(
+template <int DIR, int UPW, int W>
+__global__ void __launch_bounds__(VJP_CHAR_LINE_MAX) __attribute__((amdgpu_waves_per_eu(2))) k_rhs_vjp_char(
     const double *__restrict__ u, const double *__restrict__ lam, const double *__restrict__ nu_in, double ca,
     double cb, double *__restrict__ nu, double *__restrict__ L, int mode, int nx, int ny, int nz,
     const double *__restrict__ dxg, double gamma, double eps) {
   extern __shared__ double smem[];
-  rhs_vjp_char_line<DIR, UPW>(u, lam, nu_in, ca, cb, nu, L, mode, nx, ny, nz, dxg, gamma, eps, xcd_block(), smem);
+  rhs_vjp_char_line<DIR, UPW, W>(u, lam, nu_in, ca, cb, nu, L, mode, nx, ny, nz, dxg, gamma, eps, xcd_block(), smem);
 }
 
 __global__ void __launch_bounds__(VT) k_visc_vjp_tile(const double *__restrict__ Q, const double *__restrict__ lam,
@@ -1215,7 +1298,7 @@ __device__ __forceinline__ int ens_find_cell(const EnsMember *mem, int nmem, siz
   a.u + 5 * M.off, a.lam + 5 * M.off, a.nu_in ? a.nu_in + 5 * M.off : nullptr, M.dt * a.A, M.dt * a.B,     \
       a.nu + 5 * M.off, a.L ? a.L + 5 * M.off : nullptr, a.mode, M.n[0], M.n[1], M.n[2], M.dxg, a.gamma, a.eps
 
-template <int DIR, int UPW>
+template <int DIR, int UPW, int W>
 __global__ void __launch_bounds__(VJP_LINE_MAX) k_ens_rhs_vjp(EnsVjp a, const unsigned *__restrict__ first,
                                                           unsigned nitems) {
   extern __shared__ double smem[];
@@ -1223,20 +1306,20 @@ __global__ void __launch_bounds__(VJP_LINE_MAX) k_ens_rhs_vjp(EnsVjp a, const un
   for (unsigned item = r.first; item < r.end; item += r.step) {
     const int m = ens_find(first, a.nmem, item);
     const EnsMember M = a.mem[m];
-    rhs_vjp_line<DIR, UPW>(ENS_MEMBER_ARGS(a, M), item - first[m], smem);
+    rhs_vjp_line<DIR, UPW, W>(ENS_MEMBER_ARGS(a, M), item - first[m], smem);
     __syncthreads();
   }
 }
 
-template <int DIR, int UPW>
-__global__ void __launch_bounds__(VJP_CHAR_LINE_MAX) k_ens_rhs_vjp_char(EnsVjp a, const unsigned *__restrict__ first,
+template <int DIR, int UPW, int W>
+__global__ void __launch_bounds__(VJP_CHAR_LINE_MAX) __attribute__((amdgpu_waves_per_eu(2))) k_ens_rhs_vjp_char(EnsVjp a, const unsigned *__restrict__ first,
                                                                     unsigned nitems) {
   extern __shared__ double smem[];
   const XcdItems r = xcd_items(nitems);
   for (unsigned item = r.first; item < r.end; item += r.step) {
     const int m = ens_find(first, a.nmem, item);
     const EnsMember M = a.mem[m];
-    rhs_vjp_char_line<DIR, UPW>(ENS_MEMBER_ARGS(a, M), item - first[m], smem);
+    rhs_vjp_char_line<DIR, UPW, W>(ENS_MEMBER_ARGS(a, M), item - first[m], smem);
     __syncthreads();
   }
 }
@@ -1303,7 +1386,6 @@ __global__ void k_axpby(const double *__restrict__ x, double a, double *__restri
 
 // ---------------------------------------------------------------------------
 bool vjp_supported(const Grid &g, const Scheme &sc) {
-  if (sc.weno != WENO_JS) return false;
   const int lmax = sc.interp == INTERP_CHAR ? VJP_CHAR_LINE_MAX : VJP_LINE_MAX;
   return g.n[0] <= lmax && g.n[1] <= lmax && g.n[2] <= lmax;
 }
@@ -1315,34 +1397,43 @@ static void allow_lds(const void *kernel, size_t bytes) {
   if (bytes > 65536) (void)hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
+// calls f(tag) with the scheme's upwinding and WENO weights as compile-time
+// constants tag.upw, tag.weno
+template <int U, int Wt>
+struct SchemeTag {
+  static constexpr int upw = U, weno = Wt;
+};
+
+template <class F>
+static void with_scheme(const Scheme &sc, F &&f) {
+  const bool roe = sc.upwind == UPWIND_ROE;
+  switch (sc.weno) {
+    case WENO_MAPPED: roe ? f(SchemeTag<UPWIND_ROE, WENO_MAPPED>{}) : f(SchemeTag<UPWIND_RUSANOV, WENO_MAPPED>{}); break;
+    case WENO_Z: roe ? f(SchemeTag<UPWIND_ROE, WENO_Z>{}) : f(SchemeTag<UPWIND_RUSANOV, WENO_Z>{}); break;
+    case WENO_YC: roe ? f(SchemeTag<UPWIND_ROE, WENO_YC>{}) : f(SchemeTag<UPWIND_RUSANOV, WENO_YC>{}); break;
+    default: roe ? f(SchemeTag<UPWIND_ROE, WENO_JS>{}) : f(SchemeTag<UPWIND_RUSANOV, WENO_JS>{}); break;
+  }
+}
+
 template <int DIR>
 static void launch_vjp_dir(const Grid &g, const Scheme &sc, const double *u, const double *lam, const double *nu_in,
                            double ca, double cb, double *nu, double *L, int mode, const double *dxg, hipStream_t st) {
   const unsigned lines = (unsigned)(g.ncell / g.n[DIR]);
   const unsigned threads = (unsigned)((g.n[DIR] + 63) / 64 * 64);
-  if (sc.interp == INTERP_COMP) {
-    const size_t lds = (size_t)23 * g.n[DIR] * sizeof(double);
-    if (sc.upwind == UPWIND_ROE) {
-      allow_lds((const void *)k_rhs_vjp<DIR, UPWIND_ROE>, lds);
-      hipLaunchKernelGGL((k_rhs_vjp<DIR, UPWIND_ROE>), dim3(lines), dim3(threads), lds, st, u, lam, nu_in, ca, cb,
-                         nu, L, mode, g.n[0], g.n[1], g.n[2], dxg, sc.gamma, sc.eps);
+  with_scheme(sc, [&](auto tag) {
+    constexpr int U = decltype(tag)::upw, W = decltype(tag)::weno;
+    if (sc.interp == INTERP_COMP) {
+      const size_t lds = (size_t)23 * g.n[DIR] * sizeof(double);
+      allow_lds((const void *)k_rhs_vjp<DIR, U, W>, lds);
+      hipLaunchKernelGGL((k_rhs_vjp<DIR, U, W>), dim3(lines), dim3(threads), lds, st, u, lam, nu_in, ca, cb, nu, L,
+                         mode, g.n[0], g.n[1], g.n[2], dxg, sc.gamma, sc.eps);
     } else {
-      allow_lds((const void *)k_rhs_vjp<DIR, UPWIND_RUSANOV>, lds);
-      hipLaunchKernelGGL((k_rhs_vjp<DIR, UPWIND_RUSANOV>), dim3(lines), dim3(threads), lds, st, u, lam, nu_in, ca,
-                         cb, nu, L, mode, g.n[0], g.n[1], g.n[2], dxg, sc.gamma, sc.eps);
+      const size_t lds = (size_t)VJP_CHAR_LDS * g.n[DIR] * sizeof(double);
+      allow_lds((const void *)k_rhs_vjp_char<DIR, U, W>, lds);
+      hipLaunchKernelGGL((k_rhs_vjp_char<DIR, U, W>), dim3(lines), dim3(threads), lds, st, u, lam, nu_in, ca, cb,
+                         nu, L, mode, g.n[0], g.n[1], g.n[2], dxg, sc.gamma, sc.eps);
     }
-    return;
-  }
-  const size_t lds = (size_t)VJP_CHAR_LDS * g.n[DIR] * sizeof(double);
-  if (sc.upwind == UPWIND_ROE) {
-    allow_lds((const void *)k_rhs_vjp_char<DIR, UPWIND_ROE>, lds);
-    hipLaunchKernelGGL((k_rhs_vjp_char<DIR, UPWIND_ROE>), dim3(lines), dim3(threads), lds, st, u, lam, nu_in, ca,
-                       cb, nu, L, mode, g.n[0], g.n[1], g.n[2], dxg, sc.gamma, sc.eps);
-  } else {
-    allow_lds((const void *)k_rhs_vjp_char<DIR, UPWIND_RUSANOV>, lds);
-    hipLaunchKernelGGL((k_rhs_vjp_char<DIR, UPWIND_RUSANOV>), dim3(lines), dim3(threads), lds, st, u, lam, nu_in,
-                       ca, cb, nu, L, mode, g.n[0], g.n[1], g.n[2], dxg, sc.gamma, sc.eps);
-  }
+  });
 }
 
 void launch_rhs_vjp(const Grid &g, const Scheme &sc, const double *u, const double *lam, const double *nu_in,
@@ -1390,29 +1481,23 @@ static void launch_ens_vjp_dir(const Scheme &sc, const EnsVjpPlan &p, const EnsV
   const unsigned items = p.nlines[DIR];
   const unsigned grid = items < 4096 ? items : 4096;  // persistent blocks stride over the lines
   const unsigned threads = (unsigned)((p.nmax[DIR] + 63) / 64 * 64);
-  if (sc.interp == INTERP_COMP) {
-    const size_t lds = (size_t)23 * p.nmax[DIR] * sizeof(double);
-    if (sc.upwind == UPWIND_ROE) {
-      allow_lds((const void *)k_ens_rhs_vjp<DIR, UPWIND_ROE>, lds);
-      hipLaunchKernelGGL((k_ens_rhs_vjp<DIR, UPWIND_ROE>), dim3(grid), dim3(threads), lds, st, a, p.lines0[DIR],
-                         items);
+  // ensembles run WENO5-JS weights (the batched passes are built for them)
+  const auto with_upwind = [&](auto &&f) {
+    sc.upwind == UPWIND_ROE ? f(SchemeTag<UPWIND_ROE, WENO_JS>{}) : f(SchemeTag<UPWIND_RUSANOV, WENO_JS>{});
+  };
+  with_upwind([&](auto tag) {
+    constexpr int U = decltype(tag)::upw, W = decltype(tag)::weno;
+    if (sc.interp == INTERP_COMP) {
+      const size_t lds = (size_t)23 * p.nmax[DIR] * sizeof(double);
+      allow_lds((const void *)k_ens_rhs_vjp<DIR, U, W>, lds);
+      hipLaunchKernelGGL((k_ens_rhs_vjp<DIR, U, W>), dim3(grid), dim3(threads), lds, st, a, p.lines0[DIR], items);
     } else {
-      allow_lds((const void *)k_ens_rhs_vjp<DIR, UPWIND_RUSANOV>, lds);
-      hipLaunchKernelGGL((k_ens_rhs_vjp<DIR, UPWIND_RUSANOV>), dim3(grid), dim3(threads), lds, st, a,
-                         p.lines0[DIR], items);
+      const size_t lds = (size_t)VJP_CHAR_LDS * p.nmax[DIR] * sizeof(double);
+      allow_lds((const void *)k_ens_rhs_vjp_char<DIR, U, W>, lds);
+      hipLaunchKernelGGL((k_ens_rhs_vjp_char<DIR, U, W>), dim3(grid), dim3(threads), lds, st, a, p.lines0[DIR],
+                         items);
     }
-    return;
-  }
-  const size_t lds = (size_t)VJP_CHAR_LDS * p.nmax[DIR] * sizeof(double);
-  if (sc.upwind == UPWIND_ROE) {
-    allow_lds((const void *)k_ens_rhs_vjp_char<DIR, UPWIND_ROE>, lds);
-    hipLaunchKernelGGL((k_ens_rhs_vjp_char<DIR, UPWIND_ROE>), dim3(grid), dim3(threads), lds, st, a,
-                       p.lines0[DIR], items);
-  } else {
-    allow_lds((const void *)k_ens_rhs_vjp_char<DIR, UPWIND_RUSANOV>, lds);
-    hipLaunchKernelGGL((k_ens_rhs_vjp_char<DIR, UPWIND_RUSANOV>), dim3(grid), dim3(threads), lds, st, a,
-                       p.lines0[DIR], items);
-  }
+  });
 }
 
 void launch_ens_rhs_vjp(const Scheme &sc, const EnsVjpPlan &p, const double *u, const double *lam,

@@ -1,6 +1,7 @@
 """Differentiable FP64 restatement of the NS3D WENO5 right-hand side and RK4
 step in torch (CPU), for reference gradients: component-wise or
-characteristic-wise WENO5-JS, Rusanov or Roe upwinding, and the viscous term.
+characteristic-wise WENO5 (JS, mapped, Z/Borges or YC weights), Rusanov or Roe
+upwinding, and the viscous term.
 
 TEST INFRASTRUCTURE ONLY: the adjoint kernels are checked against
 torch.autograd through this module (and against central finite
@@ -9,7 +10,8 @@ to the C oracle (and so to the reference) by tests/test_adjoint.py.
 
 Follows the same reference files as ns3d_oracle.c: flux and Roe average
 include/physicalmodels/navierstokes3d.h:98-169, eigenvectors :254-470,
-WENO5-JS include/interpolation.h:265-285 and Interp1PrimFifthOrderWENO.c:146-158,
+WENO5 weights include/interpolation.h:265-285 (JS), :314-341 (mapped), :426-447
+(Z), :479-500 (YC), interpolants Interp1PrimFifthOrderWENO.c:146-158,
 characteristic interpolation Interp1PrimFifthOrderWENOChar.c:86-198 with the
 weights of WENOFifthOrderCalculateWeights.c:760-896, Rusanov and Roe
 NavierStokes3DUpwind.c:40-124,349-420, flux difference
@@ -51,14 +53,23 @@ def modified(u, gamma=GAMMA):
     return torch.cat([u[:4], e2[None]])
 
 
-def weno5(m3, m2, m1, p1, p2, g3, g2, g1, q1, q2, eps):
+def weno5(m3, m2, m1, p1, p2, g3, g2, g1, q1, q2, eps, weights="js"):
+    """WENO5 value at the interface j+1/2 from smoothness stencil m3..p2 and
+    interpolated values g3..q2; weights "js", "mapped", "borges" (WENO-Z) or "yc"."""
     c1312, c14, s6 = 13.0 / 12.0, 0.25, 1.0 / 6.0
+    c = (0.1, 0.6, 0.3)
     b1 = c1312 * (m3 - 2 * m2 + m1) ** 2 + c14 * (m3 - 4 * m2 + 3 * m1) ** 2
     b2 = c1312 * (m2 - 2 * m1 + p1) ** 2 + c14 * (m2 - p1) ** 2
     b3 = c1312 * (m1 - 2 * p1 + p2) ** 2 + c14 * (3 * m1 - 4 * p1 + p2) ** 2
-    a1 = 0.1 / (b1 + eps) ** 2
-    a2 = 0.6 / (b2 + eps) ** 2
-    a3 = 0.3 / (b3 + eps) ** 2
+    if weights in ("js", "mapped"):
+        a1, a2, a3 = (ck / (bk + eps) ** 2 for ck, bk in zip(c, (b1, b2, b3)))
+        if weights == "mapped":
+            s = a1 + a2 + a3
+            a1, a2, a3 = ((ak / s) * (ck + ck * ck - 3 * ck * (ak / s) + (ak / s) ** 2)
+                          / (ck * ck + (ak / s) * (1.0 - 2.0 * ck)) for ck, ak in zip(c, (a1, a2, a3)))
+    else:
+        tau = torch.abs(b3 - b1) if weights == "borges" else (m3 - 4 * m2 + 6 * m1 - 4 * p1 + p2) ** 2
+        a1, a2, a3 = (ck * (1.0 + (tau / (bk + eps)) ** 2) for ck, bk in zip(c, (b1, b2, b3)))
     s = a1 + a2 + a3
     f1 = (2 * s6) * g3 + (-7 * s6) * g2 + (11 * s6) * g1
     f2 = (-s6) * g2 + (5 * s6) * g1 + (2 * s6) * q1
@@ -155,8 +166,8 @@ def harten(l, delta=1e-6):
     return torch.where(al < delta, (l * l + delta * delta) / (2 * delta), al)
 
 
-def interface_flux(u, d, eps=1e-6, gamma=GAMMA, interp="components", upwind="rusanov"):
-    """WENO5-JS interface flux at the interface left of every cell along
+def interface_flux(u, d, eps=1e-6, gamma=GAMMA, interp="components", upwind="rusanov", weights="js"):
+    """WENO5 interface flux at the interface left of every cell along
     direction d (component-wise or characteristic-wise; Rusanov or Roe)."""
     ax = AXIS[d]
     U = [torch.roll(u, shifts=3 - s, dims=ax) for s in range(6)]   # cells i-3 .. i+2
@@ -171,10 +182,10 @@ def interface_flux(u, d, eps=1e-6, gamma=GAMMA, interp="components", upwind="rus
         Um = [matvec(L, x) for x in U]
     else:
         Um = U
-    fL = weno5(*F[0:5], *F[0:5], eps)
-    fR = weno5(F[5], F[4], F[3], F[2], F[1], F[5], F[4], F[3], F[2], F[1], eps)
-    uL = weno5(*Um[0:5], *C[0:5], eps)
-    uR = weno5(Um[5], Um[4], Um[3], Um[2], Um[1], C[5], C[4], C[3], C[2], C[1], eps)
+    fL = weno5(*F[0:5], *F[0:5], eps, weights)
+    fR = weno5(F[5], F[4], F[3], F[2], F[1], F[5], F[4], F[3], F[2], F[1], eps, weights)
+    uL = weno5(*Um[0:5], *C[0:5], eps, weights)
+    uR = weno5(Um[5], Um[4], Um[3], Um[2], Um[1], C[5], C[4], C[3], C[2], C[1], eps, weights)
     if interp == "characteristic":
         fL, fR, uL, uR = (matvec(R, x) for x in (fL, fR, uL, uR))
     du = 0.5 * (uR - uL)
@@ -268,13 +279,14 @@ def parabolic(u, dxinv, Re, Pr=0.72, gamma=GAMMA):
     return torch.stack(par)
 
 
-def rhs(u, dxinv, eps=1e-6, gamma=GAMMA, interp="components", upwind="rusanov", Re=-1.0, Pr=0.72):
+def rhs(u, dxinv, eps=1e-6, gamma=GAMMA, interp="components", upwind="rusanov", Re=-1.0, Pr=0.72,
+        weights="js"):
     """-hyperbolic + viscous term; dxinv = (dx, dy, dz) interior inverse
     spacings; Re <= 0 is inviscid (Re already divided by Minf)."""
     hyp = torch.zeros_like(u)
     for d in range(3):
         ax = AXIS[d]
-        fI = interface_flux(u, d, eps, gamma, interp, upwind)
+        fI = interface_flux(u, d, eps, gamma, interp, upwind, weights)
         shape = [1, 1, 1, 1]
         shape[ax] = -1
         hyp = hyp + dxinv[d].reshape(shape) * (torch.roll(fI, shifts=-1, dims=ax) - fI)

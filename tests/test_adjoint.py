@@ -1,5 +1,6 @@
-"""Discrete adjoint of the WENO5-JS RK4 step: component-wise and
-characteristic-wise, Roe and Rusanov upwinding, inviscid and viscous.
+"""Discrete adjoint of the WENO5 RK4 step: component-wise and
+characteristic-wise, Roe and Rusanov upwinding, inviscid and viscous, with
+JS, mapped, WENO-Z and YC weights.
 
 The reference has no adjoint, so the checks are (a) the FP64 CPU reference
 gradient: torch.autograd through oracle/torch_ref.py, whose forward is pinned
@@ -24,12 +25,19 @@ SCHEMES = [("components", "rusanov", -1.0), ("characteristic", "roe", -1.0), ("c
            ("components", "roe", -1.0), ("characteristic", "roe", 1000.0), ("components", "rusanov", 1000.0)]
 
 
-def _scheme_case(interp, upwind, Re, dims=DIMS):
-    w = dataclasses.replace(decks.config1(12, 1), dims=dims, dt=None, interp=interp, upwind=upwind, Re=Re)
+def _scheme_case(interp, upwind, Re, dims=DIMS, weno_type="js"):
+    w = dataclasses.replace(decks.config1(12, 1), dims=dims, dt=None, interp=interp, upwind=upwind, Re=Re,
+                            weno_type=weno_type)
     u0, dt = decks.prepare(w)
     dx = [torch.tensor(decks.dxinv_1d(decks.grid_1d(n))[3:-3]) for n in dims]
-    scheme = dict(interp=interp, upwind=upwind, Re=Re / w.Minf if Re > 0 else -1.0, Pr=w.Pr)
+    scheme = dict(interp=interp, upwind=upwind, Re=Re / w.Minf if Re > 0 else -1.0, Pr=w.Pr, weights=weno_type)
     return w, u0, dt, dx, scheme
+
+
+# mapped, WENO-Z (Borges) and Yamaleev-Carpenter weights, on one scheme each
+WEIGHT_SCHEMES = [("components", "rusanov", -1.0, "mapped"), ("characteristic", "roe", 1000.0, "borges"),
+                  ("characteristic", "rusanov", -1.0, "yc"), ("components", "roe", 1000.0, "yc"),
+                  ("characteristic", "roe", -1.0, "mapped"), ("components", "rusanov", 1000.0, "borges")]
 
 
 @pytest.mark.parametrize("interp,upwind,Re", SCHEMES)
@@ -41,6 +49,16 @@ def test_torch_reference_matches_oracle(interp, upwind, Re):
     if Re > 0:
         par = T.to_aos(T.parabolic(T.to_soa(u0), dx, sch["Re"], sch["Pr"]))
         assert decks.rel_l2(par, o.parabolic(u0)) < 1e-14
+    u2 = T.to_aos(T.rk4_step(T.rk4_step(T.to_soa(u0), dt, dx, **sch), dt, dx, **sch))
+    assert decks.rel_l2(u2, o.rk4(u0, dt, 2)) < 1e-14
+
+
+@pytest.mark.parametrize("interp,upwind,Re,weno", WEIGHT_SCHEMES)
+def test_torch_reference_weight_variants_match_oracle(interp, upwind, Re, weno):
+    w, u0, dt, dx, sch = _scheme_case(interp, upwind, Re, weno_type=weno)
+    o = Oracle.for_workload(w)
+    r = T.to_aos(T.rhs(T.to_soa(u0), dx, **sch))
+    assert decks.rel_l2(r, o.rhs(u0)) < 1e-13
     u2 = T.to_aos(T.rk4_step(T.rk4_step(T.to_soa(u0), dt, dx, **sch), dt, dx, **sch))
     assert decks.rel_l2(u2, o.rk4(u0, dt, 2)) < 1e-14
 
@@ -117,15 +135,39 @@ def test_rhs_vjp_on_ragged_lines(interp, upwind, Re):
 
 
 @pytest.mark.gpu
-def test_vjp_rejects_unsupported_schemes():
+def test_vjp_rejects_lines_longer_than_a_workgroup():
+    """One workgroup holds one grid line: 512 cells component-wise, 256
+    characteristic-wise."""
     from hypar_amd import lib
     from hypar_amd.solver import NS3D
-    for kw in [dict(weno_type="mapped"), dict(weno_type="yc", interp="characteristic", upwind="roe")]:
-        s = NS3D(DIMS, **kw)
+    for dims, kw in [((513, 5, 5), {}), ((5, 257, 5), dict(interp="characteristic", upwind="roe"))]:
+        s = NS3D(dims, **kw)
         u = torch.ones(5 * s.ncell, dtype=torch.float64, device="cuda")
         with pytest.raises(lib.HMError) as ex:
             s.rhs_vjp(u, u.clone())
         assert ex.value.code == lib.HM_ERR_UNSUPPORTED
+        s.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("interp,upwind,Re,weno", WEIGHT_SCHEMES)
+def test_weight_variant_vjps_match_autograd(interp, upwind, Re, weno):
+    """Mapped, WENO-Z and YC weights: the RHS VJP and the RK4-step VJP against
+    torch autograd through the CPU reference (pinned to the oracle above)."""
+    w, u0, dt, dx, sch = _scheme_case(interp, upwind, Re, weno_type=weno)
+    x = T.to_soa(u0).requires_grad_(True)
+    lam = torch.tensor(np.random.default_rng(5).normal(size=x.shape))
+    (ref,) = torch.autograd.grad(T.rhs(x, dx, **sch), x, lam)
+    (ref_step,) = torch.autograd.grad(T.rk4_step(x, dt, dx, **sch), x, lam)
+    print(f"[gpu] rhs and rk4-step vjp {interp}/{upwind} Re={Re} weights={weno}", flush=True)
+    s = _gpu_solver(w)
+    err = _rel(s.rhs_vjp(_hm(x), _hm(lam)).cpu().view_as(ref), ref)
+    g = _hm(lam)
+    s.rk4_vjp(_hm(x), dt, g)
+    err_step = _rel(g.cpu().view_as(ref_step), ref_step)
+    print(f"[gpu]   rel err {err:.2e}, step {err_step:.2e}", flush=True)
+    assert err <= 1e-12 and err_step <= 1e-12
+    s.close()
 
 
 @pytest.mark.gpu
@@ -145,18 +187,20 @@ def test_rk4_step_vjp_matches_autograd(interp, upwind, Re):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("interp,upwind,Re", [("components", "rusanov", -1.0), ("characteristic", "roe", 1000.0)])
-def test_checkpointed_trajectory_gradient_matches_autograd(interp, upwind, Re):
+@pytest.mark.parametrize("interp,upwind,Re,weno", [("components", "rusanov", -1.0, "js"),
+                                                   ("characteristic", "roe", 1000.0, "js"),
+                                                   ("characteristic", "roe", 1000.0, "mapped")])
+def test_checkpointed_trajectory_gradient_matches_autograd(interp, upwind, Re, weno):
     """7 steps with a checkpoint every 3 (segments 3, 3, 1) against torch
     autograd through the unrolled CPU trajectory."""
     from hypar_amd.adjoint import kinetic_energy, trajectory
-    w, u0, dt, dx, sch = _scheme_case(interp, upwind, Re)
+    w, u0, dt, dx, sch = _scheme_case(interp, upwind, Re, weno_type=weno)
     x = T.to_soa(u0).requires_grad_(True)
     y = x
     for _ in range(7):
         y = T.rk4_step(y, dt, dx, **sch)
     (ref,) = torch.autograd.grad(T.kinetic_energy(y, dx), x)
-    print(f"[gpu] trajectory gradient {interp}/{upwind} Re={Re}", flush=True)
+    print(f"[gpu] trajectory gradient {interp}/{upwind} Re={Re} weights={weno}", flush=True)
     s = _gpu_solver(w)
     xg = _hm(x).requires_grad_(True)
     yN = trajectory(xg, s, dt, 7, 3)
