@@ -2,8 +2,8 @@
 //
 // The reference has no adjoint; this differentiates this library's own
 // discretisation of the NavierStokes3D right-hand side by hand, in reverse
-// mode: component-wise WENO5-JS with Rusanov upwinding, characteristic-wise
-// WENO5-JS with Roe or Rusanov upwinding (the operator of
+// mode: component-wise and characteristic-wise WENO5 (JS, mapped, Z and YC
+// weights) with Roe or Rusanov upwinding (the operator of
 // src/HyParFunctions/HyperbolicFunction.c:31-222, NavierStokes3DUpwind.c:40-124,
 // 349-420), and the viscous term (NavierStokes3DParabolicFunction.c:50-325).
 //
