@@ -37,9 +37,17 @@ def _run(cmd, cwd=ROOT):
 
 
 def build_hip():
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
-    tmp = LIB + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *srcs])
+    """Compile the HIP sources in parallel (one hipcc per source), then link."""
+    import concurrent.futures
+    import tempfile
+    with tempfile.TemporaryDirectory(prefix="hm_build_") as tdir:
+        objs = [os.path.join(tdir, os.path.splitext(s)[0] + ".o") for s in HIP_SOURCES]
+        cmds = [[HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, s), "-o", o] for s, o in zip(HIP_SOURCES, objs)]
+        with concurrent.futures.ThreadPoolExecutor(max_workers=min(len(cmds), os.cpu_count() or 1)) as ex:
+            for f in [ex.submit(_run, c) for c in cmds]:
+                f.result()
+        tmp = LIB + ".tmp"
+        _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *objs])
     os.replace(tmp, LIB)
     return LIB
 

@@ -1481,11 +1481,7 @@ static void launch_ens_vjp_dir(const Scheme &sc, const EnsVjpPlan &p, const EnsV
   const unsigned items = p.nlines[DIR];
   const unsigned grid = items < 4096 ? items : 4096;  // persistent blocks stride over the lines
   const unsigned threads = (unsigned)((p.nmax[DIR] + 63) / 64 * 64);
-  // ensembles run WENO5-JS weights (the batched passes are built for them)
-  const auto with_upwind = [&](auto &&f) {
-    sc.upwind == UPWIND_ROE ? f(SchemeTag<UPWIND_ROE, WENO_JS>{}) : f(SchemeTag<UPWIND_RUSANOV, WENO_JS>{});
-  };
-  with_upwind([&](auto tag) {
+  with_scheme(sc, [&](auto tag) {
     constexpr int U = decltype(tag)::upw, W = decltype(tag)::weno;
     if (sc.interp == INTERP_COMP) {
       const size_t lds = (size_t)23 * p.nmax[DIR] * sizeof(double);

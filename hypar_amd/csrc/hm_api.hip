@@ -907,14 +907,14 @@ hm_status hm_ensemble_create(const hm_config *members, int nmembers, hm_ensemble
     const hm_config &c = members[m];
     for (int d = 0; d < 3; d++)
       if (c.dim[d] < 5) return fail(HM_ERR_ARG, "hm_ensemble_create: every grid dimension must be >= 5");
-    if (c.interp != c0.interp || c.upwind != c0.upwind || c.gamma != c0.gamma || c.weno_eps != c0.weno_eps)
-      return fail(HM_ERR_ARG, "hm_ensemble_create: members must share interpolation, upwinding, gamma and eps");
+    if (c.interp != c0.interp || c.upwind != c0.upwind || c.gamma != c0.gamma || c.weno_eps != c0.weno_eps ||
+        c.weno_type != c0.weno_type)
+      return fail(HM_ERR_ARG,
+                  "hm_ensemble_create: members must share interpolation, upwinding, WENO weights, gamma and eps");
     if ((c.Re > 0) != (c0.Re > 0) || (c0.Re > 0 && (c.Re != c0.Re || c.Minf != c0.Minf || c.Pr != c0.Pr)))
       return fail(HM_ERR_ARG, "hm_ensemble_create: members must share Re, Minf and Pr");
     if (c.Re > 0 && !(c.Minf > 0 && c.Pr > 0))
       return fail(HM_ERR_ARG, "hm_ensemble_create: viscous members need Minf > 0 and Pr > 0");
-    if (c.weno_type != HM_WENO_JS)
-      return fail(HM_ERR_UNSUPPORTED, "hm_ensemble_create: batched members use Jiang-Shu weights");
   }
   if (c0.interp != HM_INTERP_COMPONENTS && c0.interp != HM_INTERP_CHARACTERISTIC)
     return fail(HM_ERR_UNSUPPORTED, "hm_ensemble_create: unknown interpolation type");
@@ -922,6 +922,8 @@ hm_status hm_ensemble_create(const hm_config *members, int nmembers, hm_ensemble
     return fail(HM_ERR_UNSUPPORTED, "hm_ensemble_create: unknown upwinding scheme");
   if (!(c0.gamma > 1.0)) return fail(HM_ERR_ARG, "hm_ensemble_create: gamma must exceed 1");
   if (!(c0.weno_eps > 0.0)) return fail(HM_ERR_ARG, "hm_ensemble_create: weno_eps must be positive");
+  if (c0.weno_type < HM_WENO_JS || c0.weno_type > HM_WENO_YC)
+    return fail(HM_ERR_UNSUPPORTED, "hm_ensemble_create: unknown WENO weight formulation");
 
   hm_ensemble *e = new hm_ensemble();
   e->sc.interp = c0.interp == HM_INTERP_CHARACTERISTIC ? hm::INTERP_CHAR : hm::INTERP_COMP;
@@ -930,7 +932,7 @@ hm_status hm_ensemble_create(const hm_config *members, int nmembers, hm_ensemble
   e->sc.Re = c0.Re > 0 ? c0.Re / c0.Minf : -1.0;
   e->sc.Pr = c0.Pr;
   e->sc.eps = c0.weno_eps;
-  e->sc.weno = hm::WENO_JS;
+  e->sc.weno = c0.weno_type;
   e->nmem = nmembers;
   hm_status st = HM_OK;
   std::vector<double> dxh;

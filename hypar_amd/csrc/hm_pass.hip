@@ -678,7 +678,7 @@ __device__ __forceinline__ void ens_setup(const EnsArgs &a, int m, int dir, Pass
   io.cb = M.dt * b[a.stage];
 }
 
-template <int INTERP, int UPW, int LPB>
+template <int INTERP, int UPW, int LPB, int W>
 __global__ void __launch_bounds__(64 * LPB) k_ens_x(EnsArgs a) {
   __shared__ XShared<LPB> sh;
   for (unsigned item = blockIdx.x; item < a.nitems; item += gridDim.x) {
@@ -686,12 +686,12 @@ __global__ void __launch_bounds__(64 * LPB) k_ens_x(EnsArgs a) {
     PassConst k;
     PassIO io;
     ens_setup(a, m, 0, k, io);
-    pass_x_body<INTERP, UPW, LPB, WENO_JS>(k, io, item - a.item0[m], sh);
+    pass_x_body<INTERP, UPW, LPB, W>(k, io, item - a.item0[m], sh);
     __syncthreads();
   }
 }
 
-template <int DIR, int INTERP, int UPW, int MR>
+template <int DIR, int INTERP, int UPW, int MR, int W>
 __global__ void __launch_bounds__(64 * MR) __attribute__((amdgpu_waves_per_eu(2))) k_ens_m(EnsArgs a) {
   __shared__ MShared<MR> sh;
   for (unsigned item = blockIdx.x; item < a.nitems; item += gridDim.x) {
@@ -699,7 +699,7 @@ __global__ void __launch_bounds__(64 * MR) __attribute__((amdgpu_waves_per_eu(2)
     PassConst k;
     PassIO io;
     ens_setup(a, m, DIR, k, io);
-    pass_m_body<DIR, INTERP, UPW, MR, WENO_JS>(k, io, item - a.item0[m], sh);
+    pass_m_body<DIR, INTERP, UPW, MR, W>(k, io, item - a.item0[m], sh);
     __syncthreads();
   }
 }
@@ -783,18 +783,26 @@ void launch_pass(const Grid &g, const Scheme &sc, int dir, const PassBuffers &b,
 
 
 namespace {
-template <int INTERP, int UPW>
-void ens_launch_t(int dir, int shape, const EnsArgs &a, unsigned grid, hipStream_t st) {
+template <int INTERP, int UPW, int W>
+void ens_launch_w(int dir, int shape, const EnsArgs &a, unsigned grid, hipStream_t st) {
   if (dir == 0) {
-    if (shape == 0) hipLaunchKernelGGL((k_ens_x<INTERP, UPW, 4>), dim3(grid), dim3(256), 0, st, a);
-    else hipLaunchKernelGGL((k_ens_x<INTERP, UPW, 2>), dim3(grid), dim3(128), 0, st, a);
+    if (shape == 0) hipLaunchKernelGGL((k_ens_x<INTERP, UPW, 4, W>), dim3(grid), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_ens_x<INTERP, UPW, 2, W>), dim3(grid), dim3(128), 0, st, a);
   } else if (dir == 1) {
-    if (shape == 0) hipLaunchKernelGGL((k_ens_m<1, INTERP, UPW, 4>), dim3(grid), dim3(256), 0, st, a);
-    else hipLaunchKernelGGL((k_ens_m<1, INTERP, UPW, 2>), dim3(grid), dim3(128), 0, st, a);
+    if (shape == 0) hipLaunchKernelGGL((k_ens_m<1, INTERP, UPW, 4, W>), dim3(grid), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_ens_m<1, INTERP, UPW, 2, W>), dim3(grid), dim3(128), 0, st, a);
   } else {
-    if (shape == 0) hipLaunchKernelGGL((k_ens_m<2, INTERP, UPW, 4>), dim3(grid), dim3(256), 0, st, a);
-    else hipLaunchKernelGGL((k_ens_m<2, INTERP, UPW, 2>), dim3(grid), dim3(128), 0, st, a);
+    if (shape == 0) hipLaunchKernelGGL((k_ens_m<2, INTERP, UPW, 4, W>), dim3(grid), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_ens_m<2, INTERP, UPW, 2, W>), dim3(grid), dim3(128), 0, st, a);
   }
+}
+
+template <int INTERP, int UPW>
+void ens_launch_t(int weno, int dir, int shape, const EnsArgs &a, unsigned grid, hipStream_t st) {
+  if (weno == WENO_MAPPED) ens_launch_w<INTERP, UPW, WENO_MAPPED>(dir, shape, a, grid, st);
+  else if (weno == WENO_Z) ens_launch_w<INTERP, UPW, WENO_Z>(dir, shape, a, grid, st);
+  else if (weno == WENO_YC) ens_launch_w<INTERP, UPW, WENO_YC>(dir, shape, a, grid, st);
+  else ens_launch_w<INTERP, UPW, WENO_JS>(dir, shape, a, grid, st);
 }
 }  // namespace
 
@@ -829,11 +837,11 @@ void launch_ens_pass(const Scheme &sc, int dir, int shape, const EnsMember *mem,
   a.visc = sc.Re > 0;
   if (grid == 0 || grid > nitems) grid = nitems;
   if (sc.interp == INTERP_CHAR) {
-    if (sc.upwind == UPWIND_ROE) ens_launch_t<INTERP_CHAR, UPWIND_ROE>(dir, shape, a, grid, st);
-    else ens_launch_t<INTERP_CHAR, UPWIND_RUSANOV>(dir, shape, a, grid, st);
+    if (sc.upwind == UPWIND_ROE) ens_launch_t<INTERP_CHAR, UPWIND_ROE>(sc.weno, dir, shape, a, grid, st);
+    else ens_launch_t<INTERP_CHAR, UPWIND_RUSANOV>(sc.weno, dir, shape, a, grid, st);
   } else {
-    if (sc.upwind == UPWIND_ROE) ens_launch_t<INTERP_COMP, UPWIND_ROE>(dir, shape, a, grid, st);
-    else ens_launch_t<INTERP_COMP, UPWIND_RUSANOV>(dir, shape, a, grid, st);
+    if (sc.upwind == UPWIND_ROE) ens_launch_t<INTERP_COMP, UPWIND_ROE>(sc.weno, dir, shape, a, grid, st);
+    else ens_launch_t<INTERP_COMP, UPWIND_RUSANOV>(sc.weno, dir, shape, a, grid, st);
   }
 }
 

@@ -18,7 +18,8 @@ from . import lib as _lib
 
 class Ensemble:
     def __init__(self, dims: Sequence[Sequence[int]], interp: str = "components", upwind: str = "rusanov",
-                 gamma: float = 1.4, eps: float = 1e-6, Re: float = -1.0, Minf: float = 1.0, Pr: float = 0.72):
+                 gamma: float = 1.4, eps: float = 1e-6, Re: float = -1.0, Minf: float = 1.0, Pr: float = 0.72,
+                 weno_type: str = "js"):
         if not torch.cuda.is_available():
             raise RuntimeError("Ensemble needs a GPU (torch.cuda.is_available() is False)")
         self.L = _lib.load()
@@ -31,6 +32,7 @@ class Ensemble:
             c.interp = _lib.INTERP[interp]
             c.upwind = _lib.UPWIND[upwind]
             c.weno_eps = eps
+            c.weno_type = _lib.WENO[weno_type]
         torch.cuda.init()
         h = ctypes.c_void_p()
         _lib.check(self.L.hm_ensemble_create(cfgs, n, ctypes.byref(h)))

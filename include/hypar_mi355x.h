@@ -209,7 +209,7 @@ hm_status hm_profile_kernels(hm_solver *s, const double *u, int reps, double *ms
  * items of all members, and one RK4 step is a captured HIP graph replayed
  * per step; viscous members add one launch per stage over the (member,
  * tile) items of the viscous fluxes.  Members share the scheme
- * (interpolation, upwinding, gamma, eps, Re, Minf, Pr; WENO5-JS weights).
+ * (interpolation, upwinding, WENO weights, gamma, eps, Re, Minf, Pr).
  * The batched state holds member m's hm-layout state at
  * u[5*offset_m .. 5*(offset_m + ncell_m)).
  * ------------------------------------------------------------------------ */

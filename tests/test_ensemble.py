@@ -22,12 +22,12 @@ TOL = 1e-12
 DIMS = [(17, 13, 11), (32, 48, 64), (5, 5, 5), (64, 32, 48), (70, 7, 9), (6, 66, 5)]
 
 
-def _members(dims, interp="components", upwind="rusanov", Re=-1.0):
+def _members(dims, interp="components", upwind="rusanov", Re=-1.0, weno="js"):
     from hypar_amd import decks
     ws, u0s, dts = [], [], []
     for m, d in enumerate(dims):
         w = dataclasses.replace(decks.config0(8, 1), dims=tuple(d), interp=interp, upwind=upwind,
-                                seed=20262 * 1000 + m, dt=None, Re=Re, Minf=0.1 if Re > 0 else 1.0)
+                                seed=20262 * 1000 + m, dt=None, Re=Re, Minf=0.1 if Re > 0 else 1.0, weno_type=weno)
         u0, dt = decks.prepare(w)
         ws.append(w)
         u0s.append(u0)
@@ -45,14 +45,18 @@ def _single(w, u0, dt, nsteps):
     return out
 
 
-@pytest.mark.parametrize("interp,upwind,Re", [("components", "rusanov", -1.0), ("characteristic", "roe", -1.0),
-                                               ("characteristic", "roe", 1000.0)])
-def test_members_match_single_grid(interp, upwind, Re):
+@pytest.mark.parametrize("interp,upwind,Re,weno", [("components", "rusanov", -1.0, "js"),
+                                                   ("characteristic", "roe", -1.0, "js"),
+                                                   ("characteristic", "roe", 1000.0, "js"),
+                                                   ("characteristic", "roe", 1000.0, "mapped"),
+                                                   ("components", "rusanov", -1.0, "yc"),
+                                                   ("characteristic", "rusanov", -1.0, "borges")])
+def test_members_match_single_grid(interp, upwind, Re, weno):
     from hypar_amd import decks
     from hypar_amd.ensemble import Ensemble
-    ws, u0s, dts = _members(DIMS, interp, upwind, Re)
-    print(f"[gpu] ensemble {interp}/{upwind} Re={Re} dims={DIMS}", flush=True)
-    e = Ensemble(DIMS, interp, upwind, Re=Re, Minf=0.1 if Re > 0 else 1.0)
+    ws, u0s, dts = _members(DIMS, interp, upwind, Re, weno)
+    print(f"[gpu] ensemble {interp}/{upwind} Re={Re} weights={weno} dims={DIMS}", flush=True)
+    e = Ensemble(DIMS, interp, upwind, Re=Re, Minf=0.1 if Re > 0 else 1.0, weno_type=weno)
     u = e.to_device(u0s)
     e.set_dt(dts)
     e.rk4(u, 3)
@@ -152,9 +156,11 @@ def test_viscous_reference_fixtures():
     assert err <= TOL
 
 
-@pytest.mark.parametrize("interp,upwind,Re", [("components", "rusanov", -1.0), ("characteristic", "roe", 1000.0),
-                                               ("components", "roe", -1.0)])
-def test_ensemble_adjoint_matches_single_grid(interp, upwind, Re):
+@pytest.mark.parametrize("interp,upwind,Re,weno", [("components", "rusanov", -1.0, "js"),
+                                                   ("characteristic", "roe", 1000.0, "js"),
+                                                   ("components", "roe", -1.0, "js"),
+                                                   ("characteristic", "roe", 1000.0, "yc")])
+def test_ensemble_adjoint_matches_single_grid(interp, upwind, Re, weno):
     """Batched trajectory gradient (5 steps, checkpoint every 2, each member
     with its own dt) against each member's single-grid adjoint, which
     test_adjoint.py checks against torch autograd.  The last member spans two
@@ -164,9 +170,9 @@ def test_ensemble_adjoint_matches_single_grid(interp, upwind, Re):
     from hypar_amd.ensemble import Ensemble
     from hypar_amd.solver import NS3D
     dims = [(12, 10, 14), (16, 12, 10), (10, 14, 12), (70, 9, 18)]
-    ws, u0s, dts = _members(dims, interp, upwind, Re)
-    print(f"[gpu] ensemble adjoint {interp}/{upwind} Re={Re} dims={dims}", flush=True)
-    e = Ensemble(dims, interp, upwind, Re=Re, Minf=0.1 if Re > 0 else 1.0)
+    ws, u0s, dts = _members(dims, interp, upwind, Re, weno)
+    print(f"[gpu] ensemble adjoint {interp}/{upwind} Re={Re} weights={weno} dims={dims}", flush=True)
+    e = Ensemble(dims, interp, upwind, Re=Re, Minf=0.1 if Re > 0 else 1.0, weno_type=weno)
     e.set_dt(dts)
     u0 = e.to_device(u0s).requires_grad_(True)
     uN = ensemble_trajectory(u0, e, 5, 2)
