@@ -1115,6 +1115,24 @@ __device__ __forceinline__ void visc_vjp_ghost_line(const double *__restrict__ Q
 #pragma unroll
     for (int k = 0; k < 4; k++) a[r][k] *= dxd[3 + rd[r]];
   }
+  // the seven cells gp-3 .. gp+3 of slot s (positions -2, -1, n, n+1,
+  // periodic) are entries s .. s+6 of the window n-5 .. n-1, 0 .. 4 (n >= 5),
+  // loaded once; the slots' adjoints onto Q accumulate per window entry and
+  // reach QB with one read-modify-write each (entries that are the same cell
+  // of a short line add in turn)
+  double f[10][4], acc[10][4];
+  size_t cw[10];
+#pragma unroll
+  for (int j = 0; j < 10; j++) {
+    cw[j] = base + (size_t)(j < 5 ? n - 5 + j : j - 5) * stride;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      f[j][c] = Q[(size_t)c * ncell + cw[j]];
+      acc[j][c] = 0.0;
+    }
+  }
+  const double r12 = 1.0 / 12.0;
+#pragma unroll
   for (int slot = 0; slot < 4; slot++) {
     const int gp = slot < 2 ? slot - 2 : n + slot - 2;
     double fb[4];
@@ -1125,46 +1143,42 @@ __device__ __forceinline__ void visc_vjp_ghost_line(const double *__restrict__ Q
       else if (slot == 2) fb[k] = d4w(1) * a[3][k] + d4w(2) * a[2][k];
       else fb[k] = d4w(2) * a[3][k];
     }
-    // the forward evaluation at this ghost position (k_visc_ghost)
-    double f[7][4];
-    size_t cw[7];
-#pragma unroll
-    for (int s = 0; s < 7; s++) {
-      cw[s] = base + (size_t)wrapn(gp - 3 + s, n) * stride;  // n >= 5: one wrap suffices
-#pragma unroll
-      for (int c = 0; c < 4; c++) f[s][c] = Q[(size_t)c * ncell + cw[s]];
-    }
+    // the forward evaluation at this ghost position (k_visc_ghost_x, k_visc_ghost)
+    const double(*w)[4] = f + slot;  // positions gp-3 .. gp+3
     // stencil weights over positions gp-3 .. gp+3
     double w7[7] = {0, 0, 0, 0, 0, 0, 0};
-    const double r12 = 1.0 / 12.0;
-    if (gp == -2) { w7[2] = -3 * r12; w7[3] = -10 * r12; w7[4] = 18 * r12; w7[5] = -6 * r12; w7[6] = r12; }
-    else if (gp == n + 1) { w7[0] = -r12; w7[1] = 6 * r12; w7[2] = -18 * r12; w7[3] = 10 * r12; w7[4] = 3 * r12; }
+    if (slot == 0) { w7[2] = -3 * r12; w7[3] = -10 * r12; w7[4] = 18 * r12; w7[5] = -6 * r12; w7[6] = r12; }
+    else if (slot == 3) { w7[0] = -r12; w7[1] = 6 * r12; w7[2] = -18 * r12; w7[3] = 10 * r12; w7[4] = 3 * r12; }
     else { w7[1] = r12; w7[2] = -8 * r12; w7[4] = 8 * r12; w7[5] = -r12; }
     const double dxgh = dxd[3 + gp];
     double gr[4][3];
 #pragma unroll
     for (int c = 0; c < 4; c++) {
       double d = 0.0;
-      if (gp == -2) d = (-3 * f[2][c] - 10 * f[3][c] + 18 * f[4][c] - 6 * f[5][c] + f[6][c]) * r12;
-      else if (gp == n + 1) d = (-f[0][c] + 6 * f[1][c] - 18 * f[2][c] + 10 * f[3][c] + 3 * f[4][c]) * r12;
-      else d = d4c(f[1][c], f[2][c], f[4][c], f[5][c]);
+      if (slot == 0) d = (-3 * w[2][c] - 10 * w[3][c] + 18 * w[4][c] - 6 * w[5][c] + w[6][c]) * r12;
+      else if (slot == 3) d = (-w[0][c] + 6 * w[1][c] - 18 * w[2][c] + 10 * w[3][c] + 3 * w[4][c]) * r12;
+      else d = d4c(w[1][c], w[2][c], w[4][c], w[5][c]);
 #pragma unroll
       for (int e = 0; e < 3; e++) gr[c][e] = e == DIR ? d * dxgh : 0.0;
     }
-    const double q[4] = {f[3][0], f[3][1], f[3][2], f[3][3]};
+    const double q[4] = {w[3][0], w[3][1], w[3][2], w[3][3]};
     const double mu = exp(0.76 * log(q[3]));
     double qb[4] = {0, 0, 0, 0}, grb[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     const double mub = visc_flux_vjp<DIR>(q, gr, mu, vc, fb, qb, grb);
     qb[3] += mub * (0.76 * mu / q[3]);
 #pragma unroll
-    for (int c = 0; c < 4; c++) QB[(size_t)c * ncell + cw[3]] += qb[c];
+    for (int c = 0; c < 4; c++) acc[slot + 3][c] += qb[c];
 #pragma unroll
     for (int s = 0; s < 7; s++) {
       if (w7[s] == 0.0) continue;
 #pragma unroll
-      for (int c = 0; c < 4; c++) QB[(size_t)c * ncell + cw[s]] += w7[s] * dxgh * grb[c][DIR];
+      for (int c = 0; c < 4; c++) acc[slot + s][c] += w7[s] * dxgh * grb[c][DIR];
     }
   }
+#pragma unroll
+  for (int j = 0; j < 10; j++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) QB[(size_t)c * ncell + cw[j]] += acc[j][c];
 }
 
 // nu = (d par/d u)^T mu: QB plus the transposed gradient stencils over GB,

@@ -10,7 +10,7 @@
 //   k_visc_flux        viscous fluxes of x, y, z at every cell from the
 //                      primitive variables the x pass stores
 //                      (NavierStokes3DParabolicFunction.c:85-195)
-//   k_visc_ghost<DIR>  viscous fluxes at the reference's ghost layers
+//   k_visc_ghost_x, k_visc_ghost<DIR>  viscous fluxes at the reference's ghost layers
 //   k_stage            flux differences in x, y, z, the viscous divergence,
 //                      rhs = -hyp + par, and the RK4 stage update in 3N form
 //                      (TimeRHSFunctionExplicit.c:88-93, TimeRK.c:126-195)
@@ -364,6 +364,71 @@ __device__ __forceinline__ void visc_ghost_one(const double *__restrict__ Q, dou
   for (int c = 0; c < 4; c++) FVg[(size_t)(slot * 4 + c) * nplane + line] = fv[c];
 }
 
+// The four ghost-layer fluxes of x-line `line` (line = j + ny*k) from one
+// thread.  Slot s (positions -2, -1, n, n+1) reads the seven cells
+// gp-3 .. gp+3, periodic; together the four read the ten cells n-5 .. n-1,
+// 0 .. 4, loaded once, with slot s taking window entries s .. s+6.  The x
+// lines lie along the contiguous direction, so the one-thread-per-(slot,
+// line) form of visc_ghost_one reads each cell's cache line from four
+// threads far apart in the grid.
+__device__ __forceinline__ void visc_ghost_xline(const double *__restrict__ Q, double *__restrict__ FVg, int nx,
+                                                 int ny, int nz, const double *__restrict__ dxg,
+                                                 const ViscParams &vp, size_t line) {
+  const size_t ncell = (size_t)nx * ny * nz;
+  const size_t nplane = ncell / nx;
+  if (line >= nplane) return;
+  const size_t base = line * nx;
+  QT f[10];  // cells n-5 .. n-1, 0 .. 4 (n >= 5)
+#pragma unroll
+  for (int j = 0; j < 5; j++) {
+    f[j] = load_q(Q, ncell, base + (size_t)(nx - 5 + j));
+    f[5 + j] = load_q(Q, ncell, base + (size_t)j);
+  }
+  const double r12 = 1.0 / 12.0;
+#pragma unroll
+  for (int slot = 0; slot < 4; slot++) {
+    const QT *w = f + slot;  // positions gp-3 .. gp+3
+    const int gp = slot < 2 ? slot - 2 : nx + slot - 2;
+    double d[4];
+    if (slot == 0) {
+      // (-3 f(-3) - 10 f(-2) + 18 f(-1) - 6 f(0) + f(1)) / 12
+      d[0] = (-3 * w[2].u - 10 * w[3].u + 18 * w[4].u - 6 * w[5].u + w[6].u) * r12;
+      d[1] = (-3 * w[2].v - 10 * w[3].v + 18 * w[4].v - 6 * w[5].v + w[6].v) * r12;
+      d[2] = (-3 * w[2].w - 10 * w[3].w + 18 * w[4].w - 6 * w[5].w + w[6].w) * r12;
+      d[3] = (-3 * w[2].T - 10 * w[3].T + 18 * w[4].T - 6 * w[5].T + w[6].T) * r12;
+    } else if (slot == 3) {
+      // (-f(n-2) + 6 f(n-1) - 18 f(n) + 10 f(n+1) + 3 f(n+2)) / 12
+      d[0] = (-w[0].u + 6 * w[1].u - 18 * w[2].u + 10 * w[3].u + 3 * w[4].u) * r12;
+      d[1] = (-w[0].v + 6 * w[1].v - 18 * w[2].v + 10 * w[3].v + 3 * w[4].v) * r12;
+      d[2] = (-w[0].w + 6 * w[1].w - 18 * w[2].w + 10 * w[3].w + 3 * w[4].w) * r12;
+      d[3] = (-w[0].T + 6 * w[1].T - 18 * w[2].T + 10 * w[3].T + 3 * w[4].T) * r12;
+    } else {
+      d[0] = d4c(w[1].u, w[2].u, w[4].u, w[5].u);
+      d[1] = d4c(w[1].v, w[2].v, w[4].v, w[5].v);
+      d[2] = d4c(w[1].w, w[2].w, w[4].w, w[5].w);
+      d[3] = d4c(w[1].T, w[2].T, w[4].T, w[5].T);
+    }
+    const double dxgh = dxg[3 + gp];  // entry 0 is position -3
+    double gr[4][3];
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+      for (int e = 0; e < 3; e++) gr[c][e] = e == 0 ? d[c] * dxgh : 0.0;
+    const QT &q = w[3];
+    const double mu = exp(0.76 * log(q.T));
+    double fv[4];
+    visc_flux<0>(fv, q, gr, mu, vp);
+#pragma unroll
+    for (int c = 0; c < 4; c++) FVg[(size_t)(slot * 4 + c) * nplane + line] = fv[c];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_visc_ghost_x(const double *__restrict__ Q, double *__restrict__ FVg, int nx,
+                                                      int ny, int nz, const double *__restrict__ dxg,
+                                                      ViscParams vp) {
+  visc_ghost_xline(Q, FVg, nx, ny, nz, dxg, vp, (size_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 template <int DIR>
 __global__ void __launch_bounds__(256) k_visc_ghost(const double *__restrict__ Q, double *__restrict__ FVg,
                                                     int nx, int ny, int nz, const double *__restrict__ dxg,
@@ -382,8 +447,12 @@ struct EnsViscArgs {
   ViscParams vp;
 };
 
-__host__ __device__ __forceinline__ unsigned ghost_blocks(size_t ncell, int n) {
-  return (unsigned)((4 * (ncell / n) + VT - 1) / VT);
+// work items of VT threads for the ghost-layer fluxes of one direction: one
+// thread per x-line (all four positions), one per (position, line) along y
+// and z
+__host__ __device__ __forceinline__ unsigned ghost_blocks(int dir, size_t ncell, int n) {
+  const size_t threads = dir == 0 ? ncell / n : 4 * (ncell / n);
+  return (unsigned)((threads + VT - 1) / VT);
 }
 
 __global__ void __launch_bounds__(VTX * VTY) k_ens_visc(EnsViscArgs a) {
@@ -404,9 +473,9 @@ __global__ void __launch_bounds__(VTX * VTY) k_ens_visc(EnsViscArgs a) {
       continue;
     }
     r -= tiles;
-    const unsigned gx = ghost_blocks(M.ncell, nx), gy = ghost_blocks(M.ncell, ny);
+    const unsigned gx = ghost_blocks(0, M.ncell, nx), gy = ghost_blocks(1, M.ncell, ny);
     const size_t t = threadIdx.x;
-    if (r < gx) visc_ghost_one<0>(M.q, M.fvg[0], nx, ny, nz, M.dxg, a.vp, (size_t)r * VT + t);
+    if (r < gx) visc_ghost_xline(M.q, M.fvg[0], nx, ny, nz, M.dxg, a.vp, (size_t)r * VT + t);
     else if (r < gx + gy) visc_ghost_one<1>(M.q, M.fvg[1], nx, ny, nz, M.dxg, a.vp, (size_t)(r - gx) * VT + t);
     else visc_ghost_one<2>(M.q, M.fvg[2], nx, ny, nz, M.dxg, a.vp, (size_t)(r - gx - gy) * VT + t);
   }
@@ -636,8 +705,8 @@ void launch_visc_flux(const Grid &g, const Scheme &sc, const double *Q, double *
   const ViscParams vp = visc_params(sc);
   const unsigned nb = visc_tiles(g.n[0], g.n[1], g.n[2]);
   hipLaunchKernelGGL(k_visc_flux, dim3(nb), dim3(VTX * VTY), 0, st, Q, FV, g.n[0], g.n[1], g.n[2], dxg, vp);
-  hipLaunchKernelGGL(k_visc_ghost<0>, dim3(nblocks(4 * (g.ncell / g.n[0]), 256)), dim3(256), 0, st, Q,
-                     FVg[0], g.n[0], g.n[1], g.n[2], dxg, vp);
+  hipLaunchKernelGGL(k_visc_ghost_x, dim3(nblocks(g.ncell / g.n[0], 256)), dim3(256), 0, st, Q, FVg[0], g.n[0],
+                     g.n[1], g.n[2], dxg, vp);
   hipLaunchKernelGGL(k_visc_ghost<1>, dim3(nblocks(4 * (g.ncell / g.n[1]), 256)), dim3(256), 0, st, Q,
                      FVg[1], g.n[0], g.n[1], g.n[2], dxg, vp);
   hipLaunchKernelGGL(k_visc_ghost<2>, dim3(nblocks(4 * (g.ncell / g.n[2]), 256)), dim3(256), 0, st, Q,
@@ -646,8 +715,8 @@ void launch_visc_flux(const Grid &g, const Scheme &sc, const double *Q, double *
 
 unsigned ens_visc_items_per_member(const int n[3]) {
   const size_t ncell = (size_t)n[0] * n[1] * n[2];
-  return visc_tiles(n[0], n[1], n[2]) + ghost_blocks(ncell, n[0]) + ghost_blocks(ncell, n[1]) +
-         ghost_blocks(ncell, n[2]);
+  return visc_tiles(n[0], n[1], n[2]) + ghost_blocks(0, ncell, n[0]) + ghost_blocks(1, ncell, n[1]) +
+         ghost_blocks(2, ncell, n[2]);
 }
 
 void launch_ens_visc(const Scheme &sc, const EnsMember *mem, int nmem, const unsigned *item0v, unsigned nitems,
