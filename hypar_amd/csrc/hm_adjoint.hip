@@ -57,6 +57,7 @@ __device__ __forceinline__ double vjp_rcp(double d) {
 //  * mapped: the JS weights u_k mapped by u (c + c^2 - 3 c u + u^2)/
 //            (c^2 + u (1 - 2c)) and renormalised; the adjoint goes back
 //            through the mapping and the JS normalisation
+//  * optimal (no_limiting): the constant weights 0.1, 0.6, 0.3
 // The a_k are formed multiplied through by their common denominator; one
 // reciprocal of (sum a_k)(prod (beta_k + eps)) gives both 1/sum a_k and the
 // 1/(beta_k + eps) (the mapped weights take a second one for the mapping).
@@ -71,7 +72,14 @@ __device__ __forceinline__ W5Side w5_side(double b1, double b2, double b3, doubl
   const double s1 = e1 * e1, s2 = e2 * e2, s3 = e3 * e3;
   W5Side r;
   r.gt = 0.0;
-  if (W == WENO_JS) {
+  if (W == WENO_OPTIMAL) {
+    // no_limiting: constant optimal weights, no dependence on beta_k
+    r.w1 = 0.1;
+    r.w2 = 0.6;
+    r.w3 = 0.3;
+    r.out = 0.1 * f1 + 0.6 * f2 + 0.3 * f3;
+    r.g1 = r.g2 = r.g3 = 0.0;
+  } else if (W == WENO_JS) {
     const double a1 = 0.1 * (s2 * s3), a2 = 0.6 * (s1 * s3), a3 = 0.3 * (s1 * s2);
     const double A = a1 + a2 + a3, e23 = e2 * e3, P = e1 * e23;
     const double q = vjp_rcp(A * P);
@@ -1425,6 +1433,9 @@ static void with_scheme(const Scheme &sc, F &&f) {
     case WENO_MAPPED: roe ? f(SchemeTag<UPWIND_ROE, WENO_MAPPED>{}) : f(SchemeTag<UPWIND_RUSANOV, WENO_MAPPED>{}); break;
     case WENO_Z: roe ? f(SchemeTag<UPWIND_ROE, WENO_Z>{}) : f(SchemeTag<UPWIND_RUSANOV, WENO_Z>{}); break;
     case WENO_YC: roe ? f(SchemeTag<UPWIND_ROE, WENO_YC>{}) : f(SchemeTag<UPWIND_RUSANOV, WENO_YC>{}); break;
+    case WENO_OPTIMAL:
+      roe ? f(SchemeTag<UPWIND_ROE, WENO_OPTIMAL>{}) : f(SchemeTag<UPWIND_RUSANOV, WENO_OPTIMAL>{});
+      break;
     default: roe ? f(SchemeTag<UPWIND_ROE, WENO_JS>{}) : f(SchemeTag<UPWIND_RUSANOV, WENO_JS>{}); break;
   }
 }

@@ -196,7 +196,7 @@ hm_status hm_create(const hm_config *cfg, hm_solver **out) {
     return fail(HM_ERR_UNSUPPORTED, "hm_create: unknown upwinding scheme");
   if (!(cfg->gamma > 1.0)) return fail(HM_ERR_ARG, "hm_create: gamma must exceed 1");
   if (!(cfg->weno_eps > 0.0)) return fail(HM_ERR_ARG, "hm_create: weno_eps must be positive");
-  if (cfg->weno_type < HM_WENO_JS || cfg->weno_type > HM_WENO_YC)
+  if (cfg->weno_type < HM_WENO_JS || cfg->weno_type > HM_WENO_OPTIMAL)
     return fail(HM_ERR_UNSUPPORTED, "hm_create: unknown WENO weight formulation");
   if (cfg->Re > 0 && !(cfg->Minf > 0 && cfg->Pr > 0))
     return fail(HM_ERR_ARG, "hm_create: viscous runs need Minf > 0 and Pr > 0");
@@ -922,7 +922,7 @@ hm_status hm_ensemble_create(const hm_config *members, int nmembers, hm_ensemble
     return fail(HM_ERR_UNSUPPORTED, "hm_ensemble_create: unknown upwinding scheme");
   if (!(c0.gamma > 1.0)) return fail(HM_ERR_ARG, "hm_ensemble_create: gamma must exceed 1");
   if (!(c0.weno_eps > 0.0)) return fail(HM_ERR_ARG, "hm_ensemble_create: weno_eps must be positive");
-  if (c0.weno_type < HM_WENO_JS || c0.weno_type > HM_WENO_YC)
+  if (c0.weno_type < HM_WENO_JS || c0.weno_type > HM_WENO_OPTIMAL)
     return fail(HM_ERR_UNSUPPORTED, "hm_ensemble_create: unknown WENO weight formulation");
 
   hm_ensemble *e = new hm_ensemble();

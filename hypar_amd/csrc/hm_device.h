@@ -164,8 +164,9 @@ __device__ __forceinline__ double harten_abs(double l) {
 // interpolated (they differ only for the solution, whose weights come from u
 // but whose interpolant uses the modified solution uC).  `type` selects the
 // weights (0 Jiang-Shu, 1 mapped, 2 WENO-Z, 3 Yamaleev-Carpenter;
-// include/interpolation.h:265-500): JS, Z and YC are applied in ratio form
-// (one division), the mapped weights as the reference forms them.
+// include/interpolation.h:265-500; 4 the frozen optimal weights of
+// no_limiting): JS, Z and YC are applied in ratio form (one division), the
+// mapped weights as the reference forms them.
 __device__ __forceinline__ double weno5(double m3, double m2, double m1, double p1, double p2,
                                         double g3, double g2, double g1, double q1, double q2,
                                         double eps, int type = 0) {
@@ -180,6 +181,7 @@ __device__ __forceinline__ double weno5(double m3, double m2, double m1, double 
   const double f1 = (2 * s6) * g3 + (-7 * s6) * g2 + (11 * s6) * g1;
   const double f2 = (-s6) * g2 + (5 * s6) * g1 + (2 * s6) * q1;
   const double f3 = (2 * s6) * g1 + (5 * s6) * q1 + (-s6) * q2;
+  if (type == 4) return 0.1 * f1 + 0.6 * f2 + 0.3 * f3;
   const double s1 = (b1 + eps) * (b1 + eps);
   const double s2 = (b2 + eps) * (b2 + eps);
   const double s3 = (b3 + eps) * (b3 + eps);

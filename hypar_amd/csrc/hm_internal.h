@@ -9,7 +9,7 @@ namespace hm {
 enum { INTERP_COMP = 0, INTERP_CHAR = 1 };
 enum { UPWIND_ROE = 0, UPWIND_RUSANOV = 1 };
 enum { OUT_STAGE = 0, OUT_RHS = 1, OUT_HYP = 2, OUT_PAR = 3 };
-enum { WENO_JS = 0, WENO_MAPPED = 1, WENO_Z = 2, WENO_YC = 3 };
+enum { WENO_JS = 0, WENO_MAPPED = 1, WENO_Z = 2, WENO_YC = 3, WENO_OPTIMAL = 4 };
 
 struct Grid {
   int n[3];

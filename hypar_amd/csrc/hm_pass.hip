@@ -127,6 +127,9 @@ __device__ __forceinline__ double weno_combine(double l1, double l2, double l3, 
   const double f1 = (2 * s6) * g3 + (-7 * s6) * g2 + (11 * s6) * g1;
   const double f2 = (-s6) * g2 + (5 * s6) * g1 + (2 * s6) * q1;
   const double f3 = (2 * s6) * g1 + (5 * s6) * q1 + (-s6) * q2;
+  // no_limiting: the weights stay at their optimal initial values
+  // (WENOFifthOrderInitializeWeights.c:33-228, HyperbolicFunction.c:62)
+  if (W == WENO_OPTIMAL) return 0.1 * f1 + 0.6 * f2 + 0.3 * f3;
   double a1, a2, a3;
   if (W == WENO_Z || W == WENO_YC) {
     // tau scales like beta for WENO-Z; the YC tau does not, hence 4 tau
@@ -735,6 +738,7 @@ void launch_w(const Grid &g, const Scheme &sc, const PassConst &k, const PassIO 
   if (sc.weno == WENO_MAPPED) launch_t<DIR, INTERP, UPW, WENO_MAPPED>(g, k, io, st);
   else if (sc.weno == WENO_Z) launch_t<DIR, INTERP, UPW, WENO_Z>(g, k, io, st);
   else if (sc.weno == WENO_YC) launch_t<DIR, INTERP, UPW, WENO_YC>(g, k, io, st);
+  else if (sc.weno == WENO_OPTIMAL) launch_t<DIR, INTERP, UPW, WENO_OPTIMAL>(g, k, io, st);
   else launch_t<DIR, INTERP, UPW, WENO_JS>(g, k, io, st);
 }
 
@@ -802,6 +806,7 @@ void ens_launch_t(int weno, int dir, int shape, const EnsArgs &a, unsigned grid,
   if (weno == WENO_MAPPED) ens_launch_w<INTERP, UPW, WENO_MAPPED>(dir, shape, a, grid, st);
   else if (weno == WENO_Z) ens_launch_w<INTERP, UPW, WENO_Z>(dir, shape, a, grid, st);
   else if (weno == WENO_YC) ens_launch_w<INTERP, UPW, WENO_YC>(dir, shape, a, grid, st);
+  else if (weno == WENO_OPTIMAL) ens_launch_w<INTERP, UPW, WENO_OPTIMAL>(dir, shape, a, grid, st);
   else ens_launch_w<INTERP, UPW, WENO_JS>(dir, shape, a, grid, st);
 }
 }  // namespace

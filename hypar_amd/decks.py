@@ -49,7 +49,7 @@ class Workload:
     Pr: float = 0.72
     gamma: float = GAMMA
     weno_eps: float = 1e-6
-    weno_type: str = "js"             # js | mapped | borges | yc
+    weno_type: str = "js"             # js | mapped | borges | yc | optimal (no_limiting 1)
     n_iter: int = 100
     cfl: float = 0.5
     seed: int = 20260
@@ -269,8 +269,9 @@ def prepare(w: Workload):
 # deck files
 # --------------------------------------------------------------------------
 def _weno_flags(weno_type: str):
-    return {"js": (0, 0, 0), "mapped": (1, 0, 0), "borges": (0, 1, 0),
-            "yc": (0, 0, 1)}[weno_type]
+    """(mapped, borges, yc, no_limiting) of weno.inp"""
+    return {"js": (0, 0, 0, 0), "mapped": (1, 0, 0, 0), "borges": (0, 1, 0, 0),
+            "yc": (0, 0, 1, 0), "optimal": (0, 0, 0, 1)}[weno_type]
 
 
 def write_solver_inp(path: str, dims_list, n_iter: int, dt: float, interp: str,
@@ -317,13 +318,13 @@ def write_physics_inp(path: str, w: Workload):
 
 
 def write_weno_inp(path: str, w: Workload):
-    m, b, y = _weno_flags(w.weno_type)
+    m, b, y, nl = _weno_flags(w.weno_type)
     with open(os.path.join(path, "weno.inp"), "w") as f:
         f.write("begin\n")
         f.write(f"  mapped        {m}\n")
         f.write(f"  borges        {b}\n")
         f.write(f"  yc            {y}\n")
-        f.write("  no_limiting   0\n")
+        f.write(f"  no_limiting   {nl}\n")
         f.write(f"  epsilon       {w.weno_eps:.17g}\n")
         f.write("  p             2.0\n")
         f.write("end\n")

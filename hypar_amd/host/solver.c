@@ -90,8 +90,6 @@ int hy_initialize_solvers(HyPar *s, const char *dir) {
     return hy_error("hyp_flux_split yes is not supported by the GPU path");
   s->interp = (WENOParameters *)calloc(1, sizeof(WENOParameters));
   if (hy_read_weno(dir, s->interp)) return 1;
-  if (s->interp->no_limiting)
-    return hy_error("no_limiting (frozen optimal WENO5 weights) is not supported");
   s->HyperbolicFunction = HyperbolicFunction;
   s->SourceFunction = SourceFunction;
   s->InterpolateInterfacesHyp = Interp1PrimFifthOrderWENO;
@@ -169,9 +167,13 @@ int NavierStokes3DInitialize(void *sv, void *m, const char *dir) {
   cfg.interp = strcmp(s->interp_type, "characteristic") ? HM_INTERP_COMPONENTS : HM_INTERP_CHARACTERISTIC;
   cfg.upwind = strcmp(p->upw_choice, "roe") ? HM_UPWIND_RUSANOV : HM_UPWIND_ROE;
   cfg.weno_eps = s->interp->eps;
-  /* the reference's precedence (WENOFifthOrderCalculateWeights.c:55-57) */
-  cfg.weno_type = s->interp->yc ? HM_WENO_YC
-                  : (s->interp->borges ? HM_WENO_Z : (s->interp->mapped ? HM_WENO_MAPPED : HM_WENO_JS));
+  /* the reference's precedence (WENOFifthOrderCalculateWeights.c:55-57);
+   * no_limiting keeps the initial optimal weights whatever the formulation
+   * (InitializeSolvers.c:219, HyperbolicFunction.c:62) */
+  cfg.weno_type = s->interp->no_limiting ? HM_WENO_OPTIMAL
+                  : (s->interp->yc ? HM_WENO_YC
+                                   : (s->interp->borges ? HM_WENO_Z
+                                                        : (s->interp->mapped ? HM_WENO_MAPPED : HM_WENO_JS)));
   cfg.dxinv = s->dxinv;
   if (hm_check(hm_create(&cfg, &s->dev), "hm_create")) return 1;
   if (alloc_device(s)) return 1;

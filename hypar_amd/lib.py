@@ -18,7 +18,7 @@ HM_OK = 0
 HM_ERR_ARG, HM_ERR_HIP, HM_ERR_NONFINITE, HM_ERR_UNSUPPORTED, HM_ERR_NOMEM = 1, 2, 3, 4, 5
 INTERP = {"components": 0, "characteristic": 1}
 UPWIND = {"roe": 0, "rusanov": 1}
-WENO = {"js": 0, "mapped": 1, "borges": 2, "yc": 3}
+WENO = {"js": 0, "mapped": 1, "borges": 2, "yc": 3, "optimal": 4}
 LAYOUT_REFERENCE, LAYOUT_AOS_INTERIOR, LAYOUT_SOA_INTERIOR = 0, 1, 2
 
 

@@ -56,8 +56,12 @@ typedef enum { HM_INTERP_COMPONENTS = 0, HM_INTERP_CHARACTERISTIC = 1 } hm_inter
 typedef enum { HM_UPWIND_ROE = 0, HM_UPWIND_RUSANOV = 1 } hm_upwind_kind;
 /* WENO weight formulation of weno.inp (WENOInitialize.c:62-96; dispatch
  * WENOFifthOrderCalculateWeights.c:55-57): Jiang-Shu, mapped (Henrick,
- * Aslam & Powers), WENO-Z (Borges et al.), Yamaleev-Carpenter */
-typedef enum { HM_WENO_JS = 0, HM_WENO_MAPPED = 1, HM_WENO_Z = 2, HM_WENO_YC = 3 } hm_weno_kind;
+ * Aslam & Powers), WENO-Z (Borges et al.), Yamaleev-Carpenter, and the
+ * optimal weights 0.1, 0.6, 0.3 that no_limiting 1 freezes
+ * (WENOFifthOrderInitializeWeights.c:33-228, InitializeSolvers.c:219) */
+typedef enum {
+  HM_WENO_JS = 0, HM_WENO_MAPPED = 1, HM_WENO_Z = 2, HM_WENO_YC = 3, HM_WENO_OPTIMAL = 4
+} hm_weno_kind;
 /* host array layouts accepted by the transfer calls */
 typedef enum {
   HM_LAYOUT_REFERENCE = 0, /* AoS, variables fastest, 3 ghost layers (HyPar CPU) */

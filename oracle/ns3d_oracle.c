@@ -327,10 +327,17 @@ static inline void weno_tau(int yc, double *w1, double *w2, double *w3, double c
   *w3 = a3 * a_sum_inv;
 }
 
-/* dispatch on the weight formulation (WENOFifthOrderCalculateWeights.c:55-57, 79-81) */
+/* dispatch on the weight formulation (WENOFifthOrderCalculateWeights.c:55-57, 79-81);
+ * with no_limiting the weights keep the optimal values they are initialised
+ * to (WENOFifthOrderInitializeWeights.c:33-228, InitializeSolvers.c:219,
+ * HyperbolicFunction.c:62) */
 static inline void weno_weights(int type, double *w1, double *w2, double *w3, double c1, double c2, double c3,
                                 double m3, double m2, double m1, double p1, double p2, double eps) {
-  if (type == ORC_WENO_YC) weno_tau(1, w1, w2, w3, c1, c2, c3, m3, m2, m1, p1, p2, eps);
+  if (type == ORC_WENO_OPTIMAL) {
+    *w1 = c1;
+    *w2 = c2;
+    *w3 = c3;
+  } else if (type == ORC_WENO_YC) weno_tau(1, w1, w2, w3, c1, c2, c3, m3, m2, m1, p1, p2, eps);
   else if (type == ORC_WENO_Z) weno_tau(0, w1, w2, w3, c1, c2, c3, m3, m2, m1, p1, p2, eps);
   else if (type == ORC_WENO_MAPPED) weno_mapped(w1, w2, w3, c1, c2, c3, m3, m2, m1, p1, p2, eps);
   else weno_js(w1, w2, w3, c1, c2, c3, m3, m2, m1, p1, p2, eps);

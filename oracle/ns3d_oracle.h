@@ -18,7 +18,7 @@ extern "C" {
 enum { ORC_INTERP_COMPONENTS = 0, ORC_INTERP_CHARACTERISTIC = 1 };
 enum { ORC_UPWIND_ROE = 0, ORC_UPWIND_RUSANOV = 1 };
 /* WENO weight formulations (weno.inp: mapped / borges / yc; default JS) */
-enum { ORC_WENO_JS = 0, ORC_WENO_MAPPED = 1, ORC_WENO_Z = 2, ORC_WENO_YC = 3 };
+enum { ORC_WENO_JS = 0, ORC_WENO_MAPPED = 1, ORC_WENO_Z = 2, ORC_WENO_YC = 3, ORC_WENO_OPTIMAL = 4 };
 
 typedef struct {
   int dim[3];          /* interior grid size                                  */

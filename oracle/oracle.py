@@ -25,7 +25,7 @@ class OrcParams(ctypes.Structure):
                 ("upwind", ctypes.c_int), ("eps", ctypes.c_double),
                 ("dxinv", ctypes.POINTER(ctypes.c_double)), ("weno_type", ctypes.c_int)]
 
-WENO_TYPES = {"js": 0, "mapped": 1, "borges": 2, "yc": 3}
+WENO_TYPES = {"js": 0, "mapped": 1, "borges": 2, "yc": 3, "optimal": 4}
 
 
 def lib():

@@ -55,13 +55,16 @@ def modified(u, gamma=GAMMA):
 
 def weno5(m3, m2, m1, p1, p2, g3, g2, g1, q1, q2, eps, weights="js"):
     """WENO5 value at the interface j+1/2 from smoothness stencil m3..p2 and
-    interpolated values g3..q2; weights "js", "mapped", "borges" (WENO-Z) or "yc"."""
+    interpolated values g3..q2; weights "js", "mapped", "borges" (WENO-Z), "yc"
+    or "optimal" (no_limiting)."""
     c1312, c14, s6 = 13.0 / 12.0, 0.25, 1.0 / 6.0
     c = (0.1, 0.6, 0.3)
     b1 = c1312 * (m3 - 2 * m2 + m1) ** 2 + c14 * (m3 - 4 * m2 + 3 * m1) ** 2
     b2 = c1312 * (m2 - 2 * m1 + p1) ** 2 + c14 * (m2 - p1) ** 2
     b3 = c1312 * (m1 - 2 * p1 + p2) ** 2 + c14 * (3 * m1 - 4 * p1 + p2) ** 2
-    if weights in ("js", "mapped"):
+    if weights == "optimal":  # no_limiting: the weights stay at c
+        a1, a2, a3 = c
+    elif weights in ("js", "mapped"):
         a1, a2, a3 = (ck / (bk + eps) ** 2 for ck, bk in zip(c, (b1, b2, b3)))
         if weights == "mapped":
             s = a1 + a2 + a3

@@ -76,6 +76,9 @@ FIXTURES = {
     "c1_mapped_16_s4": ("c1+mapped", (16, 16, 16), 4, True),
     "c1_borges_16_s4": ("c1+borges", (16, 16, 16), 4, True),
     "c1_yc_16_s4": ("c1+yc", (16, 16, 16), 4, True),
+    # no_limiting 1: the weights stay at their optimal values
+    "c0_optimal_16_s4": ("c0+optimal", (16, 16, 16), 4, True),
+    "c1_optimal_16_s4": ("c1+optimal", (16, 16, 16), 4, True),
 }
 
 # fixtures whose decks turn on the conservation check; they also record the

@@ -34,10 +34,11 @@ def _scheme_case(interp, upwind, Re, dims=DIMS, weno_type="js"):
     return w, u0, dt, dx, scheme
 
 
-# mapped, WENO-Z (Borges) and Yamaleev-Carpenter weights, on one scheme each
+# mapped, WENO-Z (Borges), Yamaleev-Carpenter and the optimal (no_limiting) weights
 WEIGHT_SCHEMES = [("components", "rusanov", -1.0, "mapped"), ("characteristic", "roe", 1000.0, "borges"),
                   ("characteristic", "rusanov", -1.0, "yc"), ("components", "roe", 1000.0, "yc"),
-                  ("characteristic", "roe", -1.0, "mapped"), ("components", "rusanov", 1000.0, "borges")]
+                  ("characteristic", "roe", -1.0, "mapped"), ("components", "rusanov", 1000.0, "borges"),
+                  ("characteristic", "roe", 1000.0, "optimal"), ("components", "rusanov", -1.0, "optimal")]
 
 
 @pytest.mark.parametrize("interp,upwind,Re", SCHEMES)
@@ -152,7 +153,7 @@ def test_vjp_rejects_lines_longer_than_a_workgroup():
 @pytest.mark.gpu
 @pytest.mark.parametrize("interp,upwind,Re,weno", WEIGHT_SCHEMES)
 def test_weight_variant_vjps_match_autograd(interp, upwind, Re, weno):
-    """Mapped, WENO-Z and YC weights: the RHS VJP and the RK4-step VJP against
+    """Mapped, WENO-Z, YC and optimal weights: the RHS VJP and the RK4-step VJP against
     torch autograd through the CPU reference (pinned to the oracle above)."""
     w, u0, dt, dx, sch = _scheme_case(interp, upwind, Re, weno_type=weno)
     x = T.to_soa(u0).requires_grad_(True)

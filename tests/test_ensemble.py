@@ -50,7 +50,8 @@ def _single(w, u0, dt, nsteps):
                                                    ("characteristic", "roe", 1000.0, "js"),
                                                    ("characteristic", "roe", 1000.0, "mapped"),
                                                    ("components", "rusanov", -1.0, "yc"),
-                                                   ("characteristic", "rusanov", -1.0, "borges")])
+                                                   ("characteristic", "rusanov", -1.0, "borges"),
+                                                   ("components", "roe", -1.0, "optimal")])
 def test_members_match_single_grid(interp, upwind, Re, weno):
     from hypar_amd import decks
     from hypar_amd.ensemble import Ensemble

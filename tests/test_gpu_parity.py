@@ -33,7 +33,7 @@ def _solver(w):
     return NS3D.from_workload(w)
 
 
-WEIGHTS = [f"{c}_{t}_16_s4" for c in ("c0", "c1") for t in ("mapped", "borges", "yc")]
+WEIGHTS = [f"{c}_{t}_16_s4" for c in ("c0", "c1") for t in ("mapped", "borges", "yc", "optimal")]
 FULL = ["c0_16_s1", "c0_16_s10", "c1_16_s1", "c1_16_s10", "c0_ragged_s3", "c1_ragged_s3"] + WEIGHTS
 
 

@@ -48,13 +48,6 @@ def test_rejects_gravity(tmp_path):
     assert r.returncode != 0 and "gravity is not supported" in r.stderr
 
 
-def test_rejects_frozen_weno_weights(tmp_path):
-    d, *_ = _deck(tmp_path)
-    _edit(os.path.join(d, "weno.inp"), "no_limiting   0", "no_limiting   1")
-    r = run_driver(d)
-    assert r.returncode != 0 and "no_limiting" in r.stderr
-
-
 def test_rejects_non_periodic_boundary(tmp_path):
     d, *_ = _deck(tmp_path)
     _edit(os.path.join(d, "boundary.inp"), "periodic 2 -1", "extrapolate 2 -1")
@@ -79,7 +72,8 @@ def test_missing_solver_inp(tmp_path):
 @pytest.mark.parametrize("name,args", [("c0_16_s10", []), ("c1_16_s10", []), ("c1_16_s10", ["--slots"]),
                                        ("c0_16_s1", ["--slots"]), ("c0_ssprk3_16_s4", []),
                                        ("c1_mapped_16_s4", []), ("c0_yc_16_s4", ["--slots"]),
-                                       ("c1_borges_16_s4", ["--slots"])])
+                                       ("c1_borges_16_s4", ["--slots"]), ("c1_optimal_16_s4", []),
+                                       ("c0_optimal_16_s4", ["--slots"])])
 def test_driver_matches_reference(tmp_path, name, args):
     d, w, meta, z = _deck(tmp_path, name)
     print(f"[gpu] driver {name} {args}", flush=True)

@@ -10,7 +10,7 @@ from conftest import load_fixture, workload_from_meta
 from hypar_amd import decks
 from oracle.oracle import Oracle
 
-WEIGHTS = [f"{c}_{t}_16_s4" for c in ("c0", "c1") for t in ("mapped", "borges", "yc")]
+WEIGHTS = [f"{c}_{t}_16_s4" for c in ("c0", "c1") for t in ("mapped", "borges", "yc", "optimal")]
 FULL = ["c0_16_s1", "c0_16_s10", "c1_16_s1", "c1_16_s10", "c0_ragged_s3", "c1_ragged_s3"] + WEIGHTS
 
 
@@ -57,8 +57,8 @@ def test_oracle_ensemble_members_share_dt():
 
 @pytest.mark.parametrize("name", WEIGHTS)
 def test_weight_formulation_changes_the_result(name):
-    """The mapped/Z/YC fixtures differ from the Jiang-Shu result of the same
-    workload, so a fixture cannot pass on JS weights."""
+    """The mapped/Z/YC/optimal fixtures differ from the Jiang-Shu result of
+    the same workload, so a fixture cannot pass on JS weights."""
     import dataclasses
     meta, z = load_fixture(name)
     w = dataclasses.replace(workload_from_meta(meta), weno_type="js")

@@ -94,7 +94,7 @@ def test_weno_fifth_order_on_smooth_data():
 
 
 # ------------------------------------------- mapped / Z / Yamaleev-Carpenter
-WTYPES = {"js": 0, "mapped": 1, "borges": 2, "yc": 3}
+WTYPES = {"js": 0, "mapped": 1, "borges": 2, "yc": 3, "optimal": 4}
 
 
 def _weights(kind, f, eps=1e-6):
@@ -103,7 +103,7 @@ def _weights(kind, f, eps=1e-6):
     return np.array(out[:])
 
 
-@pytest.mark.parametrize("kind", ["js", "mapped", "borges", "yc"])
+@pytest.mark.parametrize("kind", ["js", "mapped", "borges", "yc", "optimal"])
 def test_weights_convex_and_optimal_on_constant_data(kind):
     c = np.array([0.1, 0.6, 0.3])
     assert np.allclose(_weights(kind, [2.5] * 5), c, rtol=0, atol=4e-16)
@@ -127,6 +127,7 @@ def test_modified_weights_approach_optimal_faster_on_smooth_data():
     assert dev["js"] > 1e-4
     for k in ("mapped", "borges", "yc"):
         assert dev[k] < 1e-2 * dev["js"], dev
+    assert dev["optimal"] == 0.0  # no_limiting: the weights are c on any data
 
 
 # ------------------------------------------------------------- eigen system
