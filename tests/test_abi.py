@@ -49,3 +49,7 @@ def test_create_rejects_bad_config_before_touching_device():
     cfg.dim[:] = [16, 16, 16]
     cfg.interp = 7
     assert L.hm_create(ctypes.byref(cfg), ctypes.byref(h)) == lib.HM_ERR_UNSUPPORTED
+    cfg.interp = 0
+    cfg.weno_type = 5  # one past the optimal (no_limiting) weights
+    assert L.hm_create(ctypes.byref(cfg), ctypes.byref(h)) == lib.HM_ERR_UNSUPPORTED
+    assert b"WENO weight" in L.hm_last_error()
