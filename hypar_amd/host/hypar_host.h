@@ -46,6 +46,8 @@ typedef struct HyPar {
   /* state and scratch on the device (hm layout) */
   double *u, *hyp, *par, *source, *uC, *fluxC, *fluxI, *fL, *fR, *uL, *uR;
   hm_solver *dev;
+  hm_config cfg;            /* the configuration dev was created with      */
+  int u_shared;             /* u lies inside the batched ensemble state    */
   /* solver.inp */
   int n_iter, restart_iter, screen_op_iter, file_op_iter;
   double dt;
@@ -93,6 +95,8 @@ typedef struct TimeIntegration {
   double *U[4], *Udot[4];   /* per member, generic path: stage arrays */
   double **Ustore, **Kstore;/* nsims*4 device arrays                   */
   double **uprev;           /* per member copy for the step norm       */
+  hm_ensemble *ens;         /* fused ensembles: all members per launch  */
+  double *ens_u;            /* batched state; member u point into it    */
   int (*RHSFunction)(double *, double *, void *, void *, double);
   int (*TimeIntegrate)(void *);
 } TimeIntegration;
@@ -142,6 +146,7 @@ int ApplyBoundaryConditions(void *s, void *m, double *x, double *xref, double wa
 /* ---- time integration (TimeInitialize.c, TimeRK.c, TimeRHSFunctionExplicit.c) ---- */
 int TimeExplicitRKInitialize(const char *cls, const char *type, ExplicitRKParameters *p);
 int TimeInitialize(TimeIntegration *TS, HyPar *sims, MPIVariables *mpi, int nsims, int fused);
+int EnsembleInitialize(TimeIntegration *TS);
 int TimeRHSFunctionExplicit(double *rhs, double *u, void *s, void *m, double t);
 int TimeRK(void *ts);
 int TimeRKFused(void *ts);

@@ -70,7 +70,8 @@ static int alloc_device(HyPar *s) {
 }
 
 void hy_cleanup(HyPar *s) {
-  double *bufs[] = {s->u, s->hyp, s->par, s->source, s->uC, s->fluxC, s->fluxI, s->fL, s->fR, s->uL, s->uR};
+  double *bufs[] = {s->u_shared ? NULL : s->u, s->hyp, s->par, s->source, s->uC, s->fluxC, s->fluxI,
+                    s->fL, s->fR, s->uL, s->uR};
   for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++) hm_free(bufs[i]);
   if (s->dev) hm_destroy(s->dev);
   free(s->x);
@@ -176,6 +177,7 @@ int NavierStokes3DInitialize(void *sv, void *m, const char *dir) {
                                                         : (s->interp->mapped ? HM_WENO_MAPPED : HM_WENO_JS)));
   cfg.dxinv = s->dxinv;
   if (hm_check(hm_create(&cfg, &s->dev), "hm_create")) return 1;
+  s->cfg = cfg;
   if (alloc_device(s)) return 1;
   /* physics slots (NavierStokes3DInitialize.c:388-506) */
   s->PreStep = NavierStokes3DPreStep;
@@ -393,10 +395,55 @@ int TimeInitialize(TimeIntegration *TS, HyPar *sims, MPIVariables *mpi, int nsim
       if (hm_check(hm_malloc(&TS->Kstore[4 * n + st], n5(&sims[n])), "hm_malloc")) return 1;
     }
   }
+  if (TS->fused && nsims > 1) return EnsembleInitialize(TS);
+  return 0;
+}
+
+/* Fused ensembles: the members (which share the scheme, TimeInitialize.c:49-52
+ * and ReadInputs.c:189-207) advance together, every direction pass of an RK
+ * stage one launch over all members.  Each member's state moves into the
+ * batched state and its u points there, so the per-member work (CFL, norm,
+ * output, conservation) reads it as before. */
+int EnsembleInitialize(TimeIntegration *TS) {
+  const int nsims = TS->nsims;
+  hm_config *cfgs = (hm_config *)calloc((size_t)nsims, sizeof(hm_config));
+  double *dts = (double *)calloc((size_t)nsims, sizeof(double));
+  for (int n = 0; n < nsims; n++) {
+    cfgs[n] = TS->sims[n].cfg;
+    dts[n] = TS->dt;
+  }
+  const hm_status st = hm_ensemble_create(cfgs, nsims, &TS->ens);
+  free(cfgs);
+  if (st != HM_OK) {
+    printf("Note: members advance one after another (%s)\n", hm_last_error());
+    TS->ens = NULL;
+    free(dts);
+    return 0;
+  }
+  if (hm_check(hm_malloc(&TS->ens_u, (size_t)NV * hm_ensemble_ncell(TS->ens)), "hm_malloc") ||
+      hm_check(hm_ensemble_set_dt(TS->ens, dts), "hm_ensemble_set_dt")) {
+    free(dts);
+    return 1;
+  }
+  free(dts);
+  for (int n = 0; n < nsims; n++) {
+    HyPar *s = &TS->sims[n];
+    size_t off, nc;
+    if (hm_check(hm_ensemble_member(TS->ens, n, &off, &nc), "hm_ensemble_member")) return 1;
+    double *dst = TS->ens_u + (size_t)NV * off;
+    if (hm_check(hm_copy_device(s->dev, dst, s->u, n5(s)), "ensemble state") ||
+        hm_check(hm_synchronize(s->dev), "ensemble state"))
+      return 1;
+    hm_free(s->u);
+    s->u = dst;
+    s->u_shared = 1;
+  }
   return 0;
 }
 
 void TimeCleanup(TimeIntegration *TS) {
+  if (TS->ens) hm_ensemble_destroy(TS->ens);
+  hm_free(TS->ens_u);
   for (int i = 0; i < 4 * TS->nsims; i++) {
     if (TS->Ustore) hm_free(TS->Ustore[i]);
     if (TS->Kstore) hm_free(TS->Kstore[i]);
@@ -431,9 +478,15 @@ int TimeRK(void *ts) {
   return 0;
 }
 
-/* one fused 3N RK4 step per member (same operation order as TimeRK) */
+/* one fused 3N RK4 step per member (same operation order as TimeRK); for
+ * ensembles one batched step of all members */
 int TimeRKFused(void *ts) {
   TimeIntegration *TS = (TimeIntegration *)ts;
+  if (TS->ens) {
+    if (hm_check(hm_ensemble_rk4(TS->ens, TS->ens_u, 1), "hm_ensemble_rk4")) return 1;
+    for (int n = 0; n < TS->nsims; n++) TS->sims[n].count_sou += 4;
+    return 0;
+  }
   for (int n = 0; n < TS->nsims; n++) {
     if (hm_check(hm_rk4(TS->sims[n].dev, TS->sims[n].u, TS->dt, 1), "hm_rk4")) return 1;
     TS->sims[n].count_sou += 4;  /* the (zero) source term of each of the four stages */

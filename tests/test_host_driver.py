@@ -88,14 +88,18 @@ def test_driver_matches_reference(tmp_path, name, args):
 
 
 @pytest.mark.gpu
-def test_driver_ensemble_matches_reference(tmp_path):
+@pytest.mark.parametrize("args", [[], ["--slots"]])
+def test_driver_ensemble_matches_reference(tmp_path, args):
     meta, z = load_fixture("ens3_s4")
     members = [dataclasses.replace(decks.config0(s[0], meta["steps"]), dims=tuple(s), seed=sd, dt=None)
                for s, sd in zip(meta["sizes"], meta["seeds"])]
     d = str(tmp_path / "ens")
     decks.write_ensemble_deck(d, members, meta["steps"], meta["dt"])
-    r = run_driver(d)
+    print(f"[gpu] driver ensemble {args}", flush=True)
+    r = run_driver(d, *args)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    # fused: one launch per pass for all members; --slots: member by member
+    assert ("batched over all members" in r.stdout) == (not args)
     for i in range(len(members)):
         _, _, u = decks.read_op_bin(os.path.join(d, f"op_{i}.bin"))
         assert decks.rel_l2(u, z[f"uN_{i}"]) <= 1e-12
