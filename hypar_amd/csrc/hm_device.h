@@ -261,19 +261,23 @@ __device__ __forceinline__ void visc_load_dir(double F[4][4], const double *__re
   const size_t base = idx - (size_t)pos * stride;
   const size_t nplane = ncell / n;
   const int offs[4] = {-2, -1, 1, 2};
+  // each point's source (interior or ghost layer) is a pointer and a
+  // component stride, so that all 16 loads issue without branches
+  const double *src[4];
+  size_t cst[4];
 #pragma unroll
   for (int s = 0; s < 4; s++) {
     const int gp = pos + offs[s];
-    if (gp >= 0 && gp < n) {
-      const size_t c = base + (size_t)gp * stride;
-#pragma unroll
-      for (int v = 0; v < 4; v++) F[s][v] = FV[(size_t)(DIR * 4 + v) * ncell + c];
-    } else {
-      const int slot = gp < 0 ? gp + 2 : gp - n + 2;
-#pragma unroll
-      for (int v = 0; v < 4; v++) F[s][v] = FVg[(size_t)(slot * 4 + v) * nplane + line];
-    }
+    const bool in = gp >= 0 && gp < n;
+    const int slot = gp < 0 ? gp + 2 : gp - n + 2;
+    src[s] = in ? FV + (size_t)(DIR * 4) * ncell + base + (size_t)gp * stride
+                : FVg + (size_t)(slot * 4) * nplane + line;
+    cst[s] = in ? ncell : nplane;
   }
+#pragma unroll
+  for (int s = 0; s < 4; s++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) F[s][v] = src[s][(size_t)v * cst[s]];
 }
 
 // d/dx_DIR of the viscous flux at cell idx, not yet scaled by dxinv:

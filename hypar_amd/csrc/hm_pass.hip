@@ -386,27 +386,64 @@ __device__ __forceinline__ void iface(const LD &ld, const PassConst &k, double f
   }
 }
 
-// Finish cell idx of the pass: flux difference, viscous divergence of the
-// pass direction, and the pass's output.
+// Inputs of cell idx's output that come from global memory: its dxinv, the
+// viscous divergence of the pass direction, the rhs so far (y, z) and the
+// RK4 operands (z).  The marching passes issue these loads in one block with
+// the next ring row's, so that their latencies overlap.
+struct EmitIn {
+  double dxi, par[NV], rin[NV], a[NV], b[NV];
+};
+
 template <int DIR>
-__device__ __forceinline__ void emit(const PassConst &k, const PassIO &io, size_t idx, int pos, size_t line,
-                                     const double (&fm)[NV], const double (&fp)[NV]) {
+__device__ __forceinline__ EmitIn emit_load(const PassConst &k, const PassIO &io, size_t idx, int pos,
+                                            size_t line) {
   const int n = k.n[DIR];
-  const double dxi = k.dxg[(DIR >= 1 ? k.n[0] + 6 : 0) + (DIR == 2 ? k.n[1] + 6 : 0) + 3 + pos];
-  double par[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  EmitIn in;
+  in.dxi = k.dxg[(DIR >= 1 ? k.n[0] + 6 : 0) + (DIR == 2 ? k.n[1] + 6 : 0) + 3 + pos];
+#pragma unroll
+  for (int v = 0; v < NV; v++) in.par[v] = in.rin[v] = in.a[v] = in.b[v] = 0.0;
+  // the rhs so far and the RK4 operands of the 3N update (u at stage 0, acc
+  // and u at stages 1-2, acc at stage 3), issued ahead of the viscous
+  // stencil, whose arithmetic waits for its loads
+  if (DIR > 0)
+#pragma unroll
+    for (int v = 0; v < NV; v++) in.rin[v] = io.R[(size_t)v * k.ncell + idx];
+  if (DIR == 2 && io.mode != OUT_RHS) {
+    // one uniform branch per stage kind, each with its loads in one block
+    if (io.stage == 0) {
+#pragma unroll
+      for (int v = 0; v < NV; v++) in.a[v] = io.u[(size_t)v * k.ncell + idx];
+    } else if (io.stage == 3) {
+#pragma unroll
+      for (int v = 0; v < NV; v++) in.a[v] = io.acc[(size_t)v * k.ncell + idx];
+    } else {
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        in.a[v] = io.acc[(size_t)v * k.ncell + idx];
+        in.b[v] = io.u[(size_t)v * k.ncell + idx];
+      }
+    }
+  }
   if (io.visc) {
     const size_t stride = DIR == 0 ? 1 : (DIR == 1 ? (size_t)k.n[0] : (size_t)k.n[0] * k.n[1]);
     double dv[4];
     visc_div_dir<DIR>(dv, io.FV, io.FVg, k.ncell, idx, pos, n, stride, line);
 #pragma unroll
-    for (int c = 0; c < 4; c++) par[c + 1] = dxi * dv[c];
+    for (int c = 0; c < 4; c++) in.par[c + 1] = in.dxi * dv[c];
   }
+  return in;
+}
+
+// Finish cell idx of the pass: flux difference, viscous divergence of the
+// pass direction, and the pass's output.
+template <int DIR>
+__device__ __forceinline__ void emit_store(const PassConst &k, const PassIO &io, size_t idx, const EmitIn &in,
+                                           const double (&fm)[NV], const double (&fp)[NV]) {
   double r[NV];
 #pragma unroll
   for (int v = 0; v < NV; v++) {
-    const double h = dxi * (fp[v] - fm[v]);
-    const size_t o = (size_t)v * k.ncell + idx;
-    r[v] = DIR == 0 ? (-h) + par[v] : (io.R[o] - h) + par[v];
+    const double h = in.dxi * (fp[v] - fm[v]);
+    r[v] = DIR == 0 ? (-h) + in.par[v] : (in.rin[v] - h) + in.par[v];
   }
   if (DIR < 2 || io.mode == OUT_RHS) {
     double *dst = DIR < 2 ? io.R : io.out;
@@ -420,16 +457,15 @@ __device__ __forceinline__ void emit(const PassConst &k, const PassIO &io, size_
   for (int v = 0; v < NV; v++) {
     const size_t o = (size_t)v * k.ncell + idx;
     if (io.stage == 0) {
-      const double u0 = io.u[o];
-      io.acc[o] = u0 + io.cb * r[v];
-      nu[v] = u0 + io.ca * r[v];
+      io.acc[o] = in.a[v] + io.cb * r[v];
+      nu[v] = in.a[v] + io.ca * r[v];
       io.S[o] = nu[v];
     } else if (io.stage < 3) {
-      io.acc[o] = io.acc[o] + io.cb * r[v];
-      nu[v] = io.u[o] + io.ca * r[v];
+      io.acc[o] = in.a[v] + io.cb * r[v];
+      nu[v] = in.b[v] + io.ca * r[v];
       io.S[o] = nu[v];
     } else {
-      nu[v] = io.acc[o] + io.cb * r[v];
+      nu[v] = in.a[v] + io.cb * r[v];
       io.u[o] = nu[v];
     }
   }
@@ -442,6 +478,12 @@ __device__ __forceinline__ void emit(const PassConst &k, const PassIO &io, size_
     io.Q[2 * k.ncell + idx] = q.vz;
     io.Q[3 * k.ncell + idx] = k.gamma * q.P / q.rho;
   }
+}
+
+template <int DIR>
+__device__ __forceinline__ void emit(const PassConst &k, const PassIO &io, size_t idx, int pos, size_t line,
+                                     const double (&fm)[NV], const double (&fp)[NV]) {
+  emit_store<DIR>(k, io, idx, emit_load<DIR>(k, io, idx, pos, line), fm, fp);
 }
 
 // ---------------------------------------------------------------------------
@@ -556,7 +598,7 @@ __device__ __forceinline__ void pass_m_body(const PassConst &k, const PassIO &io
   const size_t stride = DIR == 1 ? (size_t)nx : (size_t)nx * ny;
   const size_t cbase = DIR == 1 ? o * nx * ny + (line - o * nx) : line;  // cell at position 0
 
-  auto load_row = [&](int r) {  // r in [-3, n+MR+1]
+  auto load_row = [&](int r) {  // r in [-3, MR+1]
     const size_t cell = cbase + (size_t)wrap(r, n) * stride;
     double u[NV], cd[NQ];
 #pragma unroll
@@ -567,10 +609,24 @@ __device__ __forceinline__ void pass_m_body(const PassConst &k, const PassIO &io
     for (int q = 0; q < NQ; q++) sh.sC[q][slot][lane] = cd[q];
   };
 
-  // rows -3..1
-  for (int r = w; r < 5; r += MR) load_row(r - 3);
+  // ring row r's state, loaded one march step ahead in the same block as
+  // the output's operands, so that their latencies overlap
+  auto fetch_row = [&](int r, double (&u)[NV]) {  // r in [-3, n+MR]
+    const size_t cell = cbase + (size_t)wrap(r, n) * stride;
+#pragma unroll
+    for (int v = 0; v < NV; v++) u[v] = io.U[(size_t)v * k.ncell + cell];
+  };
+  auto stage_row = [&](int r, const double (&u)[NV]) {
+    double cd[NQ];
+    cell_data<DIR>(u, k, cd);
+    const int slot = (r + NRING) % NRING;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) sh.sC[q][slot][lane] = cd[q];
+  };
+
+  // rows -3..MR+1
+  for (int r = w; r < MR + 5; r += MR) load_row(r - 3);
   for (int t0 = 0; t0 < n; t0 += MR) {
-    load_row(t0 + 2 + w);
     __syncthreads();
     const int t = t0 + w;
     if (t < n) {
@@ -589,9 +645,21 @@ __device__ __forceinline__ void pass_m_body(const PassConst &k, const PassIO &io
 #pragma unroll
         for (int v = 0; v < NV; v++) sh.sF0[v][lane] = f[v];
     }
-    __syncthreads();
     const int e = t0 - 1 + w;  // row emitted by this wavefront
-    if (col && e >= 0 && e <= n - 2) {
+    const bool emits = col && e >= 0 && e <= n - 2;
+    __syncthreads();
+    // the next step's ring row (its slot held row t0-3+w, which no later
+    // step reads) and the output's operands, loaded by every lane (a stand-in
+    // row 0 after the last step, output rows clamped into the line) so that
+    // no divergent branch separates the loads
+    const bool more = t0 + MR < n;
+    const int rn = more ? t0 + MR + 2 + w : 0;  // at most n+MR
+    double un[NV];
+    fetch_row(rn, un);
+    const int ec = e < 0 ? 0 : (e > n - 2 ? n - 2 : e);
+    const EmitIn in = emit_load<DIR>(k, io, cbase + (size_t)ec * stride, ec, line);
+    if (more) stage_row(rn, un);
+    if (emits) {
       double fm[NV], fp[NV];
       const int a = e % NFR, b = (e + 1) % NFR;
 #pragma unroll
@@ -599,7 +667,7 @@ __device__ __forceinline__ void pass_m_body(const PassConst &k, const PassIO &io
         fm[v] = sh.sF[v][a][lane];
         fp[v] = sh.sF[v][b][lane];
       }
-      emit<DIR>(k, io, cbase + (size_t)e * stride, e, line, fm, fp);
+      emit_store<DIR>(k, io, cbase + (size_t)e * stride, in, fm, fp);
     }
   }
   // last row: fI[n-1] and fI[n] = fI[0]
