@@ -15,6 +15,14 @@
 #include "../../include/hypar_mi355x.h"
 #include "hm_internal.h"
 
+// Segments of the adjoint tape whose RK stage states are in HBM at once
+// (4K states each, K = checkpoint_every): the forward sweep keeps the last
+// TAPE_SETS segments, and the reverse sweep recomputes every earlier segment
+// from its checkpoint into the set that segment's later neighbour freed.  A
+// third set costs 4K more states (128^3, K = 10: 3.4 GB) and saves K
+// recomputed steps.
+constexpr int TAPE_SETS = 3;
+
 struct hm_solver {
   hm_solver() : g(), sc(), device(0), stream(nullptr), dxg(nullptr), fI{nullptr, nullptr, nullptr},
                 FV(nullptr), FVg{nullptr, nullptr, nullptr}, Q(nullptr), S{nullptr, nullptr},
@@ -44,7 +52,7 @@ struct hm_solver {
   // reverse sweep: a second stream recomputes the next segment while the
   // solver's stream runs the adjoint of the current one
   hipStream_t aux = nullptr;
-  hipEvent_t ev_ready[2] = {nullptr, nullptr}, ev_done[2] = {nullptr, nullptr}, ev_start = nullptr;
+  hipEvent_t ev_ready[TAPE_SETS] = {}, ev_done[TAPE_SETS] = {}, ev_start = nullptr;
 };
 
 namespace {
@@ -179,7 +187,10 @@ void hm_destroy(hm_solver *s) {
   if (s->part_host) (void)hipHostFree(s->part_host);
   for (auto *v : {&s->ckpt, &s->stages, &s->vscratch, &s->vwork})
     for (double *b : *v) (void)hipFree(b);
-  for (hipEvent_t e : {s->ev_ready[0], s->ev_ready[1], s->ev_done[0], s->ev_done[1], s->ev_start})
+  for (int i = 0; i < TAPE_SETS; i++)
+    for (hipEvent_t e : {s->ev_ready[i], s->ev_done[i]})
+      if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : {s->ev_start})
     if (e) (void)hipEventDestroy(e);
   if (s->aux) (void)hipStreamDestroy(s->aux);
   delete s;
@@ -615,9 +626,12 @@ hm_status hm_rk4_vjp(hm_solver *s, const double *u_n, double dt, double *lam) {
   return HM_OK;
 }
 
-// Two sets of segment stage states: segment seg uses set seg % 2; step m of
-// a segment keeps u_n, U1, U2, U3 at stages[set*4K + 4m .. +3].
-static double **segment_stages(hm_solver *s, int seg, int K) { return &s->stages[(size_t)4 * K * (seg % 2)]; }
+// TAPE_SETS sets of segment stage states: segment seg uses set
+// seg % TAPE_SETS; step m of a segment keeps u_n, U1, U2, U3 at
+// stages[set*4K + 4m .. +3].
+static double **segment_stages(hm_solver *s, int seg, int K) {
+  return &s->stages[(size_t)4 * K * (seg % TAPE_SETS)];
+}
 
 hm_status hm_adjoint_forward(hm_solver *s, double *u, double dt, int nsteps, int every) {
   hm_status st;
@@ -627,10 +641,10 @@ hm_status hm_adjoint_forward(hm_solver *s, double *u, double dt, int nsteps, int
   const size_t n5 = 5 * s->g.ncell;
   const int nck = (nsteps + every - 1) / every;
   if ((st = ensure_buffers(s->ckpt, (size_t)nck, n5)) != HM_OK) return st;
-  if ((st = ensure_buffers(s->stages, (size_t)8 * every, n5)) != HM_OK) return st;
-  // the last two segments keep their stage states in the two stage sets:
+  if ((st = ensure_buffers(s->stages, (size_t)4 * TAPE_SETS * every, n5)) != HM_OK) return st;
+  // the last TAPE_SETS segments keep their stage states, one in each set:
   // the reverse sweep starts on them without recomputation
-  const int nkeep = (nck >= 2 ? nck - 2 : 0) * every;
+  const int nkeep = (nck >= TAPE_SETS ? nck - TAPE_SETS : 0) * every;
   for (int n = 0; n < nsteps; n++) {
     if (n % every == 0)
       HM_HIP(hipMemcpyAsync(s->ckpt[n / every], u, n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
@@ -678,26 +692,30 @@ hm_status hm_adjoint_reverse(hm_solver *s, double *lam) {
   const int N = s->tape_nsteps, K = s->tape_every;
   const double dt = s->tape_dt;
   const size_t n5 = 5 * s->g.ncell;
-  if ((st = ensure_buffers(s->stages, (size_t)8 * K, n5)) != HM_OK) return st;
+  if ((st = ensure_buffers(s->stages, (size_t)4 * TAPE_SETS * K, n5)) != HM_OK) return st;
   if ((st = ensure_buffers(s->vscratch, 4, n5)) != HM_OK) return st;
   if (!s->aux) HM_HIP(hipStreamCreate(&s->aux));  // default priority
-  for (hipEvent_t *e : {&s->ev_ready[0], &s->ev_ready[1], &s->ev_done[0], &s->ev_done[1], &s->ev_start})
-    if (!*e) HM_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+  for (int i = 0; i < TAPE_SETS; i++)
+    for (hipEvent_t *e : {&s->ev_ready[i], &s->ev_done[i]})
+      if (!*e) HM_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+  if (!s->ev_start) HM_HIP(hipEventCreateWithFlags(&s->ev_start, hipEventDisableTiming));
   const int nck = (N + K - 1) / K;
   hipStream_t main = s->stream;
   double *cur = lam;
   // the recomputation stream starts after everything queued so far
   HM_HIP(hipEventRecord(s->ev_start, main));
   HM_HIP(hipStreamWaitEvent(s->aux, s->ev_start, 0));
-  // segments nck-2 and nck-1 kept their stage states in the forward sweep
-  const int kept = nck >= 2 ? nck - 2 : 0;
+  // segments nck-TAPE_SETS .. nck-1 kept their stage states in the forward
+  // sweep; segments below kept are recomputed
+  const int kept = nck >= TAPE_SETS ? nck - TAPE_SETS : 0;
   for (int seg = nck - 1; seg >= 0; seg--) {
     const int n0 = seg * K, n1 = (n0 + K < N) ? n0 + K : N;
     if (seg > 0 && seg - 1 < kept) {
-      // recompute the previous segment into the other set on the second
-      // stream, once the adjoint that last read that set (segment seg+1) is done
-      const int o = (seg - 1) % 2;
-      if (seg + 1 < nck) HM_HIP(hipStreamWaitEvent(s->aux, s->ev_done[o], 0));
+      // recompute the previous segment into its set on the second stream,
+      // once the adjoint that last read that set (segment seg-1+TAPE_SETS)
+      // is done
+      const int o = (seg - 1) % TAPE_SETS;
+      if (seg - 1 + TAPE_SETS < nck) HM_HIP(hipStreamWaitEvent(s->aux, s->ev_done[o], 0));
       s->stream = s->aux;
       st = recompute_segment(s, seg - 1, n0 - K, n0, dt);
       s->stream = main;
@@ -706,7 +724,7 @@ hm_status hm_adjoint_reverse(hm_solver *s, double *lam) {
     }
     // kept segments come from the forward sweep; the others from the
     // recomputation queued in the previous iteration
-    if (seg < kept) HM_HIP(hipStreamWaitEvent(main, s->ev_ready[seg % 2], 0));
+    if (seg < kept) HM_HIP(hipStreamWaitEvent(main, s->ev_ready[seg % TAPE_SETS], 0));
     double **S = segment_stages(s, seg, K);
     for (int n = n1 - 1; n >= n0; n--) {
       // the step adjoint alternates between the caller's array and scratch
@@ -714,7 +732,7 @@ hm_status hm_adjoint_reverse(hm_solver *s, double *lam) {
       if ((st = rk4_step_vjp(s, S + (size_t)4 * (n - n0), dt, cur, out)) != HM_OK) return st;
       cur = out;
     }
-    HM_HIP(hipEventRecord(s->ev_done[seg % 2], main));
+    HM_HIP(hipEventRecord(s->ev_done[seg % TAPE_SETS], main));
   }
   if (cur != lam) HM_HIP(hipMemcpyAsync(lam, cur, n5 * sizeof(double), hipMemcpyDeviceToDevice, main));
   return HM_OK;
@@ -1205,9 +1223,10 @@ hm_status hm_ensemble_adjoint_forward(hm_ensemble *e, double *u, int nsteps, int
   const size_t n5 = 5 * e->ncell;
   const int nck = (nsteps + every - 1) / every;
   if ((st = ens_buffers(e->ckpt, (size_t)nck, n5)) != HM_OK) return st;
-  if ((st = ens_buffers(e->stages, (size_t)8 * every, n5)) != HM_OK) return st;
-  // the last two segments keep their stage states (segment seg in set seg % 2)
-  const int nkeep = (nck >= 2 ? nck - 2 : 0) * every;
+  if ((st = ens_buffers(e->stages, (size_t)4 * TAPE_SETS * every, n5)) != HM_OK) return st;
+  // the last TAPE_SETS segments keep their stage states (segment seg in set
+  // seg % TAPE_SETS)
+  const int nkeep = (nck >= TAPE_SETS ? nck - TAPE_SETS : 0) * every;
   for (int n = 0; n < nsteps; n++) {
     if (n % every == 0)
       HM_HIP(hipMemcpyAsync(e->ckpt[n / every], u, n5 * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
@@ -1215,7 +1234,7 @@ hm_status hm_ensemble_adjoint_forward(hm_ensemble *e, double *u, int nsteps, int
       if ((st = hm_ensemble_rk4(e, u, 1)) != HM_OK) return st;
     } else {
       const int seg = n / every;
-      double **U = &e->stages[(size_t)4 * every * (seg % 2) + (size_t)4 * (n - seg * every)];
+      double **U = &e->stages[(size_t)4 * every * (seg % TAPE_SETS) + (size_t)4 * (n - seg * every)];
       HM_HIP(hipMemcpyAsync(U[0], u, n5 * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
       double *const Us[3] = {U[1], U[2], U[3]};
       if ((st = ens_step_store(e, u, Us)) != HM_OK) return st;
@@ -1251,11 +1270,11 @@ hm_status hm_ensemble_adjoint_reverse(hm_ensemble *e, double *lam) {
   plan.tiles0 = e->tiles0;
   plan.ntiles = e->ntiles;
   const int nck = (N + K - 1) / K;
-  const int kept = nck >= 2 ? nck - 2 : 0;  // segments the forward sweep kept
+  const int kept = nck >= TAPE_SETS ? nck - TAPE_SETS : 0;  // segments below kept are recomputed
   double *cur = lam;
   for (int seg = nck - 1; seg >= 0; seg--) {
     const int n0 = seg * K, n1 = (n0 + K < N) ? n0 + K : N;
-    double **S = &e->stages[(size_t)4 * K * (seg % 2)];
+    double **S = &e->stages[(size_t)4 * K * (seg % TAPE_SETS)];
     if (seg < kept) {
       // recompute the segment from its checkpoint, keeping the stage states
       for (int n = n0; n < n1; n++) {

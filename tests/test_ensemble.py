@@ -162,8 +162,9 @@ def test_viscous_reference_fixtures():
                                                    ("components", "roe", -1.0, "js"),
                                                    ("characteristic", "roe", 1000.0, "yc")])
 def test_ensemble_adjoint_matches_single_grid(interp, upwind, Re, weno):
-    """Batched trajectory gradient (5 steps, checkpoint every 2, each member
-    with its own dt) against each member's single-grid adjoint, which
+    """Batched trajectory gradient (5 steps, checkpoint every step: the last
+    three segments kept by the forward sweep, the first two recomputed; each
+    member with its own dt) against each member's single-grid adjoint, which
     test_adjoint.py checks against torch autograd.  The last member spans two
     x tiles and two z segments of the viscous kernels."""
     from hypar_amd import decks
@@ -176,13 +177,13 @@ def test_ensemble_adjoint_matches_single_grid(interp, upwind, Re, weno):
     e = Ensemble(dims, interp, upwind, Re=Re, Minf=0.1 if Re > 0 else 1.0, weno_type=weno)
     e.set_dt(dts)
     u0 = e.to_device(u0s).requires_grad_(True)
-    uN = ensemble_trajectory(u0, e, 5, 2)
+    uN = ensemble_trajectory(u0, e, 5, 1)
     J = sum(kinetic_energy(e.member(uN, m), dims[m]) for m in range(len(dims)))
     (g,) = torch.autograd.grad(J, u0)
     for m, w in enumerate(ws):
         s = NS3D.from_workload(w)
         x = s.to_device(u0s[m]).requires_grad_(True)
-        y = trajectory(x, s, dts[m], 5, 2)
+        y = trajectory(x, s, dts[m], 5, 1)
         assert decks.rel_l2(e.member(uN, m).detach().cpu().numpy(), y.detach().cpu().numpy()) <= 1e-14
         (gs,) = torch.autograd.grad(kinetic_energy(y, w.dims), x)
         err = float((e.member(g, m) - gs).norm() / gs.norm())
