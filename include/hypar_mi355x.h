@@ -183,11 +183,14 @@ hm_status hm_rhs_vjp(hm_solver *s, double *out, const double *u, const double *l
 /* lam <- (d u_{n+1} / d u_n)^T lam for one RK4 step of size dt from u_n */
 hm_status hm_rk4_vjp(hm_solver *s, const double *u_n, double dt, double *lam);
 /* Forward sweep of nsteps RK4 steps in place on u, keeping a checkpoint of
- * the state every `checkpoint_every` steps in HBM (the solver owns them). */
+ * the state every `checkpoint_every` steps in HBM (the solver owns them),
+ * and the RK stage states of the last three checkpoint segments
+ * (12 * checkpoint_every states). */
 hm_status hm_adjoint_forward(hm_solver *s, double *u, double dt, int nsteps, int checkpoint_every);
 /* Reverse sweep of the last hm_adjoint_forward: lam holds dJ/du_N on entry
- * and dJ/du_0 on return.  Each checkpoint segment is recomputed once with
- * its RK stage states kept in HBM, then stepped back. */
+ * and dJ/du_0 on return.  Every segment before the last three is recomputed
+ * once from its checkpoint with its RK stage states kept in HBM, then
+ * stepped back. */
 hm_status hm_adjoint_reverse(hm_solver *s, double *lam);
 
 /* Kernel timing for roofline reports (no counterpart in the reference; the
@@ -242,8 +245,10 @@ hm_status hm_ensemble_tune(hm_ensemble *e, const double *u, int reps, int *shape
 /* Batched discrete adjoint (no counterpart in the reference): the adjoint
  * of hm_adjoint_forward / hm_adjoint_reverse for all members at once, each
  * with its own dt.  The forward sweep advances the batched state nsteps RK4
- * steps in place with a checkpoint every `checkpoint_every` steps; the
- * reverse sweep takes lam = dJ/du_N of all members and returns dJ/du_0.
+ * steps in place with a checkpoint every `checkpoint_every` steps and the
+ * RK stage states of the last three segments; the reverse sweep recomputes
+ * the earlier segments, takes lam = dJ/du_N of all members and returns
+ * dJ/du_0.
  * Every VJP launch of a stage covers the (member, line) work items of all
  * members.  Built for the schemes hm_rhs_vjp supports. */
 hm_status hm_ensemble_adjoint_forward(hm_ensemble *e, double *u, int nsteps, int checkpoint_every);
