@@ -403,6 +403,37 @@ __device__ __forceinline__ double harten_d(double l) {
 // contributions of the flux and solution adjoints [6][n] each.
 enum { VJP_WRITE = 0, VJP_ADD = 1, VJP_ADD_L = 2, VJP_ADD_LSET = 3 };
 
+// the finished adjoint acc of cell c into nu (and L) by mode; every load
+// issues before the first store, so that the stores (nu, L and lam may
+// alias as far as the compiler knows) do not serialise the loads
+__device__ __forceinline__ void vjp_store(double *__restrict__ nu, double *__restrict__ L,
+                                          const double *__restrict__ lam, int mode, size_t ncell, size_t c,
+                                          const double (&acc)[NV]) {
+  if (mode == VJP_WRITE) {
+#pragma unroll
+    for (int v = 0; v < NV; v++) nu[(size_t)v * ncell + c] = acc[v];
+    return;
+  }
+  double r[NV], l[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int v = 0; v < NV; v++) r[v] = nu[(size_t)v * ncell + c];
+  if (mode == VJP_ADD_L) {
+#pragma unroll
+    for (int v = 0; v < NV; v++) l[v] = L[(size_t)v * ncell + c];
+  } else if (mode == VJP_ADD_LSET) {
+#pragma unroll
+    for (int v = 0; v < NV; v++) l[v] = lam[(size_t)v * ncell + c];
+  }
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    r[v] = r[v] + acc[v];
+    nu[(size_t)v * ncell + c] = r[v];
+  }
+  if (mode == VJP_ADD_L || mode == VJP_ADD_LSET)
+#pragma unroll
+    for (int v = 0; v < NV; v++) L[(size_t)v * ncell + c] = l[v] + r[v];
+}
+
 template <int DIR, int UPW, int W>
 __device__ __forceinline__ void rhs_vjp_line(const double *__restrict__ u, const double *__restrict__ lam,
                                              const double *__restrict__ nu_in, double ca, double cb,
@@ -624,18 +655,7 @@ __device__ __forceinline__ void rhs_vjp_line(const double *__restrict__ u, const
     }
     flux_jac_T<DIR>(uc, gamma, accF, accU);
     const size_t c = base + (size_t)t * stride;
-#pragma unroll
-    for (int v = 0; v < NV; v++) {
-      const size_t o = (size_t)v * ncell + c;
-      if (mode == VJP_WRITE) {
-        nu[o] = accU[v];
-      } else {
-        const double r = nu[o] + accU[v];
-        nu[o] = r;
-        if (mode == VJP_ADD_L) L[o] += r;
-        else if (mode == VJP_ADD_LSET) L[o] = lam[o] + r;
-      }
-    }
+    vjp_store(nu, L, lam, mode, ncell, c, accU);
   }
 }
 
@@ -903,18 +923,7 @@ __device__ __forceinline__ void rhs_vjp_char_line(
 #pragma unroll
     for (int v = 0; v < NV; v++) acc[v] += E[tp * ES + v] + E[t * ES + NV + v];
     const size_t c = base + (size_t)t * stride;
-#pragma unroll
-    for (int v = 0; v < NV; v++) {
-      const size_t o = (size_t)v * ncell + c;
-      if (mode == VJP_WRITE) {
-        nu[o] = acc[v];
-      } else {
-        const double r = nu[o] + acc[v];
-        nu[o] = r;
-        if (mode == VJP_ADD_L) L[o] += r;
-        else if (mode == VJP_ADD_LSET) L[o] = lam[o] + r;
-      }
-    }
+    vjp_store(nu, L, lam, mode, ncell, c, acc);
   }
 }
 
