@@ -403,6 +403,23 @@ __device__ __forceinline__ double harten_d(double l) {
 // contributions of the flux and solution adjoints [6][n] each.
 enum { VJP_WRITE = 0, VJP_ADD = 1, VJP_ADD_L = 2, VJP_ADD_LSET = 3 };
 
+// state, lam and (when given) nu_in of cell c, every load issued before any
+// is used: a null check inside the variable loop would put a wait in each
+// iteration
+__device__ __forceinline__ void vjp_load_cell(const double *__restrict__ u, const double *__restrict__ lam,
+                                              const double *__restrict__ nu_in, size_t ncell, size_t c,
+                                              double (&us)[NV], double (&lm)[NV], double (&ni)[NV]) {
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    us[v] = u[(size_t)v * ncell + c];
+    lm[v] = lam[(size_t)v * ncell + c];
+    ni[v] = 0.0;
+  }
+  if (nu_in)
+#pragma unroll
+    for (int v = 0; v < NV; v++) ni[v] = nu_in[(size_t)v * ncell + c];
+}
+
 // the finished adjoint acc of cell c into nu (and L) by mode; every load
 // issues before the first store, so that the stores (nu, L and lam may
 // alias as far as the compiler knows) do not serialise the loads
@@ -459,14 +476,13 @@ __device__ __forceinline__ void rhs_vjp_line(const double *__restrict__ u, const
   double g[NV] = {0, 0, 0, 0, 0};
   if (active) {
     const size_t c = base + (size_t)t * stride;
-    double us[NV];
+    double us[NV], lm[NV], ni[NV];
+    vjp_load_cell(u, lam, nu_in, ncell, c, us, lm, ni);
 #pragma unroll
     for (int v = 0; v < NV; v++) {
-      us[v] = u[(size_t)v * ncell + c];
       sU[v * n + t] = us[v];
-      const size_t o = (size_t)v * ncell;
       // dxinv mu of this cell, staged in the (not yet used) slot buffer
-      cF[v * n + t] = dxd[t] * (cb * lam[o + c] + (nu_in ? ca * nu_in[o + c] : 0.0));
+      cF[v * n + t] = dxd[t] * (cb * lm[v] + (nu_in ? ca * ni[v] : 0.0));
     }
     const Prim q = flowvar(us, gamma);
     double fl[NV];
@@ -714,14 +730,13 @@ __device__ __forceinline__ void rhs_vjp_char_line(
   double lF[NV] = {0, 0, 0, 0, 0};  // adjoint of the interface flux
   if (active) {
     const size_t c = base + (size_t)t * stride;
-    double us[NV];
+    double us[NV], lm[NV], ni[NV];
+    vjp_load_cell(u, lam, nu_in, ncell, c, us, lm, ni);
 #pragma unroll
     for (int v = 0; v < NV; v++) {
-      us[v] = u[(size_t)v * ncell + c];
       sC[t * CS + v] = us[v];
-      const size_t o = (size_t)v * ncell;
       // dxinv mu of this cell, staged in the (not yet used) exchange buffer
-      E[t * ES + v] = dxd[t] * (cb * lam[o + c] + (nu_in ? ca * nu_in[o + c] : 0.0));
+      E[t * ES + v] = dxd[t] * (cb * lm[v] + (nu_in ? ca * ni[v] : 0.0));
     }
     const Prim q = flowvar(us, gamma);
     double fl[NV];
