@@ -1015,11 +1015,16 @@ __device__ __forceinline__ double visc_flux_vjp(const double q[4], const double 
 // mu_c at a cell: momentum and energy components of cb*lam + ca*nu_in
 __device__ __forceinline__ void visc_mu(const double *__restrict__ lam, const double *__restrict__ nu_in, double ca,
                                         double cb, size_t ncell, size_t c, double m[4]) {
+  // all loads before any use: a null check of nu_in inside the loop would
+  // put a wait in each iteration
+  double l[4], n[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-  for (int k = 0; k < 4; k++) {
-    const size_t o = (size_t)(1 + k) * ncell + c;
-    m[k] = cb * lam[o] + (nu_in ? ca * nu_in[o] : 0.0);
-  }
+  for (int k = 0; k < 4; k++) l[k] = lam[(size_t)(1 + k) * ncell + c];
+  if (nu_in)
+#pragma unroll
+    for (int k = 0; k < 4; k++) n[k] = nu_in[(size_t)(1 + k) * ncell + c];
+#pragma unroll
+  for (int k = 0; k < 4; k++) m[k] = cb * l[k] + (nu_in ? ca * n[k] : 0.0);
 }
 
 __device__ __forceinline__ void visc_vjp_tile(const double *__restrict__ Q, const double *__restrict__ lam,
