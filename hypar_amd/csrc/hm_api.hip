@@ -27,7 +27,9 @@ struct hm_solver {
   hm_solver() : g(), sc(), device(0), stream(nullptr), dxg(nullptr), fI{nullptr, nullptr, nullptr},
                 FV(nullptr), FVg{nullptr, nullptr, nullptr}, Q(nullptr), S{nullptr, nullptr},
                 acc(nullptr), part(nullptr), part_host(nullptr), npart(0), tmp(nullptr), tmp_elems(0),
-                tape_nsteps(0), tape_every(0), tape_dt(0.0) {}
+                tape_nsteps(0), tape_every(0), tape_dt(0.0) {
+    for (int i = 0; i < TAPE_SETS; i++) tape_holds[i] = -1;
+  }
   hm::Grid g;
   hm::Scheme sc;
   int device;
@@ -49,6 +51,10 @@ struct hm_solver {
   std::vector<double *> vwork;  // viscous adjoint work space (one buffer)
   int tape_nsteps, tape_every;
   double tape_dt;
+  // segment whose RK stage states set i holds (-1: none).  A reverse sweep
+  // recomputes every segment its set does not hold, so a later sweep over
+  // the same forward (or one after hm_rk4_vjp reused set 0) stays exact.
+  int tape_holds[TAPE_SETS];
   // reverse sweep: a second stream recomputes the next segment while the
   // solver's stream runs the adjoint of the current one
   hipStream_t aux = nullptr;
@@ -620,6 +626,9 @@ hm_status hm_rk4_vjp(hm_solver *s, const double *u_n, double dt, double *lam) {
   HM_HIP(hipMemcpyAsync(w, u_n, n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
   double *const Us[3] = {U[1], U[2], U[3]};
   if ((st = rk4_step_store(s, w, dt, Us)) != HM_OK) return st;
+  // U[0..3] are the first step of stage set 0: a recorded trajectory no
+  // longer finds its segment there and recomputes it
+  s->tape_holds[0] = -1;
   double *L = s->vscratch[2];
   if ((st = rk4_step_vjp(s, U, dt, lam, L)) != HM_OK) return st;
   HM_HIP(hipMemcpyAsync(lam, L, n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
@@ -628,9 +637,14 @@ hm_status hm_rk4_vjp(hm_solver *s, const double *u_n, double dt, double *lam) {
 
 // TAPE_SETS sets of segment stage states: segment seg uses set
 // seg % TAPE_SETS; step m of a segment keeps u_n, U1, U2, U3 at
-// stages[set*4K + 4m .. +3].
+// stages[set*4K + 4m .. +3].  A trajectory of N steps needs
+// 4*min(N, TAPE_SETS*K) stage states.
 static double **segment_stages(hm_solver *s, int seg, int K) {
   return &s->stages[(size_t)4 * K * (seg % TAPE_SETS)];
+}
+
+static size_t tape_states(int nsteps, int every) {
+  return (size_t)4 * (size_t)std::min(nsteps, TAPE_SETS * every);
 }
 
 hm_status hm_adjoint_forward(hm_solver *s, double *u, double dt, int nsteps, int every) {
@@ -638,10 +652,13 @@ hm_status hm_adjoint_forward(hm_solver *s, double *u, double dt, int nsteps, int
   if ((st = check_vjp(s, "hm_adjoint_forward")) != HM_OK) return st;
   if ((st = check_state(s, u, "hm_adjoint_forward: u")) != HM_OK) return st;
   if (nsteps < 1 || every < 1) return fail(HM_ERR_ARG, "hm_adjoint_forward: nsteps and checkpoint_every must be >= 1");
+  // a failed or partial sweep leaves no tape behind
+  s->tape_nsteps = 0;
+  for (int i = 0; i < TAPE_SETS; i++) s->tape_holds[i] = -1;
   const size_t n5 = 5 * s->g.ncell;
   const int nck = (nsteps + every - 1) / every;
   if ((st = ensure_buffers(s->ckpt, (size_t)nck, n5)) != HM_OK) return st;
-  if ((st = ensure_buffers(s->stages, (size_t)4 * TAPE_SETS * every, n5)) != HM_OK) return st;
+  if ((st = ensure_buffers(s->stages, tape_states(nsteps, every), n5)) != HM_OK) return st;
   // the last TAPE_SETS segments keep their stage states, one in each set:
   // the reverse sweep starts on them without recomputation
   const int nkeep = (nck >= TAPE_SETS ? nck - TAPE_SETS : 0) * every;
@@ -656,6 +673,7 @@ hm_status hm_adjoint_forward(hm_solver *s, double *u, double dt, int nsteps, int
       HM_HIP(hipMemcpyAsync(U[0], u, n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
       double *const Us[3] = {U[1], U[2], U[3]};
       if ((st = rk4_step_store(s, u, dt, Us)) != HM_OK) return st;
+      s->tape_holds[seg % TAPE_SETS] = seg;
     }
   }
   s->tape_nsteps = nsteps;
@@ -684,15 +702,12 @@ static hm_status recompute_segment(hm_solver *s, int seg, int n0, int n1, double
   return HM_OK;
 }
 
-hm_status hm_adjoint_reverse(hm_solver *s, double *lam) {
+static hm_status adjoint_reverse_sweep(hm_solver *s, double *lam) {
   hm_status st;
-  if ((st = check_vjp(s, "hm_adjoint_reverse")) != HM_OK) return st;
-  if ((st = check_state(s, lam, "hm_adjoint_reverse: lam")) != HM_OK) return st;
-  if (s->tape_nsteps < 1) return fail(HM_ERR_ARG, "hm_adjoint_reverse: no forward sweep recorded");
   const int N = s->tape_nsteps, K = s->tape_every;
   const double dt = s->tape_dt;
   const size_t n5 = 5 * s->g.ncell;
-  if ((st = ensure_buffers(s->stages, (size_t)4 * TAPE_SETS * K, n5)) != HM_OK) return st;
+  if ((st = ensure_buffers(s->stages, tape_states(N, K), n5)) != HM_OK) return st;
   if ((st = ensure_buffers(s->vscratch, 4, n5)) != HM_OK) return st;
   if (!s->aux) HM_HIP(hipStreamCreate(&s->aux));  // default priority
   for (int i = 0; i < TAPE_SETS; i++)
@@ -705,26 +720,42 @@ hm_status hm_adjoint_reverse(hm_solver *s, double *lam) {
   // the recomputation stream starts after everything queued so far
   HM_HIP(hipEventRecord(s->ev_start, main));
   HM_HIP(hipStreamWaitEvent(s->aux, s->ev_start, 0));
-  // segments nck-TAPE_SETS .. nck-1 kept their stage states in the forward
-  // sweep; segments below kept are recomputed
-  const int kept = nck >= TAPE_SETS ? nck - TAPE_SETS : 0;
+  // After a forward sweep the sets hold segments nck-TAPE_SETS .. nck-1;
+  // every other segment, and any whose set was reused since (an earlier
+  // reverse sweep, hm_rk4_vjp), is recomputed from its checkpoint.  The last
+  // segment is recomputed on the solver's stream, every earlier one on the
+  // second stream while the adjoint of the segment after it runs.
+  bool on_aux[TAPE_SETS] = {};
+  {
+    const int seg = nck - 1, o = seg % TAPE_SETS;
+    if (s->tape_holds[o] != seg) {
+      s->tape_holds[o] = -1;
+      if ((st = recompute_segment(s, seg, seg * K, N, dt)) != HM_OK) return st;
+      s->tape_holds[o] = seg;
+    }
+  }
   for (int seg = nck - 1; seg >= 0; seg--) {
     const int n0 = seg * K, n1 = (n0 + K < N) ? n0 + K : N;
-    if (seg > 0 && seg - 1 < kept) {
+    if (seg > 0 && s->tape_holds[(seg - 1) % TAPE_SETS] != seg - 1) {
       // recompute the previous segment into its set on the second stream,
       // once the adjoint that last read that set (segment seg-1+TAPE_SETS)
       // is done
       const int o = (seg - 1) % TAPE_SETS;
+      s->tape_holds[o] = -1;
       if (seg - 1 + TAPE_SETS < nck) HM_HIP(hipStreamWaitEvent(s->aux, s->ev_done[o], 0));
       s->stream = s->aux;
       st = recompute_segment(s, seg - 1, n0 - K, n0, dt);
       s->stream = main;
       if (st != HM_OK) return st;
       HM_HIP(hipEventRecord(s->ev_ready[o], s->aux));
+      s->tape_holds[o] = seg - 1;
+      on_aux[o] = true;
     }
-    // kept segments come from the forward sweep; the others from the
-    // recomputation queued in the previous iteration
-    if (seg < kept) HM_HIP(hipStreamWaitEvent(main, s->ev_ready[seg % TAPE_SETS], 0));
+    const int o = seg % TAPE_SETS;
+    if (on_aux[o]) {
+      HM_HIP(hipStreamWaitEvent(main, s->ev_ready[o], 0));
+      on_aux[o] = false;
+    }
     double **S = segment_stages(s, seg, K);
     for (int n = n1 - 1; n >= n0; n--) {
       // the step adjoint alternates between the caller's array and scratch
@@ -732,10 +763,23 @@ hm_status hm_adjoint_reverse(hm_solver *s, double *lam) {
       if ((st = rk4_step_vjp(s, S + (size_t)4 * (n - n0), dt, cur, out)) != HM_OK) return st;
       cur = out;
     }
-    HM_HIP(hipEventRecord(s->ev_done[seg % TAPE_SETS], main));
+    HM_HIP(hipEventRecord(s->ev_done[o], main));
   }
   if (cur != lam) HM_HIP(hipMemcpyAsync(lam, cur, n5 * sizeof(double), hipMemcpyDeviceToDevice, main));
   return HM_OK;
+}
+
+hm_status hm_adjoint_reverse(hm_solver *s, double *lam) {
+  hm_status st;
+  if ((st = check_vjp(s, "hm_adjoint_reverse")) != HM_OK) return st;
+  if ((st = check_state(s, lam, "hm_adjoint_reverse: lam")) != HM_OK) return st;
+  if (s->tape_nsteps < 1) return fail(HM_ERR_ARG, "hm_adjoint_reverse: no forward sweep recorded");
+  hipStream_t main = s->stream;
+  st = adjoint_reverse_sweep(s, lam);
+  s->stream = main;
+  if (st != HM_OK)
+    for (int i = 0; i < TAPE_SETS; i++) s->tape_holds[i] = -1;  // contents unknown after a failure
+  return st;
 }
 
 hm_status hm_profile_kernels(hm_solver *s, const double *u, int reps, double *ms) {
@@ -822,6 +866,8 @@ struct hm_ensemble {
   int nmax[3] = {0, 0, 0};
   std::vector<double *> ckpt, stages, vscratch, vwork;
   int tape_nsteps = 0, tape_every = 0;
+  std::vector<double> tape_dt;        // the members' dt of the recorded trajectory
+  int tape_holds[TAPE_SETS] = {-1, -1, -1};  // segment held by each stage set (-1: none)
   int shape = 0;
   unsigned grid[3] = {0, 0, 0};
   bool dt_set = false;
@@ -1220,10 +1266,12 @@ hm_status hm_ensemble_adjoint_forward(hm_ensemble *e, double *u, int nsteps, int
   if (st != HM_OK) return st;
   if (nsteps < 1 || every < 1)
     return fail(HM_ERR_ARG, "hm_ensemble_adjoint_forward: nsteps and checkpoint_every must be >= 1");
+  e->tape_nsteps = 0;
+  for (int i = 0; i < TAPE_SETS; i++) e->tape_holds[i] = -1;
   const size_t n5 = 5 * e->ncell;
   const int nck = (nsteps + every - 1) / every;
   if ((st = ens_buffers(e->ckpt, (size_t)nck, n5)) != HM_OK) return st;
-  if ((st = ens_buffers(e->stages, (size_t)4 * TAPE_SETS * every, n5)) != HM_OK) return st;
+  if ((st = ens_buffers(e->stages, tape_states(nsteps, every), n5)) != HM_OK) return st;
   // the last TAPE_SETS segments keep their stage states (segment seg in set
   // seg % TAPE_SETS)
   const int nkeep = (nck >= TAPE_SETS ? nck - TAPE_SETS : 0) * every;
@@ -1238,19 +1286,21 @@ hm_status hm_ensemble_adjoint_forward(hm_ensemble *e, double *u, int nsteps, int
       HM_HIP(hipMemcpyAsync(U[0], u, n5 * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
       double *const Us[3] = {U[1], U[2], U[3]};
       if ((st = ens_step_store(e, u, Us)) != HM_OK) return st;
+      e->tape_holds[seg % TAPE_SETS] = seg;
     }
   }
+  e->tape_dt.resize(e->nmem);
+  for (int m = 0; m < e->nmem; m++) e->tape_dt[m] = e->mem[m].dt;
   e->tape_nsteps = nsteps;
   e->tape_every = every;
   return HM_OK;
 }
 
-hm_status hm_ensemble_adjoint_reverse(hm_ensemble *e, double *lam) {
-  hm_status st = ens_check_vjp(e, lam, "hm_ensemble_adjoint_reverse");
-  if (st != HM_OK) return st;
-  if (e->tape_nsteps < 1) return fail(HM_ERR_ARG, "hm_ensemble_adjoint_reverse: no forward sweep recorded");
+static hm_status ens_reverse_sweep(hm_ensemble *e, double *lam) {
+  hm_status st;
   const int N = e->tape_nsteps, K = e->tape_every;
   const size_t n5 = 5 * e->ncell;
+  if ((st = ens_buffers(e->stages, tape_states(N, K), n5)) != HM_OK) return st;
   if ((st = ens_buffers(e->vscratch, 4, n5)) != HM_OK) return st;
   double *work = nullptr;
   const size_t nw = hm::ens_vjp_work_size(e->sc, e->ncell);
@@ -1270,13 +1320,14 @@ hm_status hm_ensemble_adjoint_reverse(hm_ensemble *e, double *lam) {
   plan.tiles0 = e->tiles0;
   plan.ntiles = e->ntiles;
   const int nck = (N + K - 1) / K;
-  const int kept = nck >= TAPE_SETS ? nck - TAPE_SETS : 0;  // segments below kept are recomputed
   double *cur = lam;
   for (int seg = nck - 1; seg >= 0; seg--) {
     const int n0 = seg * K, n1 = (n0 + K < N) ? n0 + K : N;
-    double **S = &e->stages[(size_t)4 * K * (seg % TAPE_SETS)];
-    if (seg < kept) {
+    const int o = seg % TAPE_SETS;
+    double **S = &e->stages[(size_t)4 * K * o];
+    if (e->tape_holds[o] != seg) {
       // recompute the segment from its checkpoint, keeping the stage states
+      e->tape_holds[o] = -1;
       for (int n = n0; n < n1; n++) {
         double **U = S + (size_t)4 * (n - n0);
         if (n == n0) HM_HIP(hipMemcpyAsync(U[0], e->ckpt[seg], n5 * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
@@ -1285,6 +1336,7 @@ hm_status hm_ensemble_adjoint_reverse(hm_ensemble *e, double *lam) {
         double *const Us[3] = {U[1], U[2], U[3]};
         if ((st = ens_step_store(e, next, Us)) != HM_OK) return st;
       }
+      e->tape_holds[o] = seg;
     }
     for (int n = n1 - 1; n >= n0; n--) {
       double **U = S + (size_t)4 * (n - n0);
@@ -1304,6 +1356,35 @@ hm_status hm_ensemble_adjoint_reverse(hm_ensemble *e, double *lam) {
   }
   if (cur != lam) HM_HIP(hipMemcpyAsync(lam, cur, n5 * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
   return HM_OK;
+}
+
+hm_status hm_ensemble_adjoint_reverse(hm_ensemble *e, double *lam) {
+  hm_status st = ens_check_vjp(e, lam, "hm_ensemble_adjoint_reverse");
+  if (st != HM_OK) return st;
+  if (e->tape_nsteps < 1) return fail(HM_ERR_ARG, "hm_ensemble_adjoint_reverse: no forward sweep recorded");
+  // the sweep differentiates the recorded trajectory: its recomputed steps
+  // and VJP kernels read each member's dt from the device member table, so
+  // a dt set after the forward sweep is swapped out for the sweep
+  bool swap = false;
+  for (int m = 0; m < e->nmem; m++) swap = swap || e->mem[m].dt != e->tape_dt[m];
+  std::vector<hm::EnsMember> saved;
+  if (swap) {
+    saved = e->mem;
+    for (int m = 0; m < e->nmem; m++) e->mem[m].dt = e->tape_dt[m];
+    HM_HIP(hipMemcpyAsync(e->mem_dev, e->mem.data(), e->nmem * sizeof(hm::EnsMember), hipMemcpyHostToDevice,
+                          e->stream));
+    HM_HIP(hipStreamSynchronize(e->stream));
+  }
+  st = ens_reverse_sweep(e, lam);
+  if (st != HM_OK)
+    for (int i = 0; i < TAPE_SETS; i++) e->tape_holds[i] = -1;
+  if (swap) {
+    e->mem = saved;
+    HM_HIP(hipMemcpyAsync(e->mem_dev, e->mem.data(), e->nmem * sizeof(hm::EnsMember), hipMemcpyHostToDevice,
+                          e->stream));
+    HM_HIP(hipStreamSynchronize(e->stream));
+  }
+  return st;
 }
 
 }  // extern "C"

@@ -295,3 +295,40 @@ def test_viscous_characteristic_gradient_matches_finite_differences():
         rel = abs(fd - ad) / abs(ad)
         print(f"[gpu] direction {k}: adjoint {ad:.12e} fd {fd:.12e} rel {rel:.2e}", flush=True)
         assert rel <= 1e-6
+
+
+@pytest.mark.gpu
+def test_reverse_sweep_repeats_and_survives_step_vjp():
+    """7 steps with a checkpoint every 2: four segments for three stage
+    sets, so the first reverse sweep recomputes segment 0 into the set that
+    held segment 3.  A second sweep over the same forward (a retain_graph
+    backward, or a second objective), and one after hm_rk4_vjp reused stage
+    set 0, must recompute what their sets no longer hold: all three give the
+    same gradient, equal to torch autograd through the CPU trajectory."""
+    from hypar_amd.adjoint import trajectory
+    w, u0, dt, dx, sch = _scheme_case("characteristic", "roe", 1000.0)
+    x = T.to_soa(u0).requires_grad_(True)
+    y = x
+    for _ in range(7):
+        y = T.rk4_step(y, dt, dx, **sch)
+    lam = torch.tensor(np.random.default_rng(11).normal(size=x.shape))
+    (ref,) = torch.autograd.grad(y, x, lam)
+    print("[gpu] repeated reverse sweeps, char/roe viscous, 7 steps, checkpoint every 2", flush=True)
+    s = _gpu_solver(w)
+    u = _hm(x)
+    s.adjoint_forward(u, dt, 7, 2)
+    g1 = s.adjoint_reverse(_hm(lam))
+    g2 = s.adjoint_reverse(_hm(lam))
+    s.rk4_vjp(_hm(x), dt, _hm(lam))
+    g3 = s.adjoint_reverse(_hm(lam))
+    err = _rel(g1.cpu().view_as(ref), ref)
+    print(f"[gpu]   rel err {err:.2e}; sweeps equal: {torch.equal(g1, g2)}, {torch.equal(g1, g3)}", flush=True)
+    assert err <= 1e-11
+    assert torch.equal(g1, g2) and torch.equal(g1, g3)
+    # the same through autograd: two backward passes over one graph
+    xg = _hm(x).requires_grad_(True)
+    yN = trajectory(xg, s, dt, 7, 2)
+    (a1,) = torch.autograd.grad(yN, xg, _hm(lam), retain_graph=True)
+    (a2,) = torch.autograd.grad(yN, xg, _hm(lam))
+    assert torch.equal(a1, g1) and torch.equal(a2, g1)
+    s.close()

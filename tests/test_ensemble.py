@@ -205,3 +205,27 @@ def test_rejections():
     assert ex.value.code == lib.HM_ERR_ARG
     with pytest.raises(ValueError):
         e.rk4(torch.zeros(10, dtype=torch.float64, device="cuda"), 1)
+
+
+def test_ensemble_tape_keeps_its_dt_and_repeats():
+    """The batched reverse sweep differentiates the recorded trajectory: a
+    dt set after the forward sweep does not enter it, and a second sweep
+    (five segments for three stage sets, so both recompute) gives the same
+    gradient as the first."""
+    from hypar_amd.ensemble import Ensemble
+    dims = [(12, 10, 14), (16, 12, 10)]
+    ws, u0s, dts = _members(dims)
+    print(f"[gpu] ensemble tape dt and repeated sweeps dims={dims}", flush=True)
+    e = Ensemble(dims)
+    e.set_dt(dts)
+    u = e.to_device(u0s)
+    e.adjoint_forward(u, 5, 1)
+    lam = torch.tensor(np.random.default_rng(3).normal(size=u.shape), device="cuda")
+    g1 = e.adjoint_reverse(lam.clone())
+    e.set_dt([2.0 * d for d in dts])
+    g2 = e.adjoint_reverse(lam.clone())
+    e.set_dt(dts)
+    g3 = e.adjoint_reverse(lam.clone())
+    print(f"[gpu]   equal: {torch.equal(g1, g2)}, {torch.equal(g1, g3)}", flush=True)
+    assert torch.equal(g1, g2) and torch.equal(g1, g3)
+    e.close()
