@@ -18,12 +18,20 @@ private), so multi-threaded characteristic runs are not reproducible.
 Small grids store the full initial and final interior states.  Large grids
 store the workload (seed, sizes, dt) plus checksums and a strided subsample of
 the final state; the initial state is regenerated from the seed.
+
+Every deck runs with ``screen_op_iter 1``, and the fixture keeps the CFL and
+norm the reference prints after each step (src/TimeIntegration/
+TimePrintStep.c:28-33, ``CFL=%1.3E`` and ``norm=%1.4E``) as the printed
+strings: the CFL of the state before the step (TimePreStep.c:93-97,
+NavierStokes3DComputeCFL.c:16-55; for ensembles the member loop leaves the
+last member's value) and the RMS of the step's change (TimePostStep.c:43-63).
 """
 from __future__ import annotations
 
 import dataclasses
 import json
 import os
+import re
 import subprocess
 import sys
 import tempfile
@@ -117,6 +125,18 @@ def conservation_outputs(deck_dir: str, stdout: str) -> dict:
     return {"volume_integral_initial": vi, "conservation_dat": fields}
 
 
+def step_trace(stdout: str) -> dict:
+    """Per-step printed CFL and norm strings of a run with screen_op_iter 1,
+    in both print formats of TimePrintStep.c:23-33 (one line per step, or
+    the ensemble block of indented lines)."""
+    its = [int(m) for m in re.findall(r"iter=\s*(\d+)", stdout)]
+    cfl = re.findall(r"CFL=([0-9.Ee+-]+)", stdout)
+    norm = re.findall(r"norm=([0-9.Ee+-]+)", stdout)
+    if not (len(its) == len(cfl) == len(norm)) or its != list(range(1, len(its) + 1)):
+        raise RuntimeError("could not parse the reference's step lines")
+    return {"cfl": cfl, "norm": norm}
+
+
 def summary(u: np.ndarray) -> dict:
     return {"sum": [float(v) for v in u.reshape(-1, 5).sum(axis=0)],
             "l2": [float(v) for v in np.sqrt((u.reshape(-1, 5) ** 2).sum(axis=0))]}
@@ -132,8 +152,10 @@ def make_single(name: str):
     cons = name in CONSERVATION
     extra = {}
     with tempfile.TemporaryDirectory() as d:
-        u0 = decks.write_deck(d, w, screen_op_iter=max(1, steps), conservation_check="yes" if cons else "no")
+        u0 = decks.write_deck(d, w, screen_op_iter=1, conservation_check="yes" if cons else "no")
         wall, out = run_reference(d, want_stdout=True)
+        trace = step_trace(out)
+        assert len(trace["cfl"]) == steps
         _, _, uN = decks.read_op_bin(os.path.join(d, "op.bin"))
         if cons:
             extra = conservation_outputs(d, out)
@@ -142,7 +164,7 @@ def make_single(name: str):
             "Minf": w.Minf, "Pr": w.Pr, "gamma": w.gamma, "eps": w.weno_eps,
             "field": w.field, "time_scheme_type": w.time_scheme_type, "weno_type": w.weno_type,
             "reference_wall_s": wall, "reference_threads": 1,
-            "u0_summary": summary(u0), "uN_summary": summary(uN), **extra}
+            "u0_summary": summary(u0), "uN_summary": summary(uN), "trace": trace, **extra}
     arrays = {"meta": np.array(json.dumps(meta))}
     if full:
         arrays["u0"] = u0
@@ -161,14 +183,16 @@ def make_ensemble(name: str):
     dt = min(decks.cfl_dt(u, m.dims) for u, m in zip(u0s, members))
     with tempfile.TemporaryDirectory() as d:
         decks.write_ensemble_deck(d, members, steps, dt)
-        wall = run_reference(d)
+        wall, out = run_reference(d, want_stdout=True)
+        trace = step_trace(out)
+        assert len(trace["cfl"]) == steps
         outs = []
         for i in range(len(members)):
             _, _, uN = decks.read_op_bin(os.path.join(d, f"op_{decks.member_index(i, len(members))}.bin"))
             outs.append(uN)
     meta = {"name": name, "sizes": [list(s) for s in sizes], "steps": steps, "dt": dt,
             "seeds": [m.seed for m in members], "reference_wall_s": wall,
-            "reference_threads": 1}
+            "reference_threads": 1, "trace": trace}
     arrays = {"meta": np.array(json.dumps(meta))}
     for i, (u0, uN) in enumerate(zip(u0s, outs)):
         arrays[f"u0_{i}"] = u0
@@ -181,6 +205,8 @@ def main(argv):
     if not os.path.exists(REF_BIN):
         raise SystemExit(f"{REF_BIN} missing: make -f oracle/Makefile.ref REF=/root/reference")
     names = argv[1:] or (list(FIXTURES) + list(ENSEMBLE))
+    if names == ["--fast"]:  # all but the two long runs (minutes each, single-threaded)
+        names = [n for n in list(FIXTURES) + list(ENSEMBLE) if n not in ("c0_64_s100", "c1_64_s200")]
     for n in names:
         if n in FIXTURES:
             make_single(n)

@@ -74,3 +74,80 @@ def test_oracle_volume_integral_matches_reference_print():
         got = Oracle.for_workload(workload_from_meta(meta)).volume_integral(z["u0"])
         ref = np.array(meta["volume_integral_initial"])
         assert np.all(np.abs(got - ref) <= 1e-15 * np.maximum(np.abs(ref), 1e-3)), (name, got, ref)
+
+
+# fixtures whose traces the oracle replays (every full fixture of the RK4 tableau)
+TRACED = FULL + ["c1_16_s4_cons", "c0_ragged_s3_cons"]
+
+
+def printed_trace(o, u0, dt, steps):
+    """The CFL before each step and the RMS of each step's change, formatted
+    as the reference prints them (TimePrintStep.c:28-33).  The norm sums the
+    squares point by point with the variables inner (ArraySumSquarenD,
+    include/arrayfunctions.h), which a sequential cumsum over the AoS array
+    reproduces; it returns the raw sums too, for the ensemble total."""
+    import math
+    u, cfl, norm, sums = u0, [], [], []
+    npts = u0.size // 5
+    for _ in range(steps):
+        cfl.append("%1.3E" % o.cfl(u, dt))
+        un = o.rk4(u, dt, 1)
+        d = (u - un).ravel()
+        sums.append(float(np.cumsum(d * d)[-1]))
+        norm.append("%1.4E" % math.sqrt(sums[-1] / npts))
+        u = un
+    return cfl, norm, sums
+
+
+@pytest.mark.parametrize("name", TRACED)
+def test_oracle_cfl_and_norm_match_reference_print(name):
+    """orc_cfl at every step's state and the RMS step change against the
+    reference's own printed CFL= and norm= values, string for string."""
+    meta, z = load_fixture(name)
+    w = workload_from_meta(meta)
+    cfl, norm, _ = printed_trace(Oracle.for_workload(w), z["u0"], w.dt, meta["steps"])
+    assert cfl == meta["trace"]["cfl"]
+    assert norm == meta["trace"]["norm"]
+
+
+def test_oracle_ensemble_cfl_and_norm_match_reference_print():
+    """Ensembles print the last member's CFL (the member loop of
+    TimePreStep.c:93-97 overwrites max_cfl) and the norm over all members,
+    sqrt(sum of every member's squares / sum of their points)
+    (TimePostStep.c:43-63)."""
+    import math
+    meta, z = load_fixture("ens3_s4")
+    per = []
+    for i, dims in enumerate(meta["sizes"]):
+        per.append(printed_trace(Oracle(tuple(dims)), z[f"u0_{i}"], meta["dt"], meta["steps"]))
+    npts = sum(int(np.prod(d)) for d in meta["sizes"])
+    cfl = per[-1][0]
+    norm = []
+    for n in range(meta["steps"]):
+        total = 0.0
+        for p in per:
+            total += p[2][n]
+        norm.append("%1.4E" % math.sqrt(total / npts))
+    assert cfl == meta["trace"]["cfl"]
+    assert norm == meta["trace"]["norm"]
+
+
+@pytest.mark.parametrize("d,dims,v", [(0, (14, 12, 10), (-0.3, 0.2, -0.1)),
+                                        (1, (10, 16, 12), (0.05, -0.4, 0.1)),
+                                        (2, (12, 10, 18), (0.1, 0.15, -0.35))])
+def test_oracle_cfl_closed_form(d, dims, v):
+    """A uniform state whose largest |v|, negative, lies along direction d,
+    the finest one: the CFL is (|v_d| + c) dt max_i dxinv_d(i) with
+    c = sqrt(gamma p / rho).  A CFL without the absolute value, or with the
+    directions' metrics swapped, gives another number."""
+    o = Oracle(dims)
+    rho, p, g = 1.25, 0.9, 1.4
+    E = p / (g - 1) + 0.5 * rho * sum(c * c for c in v)
+    u = np.empty(dims[::-1] + (5,))
+    u[...] = [rho, rho * v[0], rho * v[1], rho * v[2], E]
+    dt = 0.01
+    c = np.sqrt(g * p / rho)
+    dx = [decks.dxinv_1d(decks.grid_1d(n))[3:-3] for n in dims]
+    want = max((abs(v[e]) + c) * dt * dx[e].max() for e in range(3))
+    assert abs(o.cfl(u, dt) - want) <= 1e-14 * want
+    assert abs(want - (abs(v[d]) + c) * dt * dx[d].max()) <= 1e-14 * want
