@@ -83,24 +83,9 @@ hm_status check_launch(const char *what) {
   return HM_OK;
 }
 
-// Uniform periodic grid x_i = i*(2*pi/n); ghost coordinates extrapolated
-// linearly (ReadArray.c:59-90); dxinv = 2/(x[i+1]-x[i-1]) inside and copies
-// of the edge value in the ghosts (InitialSolution.c:74-113).
 void uniform_dxinv(int n, int g, double *out) {
   std::vector<double> X(n + 2 * g);
-  const double h = 2.0 * M_PI / n;
-  for (int i = 0; i < n; i++) X[g + i] = (double)i * h;
-  for (int i = 0; i < g; i++) {
-    int delta = g - i;
-    X[i] = X[g] + ((double)delta) * (X[g] - X[g + 1]);
-  }
-  for (int i = n + g; i < n + 2 * g; i++) {
-    int delta = i - (n + g - 1);
-    X[i] = X[n + g - 1] + ((double)delta) * (X[n + g - 1] - X[n + g - 2]);
-  }
-  for (int i = 0; i < n; i++) out[i + g] = 2.0 / (X[i + 1 + g] - X[i - 1 + g]);
-  for (int i = 0; i < g; i++) out[i] = out[g];
-  for (int i = n + g; i < n + 2 * g; i++) out[i] = out[n + g - 1];
+  hm::uniform_dxinv(n, g, X.data(), out);
 }
 
 hm_status dmalloc(double **p, size_t n) {
@@ -178,6 +163,8 @@ hm_status ensure_tmp(hm_solver *s, size_t n) {
 }
 
 }  // namespace
+
+int hm::set_error(int code, const std::string &msg) { return (int)fail((hm_status)code, msg); }
 
 extern "C" {
 

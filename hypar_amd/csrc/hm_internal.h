@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 
+#include <cmath>
+#include <string>
+
 namespace hm {
 
 enum { INTERP_COMP = 0, INTERP_CHAR = 1 };
@@ -24,6 +27,31 @@ struct Scheme {
   double eps;
   int weno;   // WENO_*
 };
+
+// records msg as the calling thread's hm_last_error() and returns code
+int set_error(int code, const std::string &msg);
+
+// Metrics of the uniform periodic grid x_i = i*(2*pi/n) with g ghost points
+// per side: ghost coordinates extrapolated linearly (ReadArray.c:59-90),
+// dxinv = 2/(x[i+1]-x[i-1]) inside and copies of the edge value in the
+// ghosts (InitialSolution.c:74-113).  X is caller scratch of n + 2g doubles;
+// out receives n + 2g doubles.  Host code (compiled without FMA contraction
+// on x86-64), so the values are those of the reference's own host code.
+inline void uniform_dxinv(int n, int g, double *X, double *out) {
+  const double h = 2.0 * M_PI / n;
+  for (int i = 0; i < n; i++) X[g + i] = (double)i * h;
+  for (int i = 0; i < g; i++) {
+    int delta = g - i;
+    X[i] = X[g] + ((double)delta) * (X[g] - X[g + 1]);
+  }
+  for (int i = n + g; i < n + 2 * g; i++) {
+    int delta = i - (n + g - 1);
+    X[i] = X[n + g - 1] + ((double)delta) * (X[n + g - 1] - X[n + g - 2]);
+  }
+  for (int i = 0; i < n; i++) out[i + g] = 2.0 / (X[i + 1 + g] - X[i - 1 + g]);
+  for (int i = 0; i < g; i++) out[i] = out[g];
+  for (int i = n + g; i < n + 2 * g; i++) out[i] = out[n + g - 1];
+}
 
 // interface fluxes along dir of the hm-layout state u into fI (5*ncell); the
 // x pass also stores the primitive variables (u,v,w,T) into Q (4*ncell)

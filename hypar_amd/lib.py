@@ -54,6 +54,8 @@ SIGNATURES = [
     ("hm_parabolic", [_VP, _DP, _DP], ctypes.c_int),
     ("hm_cfl", [_VP, _DP, ctypes.c_double, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     ("hm_step_norm", [_VP, _DP, _DP, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    ("hm_cfl_noalloc", [ctypes.POINTER(HMConfig), _DP, ctypes.c_double, ctypes.POINTER(ctypes.c_double), _VP],
+     ctypes.c_int),
     ("hm_volume_integral", [_VP, _DP, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     ("hm_profile_kernels", [_VP, _DP, ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     ("hm_malloc", [ctypes.POINTER(ctypes.c_void_p), _SZ], ctypes.c_int),

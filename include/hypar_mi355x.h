@@ -129,6 +129,22 @@ hm_status hm_step_norm(hm_solver *s, const double *a, const double *b, double *n
  * check of TimePostStep.c:81-93).  Synchronises; HM_ERR_NONFINITE on NaN/Inf. */
 hm_status hm_volume_integral(hm_solver *s, const double *u, double *out);
 
+/* ---- an entry point that allocates nothing ----------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): hm_cfl without a solver object.  The grid metrics and
+ * the reduction partials live in a static device array of
+ * HM_NOALLOC_SCRATCH_DOUBLES doubles (66 KiB) that each GPU the library
+ * runs on holds until the process ends; grid sides above
+ * HM_NOALLOC_MAX_SIDE are rejected with HM_ERR_ARG.  Enqueued on `stream`
+ * (a hipStream_t; NULL = default stream), then synchronised.  Calls that
+ * overlap on one GPU share that array and race.  The result equals
+ * hm_cfl's for a solver created from the same cfg (its dxinv, or the
+ * uniform periodic grid's metrics when cfg->dxinv is NULL). */
+#define HM_NOALLOC_MAX_SIDE 2048
+#define HM_NOALLOC_SCRATCH_DOUBLES 8448
+hm_status hm_cfl_noalloc(const hm_config *cfg, const double *u, double dt, double *cfl, void *stream);
+
 /* ---- device memory (so that host code needs no HIP headers) ---------- */
 /* Allocate / free n doubles of device memory on the current device. */
 hm_status hm_malloc(double **dev, size_t n);

@@ -53,3 +53,15 @@ def test_create_rejects_bad_config_before_touching_device():
     cfg.weno_type = 5  # one past the optimal (no_limiting) weights
     assert L.hm_create(ctypes.byref(cfg), ctypes.byref(h)) == lib.HM_ERR_UNSUPPORTED
     assert b"WENO weight" in L.hm_last_error()
+
+
+def test_noalloc_cfl_rejects_oversized_grid_before_touching_device():
+    from hypar_amd import lib
+    L = lib.load()
+    cfg = lib.HMConfig()
+    cfg.dim[:] = [16, 16, 4096]
+    cfg.gamma, cfg.Re, cfg.Minf, cfg.Pr, cfg.weno_eps = 1.4, -1.0, 1.0, 0.72, 1e-6
+    out = ctypes.c_double()
+    code = L.hm_cfl_noalloc(ctypes.byref(cfg), ctypes.c_void_p(8), 0.01, ctypes.byref(out), None)
+    assert code == lib.HM_ERR_ARG
+    assert b"HM_NOALLOC_MAX_SIDE" in L.hm_last_error()
