@@ -26,21 +26,25 @@ sys.path.insert(0, ROOT)
 METRIC = "NS3D WENO5-RK4 FP64 cell-updates/s at 256³; adjoint/forward time ratio"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
-# algorithmic HBM bytes per cell of one launch of each kernel of an RK stage
-# (5 FP64 conserved variables = 40 B; viscous runs add the viscous flux of
-# the pass direction, 4 components = 32 B, and the z pass writes the four
-# primitive variables of the new state, 32 B):
-#   pass_x     read U_s, write R
-#   pass_y     read U_s and R, write R
-#   pass_z     read U_s, R, u, acc; write acc, U_{s+1}   (stage-1 form)
+# algorithmic HBM bytes per cell-update of a fused 3N RK4 step (5 FP64
+# variables = 40 B per array; stage 1 reads u, writes acc and U2 = 120 B;
+# stages 2-3 read U_s, u, acc, write acc, U_{s+1} = 200 B each; stage 4
+# reads U4, acc, writes u = 120 B).  Halo re-reads, metrics and every
+# intermediate (the rhs accumulator between passes, the viscous fluxes and
+# primitive variables) count zero.
+STEP_BYTES_PER_CELL_UPDATE = 640
+# one launch of a stage kernel processes one stage of every cell: its share
+# of the step's algorithmic bytes is a quarter of 640 B per cell
+STAGE_BYTES_PER_CELL = STEP_BYTES_PER_CELL_UPDATE // 4
+# operand bytes per cell one launch of each kernel reads and writes in this
+# implementation (not the algorithmic figure; reported beside it):
+#   pass_x     read U_s, write R                              (+ FV_x 32 B viscous)
+#   pass_y     read U_s and R, write R                         (+ FV_y 32 B)
+#   pass_z     read U_s, R, u, acc; write acc, U_{s+1}         (+ FV_z 32 B, write Q 32 B)
 #   visc_flux  read Q (4 primitives), write the viscous fluxes of x, y, z (12)
 #   prims      read u, write Q
-PASS_BYTES = {"pass_x": (80, 32), "pass_y": (120, 32), "pass_z": (240, 64), "visc_flux": (128, 0),
-              "prims": (72, 0)}
-# algorithmic HBM bytes per cell-update of a fused 3N RK4 step (5 FP64
-# variables; stage 1 reads u, writes acc and U2; stages 2-3 read U_s, u, acc,
-# write acc, U_{s+1}; stage 4 reads U4, acc, writes u)
-STEP_BYTES_PER_CELL_UPDATE = 640
+OPERAND_BYTES = {"pass_x": (80, 32), "pass_y": (120, 32), "pass_z": (240, 64), "visc_flux": (128, 0),
+                 "prims": (72, 0)}
 
 
 def cpu_baseline(n=64, steps=2):
@@ -74,6 +78,41 @@ def cpu_baseline(n=64, steps=2):
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": n ** 3 * steps / t, "unit": "cell-updates/s", "cores": cores, "kind": "port",
             "sample": f"CPU oracle, same scheme on a periodic {n}^3 grid, {steps} RK4 steps, {t:.2f} s"}
+
+
+def host_cores():
+    """Threads this process may use: OMP_NUM_THREADS when set (the GPU box
+    sets it to the box's CPU share), else the CPUs of the affinity mask."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_baseline_components(n=64, steps=2):
+    """The reference on all host cores: its component-wise scheme (the
+    inviscid WENO5-JS + Rusanov configuration, whose OpenMP loops share no
+    scratch between threads) on a periodic n^3 grid for `steps` RK4 steps,
+    OMP_NUM_THREADS = host_cores().  None when the reference binary is
+    absent."""
+    from hypar_amd import decks
+    ref = os.path.join(ROOT, "oracle", "_ref", "HyPar")
+    if not os.path.exists(ref):
+        return None
+    w = decks.config0(n, steps)
+    cores = host_cores()
+    with tempfile.TemporaryDirectory() as d:
+        decks.write_deck(d, w, screen_op_iter=1)
+        env = dict(os.environ, OMP_NUM_THREADS=str(cores))
+        r = subprocess.run([ref], cwd=d, capture_output=True, text=True, env=env)
+        m = re.search(r"total wctime:\s*([0-9.eE+-]+)", r.stdout)
+        if r.returncode != 0 or not m:
+            return None
+        t = float(m.group(1))
+    return {"value": n ** 3 * steps / t, "unit": "cell-updates/s", "cores": cores, "kind": "reference",
+            "affinity_cpus": len(os.sched_getaffinity(0)),
+            "sample": f"reference HyPar (gcc -O3, OpenMP, {cores} threads), inviscid component-wise WENO5-JS + "
+                      f"Rusanov RK4 on a periodic {n}^3 grid, {steps} RK4 steps, {t:.2f} s of time integration"}
 
 
 def adjoint_ratio(n=128, nsteps=50, every=10):
@@ -316,8 +355,8 @@ def main():
     kt = s.profile_kernels(u, reps=5)
     per_step = {k: 4 * v for k, v in kt.items() if k != "prims"}  # once per stage
     dom = max(per_step, key=per_step.get)
-    base, visc_extra = PASS_BYTES[dom]
-    bytes_launch = (base + visc_extra) * ncell
+    base, visc_extra = OPERAND_BYTES[dom]
+    bytes_launch = STAGE_BYTES_PER_CELL * ncell
     achieved = bytes_launch / (kt[dom] / 1e3) / 1e9
     prof = {}
     pfile = os.path.join(ROOT, "profiles", f"counters_{args.n}.json")
@@ -326,7 +365,10 @@ def main():
     traffic = prof.get("hbm_bytes_per_launch", {}).get(dom)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                "avg_launch_ms": kt[dom], "algorithmic_bytes_per_launch": bytes_launch}
+                "avg_launch_ms": kt[dom], "algorithmic_bytes_per_launch": bytes_launch,
+                "algorithmic_bytes_basis": f"{STEP_BYTES_PER_CELL_UPDATE}/4 B per cell per stage launch",
+                "operand_bytes_per_launch": (base + visc_extra) * ncell,
+                "operand_GBs": (base + visc_extra) * ncell / (kt[dom] / 1e3) / 1e9}
     flops = prof.get("fp64_flops_per_launch", {}).get(dom)
     roofline_fp64 = None
     if flops:
@@ -356,6 +398,10 @@ def main():
                           "achieved_GBs": step_bw, "peak_GBs": HBM_PEAK_GBS,
                           "frac": step_bw / HBM_PEAK_GBS},
         "kernel_ms_per_step": per_step,
+        # the kernels timed one at a time (each launch alone on the stream);
+        # inside the step consecutive launches overlap at their edges, so
+        # this sum exceeds ms_per_step
+        "kernel_ms_sum_isolated": sum(per_step.values()),
     }
     s.close()
     del u
@@ -368,6 +414,7 @@ def main():
         line["ensemble_adjoint"] = ensemble_adjoint()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
+        line["cpu_baseline_components_all_cores"] = cpu_baseline_components()
     print(json.dumps(line), flush=True)
 
 
