@@ -24,8 +24,8 @@
 constexpr int TAPE_SETS = 3;
 
 struct hm_solver {
-  hm_solver() : g(), sc(), device(0), stream(nullptr), dxg(nullptr), fI{nullptr, nullptr, nullptr},
-                FV(nullptr), FVg{nullptr, nullptr, nullptr}, Q(nullptr), S{nullptr, nullptr},
+  hm_solver() : g(), sc(), device(0), stream(nullptr), dxg(nullptr), R(nullptr), par(nullptr), Q(nullptr),
+                S{nullptr, nullptr},
                 acc(nullptr), part(nullptr), part_host(nullptr), npart(0), tmp(nullptr), tmp_elems(0),
                 tape_nsteps(0), tape_every(0), tape_dt(0.0) {
     for (int i = 0; i < TAPE_SETS; i++) tape_holds[i] = -1;
@@ -35,10 +35,9 @@ struct hm_solver {
   int device;
   hipStream_t stream;
   double *dxg;        // ghosted dxinv x|y|z on the device
-  double *fI[3];      // interface fluxes, 5*ncell each
-  double *FV;         // viscous fluxes, 12*ncell (viscous runs only)
-  double *FVg[3];     // ghost-layer viscous fluxes per direction (viscous only)
-  double *Q;          // primitive variables u,v,w,T, 4*ncell (viscous only)
+  double *R;          // hyperbolic-term accumulator of the passes, 5*ncell
+  double *par;        // viscous term, 4*ncell (viscous runs only)
+  double *Q;          // primitive variables u, v, w, T, 4*ncell (viscous only)
   double *S[2];       // RK stage states (double-buffered)
   double *acc;        // RK accumulator
   double *part;       // reduction partials (device)
@@ -114,14 +113,6 @@ hm_status check_state(const hm_solver *s, const void *p, const char *name) {
   return HM_OK;
 }
 
-// right-hand side pieces of the stage state U into fI / FV
-hm_status stage_fluxes(hm_solver *s, const double *U) {
-  const bool visc = s->sc.Re > 0;
-  for (int d = 0; d < 3; d++) hm::launch_iface(s->g, s->sc, d, U, s->fI[d], visc ? s->Q : nullptr, s->stream);
-  if (visc) hm::launch_visc_flux(s->g, s->sc, s->Q, s->FV, s->FVg, s->dxg, s->stream);
-  return check_launch("interface/viscous flux kernels");
-}
-
 // "rk 44" tableau (TimeExplicitRKInitialize.c:58-68)
 const double RK_A_SUB[3] = {0.5, 0.5, 1.0};  // A[s+1][s]
 const double RK_B[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
@@ -135,15 +126,14 @@ hm_status rk4_step(hm_solver *s, double *u, double dt, double *const Sx[3]) {
   for (int stage = 0; stage < 4; stage++) {
     hm::PassBuffers b;
     b.U = stage == 0 ? u : Sx[stage - 1];
-    b.R = s->fI[0];
-    b.FV = s->FV;
-    b.FVg = s->FVg;
+    b.R = s->R;
+    b.par = s->par;
     b.u = u;
     b.S = stage < 3 ? Sx[stage] : nullptr;
     b.acc = s->acc;
     b.out = nullptr;
     b.Q = s->Q;
-    if (visc) hm::launch_visc_flux(s->g, s->sc, s->Q, s->FV, s->FVg, s->dxg, s->stream);
+    if (visc) hm::launch_visc_par(s->g, s->sc, s->Q, s->par, s->dxg, s->stream);
     const double ca = stage < 3 ? dt * RK_A_SUB[stage] : 0.0, cb = dt * RK_B[stage];
     for (int d = 0; d < 3; d++) hm::launch_pass(s->g, s->sc, d, b, hm::OUT_STAGE, stage, ca, cb, s->dxg, s->stream);
     hm_status st = check_launch("RK4 stage passes");
@@ -173,8 +163,7 @@ const char *hm_version(void) { return "hypar-mi355x 0.1 (gfx950, fp64)"; }
 
 void hm_destroy(hm_solver *s) {
   if (!s) return;
-  double *bufs[] = {s->dxg, s->fI[0], s->fI[1], s->fI[2], s->FV, s->FVg[0], s->FVg[1], s->FVg[2], s->Q,
-                    s->S[0], s->S[1], s->acc, s->part, s->tmp};
+  double *bufs[] = {s->dxg, s->R, s->par, s->Q, s->S[0], s->S[1], s->acc, s->part, s->tmp};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   if (s->part_host) (void)hipHostFree(s->part_host);
@@ -236,13 +225,10 @@ hm_status hm_create(const hm_config *cfg, hm_solver **out) {
   const size_t n5 = 5 * s->g.ncell;
   hm_status st = HM_OK;
   if ((st = dmalloc(&s->dxg, ndx)) != HM_OK) goto bad;
-  for (int d = 0; d < 3; d++)
-    if ((st = dmalloc(&s->fI[d], n5)) != HM_OK) goto bad;
+  if ((st = dmalloc(&s->R, n5)) != HM_OK) goto bad;
   if (s->sc.Re > 0) {
-    if ((st = dmalloc(&s->FV, 12 * s->g.ncell)) != HM_OK) goto bad;
+    if ((st = dmalloc(&s->par, 4 * s->g.ncell)) != HM_OK) goto bad;
     if ((st = dmalloc(&s->Q, 4 * s->g.ncell)) != HM_OK) goto bad;
-    for (int d = 0; d < 3; d++)
-      if ((st = dmalloc(&s->FVg[d], hm::visc_ghost_size(s->g, d))) != HM_OK) goto bad;
   }
   if ((st = dmalloc(&s->S[0], n5)) != HM_OK) goto bad;
   if ((st = dmalloc(&s->S[1], n5)) != HM_OK) goto bad;
@@ -329,41 +315,44 @@ hm_status hm_rk4(hm_solver *s, double *u, double dt, int nsteps) {
   return HM_OK;
 }
 
-static hm_status rhs_part(hm_solver *s, double *out, const double *u, int mode, const char *name) {
+// -hyp + par (OUT_RHS) or hyp (OUT_HYP) of u into out through the passes
+static hm_status rhs_passes(hm_solver *s, double *out, const double *u, int mode, const char *name) {
   hm_status st = check_state(s, u, name);
   if (st != HM_OK) return st;
   if ((st = check_state(s, out, name)) != HM_OK) return st;
-  if ((st = stage_fluxes(s, u)) != HM_OK) return st;
-  hm::launch_stage(s->g, s->sc, s->fI[0], s->fI[1], s->fI[2], s->FV, s->FVg, nullptr, nullptr, nullptr,
-                   out, mode, s->dxg, 0, 0.0, 0.0, s->stream);
+  if (out == u) return fail(HM_ERR_ARG, std::string(name) + ": the output must not alias u");
+  if (s->sc.Re > 0 && mode == hm::OUT_RHS) {
+    hm::launch_prims(s->g, s->sc, u, s->Q, s->stream);
+    hm::launch_visc_par(s->g, s->sc, s->Q, s->par, s->dxg, s->stream);
+  }
+  hm::PassBuffers b;
+  b.U = u;
+  b.R = s->R;
+  b.par = s->par;
+  b.u = b.S = b.acc = b.Q = nullptr;
+  b.out = out;
+  for (int d = 0; d < 3; d++) hm::launch_pass(s->g, s->sc, d, b, mode, 0, 0.0, 0.0, s->dxg, s->stream);
   return check_launch(name);
 }
 
 hm_status hm_rhs(hm_solver *s, double *rhs, const double *u) {
-  hm_status st = check_state(s, u, "hm_rhs: u");
-  if (st != HM_OK) return st;
-  if ((st = check_state(s, rhs, "hm_rhs: rhs")) != HM_OK) return st;
-  if (rhs == u) return fail(HM_ERR_ARG, "hm_rhs: rhs must not alias u");
-  const bool visc = s->sc.Re > 0;
-  if (visc) {
-    hm::launch_prims(s->g, s->sc, u, s->Q, s->stream);
-    hm::launch_visc_flux(s->g, s->sc, s->Q, s->FV, s->FVg, s->dxg, s->stream);
-  }
-  hm::PassBuffers b;
-  b.U = u;
-  b.R = s->fI[0];
-  b.FV = s->FV;
-  b.FVg = s->FVg;
-  b.u = b.S = b.acc = b.Q = nullptr;
-  b.out = rhs;
-  for (int d = 0; d < 3; d++) hm::launch_pass(s->g, s->sc, d, b, hm::OUT_RHS, 0, 0.0, 0.0, s->dxg, s->stream);
-  return check_launch("hm_rhs");
+  return rhs_passes(s, rhs, u, hm::OUT_RHS, "hm_rhs");
 }
 hm_status hm_hyperbolic(hm_solver *s, double *hyp, const double *u) {
-  return rhs_part(s, hyp, u, hm::OUT_HYP, "hm_hyperbolic");
+  return rhs_passes(s, hyp, u, hm::OUT_HYP, "hm_hyperbolic");
 }
 hm_status hm_parabolic(hm_solver *s, double *par, const double *u) {
-  return rhs_part(s, par, u, hm::OUT_PAR, "hm_parabolic");
+  hm_status st = check_state(s, u, "hm_parabolic: u");
+  if (st != HM_OK) return st;
+  if ((st = check_state(s, par, "hm_parabolic: par")) != HM_OK) return st;
+  if (s->sc.Re > 0) {
+    hm::launch_prims(s->g, s->sc, u, s->Q, s->stream);
+    hm::launch_visc_par(s->g, s->sc, s->Q, s->par, s->dxg, s->stream);
+    hm::launch_par_out(s->g, s->par, par, s->stream);
+  } else {
+    hm::launch_fill(par, 0.0, 5 * s->g.ncell, s->stream);
+  }
+  return check_launch("hm_parabolic");
 }
 
 hm_status hm_cfl(hm_solver *s, const double *u, double dt, double *cfl) {
@@ -790,27 +779,26 @@ hm_status hm_profile_kernels(hm_solver *s, const double *u, int reps, double *ms
   const bool visc = s->sc.Re > 0;
   if (visc) {
     hm::launch_prims(s->g, s->sc, u, s->Q, s->stream);
-    hm::launch_visc_flux(s->g, s->sc, s->Q, s->FV, s->FVg, s->dxg, s->stream);
+    hm::launch_visc_par(s->g, s->sc, s->Q, s->par, s->dxg, s->stream);
   }
   // the passes as stage 1 of an RK step runs them: the z pass reads u and
   // acc and writes acc, the next stage state (and Q); u is not written
   HM_HIP(hipMemsetAsync(s->acc, 0, 5 * s->g.ncell * sizeof(double), s->stream));
   hm::PassBuffers b;
   b.U = u;
-  b.R = s->fI[0];
-  b.FV = s->FV;
-  b.FVg = s->FVg;
+  b.R = s->R;
+  b.par = s->par;
   b.u = const_cast<double *>(u);
   b.S = s->S[1];
   b.acc = s->acc;
   b.out = nullptr;
-  b.Q = s->fI[1];  // primitive variables of the stage state go to scratch
+  b.Q = s->Q;  // primitive variables of the stage state (solver scratch)
   for (int d = 0; d < 3 && st == HM_OK; d++)
     st = timed(d, [&] { hm::launch_pass(s->g, s->sc, d, b, hm::OUT_STAGE, 1, 0.0, 0.0, s->dxg, s->stream); });
   if (st == HM_OK) {
     if (visc) {
-      st = timed(3, [&] { hm::launch_visc_flux(s->g, s->sc, s->Q, s->FV, s->FVg, s->dxg, s->stream); });
-      if (st == HM_OK) st = timed(4, [&] { hm::launch_prims(s->g, s->sc, u, s->fI[1], s->stream); });
+      st = timed(3, [&] { hm::launch_visc_par(s->g, s->sc, s->Q, s->par, s->dxg, s->stream); });
+      if (st == HM_OK) st = timed(4, [&] { hm::launch_prims(s->g, s->sc, u, s->Q, s->stream); });
     } else {
       ms[3] = ms[4] = 0.0;
     }
@@ -837,13 +825,12 @@ struct hm_ensemble {
   double *dxg = nullptr;
   size_t ncell = 0;
   double *R = nullptr, *S[2] = {nullptr, nullptr}, *acc = nullptr;
-  // viscous members: fluxes, ghost-layer fluxes, primitive variables of all
-  // members, and the work list of the viscous-flux launch
-  double *FV = nullptr, *FVg[3] = {nullptr, nullptr, nullptr}, *Q = nullptr;
-  unsigned *item0v = nullptr;
-  unsigned nitems_v = 0;
-  unsigned *tiles0 = nullptr;  // adjoint: each member's first viscous tile
-  unsigned ntiles = 0;
+  // viscous members: viscous terms and primitive variables of all members,
+  // each member's first tile of the viscous-term launch, and (adjoint) of
+  // the viscous VJP launches
+  double *par = nullptr, *Q = nullptr;
+  unsigned *ptiles0 = nullptr, *tiles0 = nullptr;
+  unsigned nptiles = 0, ntiles = 0;
   unsigned *item0[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   unsigned nitems[2][3] = {{0, 0, 0}, {0, 0, 0}};
   // adjoint: first line of every member along each direction, longest
@@ -883,7 +870,7 @@ void ens_step_launches(hm_ensemble *e, double *u, hipStream_t st) {
   for (int stage = 0; stage < 4; stage++) {
     const double *U = stage == 0 ? u : e->S[(stage - 1) % 2];
     double *Sn = stage < 3 ? e->S[stage % 2] : nullptr;
-    if (e->sc.Re > 0) hm::launch_ens_visc(e->sc, e->mem_dev, e->nmem, e->item0v, e->nitems_v, st);
+    if (e->sc.Re > 0) hm::launch_ens_visc(e->sc, e->mem_dev, e->nmem, e->ptiles0, e->nptiles, st);
     for (int d = 0; d < 3; d++)
       hm::launch_ens_pass(e->sc, d, e->shape, e->mem_dev, e->nmem, e->item0[e->shape][d], e->nitems[e->shape][d],
                           e->grid[d], U, e->R, u, Sn, e->acc, stage, st);
@@ -911,7 +898,7 @@ hm_status ens_step_store(hm_ensemble *e, double *u, double *const U[3]) {
   for (int stage = 0; stage < 4; stage++) {
     const double *Uin = stage == 0 ? u : U[stage - 1];
     double *Sn = stage < 3 ? U[stage] : nullptr;
-    if (e->sc.Re > 0) hm::launch_ens_visc(e->sc, e->mem_dev, e->nmem, e->item0v, e->nitems_v, st);
+    if (e->sc.Re > 0) hm::launch_ens_visc(e->sc, e->mem_dev, e->nmem, e->ptiles0, e->nptiles, st);
     for (int d = 0; d < 3; d++)
       hm::launch_ens_pass(e->sc, d, e->shape, e->mem_dev, e->nmem, e->item0[e->shape][d], e->nitems[e->shape][d],
                           e->grid[d], Uin, e->R, u, Sn, e->acc, stage, st);
@@ -934,12 +921,12 @@ void hm_ensemble_destroy(hm_ensemble *e) {
   if (!e) return;
   ens_drop_graph(e);
   if (e->cap) (void)hipStreamDestroy(e->cap);
-  double *bufs[] = {e->dxg, e->R, e->S[0], e->S[1], e->acc, e->FV, e->FVg[0], e->FVg[1], e->FVg[2], e->Q};
+  double *bufs[] = {e->dxg, e->R, e->S[0], e->S[1], e->acc, e->par, e->Q};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   if (e->mem_dev) (void)hipFree(e->mem_dev);
-  if (e->item0v) (void)hipFree(e->item0v);
   if (e->tiles0) (void)hipFree(e->tiles0);
+  if (e->ptiles0) (void)hipFree(e->ptiles0);
   for (int sh = 0; sh < 2; sh++)
     for (int d = 0; d < 3; d++)
       if (e->item0[sh][d]) (void)hipFree(e->item0[sh][d]);
@@ -996,8 +983,7 @@ hm_status hm_ensemble_create(const hm_config *members, int nmembers, hm_ensemble
     M.ncell = (size_t)c.dim[0] * c.dim[1] * c.dim[2];
     M.off = e->ncell;
     M.dt = 0.0;
-    M.fv = M.q = nullptr;
-    M.fvg[0] = M.fvg[1] = M.fvg[2] = nullptr;
+    M.par = M.q = nullptr;
     e->ncell += M.ncell;
     const size_t ndx = (size_t)(c.dim[0] + 6) + (c.dim[1] + 6) + (c.dim[2] + 6);
     dxoff[m] = dxh.size();
@@ -1028,29 +1014,19 @@ hm_status hm_ensemble_create(const hm_config *members, int nmembers, hm_ensemble
   }
   for (int m = 0; m < nmembers; m++) e->mem[m].dxg = e->dxg + dxoff[m];
   if (e->sc.Re > 0) {
-    size_t ng[3] = {0, 0, 0};
-    for (int m = 0; m < nmembers; m++)
-      for (int d = 0; d < 3; d++) ng[d] += 16 * (e->mem[m].ncell / e->mem[m].n[d]);
-    if ((st = dmalloc(&e->FV, 12 * e->ncell)) != HM_OK) goto bad;
+    if ((st = dmalloc(&e->par, 4 * e->ncell)) != HM_OK) goto bad;
     if ((st = dmalloc(&e->Q, 4 * e->ncell)) != HM_OK) goto bad;
-    for (int d = 0; d < 3; d++)
-      if ((st = dmalloc(&e->FVg[d], ng[d])) != HM_OK) goto bad;
-    size_t og[3] = {0, 0, 0};
     std::vector<unsigned> pre(nmembers + 1, 0);
     for (int m = 0; m < nmembers; m++) {
       hm::EnsMember &M = e->mem[m];
-      M.fv = e->FV + 12 * M.off;
+      M.par = e->par + 4 * M.off;
       M.q = e->Q + 4 * M.off;
-      for (int d = 0; d < 3; d++) {
-        M.fvg[d] = e->FVg[d] + og[d];
-        og[d] += 16 * (M.ncell / M.n[d]);
-      }
-      pre[m + 1] = pre[m] + hm::ens_visc_items_per_member(M.n);
+      pre[m + 1] = pre[m] + hm::par_tiles(M.n[0], M.n[1], M.n[2]);
     }
-    e->nitems_v = pre[nmembers];
-    if (hipMalloc((void **)&e->item0v, pre.size() * sizeof(unsigned)) != hipSuccess ||
-        hipMemcpy(e->item0v, pre.data(), pre.size() * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess) {
-      st = fail(HM_ERR_NOMEM, "hm_ensemble_create: viscous work list");
+    e->nptiles = pre[nmembers];
+    if (hipMalloc((void **)&e->ptiles0, pre.size() * sizeof(unsigned)) != hipSuccess ||
+        hipMemcpy(e->ptiles0, pre.data(), pre.size() * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess) {
+      st = fail(HM_ERR_NOMEM, "hm_ensemble_create: viscous-term tile list");
       goto bad;
     }
     for (int m = 0; m < nmembers; m++) {

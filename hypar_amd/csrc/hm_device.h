@@ -249,47 +249,4 @@ __device__ __forceinline__ double d4c(double fm2, double fm1, double fp1, double
   return (fm2 - 8 * fm1 + 8 * fp1 - fp2) * (1.0 / 12.0);
 }
 
-// Viscous flux along DIR at offsets -2,-1,+1,+2 from cell idx along DIR,
-// from FV inside the grid and from the ghost-layer values of FVg where the
-// stencil leaves it.  FV[(DIR*4+c)*ncell + cell]; FVg[(slot*4+c)*nplane +
-// line], slot 0..3 for positions -2,-1,n,n+1 of the line (line = j+ny*k
-// along x, i+nx*k along y, i+nx*j along z).
-template <int DIR>
-__device__ __forceinline__ void visc_load_dir(double F[4][4], const double *__restrict__ FV,
-                                              const double *__restrict__ FVg, size_t ncell, size_t idx, int pos,
-                                              int n, size_t stride, size_t line) {
-  const size_t base = idx - (size_t)pos * stride;
-  const size_t nplane = ncell / n;
-  const int offs[4] = {-2, -1, 1, 2};
-  // each point's source (interior or ghost layer) is a pointer and a
-  // component stride, so that all 16 loads issue without branches
-  const double *src[4];
-  size_t cst[4];
-#pragma unroll
-  for (int s = 0; s < 4; s++) {
-    const int gp = pos + offs[s];
-    const bool in = gp >= 0 && gp < n;
-    const int slot = gp < 0 ? gp + 2 : gp - n + 2;
-    src[s] = in ? FV + (size_t)(DIR * 4) * ncell + base + (size_t)gp * stride
-                : FVg + (size_t)(slot * 4) * nplane + line;
-    cst[s] = in ? ncell : nplane;
-  }
-#pragma unroll
-  for (int s = 0; s < 4; s++)
-#pragma unroll
-    for (int v = 0; v < 4; v++) F[s][v] = src[s][(size_t)v * cst[s]];
-}
-
-// d/dx_DIR of the viscous flux at cell idx, not yet scaled by dxinv:
-// (F(-2) - 8F(-1) + 8F(+1) - F(+2))/12
-template <int DIR>
-__device__ __forceinline__ void visc_div_dir(double out[4], const double *__restrict__ FV,
-                                             const double *__restrict__ FVg, size_t ncell, size_t idx,
-                                             int pos, int n, size_t stride, size_t line) {
-  double F[4][4];
-  visc_load_dir<DIR>(F, FV, FVg, ncell, idx, pos, n, stride, line);
-#pragma unroll
-  for (int v = 0; v < 4; v++) out[v] = d4c(F[0][v], F[1][v], F[2][v], F[3][v]);
-}
-
 }  // namespace hm

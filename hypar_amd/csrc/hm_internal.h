@@ -53,22 +53,12 @@ inline void uniform_dxinv(int n, int g, double *X, double *out) {
   for (int i = n + g; i < n + 2 * g; i++) out[i] = out[n + g - 1];
 }
 
-// interface fluxes along dir of the hm-layout state u into fI (5*ncell); the
-// x pass also stores the primitive variables (u,v,w,T) into Q (4*ncell)
-// when Q is not null
-void launch_iface(const Grid &g, const Scheme &sc, int dir, const double *u, double *fI, double *Q,
-                  hipStream_t st);
-// doubles of the ghost viscous-flux array of direction dir
-size_t visc_ghost_size(const Grid &g, int dir);
-// viscous fluxes (12*ncell) and ghost-layer viscous fluxes from Q
-void launch_visc_flux(const Grid &g, const Scheme &sc, const double *Q, double *FV, double *const FVg[3],
-                      const double *dxg, hipStream_t st);
-// flux differences, viscous divergence and RK4 stage update (mode OUT_STAGE),
-// or the right-hand side / hyperbolic / parabolic term into `out`
-void launch_stage(const Grid &g, const Scheme &sc, const double *fIx, const double *fIy,
-                  const double *fIz, const double *FV, double *const FVg[3], double *u, double *S,
-                  double *acc, double *out, int mode, const double *dxg, int stage, double ca,
-                  double cb, hipStream_t st);
+// the viscous term par (4*ncell: momentum x, y, z, energy) of the state
+// whose primitive variables (u, v, w, T) are in Q (4*ncell)
+void launch_visc_par(const Grid &g, const Scheme &sc, const double *Q, double *par, const double *dxg,
+                     hipStream_t st);
+// par as a five-variable array with a zero mass component
+void launch_par_out(const Grid &g, const double *par, double *out, hipStream_t st);
 unsigned reduction_blocks(size_t n);
 // volume integral partials: part[v*nb + block], 5*nb entries
 void launch_volint(const Grid &g, const double *u, double *part, unsigned nb, const double *dxg, hipStream_t st);
@@ -95,24 +85,23 @@ void launch_fill(double *x, double value, size_t n, hipStream_t st);
 // fused direction passes (hm_pass.hip)
 struct PassBuffers {
   const double *U;        // stage state
-  double *R;              // rhs accumulator, 5*ncell
-  const double *FV;       // viscous fluxes (viscous runs)
-  double *const *FVg;     // ghost-layer viscous fluxes per direction (viscous runs)
+  double *R;              // hyperbolic-term accumulator, 5*ncell
+  const double *par;      // viscous term, 4*ncell (viscous runs; read by the z pass)
   double *u, *S, *acc;    // RK4 state, next stage state, accumulator (z pass, OUT_STAGE)
-  double *out;            // right-hand side (z pass, OUT_RHS)
-  double *Q;              // primitive variables of the new state (z pass, viscous OUT_STAGE)
+  double *out;            // right-hand side or hyperbolic term (z pass, OUT_RHS / OUT_HYP)
+  double *Q;              // u, v, w, T of the new state (z pass, viscous OUT_STAGE)
 };
-// pass along dir: x initialises R, y accumulates, z finishes the stage
-// (mode OUT_STAGE with stage, ca = dt*A[s+1][s], cb = dt*b[s]) or writes
-// the right-hand side (mode OUT_RHS)
+// pass along dir: x stores its flux differences in R, y adds its own, z adds
+// its own and forms rhs = -hyp + par, then finishes the stage (mode
+// OUT_STAGE with stage, ca = dt*A[s+1][s], cb = dt*b[s]) or writes rhs
+// (OUT_RHS) or hyp (OUT_HYP)
 void launch_pass(const Grid &g, const Scheme &sc, int dir, const PassBuffers &b, int mode, int stage, double ca,
                  double cb, const double *dxg, hipStream_t st);
-// primitive variables (u,v,w,T) of u into Q
+// primitive variables (u, v, w, T) of u into Q
 void launch_prims(const Grid &g, const Scheme &sc, const double *u, double *Q, hipStream_t st);
 
 // tiles of the viscous kernels (hm_kernels.hip, hm_adjoint.hip): a block
-// owns VTX x VTY (x, y) columns and marches through VKS z-planes with a
-// 2-cell halo in x and y
+// owns VTX x VTY (x, y) columns and marches through VKS z-planes
 constexpr int VTX = 64, VTY = 8, VKS = 16;
 constexpr int VT = VTX * VTY;  // threads of a viscous block
 constexpr int VHX = VTX + 4, VHY = VTY + 4;
@@ -122,6 +111,19 @@ __host__ __device__ inline unsigned visc_tiles(int nx, int ny, int nz) {
   return (unsigned)(((nx + VTX - 1) / VTX) * ((ny + VTY - 1) / VTY) * ((nz + VKS - 1) / VKS));
 }
 
+// tiles of the viscous-term kernel (hm_kernels.hip): a block owns PTX x PTY
+// (x, y) columns and marches through PKS z-planes
+#ifndef HM_PTX
+#define HM_PTX 32
+#endif
+#ifndef HM_PTY
+#define HM_PTY 16
+#endif
+constexpr int PTX = HM_PTX, PTY = HM_PTY, PKS = 16;
+__host__ __device__ inline unsigned par_tiles(int nx, int ny, int nz) {
+  return (unsigned)(((nx + PTX - 1) / PTX) * ((ny + PTY - 1) / PTY) * ((nz + PKS - 1) / PKS));
+}
+
 // batched ensemble passes (hm_pass.hip): members of one scheme
 struct EnsMember {
   int n[3];
@@ -129,9 +131,9 @@ struct EnsMember {
   size_t off;         // first cell of the member; its state is u[5*off .. 5*(off+ncell))
   const double *dxg;  // ghosted dxinv x|y|z of the member (device)
   double dt;          // the member's time step
-  // viscous members: viscous fluxes (12*ncell), ghost-layer fluxes per
-  // direction, primitive variables (4*ncell); null when inviscid
-  double *fv, *fvg[3], *q;
+  // viscous members: viscous term (4*ncell) and primitive variables
+  // (4*ncell); null when inviscid
+  double *par, *q;
 };
 // work items of one member in pass dir with tile shape `shape` (0: 4
 // wavefronts per block, 1: 2 wavefronts per block)
@@ -142,10 +144,10 @@ void launch_ens_pass(const Scheme &sc, int dir, int shape, const EnsMember *mem,
                      unsigned nitems, unsigned grid, const double *U, double *R, double *u, double *S, double *acc,
                      int stage, hipStream_t st);
 
-// viscous fluxes of all members (hm_kernels.hip): work items per member and
-// one launch over all of them; primitive variables of all members
-unsigned ens_visc_items_per_member(const int n[3]);
-void launch_ens_visc(const Scheme &sc, const EnsMember *mem, int nmem, const unsigned *item0v, unsigned nitems,
+// viscous terms of all members (hm_kernels.hip): one launch over the
+// (member, par_tiles) items, ptiles0 = each member's first item (nmem + 1
+// entries); primitive variables of all members
+void launch_ens_visc(const Scheme &sc, const EnsMember *mem, int nmem, const unsigned *ptiles0, unsigned nitems,
                      hipStream_t st);
 void launch_ens_prims(const Scheme &sc, const EnsMember *mem, int nmem, const double *u, size_t ncell,
                       hipStream_t st);

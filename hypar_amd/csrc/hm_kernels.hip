@@ -1,19 +1,18 @@
-// hm_kernels.hip -- gfx950 kernels of the NS3D WENO5 + RK4 path.
+// hm_kernels.hip -- gfx950 kernels of the NS3D WENO5 + RK4 path besides the
+// direction passes (hm_pass.hip).
 //
 // State layout: structure of arrays without ghosts (see include/hypar_mi355x.h).
 // All boundaries are periodic; stencil indices wrap.
 //
-// Kernels per RK stage:
-//   k_iface<DIR,...>   interface fluxes F_{i-1/2} along DIR for every cell i
-//                      (Roe/Rusanov upwinding of WENO5 reconstructions);
-//                      HyperbolicFunction.c:75-93 with ReconstructHyperbolic
-//   k_visc_flux        viscous fluxes of x, y, z at every cell from the
-//                      primitive variables the x pass stores
-//                      (NavierStokes3DParabolicFunction.c:85-195)
-//   k_visc_ghost_x, k_visc_ghost<DIR>  viscous fluxes at the reference's ghost layers
-//   k_stage            flux differences in x, y, z, the viscous divergence,
-//                      rhs = -hyp + par, and the RK4 stage update in 3N form
-//                      (TimeRHSFunctionExplicit.c:88-93, TimeRK.c:126-195)
+//   k_visc_par, k_ens_visc  the viscous term of every cell from the primitive
+//                           variables the previous z pass stored
+//                           (NavierStokes3DParabolicFunction.c:50-325), with
+//                           the reference's ghost-layer fluxes at the faces
+//   k_par_out               the viscous term as a five-variable array
+//   k_cfl, k_sqdiff, k_volint  reductions: CFL number, step norm, volume
+//                           integrals (NavierStokes3DComputeCFL.c:16-55,
+//                           TimePostStep.c:43-63, VolumeIntegral.c:14-50)
+//   k_aos_to_soa, k_soa_to_aos  layout conversion at the host transfers
 #include <hip/hip_runtime.h>
 
 #include "hm_device.h"
@@ -22,160 +21,13 @@
 namespace hm {
 
 // ---------------------------------------------------------------------------
-// interface flux kernel
-// ---------------------------------------------------------------------------
-template <int DIR, int INTERP, int UPW>
-__device__ __forceinline__ void iface_flux(const double (&U)[6][NV], double gamma, double eps, int weno,
-                                           double fI[NV]) {
-  const double inv_gm1 = 1.0 / (gamma - 1.0);
-  double F[6][NV];
-  double UC4[6];
-  for (int s = 0; s < 6; s++) {
-    Prim q = flowvar(U[s], gamma);
-    euler_flux<DIR>(F[s], q);
-    UC4[s] = modified_energy(q, inv_gm1);
-  }
-  // upwinding states are the cells adjacent to the interface (I-1, I)
-  Prim avg = roe_average(U[2], U[3], gamma);
-
-  if (INTERP == INTERP_CHAR) {
-    Eigen E;
-    eigen<DIR>(E, avg, gamma);
-    double alpha = 0.0;
-    if (UPW == UPWIND_RUSANOV) {
-      Prim l = flowvar(U[2], gamma), r = flowvar(U[3], gamma);
-      double aL = sqrt(gamma * l.P / l.rho) + fabs(vel<DIR>(l));
-      double aR = sqrt(gamma * r.P / r.rho) + fabs(vel<DIR>(r));
-      double aA = sqrt(gamma * avg.P / avg.rho) + fabs(vel<DIR>(avg));
-      alpha = fmax(fmax(aL, aR), fmax(aR, aA));
-    }
-    double g[NV];
-#pragma unroll
-    for (int v = 0; v < NV; v++) {
-      double pf[6], pu[6], pc[6];
-#pragma unroll
-      for (int s = 0; s < 6; s++) {
-        double af = 0.0, au = 0.0;
-#pragma unroll
-        for (int k = 0; k < NV; k++) {
-          af += E.L[v][k] * F[s][k];
-          au += E.L[v][k] * U[s][k];
-        }
-        pf[s] = af;
-        pu[s] = au;
-        pc[s] = au + E.L[v][4] * (UC4[s] - U[s][4]);
-      }
-      double fcL = weno5(pf[0], pf[1], pf[2], pf[3], pf[4], pf[0], pf[1], pf[2], pf[3], pf[4], eps, weno);
-      double fcR = weno5(pf[5], pf[4], pf[3], pf[2], pf[1], pf[5], pf[4], pf[3], pf[2], pf[1], eps, weno);
-      double ucL = weno5(pu[0], pu[1], pu[2], pu[3], pu[4], pc[0], pc[1], pc[2], pc[3], pc[4], eps, weno);
-      double ucR = weno5(pu[5], pu[4], pu[3], pu[2], pu[1], pc[5], pc[4], pc[3], pc[2], pc[1], eps, weno);
-      double lam = (UPW == UPWIND_ROE) ? harten_abs(E.lam[v]) : alpha;
-      g[v] = 0.5 * (fcL + fcR) - lam * (0.5 * (ucR - ucL));
-    }
-#pragma unroll
-    for (int r = 0; r < NV; r++) {
-      double acc = 0.0;
-#pragma unroll
-      for (int c = 0; c < NV; c++) acc += E.R[r][c] * g[c];
-      fI[r] = acc;
-    }
-  } else {
-    double fL[NV], fR[NV], uL[NV], uR[NV];
-#pragma unroll
-    for (int v = 0; v < NV; v++) {
-      double c0 = (v == 4) ? UC4[0] : U[0][v], c1 = (v == 4) ? UC4[1] : U[1][v];
-      double c2 = (v == 4) ? UC4[2] : U[2][v], c3 = (v == 4) ? UC4[3] : U[3][v];
-      double c4 = (v == 4) ? UC4[4] : U[4][v], c5 = (v == 4) ? UC4[5] : U[5][v];
-      fL[v] = weno5(F[0][v], F[1][v], F[2][v], F[3][v], F[4][v], F[0][v], F[1][v], F[2][v], F[3][v], F[4][v], eps, weno);
-      fR[v] = weno5(F[5][v], F[4][v], F[3][v], F[2][v], F[1][v], F[5][v], F[4][v], F[3][v], F[2][v], F[1][v], eps, weno);
-      uL[v] = weno5(U[0][v], U[1][v], U[2][v], U[3][v], U[4][v], c0, c1, c2, c3, c4, eps, weno);
-      uR[v] = weno5(U[5][v], U[4][v], U[3][v], U[2][v], U[1][v], c5, c4, c3, c2, c1, eps, weno);
-    }
-    if (UPW == UPWIND_RUSANOV) {
-      Prim l = flowvar(U[2], gamma), r = flowvar(U[3], gamma);
-      double aL = sqrt(gamma * l.P / l.rho) + fabs(vel<DIR>(l));
-      double aR = sqrt(gamma * r.P / r.rho) + fabs(vel<DIR>(r));
-      double aA = sqrt(gamma * avg.P / avg.rho) + fabs(vel<DIR>(avg));
-      double alpha = fmax(fmax(aL, aR), fmax(aR, aA));
-#pragma unroll
-      for (int v = 0; v < NV; v++) fI[v] = 0.5 * (fL[v] + fR[v]) - alpha * (0.5 * (uR[v] - uL[v]));
-    } else {
-      Eigen E;
-      eigen<DIR>(E, avg, gamma);
-      double c[NV];
-#pragma unroll
-      for (int r = 0; r < NV; r++) {
-        double acc = 0.0;
-#pragma unroll
-        for (int k = 0; k < NV; k++) acc += E.L[r][k] * (0.5 * (uR[k] - uL[k]));
-        c[r] = harten_abs(E.lam[r]) * acc;
-      }
-#pragma unroll
-      for (int r = 0; r < NV; r++) {
-        double acc = 0.0;
-#pragma unroll
-        for (int k = 0; k < NV; k++) acc += E.R[r][k] * c[k];
-        fI[r] = 0.5 * (fL[r] + fR[r]) - acc;
-      }
-    }
-  }
-}
-
-// one thread per interface i-1/2 (stored at the index of cell i).  The
-// x-direction pass also stores the primitive variables (u, v, w, T) of its
-// cell for the viscous kernels (Q != nullptr).
-template <int DIR, int INTERP, int UPW>
-__global__ void __launch_bounds__(256) k_iface(const double *__restrict__ u, double *__restrict__ fI,
-                                               double *__restrict__ Q, int nx, int ny, int nz,
-                                               double gamma, double eps, int weno) {
-  const size_t ncell = (size_t)nx * ny * nz;
-  const size_t idx = xcd_block() * blockDim.x + threadIdx.x;
-  if (idx >= ncell) return;
-  const int i = (int)(idx % nx);
-  const int j = (int)((idx / nx) % ny);
-  const int k = (int)(idx / ((size_t)nx * ny));
-  const int n = DIR == 0 ? nx : (DIR == 1 ? ny : nz);
-  const int pos = DIR == 0 ? i : (DIR == 1 ? j : k);
-  const size_t stride = DIR == 0 ? 1 : (DIR == 1 ? (size_t)nx : (size_t)nx * ny);
-  const size_t base = idx - (size_t)pos * stride;
-  double U[6][NV];
-#pragma unroll
-  for (int s = 0; s < 6; s++) {
-    const size_t c = base + (size_t)wrap(pos + s - 3, n) * stride;
-#pragma unroll
-    for (int v = 0; v < NV; v++) U[s][v] = u[(size_t)v * ncell + c];
-  }
-  if (Q) {
-    Prim q = flowvar(U[3], gamma);
-    Q[idx] = q.vx;
-    Q[ncell + idx] = q.vy;
-    Q[2 * ncell + idx] = q.vz;
-    Q[3 * ncell + idx] = gamma * q.P / q.rho;
-  }
-  double f[NV];
-  iface_flux<DIR, INTERP, UPW>(U, gamma, eps, weno, f);
-#pragma unroll
-  for (int v = 0; v < NV; v++) fI[(size_t)v * ncell + idx] = f[v];
-}
-
-// ---------------------------------------------------------------------------
 // viscous terms
 // ---------------------------------------------------------------------------
 // Primitive variables used by the viscous term: (u, v, w, T) with
-// T = gamma*P/rho (NavierStokes3DParabolicFunction.c:85-108), stored as
-// four arrays Q[c*ncell + cell].
+// T = gamma*P/rho (NavierStokes3DParabolicFunction.c:85-108).
 struct QT {
   double u, v, w, T;
 };
-
-__device__ __forceinline__ QT load_q(const double *__restrict__ Q, size_t ncell, size_t c) {
-  QT q;
-  q.u = Q[c];
-  q.v = Q[ncell + c];
-  q.w = Q[2 * ncell + c];
-  q.T = Q[3 * ncell + c];
-  return q;
-}
 
 struct ViscParams {
   double gamma, inv_Re, inv_gm1, inv_Pr;
@@ -214,270 +66,369 @@ __device__ __forceinline__ void visc_flux(double fv[4], const QT &q, const doubl
   fv[3] = q.u * t1 + q.v * t2 + q.w * t3 + qd;
 }
 
-// Viscous fluxes of x, y, z at every cell: FV[(d*4+c)*ncell + cell],
-// c = momentum x, y, z, energy.  Gradients use the 4th-order central
-// difference with periodic neighbours, scaled by the cell's dxinv.
+// Viscous term of the right-hand side (NavierStokes3DParabolicFunction.c:50-325):
+//   par = (dxinv Dx FVx + dyinv Dy FVy) + dzinv Dz FVz,
+// summed in the reference's order, with FVd the viscous flux along d of the
+// primitive variables (u, v, w, T) and their gradient (4th-order central
+// differences with periodic neighbours, scaled by the point's dxinv;
+// FirstDerivativeFourthOrder.c:103) and Dd the same central difference of
+// the flux.  par[c*ncell + cell], c = momentum x, y, z, energy (the mass
+// component is zero).  Q holds u, v, w, T of every cell, Q[c*ncell + cell];
+// the viscosity is mu = T^0.76 = exp(0.76 log T) (raiseto,
+// include/math_ops.h).
 //
-// A block owns a VTX x VTY tile of (x, y) columns and marches through VKS
-// z-planes.  Each thread keeps its column's (u, v, w, T) at k-2..k+2 in
-// registers (the z-derivative stencil, one new plane loaded per step); the
-// tile's plane k with a 2-cell halo in x and y is staged in LDS for the x-
-// and y-derivatives.  Every plane of Q is thus read from HBM about once per
-// tile, instead of once per stencil point.
-// (tile constants VTX, VTY, VKS and visc_tiles: hm_internal.h)
+// Where the stencil of Dd leaves the grid (positions -2, -1, n, n+1 along
+// d) the reference reads the fluxes of its ghost layers: the primitive
+// values there are the periodic copies, the derivative along d uses the
+// biased stencils of FirstDerivativeFourthOrder.c:74-93/106-125 in the outer
+// layer and the central one in the inner layer, scaled by the ghost dxinv
+// (a copy of the nearest interior value, InitialSolution.c:101-113), and the
+// derivatives across d are zero, because the reference computes those only
+// on lines whose other indices are interior.  The kernel evaluates these
+// ghost-layer fluxes where its stencils need them.
+//
+// A block owns a TX x TY tile of (x, y) columns and marches through the
+// planes of one z segment [k0, k1) (PKS planes).  Plane q of Q with a
+// 4-cell halo in x and y is staged in LDS.  Every thread evaluates the
+// gradient at its own column and at most one halo point (the rows y0-2,
+// y0-1, y0+TY, y0+TY+1 and the columns x0-2, x0-1, x0+TX, x0+TX+1 of the
+// tile), taking the z derivative from a ring of that point's values on
+// planes q-2..q+2 held in registers (in a thread-private part of LDS for
+// the halo point).  The next plane's loads are issued before the current
+// plane's arithmetic and stored to LDS after it.  The x and y fluxes of plane q go to
+// LDS, where their x and y differences are formed.  The z fluxes of the
+// column stay in a five-plane register ring, so the term of plane q-2 is
+// complete once plane q is done.  The z differences of the segment need FVz
+// on planes k0-2..k1+1, so the march covers them; on the ghost planes (-2,
+// -1, nz, nz+1) the z flux comes from direct loads of the column.
+template <int TX, int TY>
+struct ViscParShared {
+  double q[4][TY + 8][TX + 8];  // plane q of Q: x0-4..x0+TX+3, y0-4..y0+TY+3
+  double fx[4][TY][TX + 4];     // x fluxes of the tile's rows at x0-2..x0+TX+1
+  double fy[4][TY + 4][TX];     // y fluxes of the tile's columns at y0-2..y0+TY+1
+  // z ring of each halo point, planes by p mod 5, and the x + y part of
+  // the own column's term on planes q-2..q, by p mod 3 (private to the thread)
+  double hr[5][4][4 * TX + 4 * TY];
+  double pxy[3][4][TX * TY];
+};
 
-// tile b of the grid; every thread of the block takes part
-__device__ __forceinline__ void visc_flux_tile(const double *__restrict__ Q, double *__restrict__ FV, int nx,
-                                               int ny, int nz,
-                                               const double *__restrict__ dxg,  // ghosted dxinv x|y|z
-                                               const ViscParams &vp, int b, double (&sQ)[4][VHY][VHX]) {
-  const size_t ncell = (size_t)nx * ny * nz;
-  const size_t plane = (size_t)nx * ny;
-  const int ntx = (nx + VTX - 1) / VTX, nty = (ny + VTY - 1) / VTY;
-  const int tx = b % ntx, ty = (b / ntx) % nty, kz = b / (ntx * nty);
-  const int x0 = tx * VTX, y0 = ty * VTY, k0 = kz * VKS;
-  const int lx = threadIdx.x % VTX, ly = threadIdx.x / VTX;
-  const int i = x0 + lx, j = y0 + ly;
-  const bool own = i < nx && j < ny;
-  const int ic = own ? i : 0, jc = own ? j : 0;
-  const size_t col = (size_t)jc * nx + ic;
-  const double dxi = dxg[3 + ic], dyi = dxg[(nx + 6) + 3 + jc];
-  const double *dzi = dxg + (nx + 6) + (ny + 6) + 3;
-  const int k1 = min(k0 + VKS, nz);
-  // window: planes k-2..k+2 of this column
-  QT q[5];
-#pragma unroll
-  for (int s = 0; s < 4; s++) q[s + 1] = load_q(Q, ncell, col + (size_t)wrap(k0 - 2 + s, nz) * plane);
-  for (int k = k0; k < k1; k++) {
-#pragma unroll
-    for (int s = 0; s < 4; s++) q[s] = q[s + 1];
-    q[4] = load_q(Q, ncell, col + (size_t)wrap(k + 2, nz) * plane);
-    // tile of plane k with halo
-    const size_t pk = (size_t)k * plane;
-    for (int c = threadIdx.x; c < VHX * VHY; c += VTX * VTY) {
-      const int hx = c % VHX, hy = c / VHX;
-      // x0 - 2 + hx spans [x0 - 2, x0 + VTX + 1] and y0 - 2 + hy spans
-      // [y0 - 2, y0 + VTY + 1], which can exceed 2 nx and 2 ny
-      const int gi = (x0 - 2 + hx + 2 * nx) % nx, gj = (y0 - 2 + hy + 2 * ny) % ny;
-      const QT h = load_q(Q, ncell, pk + (size_t)gj * nx + gi);
-      sQ[0][hy][hx] = h.u;
-      sQ[1][hy][hx] = h.v;
-      sQ[2][hy][hx] = h.w;
-      sQ[3][hy][hx] = h.T;
-    }
-    __syncthreads();
-    if (own) {
-      double gr[4][3];
-      const double dzk = dzi[k];
-#pragma unroll
-      for (int c = 0; c < 4; c++) {
-        const double *r = &sQ[c][ly + 2][0];
-        gr[c][0] = d4c(r[lx], r[lx + 1], r[lx + 3], r[lx + 4]) * dxi;
-        gr[c][1] = d4c(sQ[c][ly][lx + 2], sQ[c][ly + 1][lx + 2], sQ[c][ly + 3][lx + 2], sQ[c][ly + 4][lx + 2]) * dyi;
-      }
-      gr[0][2] = d4c(q[0].u, q[1].u, q[3].u, q[4].u) * dzk;
-      gr[1][2] = d4c(q[0].v, q[1].v, q[3].v, q[4].v) * dzk;
-      gr[2][2] = d4c(q[0].w, q[1].w, q[3].w, q[4].w) * dzk;
-      gr[3][2] = d4c(q[0].T, q[1].T, q[3].T, q[4].T) * dzk;
-      const QT &q0 = q[2];
-      const double mu = exp(0.76 * log(q0.T));
-      const size_t idx = pk + col;
-      double fv[4];
-      visc_flux<0>(fv, q0, gr, mu, vp);
-      for (int c = 0; c < 4; c++) FV[(size_t)(0 * 4 + c) * ncell + idx] = fv[c];
-      visc_flux<1>(fv, q0, gr, mu, vp);
-      for (int c = 0; c < 4; c++) FV[(size_t)(1 * 4 + c) * ncell + idx] = fv[c];
-      visc_flux<2>(fv, q0, gr, mu, vp);
-      for (int c = 0; c < 4; c++) FV[(size_t)(2 * 4 + c) * ncell + idx] = fv[c];
-    }
-    __syncthreads();
-  }
+__device__ __forceinline__ int wrapi(int p, int n) {
+  const int r = p % n;
+  return r < 0 ? r + n : r;
 }
 
-__global__ void __launch_bounds__(VTX * VTY) k_visc_flux(const double *__restrict__ Q, double *__restrict__ FV,
-                                                         int nx, int ny, int nz, const double *__restrict__ dxg,
-                                                         ViscParams vp) {
-  __shared__ double sQ[4][VHY][VHX];
-  visc_flux_tile(Q, FV, nx, ny, nz, dxg, vp, blockIdx.x, sQ);
-}
-
-// Viscous flux along DIR at the ghost positions gp in {-2,-1,n,n+1} of every
-// line along DIR (other indices interior), as the reference evaluates it
-// from its ghost layers: primitive values are the periodic copies, the
-// derivative along DIR uses the biased stencils of
-// FirstDerivativeFourthOrder.c:74-93/106-125 in the second-outermost layer
-// and the central one otherwise, scaled by the ghost dxinv (a copy of the
-// nearest interior value, InitialSolution.c:101-113), and the derivatives
-// across DIR are zero because the reference computes them only on lines whose
-// other indices are interior.
-// Output FVg[(slot*4 + c)*nplane + line], slot 0..3 for gp = -2,-1,n,n+1.
-template <int DIR>
-__device__ __forceinline__ void visc_ghost_one(const double *__restrict__ Q, double *__restrict__ FVg, int nx,
-                                               int ny, int nz, const double *__restrict__ dxg, const ViscParams &vp,
-                                               size_t t) {
-  const size_t ncell = (size_t)nx * ny * nz;
-  const int n = DIR == 0 ? nx : (DIR == 1 ? ny : nz);
-  const size_t nplane = ncell / n;
-  if (t >= 4 * nplane) return;
-  const int slot = (int)(t / nplane);
-  const size_t line = t % nplane;
-  size_t base, stride;
-  if (DIR == 0) { base = line * nx; stride = 1; }                          // line = j + ny*k
-  else if (DIR == 1) { base = (line % nx) + (line / nx) * (size_t)nx * ny; stride = nx; }  // i + nx*k
-  else { base = line; stride = (size_t)nx * ny; }                           // i + nx*j
-  const int gp = slot < 2 ? slot - 2 : n + slot - 2;
-  const double *dxd = dxg + (DIR >= 1 ? nx + 6 : 0) + (DIR == 2 ? ny + 6 : 0);
-  QT f[7];  // positions gp-3 .. gp+3 (periodic copies)
-#pragma unroll
-  for (int s = 0; s < 7; s++) f[s] = load_q(Q, ncell, base + (size_t)wrap(gp - 3 + s, n) * stride);
-  double d[4];
+// d/dx_d of one primitive at the ghost position of slot s (0..3 for the
+// positions -2, -1, n, n+1) from f(o), its values at offsets o from that
+// position along d (periodic copies); not yet scaled by dxinv.  Only the
+// offsets a slot's stencil uses are read: -1..3 (slot 0), -3..1 (slot 3),
+// -2..2 (slots 1, 2).
+template <class F>
+__device__ __forceinline__ double ghost_deriv(int slot, const F &f) {
   const double r12 = 1.0 / 12.0;
-  if (gp == -2) {
-    // (-3 f(-3) - 10 f(-2) + 18 f(-1) - 6 f(0) + f(1)) / 12
-    d[0] = (-3 * f[2].u - 10 * f[3].u + 18 * f[4].u - 6 * f[5].u + f[6].u) * r12;
-    d[1] = (-3 * f[2].v - 10 * f[3].v + 18 * f[4].v - 6 * f[5].v + f[6].v) * r12;
-    d[2] = (-3 * f[2].w - 10 * f[3].w + 18 * f[4].w - 6 * f[5].w + f[6].w) * r12;
-    d[3] = (-3 * f[2].T - 10 * f[3].T + 18 * f[4].T - 6 * f[5].T + f[6].T) * r12;
-  } else if (gp == n + 1) {
-    // (-f(n-2) + 6 f(n-1) - 18 f(n) + 10 f(n+1) + 3 f(n+2)) / 12
-    d[0] = (-f[0].u + 6 * f[1].u - 18 * f[2].u + 10 * f[3].u + 3 * f[4].u) * r12;
-    d[1] = (-f[0].v + 6 * f[1].v - 18 * f[2].v + 10 * f[3].v + 3 * f[4].v) * r12;
-    d[2] = (-f[0].w + 6 * f[1].w - 18 * f[2].w + 10 * f[3].w + 3 * f[4].w) * r12;
-    d[3] = (-f[0].T + 6 * f[1].T - 18 * f[2].T + 10 * f[3].T + 3 * f[4].T) * r12;
-  } else {
-    d[0] = d4c(f[1].u, f[2].u, f[4].u, f[5].u);
-    d[1] = d4c(f[1].v, f[2].v, f[4].v, f[5].v);
-    d[2] = d4c(f[1].w, f[2].w, f[4].w, f[5].w);
-    d[3] = d4c(f[1].T, f[2].T, f[4].T, f[5].T);
-  }
-  const double dxgh = dxd[3 + gp];  // entry 0 is position -3
+  if (slot == 0) return (-3 * f(-1) - 10 * f(0) + 18 * f(1) - 6 * f(2) + f(3)) * r12;
+  if (slot == 3) return (-f(-3) + 6 * f(-2) - 18 * f(-1) + 10 * f(0) + 3 * f(1)) * r12;
+  return d4c(f(-2), f(-1), f(1), f(2));
+}
+
+// Viscous flux along DIR at a ghost position: derivative along DIR only
+// (d[c], already scaled), primitive values q.
+template <int DIR>
+__device__ __forceinline__ void ghost_flux(double fv[4], const QT &q, const double (&d)[4], const ViscParams &vp) {
+  const double mu = exp(0.76 * log(q.T));
   double gr[4][3];
 #pragma unroll
   for (int c = 0; c < 4; c++)
 #pragma unroll
-    for (int e = 0; e < 3; e++) gr[c][e] = (e == DIR) ? d[c] * dxgh : 0.0;
-  const QT &q = f[3];
-  const double mu = exp(0.76 * log(q.T));
-  double fv[4];
+    for (int e = 0; e < 3; e++) gr[c][e] = e == DIR ? d[c] : 0.0;
   visc_flux<DIR>(fv, q, gr, mu, vp);
-#pragma unroll
-  for (int c = 0; c < 4; c++) FVg[(size_t)(slot * 4 + c) * nplane + line] = fv[c];
 }
 
-// The four ghost-layer fluxes of x-line `line` (line = j + ny*k) from one
-// thread.  Slot s (positions -2, -1, n, n+1) reads the seven cells
-// gp-3 .. gp+3, periodic; together the four read the ten cells n-5 .. n-1,
-// 0 .. 4, loaded once, with slot s taking window entries s .. s+6.  The x
-// lines lie along the contiguous direction, so the one-thread-per-(slot,
-// line) form of visc_ghost_one reads each cell's cache line from four
-// threads far apart in the grid.
-__device__ __forceinline__ void visc_ghost_xline(const double *__restrict__ Q, double *__restrict__ FVg, int nx,
-                                                 int ny, int nz, const double *__restrict__ dxg,
-                                                 const ViscParams &vp, size_t line) {
+// tile b of the grid; every thread of the block takes part
+template <int TX, int TY>
+__device__ __forceinline__ void visc_par_tile(const double *__restrict__ Q, double *__restrict__ par, int nx,
+                                              int ny, int nz,
+                                              const double *__restrict__ dxg,  // ghosted dxinv x|y|z
+                                              const ViscParams &vp, int b, ViscParShared<TX, TY> &sh) {
+  constexpr int NT = TX * TY;
+  static_assert(4 * TX + 4 * TY <= NT, "every halo point needs a thread");
   const size_t ncell = (size_t)nx * ny * nz;
-  const size_t nplane = ncell / nx;
-  if (line >= nplane) return;
-  const size_t base = line * nx;
-  QT f[10];  // cells n-5 .. n-1, 0 .. 4 (n >= 5)
-#pragma unroll
-  for (int j = 0; j < 5; j++) {
-    f[j] = load_q(Q, ncell, base + (size_t)(nx - 5 + j));
-    f[5 + j] = load_q(Q, ncell, base + (size_t)j);
+  const size_t plane = (size_t)nx * ny;
+  const int ntx = (nx + TX - 1) / TX, nty = (ny + TY - 1) / TY;
+  const int tx = b % ntx, ty = (b / ntx) % nty, kz = b / (ntx * nty);
+  const int x0 = tx * TX, y0 = ty * TY, k0 = kz * PKS;
+  const int k1 = min(k0 + PKS, nz);
+  const double *dxa = dxg, *dya = dxg + (nx + 6), *dza = dxg + (nx + 6) + (ny + 6);  // entry 3+p: position p
+  const int t = threadIdx.x;
+  const int lx = t % TX, ly = t / TX;
+  const int i = x0 + lx, j = y0 + ly;
+  const bool own = i < nx && j < ny;
+  const size_t col = (size_t)wrapi(j, ny) * nx + wrapi(i, nx);
+  // halo point: y fluxes of rows y0-2, y0-1, y0+TY, y0+TY+1 (threads
+  // 0..4*TX-1), x fluxes of columns x0-2, x0-1, x0+TX, x0+TX+1 (the next
+  // 4*TY threads)
+  int hkind = -1, hx = 0, hy = 0;
+  if (t < 4 * TX) {
+    const int r = t / TX;
+    hkind = 1;
+    hx = t % TX;
+    hy = r < 2 ? r - 2 : TY + r - 2;
+  } else if (t < 4 * TX + 4 * TY) {
+    const int u = t - 4 * TX, r = u % 4;
+    hkind = 0;
+    hy = u / 4;
+    hx = r < 2 ? r - 2 : TX + r - 2;
   }
-  const double r12 = 1.0 / 12.0;
+  const int hi = x0 + hx, hj = y0 + hy;
+  const size_t hcol = (size_t)wrapi(hj, ny) * nx + wrapi(hi, nx);
+  // does the halo point carry a flux the tile needs, and is it interior?
+  const bool hneed = hkind == 1 ? (hi < nx && hj >= -2 && hj < ny + 2)
+                                : (hkind == 0 && hj < ny && hi >= -2 && hi < nx + 2);
+  const bool hint = hkind == 1 ? (hj >= 0 && hj < ny) : (hi >= 0 && hi < nx);
+
+  auto ldq = [&](size_t c, int k, double (&r)[4]) {
+    const size_t o = (size_t)wrapi(k, nz) * plane + c;
 #pragma unroll
-  for (int slot = 0; slot < 4; slot++) {
-    const QT *w = f + slot;  // positions gp-3 .. gp+3
-    const int gp = slot < 2 ? slot - 2 : nx + slot - 2;
-    double d[4];
-    if (slot == 0) {
-      // (-3 f(-3) - 10 f(-2) + 18 f(-1) - 6 f(0) + f(1)) / 12
-      d[0] = (-3 * w[2].u - 10 * w[3].u + 18 * w[4].u - 6 * w[5].u + w[6].u) * r12;
-      d[1] = (-3 * w[2].v - 10 * w[3].v + 18 * w[4].v - 6 * w[5].v + w[6].v) * r12;
-      d[2] = (-3 * w[2].w - 10 * w[3].w + 18 * w[4].w - 6 * w[5].w + w[6].w) * r12;
-      d[3] = (-3 * w[2].T - 10 * w[3].T + 18 * w[4].T - 6 * w[5].T + w[6].T) * r12;
-    } else if (slot == 3) {
-      // (-f(n-2) + 6 f(n-1) - 18 f(n) + 10 f(n+1) + 3 f(n+2)) / 12
-      d[0] = (-w[0].u + 6 * w[1].u - 18 * w[2].u + 10 * w[3].u + 3 * w[4].u) * r12;
-      d[1] = (-w[0].v + 6 * w[1].v - 18 * w[2].v + 10 * w[3].v + 3 * w[4].v) * r12;
-      d[2] = (-w[0].w + 6 * w[1].w - 18 * w[2].w + 10 * w[3].w + 3 * w[4].w) * r12;
-      d[3] = (-w[0].T + 6 * w[1].T - 18 * w[2].T + 10 * w[3].T + 3 * w[4].T) * r12;
-    } else {
-      d[0] = d4c(w[1].u, w[2].u, w[4].u, w[5].u);
-      d[1] = d4c(w[1].v, w[2].v, w[4].v, w[5].v);
-      d[2] = d4c(w[1].w, w[2].w, w[4].w, w[5].w);
-      d[3] = d4c(w[1].T, w[2].T, w[4].T, w[5].T);
+    for (int v = 0; v < 4; v++) r[v] = Q[(size_t)v * ncell + o];
+  };
+  // Q on LDS at tile offsets (ox, oy) (-4 <= ox < TX + 4, -4 <= oy < TY + 4)
+  auto sq = [&](int v, int oy, int ox) { return sh.q[v][oy + 4][ox + 4]; };
+  auto qat = [&](int ox, int oy) {
+    QT q;
+    q.u = sq(0, oy, ox);
+    q.v = sq(1, oy, ox);
+    q.w = sq(2, oy, ox);
+    q.T = sq(3, oy, ox);
+    return q;
+  };
+  // gradient at tile offset (ox, oy) of interior plane q; zr(s, v) is
+  // component v on plane q-2+s of the point
+  auto grad = [&](int ox, int oy, int q, const auto &zr, double (&gr)[4][3]) {
+    const double dxi = dxa[3 + x0 + ox], dyi = dya[3 + y0 + oy], dzi = dza[3 + q];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      gr[v][0] = d4c(sq(v, oy, ox - 2), sq(v, oy, ox - 1), sq(v, oy, ox + 1), sq(v, oy, ox + 2)) * dxi;
+      gr[v][1] = d4c(sq(v, oy - 2, ox), sq(v, oy - 1, ox), sq(v, oy + 1, ox), sq(v, oy + 2, ox)) * dyi;
+      gr[v][2] = d4c(zr(0, v), zr(1, v), zr(3, v), zr(4, v)) * dzi;
     }
-    const double dxgh = dxg[3 + gp];  // entry 0 is position -3
-    double gr[4][3];
+  };
+  // ghost-layer x / y flux at tile offset (ox, oy)
+  auto ghost_x = [&](int ox, int oy) {
+    const int gx = x0 + ox, slot = gx < 0 ? gx + 2 : gx - nx + 2;
+    double d[4];
 #pragma unroll
-    for (int c = 0; c < 4; c++)
-#pragma unroll
-      for (int e = 0; e < 3; e++) gr[c][e] = e == 0 ? d[c] * dxgh : 0.0;
-    const QT &q = w[3];
-    const double mu = exp(0.76 * log(q.T));
+    for (int v = 0; v < 4; v++) d[v] = ghost_deriv(slot, [&](int o) { return sq(v, oy, ox + o); }) * dxa[3 + gx];
     double fv[4];
-    visc_flux<0>(fv, q, gr, mu, vp);
+    ghost_flux<0>(fv, qat(ox, oy), d, vp);
 #pragma unroll
-    for (int c = 0; c < 4; c++) FVg[(size_t)(slot * 4 + c) * nplane + line] = fv[c];
+    for (int v = 0; v < 4; v++) sh.fx[v][oy][ox + 2] = fv[v];
+  };
+  auto ghost_y = [&](int ox, int oy) {
+    const int gy = y0 + oy, slot = gy < 0 ? gy + 2 : gy - ny + 2;
+    double d[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) d[v] = ghost_deriv(slot, [&](int o) { return sq(v, oy + o, ox); }) * dya[3 + gy];
+    double fv[4];
+    ghost_flux<1>(fv, qat(ox, oy), d, vp);
+#pragma unroll
+    for (int v = 0; v < 4; v++) sh.fy[v][oy + 2][ox] = fv[v];
+  };
+
+  // staging of plane q into LDS: every thread takes NR points of the
+  // (TX + 8) x (TY + 8) region; the loads are issued a plane ahead into
+  // registers and stored once the previous plane's reads are done
+  constexpr int NQP = (TX + 8) * (TY + 8), NR = (NQP + NT - 1) / NT;
+  double pre[NR][4];
+  auto fetch_plane = [&](int q) {
+    const size_t pk = (size_t)q * plane;
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      const int c = t + r * NT;
+      if (c < NQP) {
+        const int ox = c % (TX + 8), oy = c / (TX + 8);
+        const size_t o = pk + (size_t)wrapi(y0 - 4 + oy, ny) * nx + wrapi(x0 - 4 + ox, nx);
+#pragma unroll
+        for (int v = 0; v < 4; v++) pre[r][v] = Q[(size_t)v * ncell + o];
+      }
+    }
+  };
+  auto store_plane = [&]() {
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      const int c = t + r * NT;
+      if (c < NQP) {
+        const int ox = c % (TX + 8), oy = c / (TX + 8);
+#pragma unroll
+        for (int v = 0; v < 4; v++) sh.q[v][oy][ox] = pre[r][v];
+      }
+    }
+  };
+
+  // rings of planes q-2..q+2 of the own column (registers) and the halo
+  // point (LDS, plane p in slot p mod 5), and the next plane of each, loaded
+  // a step ahead
+  double rq[5][4], nq[4], nh[4];
+  double fz[5][4];
+  auto hslot = [&](int p, int v) -> double & { return sh.hr[wrapi(p, 5)][v][t]; };
+#pragma unroll
+  for (int s = 0; s < 5; s++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) rq[s][v] = fz[s][v] = 0.0;
+#pragma unroll
+  for (int v = 0; v < 4; v++) nh[v] = 0.0;
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    ldq(col, k0 - 4 + s, rq[s + 1]);
+    if (hkind >= 0) {
+      double r[4];
+      ldq(hcol, k0 - 4 + s, r);
+#pragma unroll
+      for (int v = 0; v < 4; v++) hslot(k0 - 4 + s, v) = r[v];
+    }
+  }
+  ldq(col, k0, nq);
+  if (hkind >= 0) ldq(hcol, k0, nh);
+  if (k0 - 2 >= 0) {  // the first plane of the march, unless a ghost plane
+    fetch_plane(k0 - 2);
+    store_plane();
+  }
+  for (int q = k0 - 2; q < k1 + 2; q++) {
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) rq[s][v] = rq[s + 1][v];
+#pragma unroll
+    for (int v = 0; v < 4; v++) rq[4][v] = nq[v];
+    if (hkind >= 0)
+#pragma unroll
+      for (int v = 0; v < 4; v++) hslot(q + 2, v) = nh[v];
+    const bool gz = q < 0 || q >= nz;     // a ghost plane along z (uniform)
+    const bool outq = q >= k0 && q < k1;  // a plane whose x and y terms are formed
+    const bool next = q + 1 < k1 + 2 && q + 1 >= 0 && q + 1 < nz;  // plane q+1 is staged
+    __syncthreads();
+    if (next) fetch_plane(q + 1);
+    if (q + 1 < k1 + 2) {
+      ldq(col, q + 3, nq);
+      if (hkind >= 0) ldq(hcol, q + 3, nh);
+    }
+    double fzq[4] = {0.0, 0.0, 0.0, 0.0};
+    if (gz) {
+      if (own) {
+        const int slot = q < 0 ? q + 2 : q - nz + 2;
+        double d[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+          d[v] = ghost_deriv(slot, [&](int o) { return Q[(size_t)v * ncell + (size_t)wrapi(q + o, nz) * plane + col]; }) *
+                 dza[3 + q];
+        const size_t o = (size_t)wrapi(q, nz) * plane + col;
+        QT qv;
+        qv.u = Q[o];
+        qv.v = Q[ncell + o];
+        qv.w = Q[2 * ncell + o];
+        qv.T = Q[3 * ncell + o];
+        ghost_flux<2>(fzq, qv, d, vp);
+      }
+    } else {
+      if (own) {
+        double gr[4][3];
+        grad(lx, ly, q, [&](int s2, int v) { return rq[s2][v]; }, gr);
+        const QT qv = qat(lx, ly);
+        const double mu = exp(0.76 * log(qv.T));
+        visc_flux<2>(fzq, qv, gr, mu, vp);
+        if (outq) {
+          double fv[4];
+          visc_flux<0>(fv, qv, gr, mu, vp);
+#pragma unroll
+          for (int v = 0; v < 4; v++) sh.fx[v][ly][lx + 2] = fv[v];
+          visc_flux<1>(fv, qv, gr, mu, vp);
+#pragma unroll
+          for (int v = 0; v < 4; v++) sh.fy[v][ly + 2][lx] = fv[v];
+        }
+      } else if (outq) {
+        // the column lies past the grid: ghost-layer x flux at i = nx, nx+1
+        // (rows inside), ghost-layer y flux at j = ny, ny+1 (columns inside)
+        if (j < ny && i < nx + 2) ghost_x(lx, ly);
+        if (i < nx && j < ny + 2) ghost_y(lx, ly);
+      }
+      if (outq && hneed) {
+        if (hint) {
+          double gr[4][3];
+          grad(hx, hy, q, [&](int s2, int v) { return hslot(q - 2 + s2, v); }, gr);
+          const QT qv = qat(hx, hy);
+          const double mu = exp(0.76 * log(qv.T));
+          double fv[4];
+          if (hkind == 0) {
+            visc_flux<0>(fv, qv, gr, mu, vp);
+#pragma unroll
+            for (int v = 0; v < 4; v++) sh.fx[v][hy][hx + 2] = fv[v];
+          } else {
+            visc_flux<1>(fv, qv, gr, mu, vp);
+#pragma unroll
+            for (int v = 0; v < 4; v++) sh.fy[v][hy + 2][hx] = fv[v];
+          }
+        } else if (hkind == 0) {
+          ghost_x(hx, hy);
+        } else {
+          ghost_y(hx, hy);
+        }
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) fz[s][v] = fz[s + 1][v];
+#pragma unroll
+    for (int v = 0; v < 4; v++) fz[4][v] = fzq[v];
+    __syncthreads();
+    if (next) store_plane();  // nothing reads plane q from LDS any more
+    if (outq && own) {
+      const double dxi = dxa[3 + i], dyi = dya[3 + j];
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const double px = dxi * d4c(sh.fx[v][ly][lx], sh.fx[v][ly][lx + 1], sh.fx[v][ly][lx + 3], sh.fx[v][ly][lx + 4]);
+        const double py = dyi * d4c(sh.fy[v][ly][lx], sh.fy[v][ly + 1][lx], sh.fy[v][ly + 3][lx], sh.fy[v][ly + 4][lx]);
+        sh.pxy[wrapi(q, 3)][v][t] = px + py;
+      }
+    }
+    const int p = q - 2;  // plane whose z difference is now complete
+    if (p >= k0 && own) {
+      const double dzi = dza[3 + p];
+      const size_t o = (size_t)p * plane + col;
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        par[(size_t)v * ncell + o] = sh.pxy[wrapi(p, 3)][v][t] + dzi * d4c(fz[0][v], fz[1][v], fz[3][v], fz[4][v]);
+    }
   }
 }
 
-__global__ void __launch_bounds__(256) k_visc_ghost_x(const double *__restrict__ Q, double *__restrict__ FVg, int nx,
-                                                      int ny, int nz, const double *__restrict__ dxg,
-                                                      ViscParams vp) {
-  visc_ghost_xline(Q, FVg, nx, ny, nz, dxg, vp, (size_t)blockIdx.x * blockDim.x + threadIdx.x);
+__global__ void __launch_bounds__(PTX * PTY) k_visc_par(const double *__restrict__ Q, double *__restrict__ par,
+                                                         int nx, int ny, int nz, const double *__restrict__ dxg,
+                                                         ViscParams vp) {
+  __shared__ ViscParShared<PTX, PTY> sh;
+  visc_par_tile<PTX, PTY>(Q, par, nx, ny, nz, dxg, vp, blockIdx.x, sh);
 }
 
-template <int DIR>
-__global__ void __launch_bounds__(256) k_visc_ghost(const double *__restrict__ Q, double *__restrict__ FVg,
-                                                    int nx, int ny, int nz, const double *__restrict__ dxg,
-                                                    ViscParams vp) {
-  visc_ghost_one<DIR>(Q, FVg, nx, ny, nz, dxg, vp, (size_t)blockIdx.x * blockDim.x + threadIdx.x);
-}
-
-// Viscous fluxes of all members of an ensemble: one persistent launch over
-// work items (member, tile) and (member, VT ghost positions of one
-// direction); item0v holds each member's first item.
+// Viscous terms of all members of an ensemble: one persistent launch over
+// the (member, tile) work items; ptiles0 holds each member's first item.
 struct EnsViscArgs {
   const EnsMember *mem;
   int nmem;
-  const unsigned *item0v;
+  const unsigned *ptiles0;
   unsigned nitems;
   ViscParams vp;
 };
 
-// work items of VT threads for the ghost-layer fluxes of one direction: one
-// thread per x-line (all four positions), one per (position, line) along y
-// and z
-__host__ __device__ __forceinline__ unsigned ghost_blocks(int dir, size_t ncell, int n) {
-  const size_t threads = dir == 0 ? ncell / n : 4 * (ncell / n);
-  return (unsigned)((threads + VT - 1) / VT);
-}
-
-__global__ void __launch_bounds__(VTX * VTY) k_ens_visc(EnsViscArgs a) {
-  __shared__ double sQ[4][VHY][VHX];
+__global__ void __launch_bounds__(PTX * PTY) k_ens_visc(EnsViscArgs a) {
+  __shared__ ViscParShared<PTX, PTY> sh;
   for (unsigned item = blockIdx.x; item < a.nitems; item += gridDim.x) {
     int lo = 0, hi = a.nmem - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
-      if (a.item0v[mid] <= item) lo = mid;
+      if (a.ptiles0[mid] <= item) lo = mid;
       else hi = mid - 1;
     }
     const EnsMember &M = a.mem[lo];
-    const int nx = M.n[0], ny = M.n[1], nz = M.n[2];
-    unsigned r = item - a.item0v[lo];
-    const unsigned tiles = visc_tiles(nx, ny, nz);
-    if (r < tiles) {
-      visc_flux_tile(M.q, M.fv, nx, ny, nz, M.dxg, a.vp, (int)r, sQ);
-      continue;
-    }
-    r -= tiles;
-    const unsigned gx = ghost_blocks(0, M.ncell, nx), gy = ghost_blocks(1, M.ncell, ny);
-    const size_t t = threadIdx.x;
-    if (r < gx) visc_ghost_xline(M.q, M.fvg[0], nx, ny, nz, M.dxg, a.vp, (size_t)r * VT + t);
-    else if (r < gx + gy) visc_ghost_one<1>(M.q, M.fvg[1], nx, ny, nz, M.dxg, a.vp, (size_t)(r - gx) * VT + t);
-    else visc_ghost_one<2>(M.q, M.fvg[2], nx, ny, nz, M.dxg, a.vp, (size_t)(r - gx - gy) * VT + t);
+    visc_par_tile<PTX, PTY>(M.q, M.par, M.n[0], M.n[1], M.n[2], M.dxg, a.vp, (int)(item - a.ptiles0[lo]), sh);
+    __syncthreads();
   }
 }
 
@@ -506,63 +457,16 @@ __global__ void __launch_bounds__(256) k_ens_prims(const EnsMember *__restrict__
 }
 
 // ---------------------------------------------------------------------------
-// flux differences + viscous divergence + RK4 stage update (3N form)
+// the parabolic term on its own (hm_parabolic): par, with the zero mass
+// component, into a five-variable array
 // ---------------------------------------------------------------------------
-// mode OUT_STAGE:
-// stage 0: acc = u + cb*k;   S = u + ca*k
-// stage 1,2: acc = acc + cb*k; S = u + ca*k
-// stage 3: u = acc + cb*k
-// with k = -hyp + par and hyp = (hx + hy) + hz.  Other modes store k, hyp
-// or par in `out` instead.  FVg holds the ghost viscous fluxes of x, y, z
-// (4 slots x 4 components x plane each).
-__global__ void __launch_bounds__(256) k_stage(
-    const double *__restrict__ fIx, const double *__restrict__ fIy, const double *__restrict__ fIz,
-    const double *__restrict__ FV, const double *__restrict__ FVgx, const double *__restrict__ FVgy,
-    const double *__restrict__ FVgz, double *__restrict__ u, double *__restrict__ S,
-    double *__restrict__ acc, double *__restrict__ out, int mode, int nx, int ny, int nz,
-    const double *__restrict__ dxg, int viscous, int stage, double ca, double cb) {
-  const size_t ncell = (size_t)nx * ny * nz;
-  const size_t idx = xcd_block() * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_par_out(const double *__restrict__ par, double *__restrict__ out,
+                                                 size_t ncell) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= ncell) return;
-  const int i = (int)(idx % nx);
-  const int j = (int)((idx / nx) % ny);
-  const int k = (int)(idx / ((size_t)nx * ny));
-  const double *dxi = dxg + 3, *dyi = dxg + (nx + 6) + 3, *dzi = dxg + (nx + 6) + (ny + 6) + 3;
-  const size_t ip = idx - i + wrap(i + 1, nx);
-  const size_t jp = idx - (size_t)j * nx + (size_t)wrap(j + 1, ny) * nx;
-  const size_t kp = idx - (size_t)k * nx * ny + (size_t)wrap(k + 1, nz) * nx * ny;
-  double par[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  if (viscous) {
-    double dv[4];
-    visc_div_dir<0>(dv, FV, FVgx, ncell, idx, i, nx, 1, (size_t)j + (size_t)ny * k);
-    for (int v = 0; v < 4; v++) par[v + 1] = dxi[i] * dv[v];
-    visc_div_dir<1>(dv, FV, FVgy, ncell, idx, j, ny, (size_t)nx, (size_t)i + (size_t)nx * k);
-    for (int v = 0; v < 4; v++) par[v + 1] += dyi[j] * dv[v];
-    visc_div_dir<2>(dv, FV, FVgz, ncell, idx, k, nz, (size_t)nx * ny, (size_t)i + (size_t)nx * j);
-    for (int v = 0; v < 4; v++) par[v + 1] += dzi[k] * dv[v];
-  }
+  out[idx] = 0.0;
 #pragma unroll
-  for (int v = 0; v < NV; v++) {
-    const size_t o = (size_t)v * ncell;
-    double hyp = dxi[i] * (fIx[o + ip] - fIx[o + idx]);
-    hyp += dyi[j] * (fIy[o + jp] - fIy[o + idx]);
-    hyp += dzi[k] * (fIz[o + kp] - fIz[o + idx]);
-    const double r = (-hyp) + par[v];
-    if (mode != OUT_STAGE) {
-      out[o + idx] = mode == OUT_RHS ? r : (mode == OUT_HYP ? hyp : par[v]);
-      continue;
-    }
-    if (stage == 0) {
-      const double u0 = u[o + idx];
-      acc[o + idx] = u0 + cb * r;
-      S[o + idx] = u0 + ca * r;
-    } else if (stage < 3) {
-      acc[o + idx] = acc[o + idx] + cb * r;
-      S[o + idx] = u[o + idx] + ca * r;
-    } else {
-      u[o + idx] = acc[o + idx] + cb * r;
-    }
-  }
+  for (int v = 0; v < 4; v++) out[(size_t)(v + 1) * ncell + idx] = par[(size_t)v * ncell + idx];
 }
 
 // ---------------------------------------------------------------------------
@@ -663,32 +567,6 @@ __global__ void k_soa_to_aos(const double *__restrict__ soa, double *__restrict_
 // ---------------------------------------------------------------------------
 static inline unsigned nblocks(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 
-template <int DIR, int INTERP, int UPW>
-static void launch_iface_t(const Grid &g, const double *u, double *fI, double *Q, double gamma,
-                           double eps, int weno, hipStream_t st) {
-  hipLaunchKernelGGL((k_iface<DIR, INTERP, UPW>), dim3(nblocks(g.ncell, 256)), dim3(256), 0, st, u, fI, Q,
-                     g.n[0], g.n[1], g.n[2], gamma, eps, weno);
-}
-
-template <int DIR>
-static void launch_iface_d(const Grid &g, const Scheme &sc, const double *u, double *fI, double *Q,
-                           hipStream_t st) {
-  if (sc.interp == INTERP_CHAR) {
-    if (sc.upwind == UPWIND_ROE) launch_iface_t<DIR, INTERP_CHAR, UPWIND_ROE>(g, u, fI, Q, sc.gamma, sc.eps, sc.weno, st);
-    else launch_iface_t<DIR, INTERP_CHAR, UPWIND_RUSANOV>(g, u, fI, Q, sc.gamma, sc.eps, sc.weno, st);
-  } else {
-    if (sc.upwind == UPWIND_ROE) launch_iface_t<DIR, INTERP_COMP, UPWIND_ROE>(g, u, fI, Q, sc.gamma, sc.eps, sc.weno, st);
-    else launch_iface_t<DIR, INTERP_COMP, UPWIND_RUSANOV>(g, u, fI, Q, sc.gamma, sc.eps, sc.weno, st);
-  }
-}
-
-void launch_iface(const Grid &g, const Scheme &sc, int dir, const double *u, double *fI, double *Q,
-                  hipStream_t st) {
-  if (dir == 0) launch_iface_d<0>(g, sc, u, fI, Q, st);
-  else if (dir == 1) launch_iface_d<1>(g, sc, u, fI, nullptr, st);
-  else launch_iface_d<2>(g, sc, u, fI, nullptr, st);
-}
-
 static ViscParams visc_params(const Scheme &sc) {
   ViscParams vp;
   vp.gamma = sc.gamma;
@@ -698,36 +576,22 @@ static ViscParams visc_params(const Scheme &sc) {
   return vp;
 }
 
-size_t visc_ghost_size(const Grid &g, int dir) { return 16 * (g.ncell / g.n[dir]); }
-
-void launch_visc_flux(const Grid &g, const Scheme &sc, const double *Q, double *FV, double *const FVg[3],
-                      const double *dxg, hipStream_t st) {
-  const ViscParams vp = visc_params(sc);
-  const unsigned nb = visc_tiles(g.n[0], g.n[1], g.n[2]);
-  hipLaunchKernelGGL(k_visc_flux, dim3(nb), dim3(VTX * VTY), 0, st, Q, FV, g.n[0], g.n[1], g.n[2], dxg, vp);
-  hipLaunchKernelGGL(k_visc_ghost_x, dim3(nblocks(g.ncell / g.n[0], 256)), dim3(256), 0, st, Q, FVg[0], g.n[0],
-                     g.n[1], g.n[2], dxg, vp);
-  hipLaunchKernelGGL(k_visc_ghost<1>, dim3(nblocks(4 * (g.ncell / g.n[1]), 256)), dim3(256), 0, st, Q,
-                     FVg[1], g.n[0], g.n[1], g.n[2], dxg, vp);
-  hipLaunchKernelGGL(k_visc_ghost<2>, dim3(nblocks(4 * (g.ncell / g.n[2]), 256)), dim3(256), 0, st, Q,
-                     FVg[2], g.n[0], g.n[1], g.n[2], dxg, vp);
+void launch_visc_par(const Grid &g, const Scheme &sc, const double *Q, double *par, const double *dxg,
+                     hipStream_t st) {
+  const unsigned nb = par_tiles(g.n[0], g.n[1], g.n[2]);
+  hipLaunchKernelGGL(k_visc_par, dim3(nb), dim3(PTX * PTY), 0, st, Q, par, g.n[0], g.n[1], g.n[2], dxg,
+                     visc_params(sc));
 }
 
-unsigned ens_visc_items_per_member(const int n[3]) {
-  const size_t ncell = (size_t)n[0] * n[1] * n[2];
-  return visc_tiles(n[0], n[1], n[2]) + ghost_blocks(0, ncell, n[0]) + ghost_blocks(1, ncell, n[1]) +
-         ghost_blocks(2, ncell, n[2]);
-}
-
-void launch_ens_visc(const Scheme &sc, const EnsMember *mem, int nmem, const unsigned *item0v, unsigned nitems,
+void launch_ens_visc(const Scheme &sc, const EnsMember *mem, int nmem, const unsigned *ptiles0, unsigned nitems,
                      hipStream_t st) {
   EnsViscArgs a;
   a.mem = mem;
   a.nmem = nmem;
-  a.item0v = item0v;
+  a.ptiles0 = ptiles0;
   a.nitems = nitems;
   a.vp = visc_params(sc);
-  hipLaunchKernelGGL(k_ens_visc, dim3(nitems), dim3(VTX * VTY), 0, st, a);
+  hipLaunchKernelGGL(k_ens_visc, dim3(nitems), dim3(PTX * PTY), 0, st, a);
 }
 
 void launch_ens_prims(const Scheme &sc, const EnsMember *mem, int nmem, const double *u, size_t ncell,
@@ -735,13 +599,8 @@ void launch_ens_prims(const Scheme &sc, const EnsMember *mem, int nmem, const do
   hipLaunchKernelGGL(k_ens_prims, dim3(nblocks(ncell, 256)), dim3(256), 0, st, mem, nmem, u, ncell, sc.gamma);
 }
 
-void launch_stage(const Grid &g, const Scheme &sc, const double *fIx, const double *fIy,
-                  const double *fIz, const double *FV, double *const FVg[3], double *u, double *S,
-                  double *acc, double *out, int mode, const double *dxg, int stage, double ca,
-                  double cb, hipStream_t st) {
-  hipLaunchKernelGGL(k_stage, dim3(nblocks(g.ncell, 256)), dim3(256), 0, st, fIx, fIy, fIz, FV,
-                     FVg ? FVg[0] : nullptr, FVg ? FVg[1] : nullptr, FVg ? FVg[2] : nullptr, u, S, acc,
-                     out, mode, g.n[0], g.n[1], g.n[2], dxg, sc.Re > 0 ? 1 : 0, stage, ca, cb);
+void launch_par_out(const Grid &g, const double *par, double *out, hipStream_t st) {
+  hipLaunchKernelGGL(k_par_out, dim3(nblocks(g.ncell, 256)), dim3(256), 0, st, par, out, g.ncell);
 }
 
 // Volume integral of every conserved variable: sum of u dV over the

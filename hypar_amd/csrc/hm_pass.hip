@@ -1,15 +1,16 @@
 // hm_pass.hip -- fused direction passes of the NS3D WENO5 right-hand side
 // and RK4 stage (gfx950, FP64).
 //
-// One RK stage is three launches, one per direction (plus the viscous-flux
-// kernel of hm_kernels.hip first when the run is viscous):
-//   pass x:  R  = -dxinv (F_{i+1/2} - F_{i-1/2}) + dxinv d/dx FV_x
-//   pass y:  R  = R - dyinv (G_{j+1/2} - G_{j-1/2}) + dyinv d/dy FV_y
-//   pass z:  r  = R - dzinv (H_{k+1/2} - H_{k-1/2}) + dzinv d/dz FV_z,
+// One RK stage is three launches, one per direction (after the viscous-term
+// kernel of hm_kernels.hip when the run is viscous):
+//   pass x:  R = dxinv (F_{i+1/2} - F_{i-1/2})
+//   pass y:  R = R + dyinv (G_{j+1/2} - G_{j-1/2})
+//   pass z:  r = -(R + dzinv (H_{k+1/2} - H_{k-1/2})) + par,
 //            then the RK4 stage update in 3N form (or r itself for hm_rhs)
-// i.e. rhs = -hyp + par of TimeRHSFunctionExplicit.c:88-93 with the
-// hyperbolic term of HyperbolicFunction.c:75-101 and the parabolic term of
-// NavierStokes3DParabolicFunction.c:50-325, summed direction by direction.
+// i.e. rhs = -hyp + par of TimeRHSFunctionExplicit.c:88-93 summed in the
+// reference's order, with the hyperbolic term of HyperbolicFunction.c:75-101
+// accumulated direction by direction and the parabolic term par of
+// NavierStokes3DParabolicFunction.c:50-325.
 //
 // Each pass stages the cells of its pencils in LDS once, evaluates every
 // cell's flux and modified energy once, computes the interface fluxes
@@ -76,9 +77,8 @@ struct PassConst {
 
 struct PassIO {
   const double *U;    // stage state
-  double *R;          // rhs accumulator
-  const double *FV;   // viscous fluxes of all directions, FV[(d*4+c)*ncell + cell]
-  const double *FVg;  // ghost-layer viscous fluxes of the pass direction
+  double *R;          // hyperbolic-term accumulator
+  const double *par;  // viscous term, par[c*ncell + cell], c = momentum x, y, z, energy
   double *u, *S, *acc, *out, *Q;
   int visc, mode, stage;
   double ca, cb;
@@ -387,29 +387,29 @@ __device__ __forceinline__ void iface(const LD &ld, const PassConst &k, double f
 }
 
 // Inputs of cell idx's output that come from global memory: its dxinv, the
-// viscous divergence of the pass direction, the rhs so far (y, z) and the
-// RK4 operands (z).  The marching passes issue these loads in one block with
-// the next ring row's, so that their latencies overlap.
+// hyperbolic term so far (y, z), the viscous term and the RK4 operands (z).
+// The marching passes issue these loads in one block with the next ring
+// row's, so that their latencies overlap.
 struct EmitIn {
   double dxi, par[NV], rin[NV], a[NV], b[NV];
 };
 
 template <int DIR>
-__device__ __forceinline__ EmitIn emit_load(const PassConst &k, const PassIO &io, size_t idx, int pos,
-                                            size_t line) {
-  const int n = k.n[DIR];
+__device__ __forceinline__ EmitIn emit_load(const PassConst &k, const PassIO &io, size_t idx, int pos) {
   EmitIn in;
   in.dxi = k.dxg[(DIR >= 1 ? k.n[0] + 6 : 0) + (DIR == 2 ? k.n[1] + 6 : 0) + 3 + pos];
 #pragma unroll
   for (int v = 0; v < NV; v++) in.par[v] = in.rin[v] = in.a[v] = in.b[v] = 0.0;
-  // the rhs so far and the RK4 operands of the 3N update (u at stage 0, acc
-  // and u at stages 1-2, acc at stage 3), issued ahead of the viscous
-  // stencil, whose arithmetic waits for its loads
   if (DIR > 0)
 #pragma unroll
     for (int v = 0; v < NV; v++) in.rin[v] = io.R[(size_t)v * k.ncell + idx];
-  if (DIR == 2 && io.mode != OUT_RHS) {
-    // one uniform branch per stage kind, each with its loads in one block
+  if (DIR == 2 && io.visc && io.mode != OUT_HYP)
+#pragma unroll
+    for (int c = 0; c < 4; c++) in.par[c + 1] = io.par[(size_t)c * k.ncell + idx];
+  // the RK4 operands of the 3N update (u at stage 0, acc and u at stages
+  // 1-2, acc at stage 3), one uniform branch per stage kind, each with its
+  // loads in one block
+  if (DIR == 2 && io.mode == OUT_STAGE) {
     if (io.stage == 0) {
 #pragma unroll
       for (int v = 0; v < NV; v++) in.a[v] = io.u[(size_t)v * k.ncell + idx];
@@ -424,18 +424,10 @@ __device__ __forceinline__ EmitIn emit_load(const PassConst &k, const PassIO &io
       }
     }
   }
-  if (io.visc) {
-    const size_t stride = DIR == 0 ? 1 : (DIR == 1 ? (size_t)k.n[0] : (size_t)k.n[0] * k.n[1]);
-    double dv[4];
-    visc_div_dir<DIR>(dv, io.FV, io.FVg, k.ncell, idx, pos, n, stride, line);
-#pragma unroll
-    for (int c = 0; c < 4; c++) in.par[c + 1] = in.dxi * dv[c];
-  }
   return in;
 }
 
-// Finish cell idx of the pass: flux difference, viscous divergence of the
-// pass direction, and the pass's output.
+// Finish cell idx of the pass: flux difference and the pass's output.
 template <int DIR>
 __device__ __forceinline__ void emit_store(const PassConst &k, const PassIO &io, size_t idx, const EmitIn &in,
                                            const double (&fm)[NV], const double (&fp)[NV]) {
@@ -443,9 +435,12 @@ __device__ __forceinline__ void emit_store(const PassConst &k, const PassIO &io,
 #pragma unroll
   for (int v = 0; v < NV; v++) {
     const double h = in.dxi * (fp[v] - fm[v]);
-    r[v] = DIR == 0 ? (-h) + in.par[v] : (in.rin[v] - h) + in.par[v];
+    // hyp = (hx + hy) + hz and rhs = (-hyp) + par (TimeRHSFunctionExplicit.c:88-93)
+    if (DIR == 0) r[v] = h;
+    else if (DIR == 1 || io.mode == OUT_HYP) r[v] = in.rin[v] + h;
+    else r[v] = (-(in.rin[v] + h)) + in.par[v];
   }
-  if (DIR < 2 || io.mode == OUT_RHS) {
+  if (DIR < 2 || io.mode != OUT_STAGE) {
     double *dst = DIR < 2 ? io.R : io.out;
 #pragma unroll
     for (int v = 0; v < NV; v++) dst[(size_t)v * k.ncell + idx] = r[v];
@@ -471,7 +466,7 @@ __device__ __forceinline__ void emit_store(const PassConst &k, const PassIO &io,
   }
   if (io.visc) {
     // primitive variables (u, v, w, T) of the new state for the next
-    // viscous-flux evaluation (NavierStokes3DParabolicFunction.c:85-108)
+    // viscous-term evaluation (NavierStokes3DParabolicFunction.c:85-108)
     const Prim q = flowvar(nu, k.gamma);
     io.Q[idx] = q.vx;
     io.Q[k.ncell + idx] = q.vy;
@@ -481,9 +476,9 @@ __device__ __forceinline__ void emit_store(const PassConst &k, const PassIO &io,
 }
 
 template <int DIR>
-__device__ __forceinline__ void emit(const PassConst &k, const PassIO &io, size_t idx, int pos, size_t line,
+__device__ __forceinline__ void emit(const PassConst &k, const PassIO &io, size_t idx, int pos,
                                      const double (&fm)[NV], const double (&fp)[NV]) {
-  emit_store<DIR>(k, io, idx, emit_load<DIR>(k, io, idx, pos, line), fm, fp);
+  emit_store<DIR>(k, io, idx, emit_load<DIR>(k, io, idx, pos), fm, fp);
 }
 
 // ---------------------------------------------------------------------------
@@ -547,7 +542,7 @@ __device__ __forceinline__ void pass_x_body(const PassConst &k, const PassIO &io
         fm[v] = sh.sF[w][v][lane];
         fp[v] = sh.sF[w][v][lane + 1];
       }
-      emit<0>(k, io, base + cell, cell, line, fm, fp);
+      emit<0>(k, io, base + cell, cell, fm, fp);
     }
     __syncthreads();
   }
@@ -560,7 +555,7 @@ __device__ __forceinline__ void pass_x_body(const PassConst &k, const PassIO &io
       fm[v] = sh.sF[w][v][nx - x0];
       fp[v] = sh.sF0[w][v];
     }
-    emit<0>(k, io, base + nx - 1, nx - 1, line, fm, fp);
+    emit<0>(k, io, base + nx - 1, nx - 1, fm, fp);
   }
 }
 
@@ -588,8 +583,7 @@ __device__ __forceinline__ void pass_m_body(const PassConst &k, const PassIO &io
   const int nx = k.n[0], ny = k.n[1];
   const int n = k.n[DIR];
   // the 64 lanes take 64 consecutive columns c = i + nx*o of the plane
-  // across DIR (o = z index in the y pass, y index in the z pass); c is
-  // also the line index of the ghost-layer viscous fluxes
+  // across DIR (o = z index in the y pass, y index in the z pass)
   const size_t ncol = k.ncell / n;
   const size_t c = group * MW + lane;
   const bool col = c < ncol;
@@ -657,7 +651,7 @@ __device__ __forceinline__ void pass_m_body(const PassConst &k, const PassIO &io
     double un[NV];
     fetch_row(rn, un);
     const int ec = e < 0 ? 0 : (e > n - 2 ? n - 2 : e);
-    const EmitIn in = emit_load<DIR>(k, io, cbase + (size_t)ec * stride, ec, line);
+    const EmitIn in = emit_load<DIR>(k, io, cbase + (size_t)ec * stride, ec);
     if (more) stage_row(rn, un);
     if (emits) {
       double fm[NV], fp[NV];
@@ -679,7 +673,7 @@ __device__ __forceinline__ void pass_m_body(const PassConst &k, const PassIO &io
       fm[v] = sh.sF[v][a][lane];
       fp[v] = sh.sF0[v][lane];
     }
-    emit<DIR>(k, io, cbase + (size_t)(n - 1) * stride, n - 1, line, fm, fp);
+    emit<DIR>(k, io, cbase + (size_t)(n - 1) * stride, n - 1, fm, fp);
   }
 }
 
@@ -734,8 +728,7 @@ __device__ __forceinline__ void ens_setup(const EnsArgs &a, int m, int dir, Pass
   const size_t o = 5 * M.off;
   io.U = a.U + o;
   io.R = a.R + o;
-  io.FV = M.fv;
-  io.FVg = M.fvg[dir];
+  io.par = M.par;
   io.u = a.u + o;
   io.S = a.S ? a.S + o : nullptr;
   io.acc = a.acc + o;
@@ -836,8 +829,7 @@ void launch_pass(const Grid &g, const Scheme &sc, int dir, const PassBuffers &b,
   PassIO io;
   io.U = b.U;
   io.R = b.R;
-  io.FV = b.FV;
-  io.FVg = b.FVg[dir];
+  io.par = b.par;
   io.u = b.u;
   io.S = b.S;
   io.acc = b.acc;

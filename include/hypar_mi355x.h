@@ -114,7 +114,8 @@ hm_status hm_rk4(hm_solver *s, double *u, double dt, int nsteps);
 /* rhs = -hyperbolic + parabolic (+ zero source) of the device state u.
  * rhs and u must be distinct arrays (HM_ERR_ARG otherwise). */
 hm_status hm_rhs(hm_solver *s, double *rhs, const double *u);
-/* hyperbolic term alone: sum over x,y,z of dxinv*(F_{i+1/2}-F_{i-1/2}). */
+/* hyperbolic term alone: sum over x,y,z of dxinv*(F_{i+1/2}-F_{i-1/2}).
+ * hyp and u must be distinct arrays (HM_ERR_ARG otherwise). */
 hm_status hm_hyperbolic(hm_solver *s, double *hyp, const double *u);
 /* viscous term alone (zero when Re <= 0). */
 hm_status hm_parabolic(hm_solver *s, double *par, const double *u);
