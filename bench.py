@@ -37,13 +37,14 @@ STEP_BYTES_PER_CELL_UPDATE = 640
 # of the step's algorithmic bytes is a quarter of 640 B per cell
 STAGE_BYTES_PER_CELL = STEP_BYTES_PER_CELL_UPDATE // 4
 # operand bytes per cell one launch of each kernel reads and writes in this
-# implementation (not the algorithmic figure; reported beside it):
-#   pass_x     read U_s, write R                              (+ FV_x 32 B viscous)
-#   pass_y     read U_s and R, write R                         (+ FV_y 32 B)
-#   pass_z     read U_s, R, u, acc; write acc, U_{s+1}         (+ FV_z 32 B, write Q 32 B)
-#   visc_flux  read Q (4 primitives), write the viscous fluxes of x, y, z (12)
+# implementation (not the algorithmic figure; reported beside it), as
+# (inviscid, viscous extra):
+#   pass_x     read U_s, write R
+#   pass_y     read U_s and R, write R
+#   pass_z     read U_s, R, u, acc; write acc, U_{s+1}   (+ read par 32 B, write Q 32 B)
+#   visc_par   read Q (4 primitives), write par (4 components)
 #   prims      read u, write Q
-OPERAND_BYTES = {"pass_x": (80, 32), "pass_y": (120, 32), "pass_z": (240, 64), "visc_flux": (128, 0),
+OPERAND_BYTES = {"pass_x": (80, 0), "pass_y": (120, 0), "pass_z": (240, 64), "visc_par": (64, 0),
                  "prims": (72, 0)}
 
 
@@ -288,6 +289,44 @@ def ensemble_adjoint(nsteps=30, every=10):
     return out
 
 
+def dns_ssprk3(n=256, nsteps=10, warmup=2):
+    """The scheme of the reference's published GPU example deck
+    (DNS_IsotropicTurbulenceDecay_CUDA: viscous NS3D, Re 333.33, Mach 0.3,
+    component-wise mapped WENO5 + Roe, SSPRK3, dt 0.005) at n^3 on a seeded
+    Taylor-Green-type field, timed with events over nsteps fused SSPRK3
+    steps.  The published figure (82.421 s for 1000 steps of 256^3 on 64
+    V100s, out.log of that example) is printed beside it as context: a
+    different machine and initial field."""
+    import torch
+
+    from hypar_amd import decks
+    from hypar_amd.solver import NS3D
+    w = decks.config_dns(n, nsteps)
+    u0, dt = decks.prepare(w)
+    s = NS3D.from_workload(w)
+    u = s.to_device(u0)
+    del u0
+    s.ssprk3(u, dt, warmup)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    s.ssprk3(u, dt, nsteps)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / nsteps
+    cfl = s.cfl(u, dt)
+    s.close()
+    del u
+    torch.cuda.empty_cache()
+    return {"workload": f"viscous NS3D periodic {n}^3 (Re 333.33, Mach 0.3, Pr 0.72), component-wise mapped "
+                        f"WENO5 + Roe, SSPRK3, dt {dt}, seeded Taylor-Green-type field",
+            "s_per_step": ms / 1e3, "cell_steps_per_s": n ** 3 / (ms / 1e3), "steps": nsteps, "final_cfl": cfl,
+            "published_reference_context": {"s_per_step": 82.421 / 1000, "hardware": "64x NVIDIA V100 (64 MPI ranks)",
+                                            "source": "Examples/3D/NavierStokes3D/DNS_IsotropicTurbulenceDecay_CUDA/"
+                                                      "out.log"}}
+
+
 def copy_probe(mib=1024, reps=3):
     """A plain device copy of known size (torch copy_ of `mib` MiB), the
     calibration run for the FETCH_SIZE / WRITE_SIZE counters."""
@@ -312,6 +351,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-ensemble-adjoint", action="store_true")
+    ap.add_argument("--no-dns", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -412,6 +452,8 @@ def main():
         line["ensemble"] = ensemble_throughput()
     if not args.no_ensemble_adjoint:
         line["ensemble_adjoint"] = ensemble_adjoint()
+    if not args.no_dns:
+        line["dns_ssprk3"] = dns_ssprk3()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
         line["cpu_baseline_components_all_cores"] = cpu_baseline_components()

@@ -116,6 +116,23 @@ hm_status check_state(const hm_solver *s, const void *p, const char *name) {
 // "rk 44" tableau (TimeExplicitRKInitialize.c:58-68)
 const double RK_A_SUB[3] = {0.5, 0.5, 1.0};  // A[s+1][s]
 const double RK_B[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+// "rk ssprk3" tableau (TimeExplicitRKInitialize.c:69-79): A[1][0] = 1,
+// A[2][0] = A[2][1] = 1/4, b = 1/6, 1/6, 2/3
+const double SSP_A10 = 1.0, SSP_A20 = 0.25, SSP_A21 = 0.25;
+const double SSP_B[3] = {1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0};
+
+hm::PassBuffers stage_buffers(hm_solver *s, double *u, const double *U, double *Sn) {
+  hm::PassBuffers b;
+  b.U = U;
+  b.R = s->R;
+  b.par = s->par;
+  b.u = u;
+  b.S = Sn;
+  b.acc = s->acc;
+  b.out = nullptr;
+  b.Q = s->Q;
+  return b;
+}
 
 // One RK4 step of u in place through the fused direction passes.  Stage s
 // reads Sx[s-1] (u for s = 0) and writes Sx[s]; the last stage writes u.
@@ -124,19 +141,40 @@ const double RK_B[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
 hm_status rk4_step(hm_solver *s, double *u, double dt, double *const Sx[3]) {
   const bool visc = s->sc.Re > 0;
   for (int stage = 0; stage < 4; stage++) {
-    hm::PassBuffers b;
-    b.U = stage == 0 ? u : Sx[stage - 1];
-    b.R = s->R;
-    b.par = s->par;
-    b.u = u;
-    b.S = stage < 3 ? Sx[stage] : nullptr;
-    b.acc = s->acc;
-    b.out = nullptr;
-    b.Q = s->Q;
+    const hm::PassBuffers b = stage_buffers(s, u, stage == 0 ? u : Sx[stage - 1], stage < 3 ? Sx[stage] : nullptr);
     if (visc) hm::launch_visc_par(s->g, s->sc, s->Q, s->par, s->dxg, s->stream);
     const double ca = stage < 3 ? dt * RK_A_SUB[stage] : 0.0, cb = dt * RK_B[stage];
     for (int d = 0; d < 3; d++) hm::launch_pass(s->g, s->sc, d, b, hm::OUT_STAGE, stage, ca, cb, s->dxg, s->stream);
     hm_status st = check_launch("RK4 stage passes");
+    if (st != HM_OK) return st;
+  }
+  return HM_OK;
+}
+
+// One SSPRK3 step of u in place (TimeRK.c:126-195 with the "ssprk3"
+// tableau):  U1 = u + dt k0,  U2 = (u + dt/4 k0) + dt/4 k1,
+// u_new = ((u + dt/6 k0) + dt/6 k1) + 2dt/3 k2, in the reference's order.
+// With the z pass's three-output update (O1 = a + c1 r, O2 = b + c2 r,
+// O3 = a + c3 r):
+//   stage 0 (from u):    acc = u + dt/6 k0, S0 = u + dt k0, S1 = u + dt/4 k0
+//   stage 1 (from S0):   acc = acc + dt/6 k1, S1 = S1 + dt/4 k1 (in place), acc again
+//   stage 2 (from S1):   u = acc + 2dt/3 k2 (all three outputs)
+hm_status ssprk3_step(hm_solver *s, double *u, double dt) {
+  const bool visc = s->sc.Re > 0;
+  double *acc = s->acc, *S0 = s->S[0], *S1 = s->S[1];
+  const double *Uin[3] = {u, S0, S1};
+  const double *A[3] = {u, acc, acc}, *B[3] = {u, S1, acc};
+  double *O1[3] = {acc, acc, u}, *O2[3] = {S0, S1, u}, *O3[3] = {S1, acc, u};
+  const double c1[3] = {dt * SSP_B[0], dt * SSP_B[1], dt * SSP_B[2]};
+  const double c2[3] = {dt * SSP_A10, dt * SSP_A21, dt * SSP_B[2]};
+  const double c3[3] = {dt * SSP_A20, dt * SSP_B[1], dt * SSP_B[2]};
+  for (int stage = 0; stage < 3; stage++) {
+    const hm::PassBuffers b = stage_buffers(s, u, Uin[stage], nullptr);
+    if (visc) hm::launch_visc_par(s->g, s->sc, s->Q, s->par, s->dxg, s->stream);
+    for (int d = 0; d < 2; d++) hm::launch_pass(s->g, s->sc, d, b, hm::OUT_STAGE, stage, 0.0, 0.0, s->dxg, s->stream);
+    hm::launch_pass_z_affine(s->g, s->sc, b, A[stage], B[stage], O1[stage], O2[stage], O3[stage], c1[stage],
+                             c2[stage], c3[stage], s->dxg, s->stream);
+    hm_status st = check_launch("SSPRK3 stage passes");
     if (st != HM_OK) return st;
   }
   return HM_OK;
@@ -312,6 +350,18 @@ hm_status hm_rk4(hm_solver *s, double *u, double dt, int nsteps) {
   double *const Sx[3] = {s->S[0], s->S[1], s->S[0]};
   for (int it = 0; it < nsteps; it++)
     if ((st = rk4_step(s, u, dt, Sx)) != HM_OK) return st;
+  return HM_OK;
+}
+
+hm_status hm_ssprk3(hm_solver *s, double *u, double dt, int nsteps) {
+  hm_status st = check_state(s, u, "hm_ssprk3: u");
+  if (st != HM_OK) return st;
+  if (nsteps < 0) return fail(HM_ERR_ARG, "hm_ssprk3: nsteps < 0");
+  if (!std::isfinite(dt)) return fail(HM_ERR_ARG, "hm_ssprk3: dt is not finite");
+  if (nsteps == 0) return HM_OK;
+  if (s->sc.Re > 0) hm::launch_prims(s->g, s->sc, u, s->Q, s->stream);
+  for (int it = 0; it < nsteps; it++)
+    if ((st = ssprk3_step(s, u, dt)) != HM_OK) return st;
   return HM_OK;
 }
 

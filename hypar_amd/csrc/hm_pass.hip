@@ -84,6 +84,22 @@ struct PassIO {
   double ca, cb;
 };
 
+// Stage update of the z pass in the separate kernel instantiations RK =
+// RK_AFFINE: with r the stage's right-hand side and a, b the operands A, B
+// at the cell,
+//   O1 = a + c1 r,  O2 = b + c2 r (the next stage state),  O3 = a + c3 r,
+// all three stored unconditionally; the caller picks operands, outputs and
+// coefficients per stage (outputs may coincide when their values do).  The
+// update has no per-stage branches: the compiler merged the stores of a
+// three-way stage branch into one store whose address register one path
+// never set.
+enum { RK_CLASSIC = 0, RK_AFFINE = 1 };
+struct SspIO {
+  const double *A, *B;
+  double *O1, *O2, *O3;
+  double c1, c2, c3;
+};
+
 // per-cell data of the pass direction (NavierStokes3DFlux.c:23-57 and
 // NavierStokes3DModifiedSolution.c:31-70)
 template <int DIR>
@@ -394,8 +410,9 @@ struct EmitIn {
   double dxi, par[NV], rin[NV], a[NV], b[NV];
 };
 
-template <int DIR>
-__device__ __forceinline__ EmitIn emit_load(const PassConst &k, const PassIO &io, size_t idx, int pos) {
+template <int DIR, int RK = RK_CLASSIC>
+__device__ __forceinline__ EmitIn emit_load(const PassConst &k, const PassIO &io, size_t idx, int pos,
+                                            const SspIO &sp = SspIO{}) {
   EmitIn in;
   in.dxi = k.dxg[(DIR >= 1 ? k.n[0] + 6 : 0) + (DIR == 2 ? k.n[1] + 6 : 0) + 3 + pos];
 #pragma unroll
@@ -409,7 +426,13 @@ __device__ __forceinline__ EmitIn emit_load(const PassConst &k, const PassIO &io
   // the RK4 operands of the 3N update (u at stage 0, acc and u at stages
   // 1-2, acc at stage 3), one uniform branch per stage kind, each with its
   // loads in one block
-  if (DIR == 2 && io.mode == OUT_STAGE) {
+  if (RK == RK_AFFINE && DIR == 2) {
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      in.a[v] = sp.A[(size_t)v * k.ncell + idx];
+      in.b[v] = sp.B[(size_t)v * k.ncell + idx];
+    }
+  } else if (DIR == 2 && io.mode == OUT_STAGE) {
     if (io.stage == 0) {
 #pragma unroll
       for (int v = 0; v < NV; v++) in.a[v] = io.u[(size_t)v * k.ncell + idx];
@@ -428,9 +451,10 @@ __device__ __forceinline__ EmitIn emit_load(const PassConst &k, const PassIO &io
 }
 
 // Finish cell idx of the pass: flux difference and the pass's output.
-template <int DIR>
+template <int DIR, int RK = RK_CLASSIC>
 __device__ __forceinline__ void emit_store(const PassConst &k, const PassIO &io, size_t idx, const EmitIn &in,
-                                           const double (&fm)[NV], const double (&fp)[NV]) {
+                                           const double (&fm)[NV], const double (&fp)[NV],
+                                           const SspIO &sp = SspIO{}) {
   double r[NV];
 #pragma unroll
   for (int v = 0; v < NV; v++) {
@@ -446,8 +470,18 @@ __device__ __forceinline__ void emit_store(const PassConst &k, const PassIO &io,
     for (int v = 0; v < NV; v++) dst[(size_t)v * k.ncell + idx] = r[v];
     return;
   }
-  // RK4 stage update in 3N form (TimeRK.c:126-195)
   double nu[NV];
+  if (RK == RK_AFFINE) {
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      const size_t o = (size_t)v * k.ncell + idx;
+      nu[v] = in.b[v] + sp.c2 * r[v];
+      sp.O1[o] = in.a[v] + sp.c1 * r[v];
+      sp.O2[o] = nu[v];
+      sp.O3[o] = in.a[v] + sp.c3 * r[v];
+    }
+  } else {
+  // RK4 stage update in 3N form (TimeRK.c:126-195)
 #pragma unroll
   for (int v = 0; v < NV; v++) {
     const size_t o = (size_t)v * k.ncell + idx;
@@ -464,6 +498,7 @@ __device__ __forceinline__ void emit_store(const PassConst &k, const PassIO &io,
       io.u[o] = nu[v];
     }
   }
+  }
   if (io.visc) {
     // primitive variables (u, v, w, T) of the new state for the next
     // viscous-term evaluation (NavierStokes3DParabolicFunction.c:85-108)
@@ -475,10 +510,11 @@ __device__ __forceinline__ void emit_store(const PassConst &k, const PassIO &io,
   }
 }
 
-template <int DIR>
+template <int DIR, int RK = RK_CLASSIC>
 __device__ __forceinline__ void emit(const PassConst &k, const PassIO &io, size_t idx, int pos,
-                                     const double (&fm)[NV], const double (&fp)[NV]) {
-  emit_store<DIR>(k, io, idx, emit_load<DIR>(k, io, idx, pos), fm, fp);
+                                     const double (&fm)[NV], const double (&fp)[NV],
+                                     const SspIO &sp = SspIO{}) {
+  emit_store<DIR, RK>(k, io, idx, emit_load<DIR, RK>(k, io, idx, pos, sp), fm, fp, sp);
 }
 
 // ---------------------------------------------------------------------------
@@ -574,9 +610,9 @@ struct MShared {
 };
 
 // columns MW*group .. MW*group + MW-1 of the plane across DIR
-template <int DIR, int INTERP, int UPW, int MR, int W>
+template <int DIR, int INTERP, int UPW, int MR, int W, int RK = RK_CLASSIC>
 __device__ __forceinline__ void pass_m_body(const PassConst &k, const PassIO &io, size_t group,
-                                            MShared<MR> &sh) {
+                                            MShared<MR> &sh, const SspIO &sp = SspIO{}) {
   constexpr int NRING = MShared<MR>::NRING, NFR = MShared<MR>::NFR;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -651,7 +687,7 @@ __device__ __forceinline__ void pass_m_body(const PassConst &k, const PassIO &io
     double un[NV];
     fetch_row(rn, un);
     const int ec = e < 0 ? 0 : (e > n - 2 ? n - 2 : e);
-    const EmitIn in = emit_load<DIR>(k, io, cbase + (size_t)ec * stride, ec);
+    const EmitIn in = emit_load<DIR, RK>(k, io, cbase + (size_t)ec * stride, ec, sp);
     if (more) stage_row(rn, un);
     if (emits) {
       double fm[NV], fp[NV];
@@ -661,7 +697,7 @@ __device__ __forceinline__ void pass_m_body(const PassConst &k, const PassIO &io
         fm[v] = sh.sF[v][a][lane];
         fp[v] = sh.sF[v][b][lane];
       }
-      emit_store<DIR>(k, io, cbase + (size_t)e * stride, in, fm, fp);
+      emit_store<DIR, RK>(k, io, cbase + (size_t)e * stride, in, fm, fp, sp);
     }
   }
   // last row: fI[n-1] and fI[n] = fI[0]
@@ -673,7 +709,7 @@ __device__ __forceinline__ void pass_m_body(const PassConst &k, const PassIO &io
       fm[v] = sh.sF[v][a][lane];
       fp[v] = sh.sF0[v][lane];
     }
-    emit<DIR>(k, io, cbase + (size_t)(n - 1) * stride, n - 1, fm, fp);
+    emit<DIR, RK>(k, io, cbase + (size_t)(n - 1) * stride, n - 1, fm, fp, sp);
   }
 }
 
@@ -689,6 +725,14 @@ template <int DIR, int INTERP, int UPW, int W>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_pass_m(PassConst k, PassIO io) {
   __shared__ MShared<4> sh;
   pass_m_body<DIR, INTERP, UPW, 4, W>(k, io, blockIdx.x, sh);
+}
+
+// the z pass with the three-output stage update (SSPRK3 stages)
+template <int INTERP, int UPW, int W>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_pass_z_affine(PassConst k, PassIO io,
+                                                                                             SspIO sp) {
+  __shared__ MShared<4> sh;
+  pass_m_body<2, INTERP, UPW, 4, W, RK_AFFINE>(k, io, blockIdx.x, sh, sp);
 }
 
 // ---------------------------------------------------------------------------
@@ -816,9 +860,9 @@ void launch_d(const Grid &g, const Scheme &sc, const PassConst &k, const PassIO 
 
 }  // namespace
 
-void launch_pass(const Grid &g, const Scheme &sc, int dir, const PassBuffers &b, int mode, int stage, double ca,
-                 double cb, const double *dxg, hipStream_t st) {
-  PassConst k;
+namespace {
+void pass_args(const Grid &g, const Scheme &sc, const PassBuffers &b, int dir, int mode, int stage, double ca,
+               double cb, const double *dxg, PassConst &k, PassIO &io) {
   for (int d = 0; d < 3; d++) k.n[d] = g.n[d];
   k.ncell = g.ncell;
   k.gamma = sc.gamma;
@@ -826,7 +870,6 @@ void launch_pass(const Grid &g, const Scheme &sc, int dir, const PassBuffers &b,
   k.inv_gm1 = 1.0 / (sc.gamma - 1.0);
   k.eps4 = 4.0 * sc.eps;
   k.dxg = dxg;
-  PassIO io;
   io.U = b.U;
   io.R = b.R;
   io.par = b.par;
@@ -840,9 +883,50 @@ void launch_pass(const Grid &g, const Scheme &sc, int dir, const PassBuffers &b,
   io.stage = stage;
   io.ca = ca;
   io.cb = cb;
+  (void)dir;
+}
+
+template <int INTERP, int UPW, int W>
+void launch_ssp_t(const Grid &g, const PassConst &k, const PassIO &io, const SspIO &sp, hipStream_t st) {
+  const size_t nb = (g.ncell / g.n[2] + MW - 1) / MW;
+  hipLaunchKernelGGL((k_pass_z_affine<INTERP, UPW, W>), dim3((unsigned)nb), dim3(256), 0, st, k, io, sp);
+}
+
+template <int INTERP, int UPW>
+void launch_ssp_w(const Grid &g, const Scheme &sc, const PassConst &k, const PassIO &io, const SspIO &sp,
+                  hipStream_t st) {
+  if (sc.weno == WENO_MAPPED) launch_ssp_t<INTERP, UPW, WENO_MAPPED>(g, k, io, sp, st);
+  else if (sc.weno == WENO_Z) launch_ssp_t<INTERP, UPW, WENO_Z>(g, k, io, sp, st);
+  else if (sc.weno == WENO_YC) launch_ssp_t<INTERP, UPW, WENO_YC>(g, k, io, sp, st);
+  else if (sc.weno == WENO_OPTIMAL) launch_ssp_t<INTERP, UPW, WENO_OPTIMAL>(g, k, io, sp, st);
+  else launch_ssp_t<INTERP, UPW, WENO_JS>(g, k, io, sp, st);
+}
+}  // namespace
+
+void launch_pass(const Grid &g, const Scheme &sc, int dir, const PassBuffers &b, int mode, int stage, double ca,
+                 double cb, const double *dxg, hipStream_t st) {
+  PassConst k;
+  PassIO io;
+  pass_args(g, sc, b, dir, mode, stage, ca, cb, dxg, k, io);
   if (dir == 0) launch_d<0>(g, sc, k, io, st);
   else if (dir == 1) launch_d<1>(g, sc, k, io, st);
   else launch_d<2>(g, sc, k, io, st);
+}
+
+void launch_pass_z_affine(const Grid &g, const Scheme &sc, const PassBuffers &b, const double *A, const double *B,
+                          double *O1, double *O2, double *O3, double c1, double c2, double c3, const double *dxg,
+                          hipStream_t st) {
+  PassConst k;
+  PassIO io;
+  pass_args(g, sc, b, 2, OUT_STAGE, 0, 0.0, 0.0, dxg, k, io);
+  const SspIO sp{A, B, O1, O2, O3, c1, c2, c3};
+  if (sc.interp == INTERP_CHAR) {
+    if (sc.upwind == UPWIND_ROE) launch_ssp_w<INTERP_CHAR, UPWIND_ROE>(g, sc, k, io, sp, st);
+    else launch_ssp_w<INTERP_CHAR, UPWIND_RUSANOV>(g, sc, k, io, sp, st);
+  } else {
+    if (sc.upwind == UPWIND_ROE) launch_ssp_w<INTERP_COMP, UPWIND_ROE>(g, sc, k, io, sp, st);
+    else launch_ssp_w<INTERP_COMP, UPWIND_RUSANOV>(g, sc, k, io, sp, st);
+  }
 }
 
 

@@ -80,6 +80,19 @@ def config1(N: int = 256, n_iter: int = 200) -> Workload:
                     seed=20261, field="taylor_green")
 
 
+def config_dns(N: int = 256, n_iter: int = 1000) -> Workload:
+    """The scheme of the reference's published GPU example deck
+    (Examples/3D/NavierStokes3D/DNS_IsotropicTurbulenceDecay_CUDA:
+    solver.inp:9-15, weno.inp:2, physics.inp:2-6): viscous NS3D (Re 333.33,
+    Mach 0.3, Pr 0.72), component-wise mapped WENO5 + Roe, SSPRK3, dt 0.005.
+    The deck's isotropic-turbulence initial condition needs FFTW
+    (Examples/.../aux/init.c), so the field is the seeded Taylor-Green type
+    at Mach 0.3 (seed 20265)."""
+    return Workload("dns-weno5mapped-components-roe-visc-ssprk3", (N, N, N), interp="components",
+                    upwind="roe", Re=333.333333333333333, Minf=0.3, Pr=0.72, weno_type="mapped",
+                    n_iter=n_iter, seed=20265, field="taylor_green", dt=0.005, time_scheme_type="ssprk3")
+
+
 def config3(N: int = 128, n_iter: int = 50) -> Workload:
     """Adjoint workload: config-0 physics at 128^3 (seed 20263)."""
     return Workload("euler-adjoint", (N, N, N), n_iter=n_iter, seed=20263,

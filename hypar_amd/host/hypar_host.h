@@ -89,7 +89,7 @@ typedef struct {
 typedef struct TimeIntegration {
   HyPar *sims;
   MPIVariables *mpi;
-  int nsims, iter, n_iter, fused;
+  int nsims, iter, n_iter, fused, ssp; /* ssp: the fused step is SSPRK3 */
   double waqt, dt, max_cfl, norm;
   ExplicitRKParameters rk;
   double *U[4], *Udot[4];   /* per member, generic path: stage arrays */

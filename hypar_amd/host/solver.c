@@ -8,8 +8,8 @@
  * when every slot is the stock NavierStokes3D/WENO5 one, calls the fused
  * device operator that computes the same thing in one kernel per direction.
  * Time integration follows TimeRK.c:126-195 (any explicit tableau of
- * TimeExplicitRKInitialize.c) or, for "rk 44", can use the fused 3N RK4
- * step of the device library.
+ * TimeExplicitRKInitialize.c) or, for "rk 44" and "rk ssprk3", can use the
+ * fused steps of the device library.
  */
 #include <math.h>
 #include <stdio.h>
@@ -379,7 +379,9 @@ int TimeInitialize(TimeIntegration *TS, HyPar *sims, MPIVariables *mpi, int nsim
   TS->n_iter = sims[0].n_iter;
   TS->waqt = sims[0].restart_iter * sims[0].dt;
   if (TimeExplicitRKInitialize(sims[0].time_scheme, sims[0].time_scheme_type, &TS->rk)) return 1;
-  TS->fused = fused && TS->rk.nstages == 4 && !strcmp(sims[0].time_scheme_type, "44");
+  const char *type = sims[0].time_scheme_type;
+  TS->ssp = !strcmp(type, "ssprk3") || !strcmp(type, "tvdrk3");
+  TS->fused = fused && ((TS->rk.nstages == 4 && !strcmp(type, "44")) || TS->ssp);
   for (int n = 0; n < nsims; n++)
     if (!is_stock(&sims[n], sims[n].FFunction, sims[n].Upwind)) TS->fused = 0;
   TS->RHSFunction = TimeRHSFunctionExplicit;
@@ -395,7 +397,9 @@ int TimeInitialize(TimeIntegration *TS, HyPar *sims, MPIVariables *mpi, int nsim
       if (hm_check(hm_malloc(&TS->Kstore[4 * n + st], n5(&sims[n])), "hm_malloc")) return 1;
     }
   }
-  if (TS->fused && nsims > 1) return EnsembleInitialize(TS);
+  /* the batched ensemble advances RK4 members; SSPRK3 members step one
+   * after another */
+  if (TS->fused && nsims > 1 && !TS->ssp) return EnsembleInitialize(TS);
   return 0;
 }
 
@@ -478,8 +482,8 @@ int TimeRK(void *ts) {
   return 0;
 }
 
-/* one fused 3N RK4 step per member (same operation order as TimeRK); for
- * ensembles one batched step of all members */
+/* one fused RK4 or SSPRK3 step per member (same operation order as
+ * TimeRK); for RK4 ensembles one batched step of all members */
 int TimeRKFused(void *ts) {
   TimeIntegration *TS = (TimeIntegration *)ts;
   if (TS->ens) {
@@ -488,8 +492,12 @@ int TimeRKFused(void *ts) {
     return 0;
   }
   for (int n = 0; n < TS->nsims; n++) {
-    if (hm_check(hm_rk4(TS->sims[n].dev, TS->sims[n].u, TS->dt, 1), "hm_rk4")) return 1;
-    TS->sims[n].count_sou += 4;  /* the (zero) source term of each of the four stages */
+    if (TS->ssp) {
+      if (hm_check(hm_ssprk3(TS->sims[n].dev, TS->sims[n].u, TS->dt, 1), "hm_ssprk3")) return 1;
+    } else if (hm_check(hm_rk4(TS->sims[n].dev, TS->sims[n].u, TS->dt, 1), "hm_rk4")) {
+      return 1;
+    }
+    TS->sims[n].count_sou += TS->rk.nstages;  /* the (zero) source term of every stage */
   }
   return 0;
 }

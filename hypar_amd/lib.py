@@ -49,6 +49,7 @@ SIGNATURES = [
     ("hm_upload", [_VP, _DP, _DP, ctypes.c_int], ctypes.c_int),
     ("hm_download", [_VP, _DP, _DP, ctypes.c_int], ctypes.c_int),
     ("hm_rk4", [_VP, _DP, ctypes.c_double, ctypes.c_int], ctypes.c_int),
+    ("hm_ssprk3", [_VP, _DP, ctypes.c_double, ctypes.c_int], ctypes.c_int),
     ("hm_rhs", [_VP, _DP, _DP], ctypes.c_int),
     ("hm_hyperbolic", [_VP, _DP, _DP], ctypes.c_int),
     ("hm_parabolic", [_VP, _DP, _DP], ctypes.c_int),

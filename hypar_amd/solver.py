@@ -104,6 +104,22 @@ class NS3D:
         _lib.check(self.L.hm_rk4(self.h, ctypes.c_void_p(u.data_ptr()), float(dt), int(nsteps)))
         return u
 
+    def ssprk3(self, u: torch.Tensor, dt: float, nsteps: int) -> torch.Tensor:
+        """nsteps SSPRK3 steps in place (enqueued on torch's current stream)."""
+        self._check_state(u)
+        self._stream()
+        _lib.check(self.L.hm_ssprk3(self.h, ctypes.c_void_p(u.data_ptr()), float(dt), int(nsteps)))
+        return u
+
+    def step(self, u: torch.Tensor, dt: float, nsteps: int, scheme: str = "44") -> torch.Tensor:
+        """nsteps steps of the fused explicit Runge-Kutta scheme named as in
+        solver.inp's time_scheme_type ("44" or "ssprk3"/"tvdrk3")."""
+        if scheme == "44":
+            return self.rk4(u, dt, nsteps)
+        if scheme in ("ssprk3", "tvdrk3"):
+            return self.ssprk3(u, dt, nsteps)
+        raise ValueError(f"no fused step for time_scheme_type {scheme}")
+
     def _term(self, fn, u):
         self._check_state(u)
         out = self.empty()
@@ -189,7 +205,7 @@ class NS3D:
         self._stream()
         ms = (ctypes.c_double * 5)()
         _lib.check(self.L.hm_profile_kernels(self.h, ctypes.c_void_p(u.data_ptr()), int(reps), ms))
-        return {"pass_x": ms[0], "pass_y": ms[1], "pass_z": ms[2], "visc_flux": ms[3], "prims": ms[4]}
+        return {"pass_x": ms[0], "pass_y": ms[1], "pass_z": ms[2], "visc_par": ms[3], "prims": ms[4]}
 
 
 def soa_to_aos(t: torch.Tensor, dims) -> np.ndarray:

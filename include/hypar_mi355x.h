@@ -16,7 +16,7 @@
  * fastest, 3 ghost layers) is accepted at the host transfer entry points.
  *
  * Reference interfaces replaced (paths relative to the reference root):
- *   hm_rk4 ............. HyPar::TimeIntegrate = TimeRK (include/hypar.h:220,
+ *   hm_rk4, hm_ssprk3 .. HyPar::TimeIntegrate = TimeRK (include/hypar.h:220,
  *                        src/TimeIntegration/TimeRK.c:35-202)
  *   hm_rhs ............. TimeIntegration::RHSFunction = TimeRHSFunctionExplicit
  *                        (src/TimeIntegration/TimeRHSFunctionExplicit.c:30-98)
@@ -110,6 +110,9 @@ hm_status hm_download(hm_solver *s, double *host, const double *dev, int layout)
 /* nsteps classical RK4 steps of size dt on the hm-layout device state u,
  * in place (TimeRK with the "rk 44" tableau). */
 hm_status hm_rk4(hm_solver *s, double *u, double dt, int nsteps);
+/* nsteps three-stage SSP Runge-Kutta steps, in place (TimeRK with the
+ * "rk ssprk3" / "tvdrk3" tableau, TimeExplicitRKInitialize.c:69-79). */
+hm_status hm_ssprk3(hm_solver *s, double *u, double dt, int nsteps);
 
 /* rhs = -hyperbolic + parabolic (+ zero source) of the device state u.
  * rhs and u must be distinct arrays (HM_ERR_ARG otherwise). */

@@ -765,34 +765,49 @@ int orc_rhs(const orc_params *p, double *rhs, double *u) {
   return ierr;
 }
 
-int orc_rk4(const orc_params *p, double *u, double dt, int nsteps) {
-  const double A[16] = {0, 0, 0, 0, 0.5, 0, 0, 0, 0, 0.5, 0, 0, 0, 0, 1.0, 0};
-  const double b[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+/* TimeRK.c:126-195 for the explicit tableau A (ns x ns, row-major, every
+ * entry below the diagonal applied, zeros included) and b */
+static int orc_rk(const orc_params *p, double *u, double dt, int nsteps, int ns, const double *A,
+                  const double *b) {
   long n = orc_size(p);
   double *U = (double *)malloc(sizeof(double) * n);
   double *K[4];
-  for (int s = 0; s < 4; s++) K[s] = (double *)malloc(sizeof(double) * n);
+  for (int s = 0; s < ns; s++) K[s] = (double *)malloc(sizeof(double) * n);
   int ierr = 0;
   for (int it = 0; it < nsteps && !ierr; it++) {
     orc_periodic_bc(p, u); /* TimePreStep */
-    for (int stage = 0; stage < 4; stage++) {
+    for (int stage = 0; stage < ns; stage++) {
       memcpy(U, u, sizeof(double) * n);
       for (int i = 0; i < stage; i++) {
-        double a = dt * A[stage * 4 + i];
+        double a = dt * A[stage * ns + i];
 #pragma omp parallel for schedule(static)
         for (long q = 0; q < n; q++) U[q] += a * K[i][q];
       }
       ierr |= orc_rhs(p, K[stage], U);
     }
-    for (int stage = 0; stage < 4; stage++) {
+    for (int stage = 0; stage < ns; stage++) {
       double a = dt * b[stage];
 #pragma omp parallel for schedule(static)
       for (long q = 0; q < n; q++) u[q] += a * K[stage][q];
     }
   }
   free(U);
-  for (int s = 0; s < 4; s++) free(K[s]);
+  for (int s = 0; s < ns; s++) free(K[s]);
   return ierr;
+}
+
+/* "rk 44" (TimeExplicitRKInitialize.c:58-68) */
+int orc_rk4(const orc_params *p, double *u, double dt, int nsteps) {
+  const double A[16] = {0, 0, 0, 0, 0.5, 0, 0, 0, 0, 0.5, 0, 0, 0, 0, 1.0, 0};
+  const double b[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+  return orc_rk(p, u, dt, nsteps, 4, A, b);
+}
+
+/* "rk ssprk3" / "tvdrk3" (TimeExplicitRKInitialize.c:69-79) */
+int orc_ssprk3(const orc_params *p, double *u, double dt, int nsteps) {
+  const double A[9] = {0, 0, 0, 1.0, 0, 0, 0.25, 0.25, 0};
+  const double b[3] = {1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0};
+  return orc_rk(p, u, dt, nsteps, 3, A, b);
 }
 
 double orc_cfl(const orc_params *p, const double *u, double dt) {

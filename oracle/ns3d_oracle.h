@@ -51,6 +51,8 @@ int orc_parabolic(const orc_params *p, double *par, const double *u);
 
 /* nsteps classical RK4 steps of size dt in place (TimeRK). */
 int orc_rk4(const orc_params *p, double *u, double dt, int nsteps);
+/* nsteps SSPRK3 steps ("rk ssprk3", TimeExplicitRKInitialize.c:69-79) */
+int orc_ssprk3(const orc_params *p, double *u, double dt, int nsteps);
 
 /* Max CFL over the interior (NavierStokes3DComputeCFL). */
 double orc_cfl(const orc_params *p, const double *u, double dt);

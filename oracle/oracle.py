@@ -42,6 +42,7 @@ def lib():
         L.orc_hyperbolic.argtypes = [pp, dp, dp]; L.orc_hyperbolic.restype = ctypes.c_int
         L.orc_parabolic.argtypes = [pp, dp, dp]; L.orc_parabolic.restype = ctypes.c_int
         L.orc_rk4.argtypes = [pp, dp, ctypes.c_double, ctypes.c_int]; L.orc_rk4.restype = ctypes.c_int
+        L.orc_ssprk3.argtypes = [pp, dp, ctypes.c_double, ctypes.c_int]; L.orc_ssprk3.restype = ctypes.c_int
         L.orc_cfl.argtypes = [pp, dp, ctypes.c_double]; L.orc_cfl.restype = ctypes.c_double
         L.orc_volume_integral.argtypes = [pp, dp, dp]; L.orc_volume_integral.restype = None
         L.orc_periodic_bc.argtypes = [pp, dp]; L.orc_periodic_bc.restype = None
@@ -103,10 +104,15 @@ class Oracle:
 
     # calls ---------------------------------------------------------------
     def rk4(self, u, dt, nsteps):
+        return self.step(u, dt, nsteps, "44")
+
+    def step(self, u, dt, nsteps, scheme="44"):
+        """nsteps steps of the rk-class scheme named as in solver.inp's
+        time_scheme_type ("44", or "ssprk3"/"tvdrk3")."""
+        fn = {"44": lib().orc_rk4, "ssprk3": lib().orc_ssprk3, "tvdrk3": lib().orc_ssprk3}[scheme]
         G = self.ghosted(u)
-        ierr = lib().orc_rk4(ctypes.byref(self.p), _dp(G), float(dt), int(nsteps))
-        if ierr:
-            raise RuntimeError("orc_rk4 failed")
+        if fn(ctypes.byref(self.p), _dp(G), float(dt), int(nsteps)):
+            raise RuntimeError(f"oracle {scheme} step failed")
         return self.interior(G)
 
     def rhs(self, u):

@@ -56,6 +56,8 @@ def _workload(kind: str, dims, n_iter: int) -> decks.Workload:
     if kind == "c0_ssprk3":
         return dataclasses.replace(decks.config0(dims[0], n_iter), dims=tuple(dims), n_iter=n_iter,
                                    dt=None, time_scheme_type="ssprk3")
+    if kind == "dns":
+        return dataclasses.replace(decks.config_dns(dims[0], n_iter), dims=tuple(dims), dt=None)
     if kind == "c0":
         w = decks.config0(dims[0], n_iter)
     elif kind == "c1":
@@ -77,6 +79,10 @@ FIXTURES = {
     "c1_32_s20": ("c1", (32, 32, 32), 20, False),
     "c1_64_s200": ("c1", (64, 64, 64), 200, False),
     "c0_ssprk3_16_s4": ("c0_ssprk3", (16, 16, 16), 4, True),
+    # the scheme of the reference's published GPU deck (viscous, mapped
+    # component-wise WENO5 + Roe, SSPRK3), dt from CFL 0.5
+    "dns_16_s4": ("dns", (16, 16, 16), 4, True),
+    "dns_ragged_s3": ("dns", (12, 20, 16), 3, True),
     # mapped (Henrick et al.), Z (Borges et al.) and Yamaleev-Carpenter weights
     "c0_mapped_16_s4": ("c0+mapped", (16, 16, 16), 4, True),
     "c0_borges_16_s4": ("c0+borges", (16, 16, 16), 4, True),

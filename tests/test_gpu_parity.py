@@ -52,6 +52,32 @@ def test_rk4_matches_reference_fixture(name):
     assert err <= TOL
 
 
+@pytest.mark.parametrize("name", ["c0_ssprk3_16_s4", "dns_16_s4", "dns_ragged_s3"])
+def test_ssprk3_matches_reference_fixture(name):
+    """The fused SSPRK3 step (hm_ssprk3) against the reference's own SSPRK3
+    runs: the component-wise JS scheme, and the scheme of the reference's
+    published GPU deck (viscous, mapped component-wise WENO5 + Roe)."""
+    from hypar_amd import decks
+    meta, z = load_fixture(name)
+    w = workload_from_meta(meta)
+    assert w.time_scheme_type == "ssprk3"
+    _say(f"[gpu] SSPRK3 fixture {name} dims={w.dims} steps={meta['steps']}")
+    s = _solver(w)
+    u = s.to_device(z["u0"])
+    s.ssprk3(u, w.dt, meta["steps"])
+    err = decks.rel_l2(s.to_host(u), z["uN"])
+    _say(f"[gpu] {name}: rel L2 = {err:.3e}")
+    assert err <= TOL
+    # the step norms against the reference's printed ones
+    u = s.to_device(z["u0"])
+    for n in range(meta["steps"]):
+        v = u.clone()
+        s.ssprk3(v, w.dt, 1)
+        norm, pn = s.step_norm(v, u), meta["trace"]["norm"][n]
+        assert _within_print(norm, pn, 4), (n, norm, pn)
+        u = v
+
+
 @pytest.mark.parametrize("name", ["c1_32_s20", "c0_64_s100", "c1_64_s200"])
 def test_rk4_long_runs_match_reference(name):
     from hypar_amd import decks

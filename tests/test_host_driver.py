@@ -71,6 +71,7 @@ def test_missing_solver_inp(tmp_path):
 @pytest.mark.gpu
 @pytest.mark.parametrize("name,args", [("c0_16_s10", []), ("c1_16_s10", []), ("c1_16_s10", ["--slots"]),
                                        ("c0_16_s1", ["--slots"]), ("c0_ssprk3_16_s4", []),
+                                       ("c0_ssprk3_16_s4", ["--slots"]), ("dns_16_s4", []), ("dns_ragged_s3", ["--slots"]),
                                        ("c1_mapped_16_s4", []), ("c0_yc_16_s4", ["--slots"]),
                                        ("c1_borges_16_s4", ["--slots"]), ("c1_optimal_16_s4", []),
                                        ("c0_optimal_16_s4", ["--slots"])])
@@ -79,8 +80,10 @@ def test_driver_matches_reference(tmp_path, name, args):
     print(f"[gpu] driver {name} {args}", flush=True)
     r = run_driver(d, *args)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
-    if args or meta.get("time_scheme_type", "44") != "44":
+    if args:
         assert "slot composition" in r.stdout
+    elif meta.get("time_scheme_type", "44") == "ssprk3":
+        assert "fused SSPRK3 step" in r.stdout
     else:
         assert "fused 3N RK4 step" in r.stdout
     _, _, u = decks.read_op_bin(os.path.join(d, "op.bin"))

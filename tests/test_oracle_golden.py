@@ -11,7 +11,8 @@ from hypar_amd import decks
 from oracle.oracle import Oracle
 
 WEIGHTS = [f"{c}_{t}_16_s4" for c in ("c0", "c1") for t in ("mapped", "borges", "yc", "optimal")]
-FULL = ["c0_16_s1", "c0_16_s10", "c1_16_s1", "c1_16_s10", "c0_ragged_s3", "c1_ragged_s3"] + WEIGHTS
+SSP = ["c0_ssprk3_16_s4", "dns_16_s4", "dns_ragged_s3"]
+FULL = ["c0_16_s1", "c0_16_s10", "c1_16_s1", "c1_16_s10", "c0_ragged_s3", "c1_ragged_s3"] + WEIGHTS + SSP
 
 
 @pytest.mark.parametrize("name", FULL)
@@ -21,7 +22,7 @@ def test_oracle_matches_reference_bitwise(name):
     u0 = z["u0"]
     # the generator reproduces the stored input exactly
     assert np.array_equal(decks.initial_state(w), u0)
-    got = Oracle.for_workload(w).rk4(u0, w.dt, meta["steps"])
+    got = Oracle.for_workload(w).step(u0, w.dt, meta["steps"], w.time_scheme_type)
     assert np.array_equal(got, z["uN"]), f"max diff {np.abs(got - z['uN']).max()}"
 
 
@@ -80,7 +81,7 @@ def test_oracle_volume_integral_matches_reference_print():
 TRACED = FULL + ["c1_16_s4_cons", "c0_ragged_s3_cons"]
 
 
-def printed_trace(o, u0, dt, steps):
+def printed_trace(o, u0, dt, steps, scheme="44"):
     """The CFL before each step and the RMS of each step's change, formatted
     as the reference prints them (TimePrintStep.c:28-33).  The norm sums the
     squares point by point with the variables inner (ArraySumSquarenD,
@@ -91,7 +92,7 @@ def printed_trace(o, u0, dt, steps):
     npts = u0.size // 5
     for _ in range(steps):
         cfl.append("%1.3E" % o.cfl(u, dt))
-        un = o.rk4(u, dt, 1)
+        un = o.step(u, dt, 1, scheme)
         d = (u - un).ravel()
         sums.append(float(np.cumsum(d * d)[-1]))
         norm.append("%1.4E" % math.sqrt(sums[-1] / npts))
@@ -105,7 +106,7 @@ def test_oracle_cfl_and_norm_match_reference_print(name):
     reference's own printed CFL= and norm= values, string for string."""
     meta, z = load_fixture(name)
     w = workload_from_meta(meta)
-    cfl, norm, _ = printed_trace(Oracle.for_workload(w), z["u0"], w.dt, meta["steps"])
+    cfl, norm, _ = printed_trace(Oracle.for_workload(w), z["u0"], w.dt, meta["steps"], w.time_scheme_type)
     assert cfl == meta["trace"]["cfl"]
     assert norm == meta["trace"]["norm"]
 
