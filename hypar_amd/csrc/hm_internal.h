@@ -127,7 +127,10 @@ __host__ __device__ inline unsigned visc_tiles(int nx, int ny, int nz) {
 #ifndef HM_PTY
 #define HM_PTY 16
 #endif
-constexpr int PTX = HM_PTX, PTY = HM_PTY, PKS = 16;
+#ifndef HM_PKS
+#define HM_PKS 16
+#endif
+constexpr int PTX = HM_PTX, PTY = HM_PTY, PKS = HM_PKS;
 __host__ __device__ inline unsigned par_tiles(int nx, int ny, int nz) {
   return (unsigned)(((nx + PTX - 1) / PTX) * ((ny + PTY - 1) / PTY) * ((nz + PKS - 1) / PKS));
 }
