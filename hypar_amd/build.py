@@ -25,7 +25,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              # no packed FP32 instructions anywhere in device code
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 HIP_SOURCES = ["hm_api.hip", "hm_kernels.hip", "hm_pass.hip", "hm_slots.hip", "hm_adjoint.hip",
-               "hm_noalloc.hip"]
+               "hm_noalloc.hip", "hm_bind.hip"]
 LIB = os.path.join(PKG, "libhypar_mi355x.so")
 
 

@@ -201,6 +201,7 @@ const char *hm_version(void) { return "hypar-mi355x 0.1 (gfx950, fp64)"; }
 
 void hm_destroy(hm_solver *s) {
   if (!s) return;
+  hm::drop_bindings(s);
   double *bufs[] = {s->dxg, s->R, s->par, s->Q, s->S[0], s->S[1], s->acc, s->part, s->tmp};
   for (double *b : bufs)
     if (b) (void)hipFree(b);

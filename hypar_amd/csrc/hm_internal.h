@@ -7,7 +7,12 @@
 #include <cmath>
 #include <string>
 
+struct hm_solver;
+
 namespace hm {
+
+// drops every slot binding of s (hm_bind.hip); hm_destroy calls it
+void drop_bindings(const ::hm_solver *s);
 
 enum { INTERP_COMP = 0, INTERP_CHAR = 1 };
 enum { UPWIND_ROE = 0, UPWIND_RUSANOV = 1 };

@@ -6,12 +6,17 @@
  * its slots (NavierStokes3DInitialize.c:388-506), the reference's input files
  * and a TimeRK that composes the slots (TimeRK.c:35-202).  All arrays the
  * slots see are device arrays in hm layout (include/hypar_mi355x.h); the
- * compute is done by libhypar_mi355x.so.
+ * compute is done by libhypar_mi355x.so.  Where a slot is the library's
+ * own (TimeIntegrate, RHSFunction, HyperbolicFunction, ParabolicFunction,
+ * ComputeCFL), the exported hm_slot_* entry point is installed or called
+ * (include/hypar_mi355x_slots.h), with each solver object bound to its
+ * device solver.
  */
 #ifndef HYPAR_HOST_H
 #define HYPAR_HOST_H
 
 #include "../../include/hypar_mi355x.h"
+#include "../../include/hypar_mi355x_slots.h"
 
 #define HY_STR 500
 
@@ -126,7 +131,6 @@ int NavierStokes3DUpwindRusanov(double *fI, double *fL, double *fR, double *uL, 
 int NavierStokes3DModifiedSolution(double *uC, double *u, int d, void *s, void *m, double t);
 int NavierStokes3DSource(double *source, double *u, void *s, void *m, double t);
 int NavierStokes3DParabolicFunction(double *par, double *u, void *s, void *m, double t);
-double NavierStokes3DComputeCFL(void *s, void *m, double dt, double t);
 int NavierStokes3DPreStep(double *u, void *s, void *m, double t);
 int Interp1PrimFifthOrderWENO(double *fI, double *fC, double *u, double *x, int upw, int dir, void *s,
                               void *m, int uflag);

@@ -179,9 +179,14 @@ int NavierStokes3DInitialize(void *sv, void *m, const char *dir) {
   if (hm_check(hm_create(&cfg, &s->dev), "hm_create")) return 1;
   s->cfg = cfg;
   if (alloc_device(s)) return 1;
+  /* the library's slots find the device solver and the state of this
+   * solver object through the binding */
+  if (hm_check(hm_bind(s, s->dev), "hm_bind") ||
+      hm_check(hm_bind_state(s, s->u, HM_SLOT_ARRAYS_DEVICE), "hm_bind_state"))
+    return 1;
   /* physics slots (NavierStokes3DInitialize.c:388-506) */
   s->PreStep = NavierStokes3DPreStep;
-  s->ComputeCFL = NavierStokes3DComputeCFL;
+  s->ComputeCFL = hm_slot_ComputeCFL;
   s->FFunction = NavierStokes3DFlux;
   s->SFunction = NavierStokes3DSource;
   s->UFunction = NavierStokes3DModifiedSolution;
@@ -226,18 +231,12 @@ int NavierStokes3DSource(double *source, double *u, void *s, void *m, double t) 
   return hm_check(hm_fill(h->dev, source, 0.0, n5(h)), "NavierStokes3DSource");
 }
 
+/* the library's slot, plus the call count the reference keeps
+ * (NavierStokes3DParabolicFunction.c:71-73) */
 int NavierStokes3DParabolicFunction(double *par, double *u, void *s, void *m, double t) {
-  (void)m; (void)t;
   HyPar *h = (HyPar *)s;
-  return hm_check(hm_parabolic(h->dev, par, u), "NavierStokes3DParabolicFunction");
-}
-
-double NavierStokes3DComputeCFL(void *s, void *m, double dt, double t) {
-  (void)m; (void)t;
-  HyPar *h = (HyPar *)s;
-  double cfl = -1.0;
-  if (hm_check(hm_cfl(h->dev, h->u, dt, &cfl), "NavierStokes3DComputeCFL")) return -1.0;
-  return cfl;
+  if (h->physics->Re > 0) h->count_par++;
+  return hm_check(hm_slot_ParabolicFunction(par, u, s, m, t), "ParabolicFunction");
 }
 
 /* The reference's PreStep builds a split-flux Jacobian that only
@@ -297,8 +296,10 @@ int HyperbolicFunction(double *hyp, double *u, void *sv, void *m, double t, int 
                        hy_flux_fn FFunction, hy_upwind_fn UpwindFunction) {
   HyPar *s = (HyPar *)sv;
   if (!FFunction) return hm_check(hm_fill(s->dev, hyp, 0.0, n5(s)), "HyperbolicFunction");
+  s->count_hyp++;  /* HyperbolicFunction.c:67 */
   if (is_stock(s, FFunction, UpwindFunction))
-    return hm_check(hm_hyperbolic(s->dev, hyp, u), "HyperbolicFunction");
+    return hm_check(hm_slot_HyperbolicFunction(hyp, u, sv, m, t, LimFlag, FFunction, UpwindFunction),
+                    "HyperbolicFunction");
   for (int d = 0; d < 3; d++) {
     if (FFunction(s->fluxC, u, d, s, t)) return 1;
     if (LimFlag && s->SetInterpLimiterVar && s->SetInterpLimiterVar(s->fluxC, u, s->x, d, s, m)) return 1;
@@ -382,10 +383,20 @@ int TimeInitialize(TimeIntegration *TS, HyPar *sims, MPIVariables *mpi, int nsim
   const char *type = sims[0].time_scheme_type;
   TS->ssp = !strcmp(type, "ssprk3") || !strcmp(type, "tvdrk3");
   TS->fused = fused && ((TS->rk.nstages == 4 && !strcmp(type, "44")) || TS->ssp);
+  int stock = 1;
   for (int n = 0; n < nsims; n++)
-    if (!is_stock(&sims[n], sims[n].FFunction, sims[n].Upwind)) TS->fused = 0;
-  TS->RHSFunction = TimeRHSFunctionExplicit;
+    if (!is_stock(&sims[n], sims[n].FFunction, sims[n].Upwind)) stock = 0;
+  TS->fused = TS->fused && stock;
+  /* Fused steps go through the library's TimeIntegrate slot.  Without
+   * --slots, the other tableaux take TimeRK with the library's fused
+   * right-hand side; with --slots (fused == 0), TimeRHSFunctionExplicit
+   * composes the slots. */
+  TS->RHSFunction = (fused && stock) ? hm_slot_RHSFunction : TimeRHSFunctionExplicit;
   TS->TimeIntegrate = TS->fused ? TimeRKFused : TimeRK;
+  if (TS->fused)
+    for (int n = 0; n < nsims; n++)
+      if (hm_check(hm_bind_time(TS, &sims[n], TS->dt, TS->ssp ? HM_TIME_SSPRK3 : HM_TIME_RK4), "hm_bind_time"))
+        return 1;
   TS->Ustore = (double **)calloc((size_t)4 * nsims, sizeof(double *));
   TS->Kstore = (double **)calloc((size_t)4 * nsims, sizeof(double *));
   TS->uprev = (double **)calloc((size_t)nsims, sizeof(double *));
@@ -441,11 +452,13 @@ int EnsembleInitialize(TimeIntegration *TS) {
     hm_free(s->u);
     s->u = dst;
     s->u_shared = 1;
+    if (hm_check(hm_bind_state(s, s->u, HM_SLOT_ARRAYS_DEVICE), "hm_bind_state")) return 1;
   }
   return 0;
 }
 
 void TimeCleanup(TimeIntegration *TS) {
+  hm_unbind(TS);
   if (TS->ens) hm_ensemble_destroy(TS->ens);
   hm_free(TS->ens_u);
   for (int i = 0; i < 4 * TS->nsims; i++) {
@@ -457,6 +470,16 @@ void TimeCleanup(TimeIntegration *TS) {
   free(TS->Ustore);
   free(TS->Kstore);
   free(TS->uprev);
+}
+
+/* The call counts of `nrhs` right-hand sides evaluated inside the library:
+ * one hyperbolic, one parabolic (viscous only) and one source term each
+ * (HyperbolicFunction.c:67, NavierStokes3DParabolicFunction.c:71-73,
+ * SourceFunction.c:46-49). */
+static void count_rhs(HyPar *s, int nrhs) {
+  s->count_hyp += nrhs;
+  if (s->physics->Re > 0) s->count_par += nrhs;
+  s->count_sou += nrhs;
 }
 
 /* TimeRK.c:126-195: stage values, right-hand sides, step completion */
@@ -474,6 +497,7 @@ int TimeRK(void *ts) {
                      "TimeRK axpy"))
           return 1;
       if (TS->RHSFunction(K[stage], U[stage], s, &TS->mpi[n], stagetime)) return 1;
+      if (TS->RHSFunction == hm_slot_RHSFunction) count_rhs(s, 1);
     }
     for (int stage = 0; stage < p->nstages; stage++)
       if (hm_check(hm_axpy(s->dev, K[stage], TS->dt * p->b[stage], s->u, n5(s)), "TimeRK update"))
@@ -488,17 +512,10 @@ int TimeRKFused(void *ts) {
   TimeIntegration *TS = (TimeIntegration *)ts;
   if (TS->ens) {
     if (hm_check(hm_ensemble_rk4(TS->ens, TS->ens_u, 1), "hm_ensemble_rk4")) return 1;
-    for (int n = 0; n < TS->nsims; n++) TS->sims[n].count_sou += 4;
-    return 0;
+  } else if (hm_check(hm_slot_TimeIntegrate(ts), "TimeIntegrate")) {
+    return 1;
   }
-  for (int n = 0; n < TS->nsims; n++) {
-    if (TS->ssp) {
-      if (hm_check(hm_ssprk3(TS->sims[n].dev, TS->sims[n].u, TS->dt, 1), "hm_ssprk3")) return 1;
-    } else if (hm_check(hm_rk4(TS->sims[n].dev, TS->sims[n].u, TS->dt, 1), "hm_rk4")) {
-      return 1;
-    }
-    TS->sims[n].count_sou += TS->rk.nstages;  /* the (zero) source term of every stage */
-  }
+  for (int n = 0; n < TS->nsims; n++) count_rhs(&TS->sims[n], TS->rk.nstages);
   return 0;
 }
 

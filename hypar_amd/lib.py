@@ -37,7 +37,7 @@ class HMError(RuntimeError):
 
 _LIB = None
 
-# (name, argtypes, restype); every symbol declared in include/hypar_mi355x.h
+# (name, argtypes, restype); every symbol declared in include/*.h
 _VP, _DP, _SZ = ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t
 SIGNATURES = [
     ("hm_create", [ctypes.POINTER(HMConfig), ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
@@ -90,7 +90,40 @@ SIGNATURES = [
     ("hm_ensemble_get_tiles", [_VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint)], ctypes.c_int),
     ("hm_ensemble_tune", [_VP, _DP, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint)],
      ctypes.c_int),
+    # include/hypar_mi355x_slots.h
+    ("hm_bind", [_VP, _VP], ctypes.c_int),
+    ("hm_bind_state", [_VP, _DP, ctypes.c_int], ctypes.c_int),
+    ("hm_bind_time", [_VP, _VP, ctypes.c_double, ctypes.c_int], ctypes.c_int),
+    ("hm_unbind", [_VP], None),
+    ("hm_slot_TimeIntegrate", [_VP], ctypes.c_int),
+    ("hm_slot_RHSFunction", [_DP, _DP, _VP, _VP, ctypes.c_double], ctypes.c_int),
+    ("hm_slot_HyperbolicFunction", [_DP, _DP, _VP, _VP, ctypes.c_double, ctypes.c_int, _VP, _VP], ctypes.c_int),
+    ("hm_slot_ParabolicFunction", [_DP, _DP, _VP, _VP, ctypes.c_double], ctypes.c_int),
+    ("hm_slot_ComputeCFL", [_VP, _VP, ctypes.c_double, ctypes.c_double], ctypes.c_double),
 ]
+
+# include/hypar_mi355x_slots.h: the arrays a bound object's slots receive,
+# and the tableaux of hm_bind_time
+SLOT_ARRAYS_DEVICE, SLOT_ARRAYS_REFERENCE_HOST = 0, 1
+TIME_RK4, TIME_SSPRK3 = 0, 1
+
+# HyPar's slot prototypes (reference include/hypar.h:220-276,
+# include/timeintegration_struct.h:96-98), for holding the hm_slot_* entry
+# points as plain function pointers
+FFunctionProto = ctypes.CFUNCTYPE(ctypes.c_int, _DP, _DP, ctypes.c_int, _VP, ctypes.c_double)
+UpwindProto = ctypes.CFUNCTYPE(ctypes.c_int, _DP, _DP, _DP, _DP, _DP, _DP, ctypes.c_int, _VP, ctypes.c_double)
+TimeIntegrateProto = ctypes.CFUNCTYPE(ctypes.c_int, _VP)
+RHSFunctionProto = ctypes.CFUNCTYPE(ctypes.c_int, _DP, _DP, _VP, _VP, ctypes.c_double)
+HyperbolicFunctionProto = ctypes.CFUNCTYPE(ctypes.c_int, _DP, _DP, _VP, _VP, ctypes.c_double, ctypes.c_int,
+                                           FFunctionProto, UpwindProto)
+ParabolicFunctionProto = ctypes.CFUNCTYPE(ctypes.c_int, _DP, _DP, _VP, _VP, ctypes.c_double)
+ComputeCFLProto = ctypes.CFUNCTYPE(ctypes.c_double, _VP, _VP, ctypes.c_double, ctypes.c_double)
+
+
+def slot_pointer(name: str, proto):
+    """The address of the library's entry point `name` as a function
+    pointer of type `proto`: what a C caller stores in a HyPar slot."""
+    return proto(ctypes.cast(getattr(load(), name), ctypes.c_void_p).value)
 
 
 def load(path: str = LIB_PATH):

@@ -41,3 +41,13 @@ def workload_from_meta(meta):
                           seed=meta["seed"], field=meta["field"], dt=meta["dt"],
                           time_scheme_type=meta.get("time_scheme_type", "44"),
                           weno_type=meta.get("weno_type", "js"))
+
+
+def within_print(got, printed, digits):
+    """|got - printed| within half a unit of the printed value's last digit
+    (plus rounding-level slack: the GPU state differs from the reference's
+    at the 1e-14 level, which may move a value across a rounding boundary)."""
+    import numpy as np
+    ref = float(printed)
+    e = np.floor(np.log10(abs(ref)))
+    return abs(got - ref) <= 0.5 * 10.0 ** (e - digits) + 1e-12 * abs(ref)

@@ -15,7 +15,7 @@ import sys
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, load_fixture, workload_from_meta
+from conftest import GOLDEN, load_fixture, within_print as _within_print, workload_from_meta
 
 torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
@@ -161,15 +161,6 @@ def test_fused_passes_on_ragged_lines(dims, interp, upwind, Re):
     err = decks.rel_l2(s.to_host(u), o.rk4(u0, dt, 3))
     _say(f"[gpu]   3 steps rel L2 = {err:.3e}")
     assert err <= TOL
-
-
-def _within_print(got, printed, digits):
-    """|got - printed| within half a unit of the printed value's last digit
-    (plus rounding-level slack: the GPU state differs from the reference's
-    at the 1e-14 level, which may move a value across a rounding boundary)."""
-    ref = float(printed)
-    e = np.floor(np.log10(abs(ref)))
-    return abs(got - ref) <= 0.5 * 10.0 ** (e - digits) + 1e-12 * abs(ref)
 
 
 TRACED = ["c0_16_s10", "c1_16_s10", "c0_ragged_s3", "c1_ragged_s3", "c0_mapped_16_s4", "c1_yc_16_s4"]

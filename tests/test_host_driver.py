@@ -88,6 +88,14 @@ def test_driver_matches_reference(tmp_path, name, args):
         assert "fused 3N RK4 step" in r.stdout
     _, _, u = decks.read_op_bin(os.path.join(d, "op.bin"))
     assert decks.rel_l2(u, z["uN"]) <= 1e-12
+    # function_counts.dat (SimulationWriteErrors.c:91-95): one hyperbolic,
+    # parabolic (viscous only) and source evaluation per stage
+    # (HyperbolicFunction.c:67, NavierStokes3DParabolicFunction.c:71-73,
+    # SourceFunction.c:46-49)
+    nrhs = meta["steps"] * (3 if meta.get("time_scheme_type", "44") == "ssprk3" else 4)
+    with open(os.path.join(d, "function_counts.dat")) as f:
+        counts = [int(x) for x in f.read().split()]
+    assert counts == [meta["steps"], nrhs, nrhs if meta["Re"] > 0 else 0, nrhs], counts
 
 
 @pytest.mark.gpu
