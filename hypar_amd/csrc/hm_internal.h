@@ -125,7 +125,8 @@ __host__ __device__ inline unsigned visc_tiles(int nx, int ny, int nz) {
 }
 
 // tiles of the viscous-term kernel (hm_kernels.hip): a block owns PTX x PTY
-// (x, y) columns and marches through PKS z-planes
+// (x, y) columns and marches through a z segment of pks planes: PKS in the
+// batched ensemble, visc_pks() of the grid for one grid
 #ifndef HM_PTX
 #define HM_PTX 32
 #endif
@@ -136,8 +137,20 @@ __host__ __device__ inline unsigned visc_tiles(int nx, int ny, int nz) {
 #define HM_PKS 16
 #endif
 constexpr int PTX = HM_PTX, PTY = HM_PTY, PKS = HM_PKS;
-__host__ __device__ inline unsigned par_tiles(int nx, int ny, int nz) {
-  return (unsigned)(((nx + PTX - 1) / PTX) * ((ny + PTY - 1) / PTY) * ((nz + PKS - 1) / PKS));
+__host__ __device__ inline unsigned par_tiles(int nx, int ny, int nz, int pks = PKS) {
+  return (unsigned)(((nx + PTX - 1) / PTX) * ((ny + PTY - 1) / PTY) * ((nz + pks - 1) / pks));
+}
+// Segment length of the one-grid viscous kernel: the longest of 128, 64, 32
+// planes that still gives at least VISC_MIN_TILES tiles (the kernel holds
+// one 512-thread block per CU, 256 CUs), else PKS.  Longer segments read
+// fewer halo planes: a segment stages 8 planes besides its own.
+#ifndef HM_VISC_MIN_TILES
+#define HM_VISC_MIN_TILES 512
+#endif
+inline int visc_pks(int nx, int ny, int nz) {
+  for (int p = 128; p > PKS; p >>= 1)
+    if (par_tiles(nx, ny, nz, p) >= (unsigned)HM_VISC_MIN_TILES) return p;
+  return PKS;
 }
 
 // batched ensemble passes (hm_pass.hip): members of one scheme

@@ -88,7 +88,7 @@ __device__ __forceinline__ void visc_flux(double fv[4], const QT &q, const doubl
 // ghost-layer fluxes where its stencils need them.
 //
 // A block owns a TX x TY tile of (x, y) columns and marches through the
-// planes of one z segment [k0, k1) (PKS planes).  Plane q of Q with a
+// planes of one z segment [k0, k1) (pks planes).  Plane q of Q with a
 // 4-cell halo in x and y is staged in LDS.  Every thread evaluates the
 // gradient at its own column and at most one halo point (the rows y0-2,
 // y0-1, y0+TY, y0+TY+1 and the columns x0-2, x0-1, x0+TX, x0+TX+1 of the
@@ -148,15 +148,15 @@ template <int TX, int TY>
 __device__ __forceinline__ void visc_par_tile(const double *__restrict__ Q, double *__restrict__ par, int nx,
                                               int ny, int nz,
                                               const double *__restrict__ dxg,  // ghosted dxinv x|y|z
-                                              const ViscParams &vp, int b, ViscParShared<TX, TY> &sh) {
+                                              const ViscParams &vp, int b, int pks, ViscParShared<TX, TY> &sh) {
   constexpr int NT = TX * TY;
   static_assert(4 * TX + 4 * TY <= NT, "every halo point needs a thread");
   const size_t ncell = (size_t)nx * ny * nz;
   const size_t plane = (size_t)nx * ny;
   const int ntx = (nx + TX - 1) / TX, nty = (ny + TY - 1) / TY;
   const int tx = b % ntx, ty = (b / ntx) % nty, kz = b / (ntx * nty);
-  const int x0 = tx * TX, y0 = ty * TY, k0 = kz * PKS;
-  const int k1 = min(k0 + PKS, nz);
+  const int x0 = tx * TX, y0 = ty * TY, k0 = kz * pks;
+  const int k1 = min(k0 + pks, nz);
   const double *dxa = dxg, *dya = dxg + (nx + 6), *dza = dxg + (nx + 6) + (ny + 6);  // entry 3+p: position p
   const int t = threadIdx.x;
   const int lx = t % TX, ly = t / TX;
@@ -401,10 +401,10 @@ __device__ __forceinline__ void visc_par_tile(const double *__restrict__ Q, doub
 }
 
 __global__ void __launch_bounds__(PTX * PTY) k_visc_par(const double *__restrict__ Q, double *__restrict__ par,
-                                                         int nx, int ny, int nz, const double *__restrict__ dxg,
-                                                         ViscParams vp) {
+                                                         int nx, int ny, int nz, int pks,
+                                                         const double *__restrict__ dxg, ViscParams vp) {
   __shared__ ViscParShared<PTX, PTY> sh;
-  visc_par_tile<PTX, PTY>(Q, par, nx, ny, nz, dxg, vp, blockIdx.x, sh);
+  visc_par_tile<PTX, PTY>(Q, par, nx, ny, nz, dxg, vp, blockIdx.x, pks, sh);
 }
 
 // Viscous terms of all members of an ensemble: one persistent launch over
@@ -427,7 +427,8 @@ __global__ void __launch_bounds__(PTX * PTY) k_ens_visc(EnsViscArgs a) {
       else hi = mid - 1;
     }
     const EnsMember &M = a.mem[lo];
-    visc_par_tile<PTX, PTY>(M.q, M.par, M.n[0], M.n[1], M.n[2], M.dxg, a.vp, (int)(item - a.ptiles0[lo]), sh);
+    visc_par_tile<PTX, PTY>(M.q, M.par, M.n[0], M.n[1], M.n[2], M.dxg, a.vp, (int)(item - a.ptiles0[lo]), PKS,
+                            sh);
     __syncthreads();
   }
 }
@@ -578,8 +579,9 @@ static ViscParams visc_params(const Scheme &sc) {
 
 void launch_visc_par(const Grid &g, const Scheme &sc, const double *Q, double *par, const double *dxg,
                      hipStream_t st) {
-  const unsigned nb = par_tiles(g.n[0], g.n[1], g.n[2]);
-  hipLaunchKernelGGL(k_visc_par, dim3(nb), dim3(PTX * PTY), 0, st, Q, par, g.n[0], g.n[1], g.n[2], dxg,
+  const int pks = visc_pks(g.n[0], g.n[1], g.n[2]);
+  const unsigned nb = par_tiles(g.n[0], g.n[1], g.n[2], pks);
+  hipLaunchKernelGGL(k_visc_par, dim3(nb), dim3(PTX * PTY), 0, st, Q, par, g.n[0], g.n[1], g.n[2], pks, dxg,
                      visc_params(sc));
 }
 

@@ -139,6 +139,27 @@ def test_terms_match_oracle(interp, upwind, Re):
             assert decks.rel_l2(got, ref) <= (TOL if term == "rhs" else 1e-13), term
 
 
+@pytest.mark.parametrize("dims", [(256, 256, 128), (256, 256, 256)])
+def test_viscous_term_with_long_z_segments(dims):
+    """The viscous kernel marches z segments of 16 to 128 planes, chosen
+    from the grid (32 planes at 256x256x128, 64 at 256^3; every smaller grid
+    of these tests uses 16).  Each segment stages four planes of Q on either
+    side and evaluates the ghost-plane fluxes at the grid faces; the term
+    must not depend on where segments start.  Against the oracle."""
+    from hypar_amd import decks
+    from oracle.oracle import Oracle
+    w = dataclasses.replace(decks.config1(16, 1), dims=dims, dt=None)
+    _say(f"[gpu] viscous term, long z segments, dims={dims}")
+    u0 = decks.initial_state(w)
+    s = _solver(w)
+    got = s.to_host(s.parabolic(s.to_device(u0)))
+    s.close()
+    ref = Oracle.for_workload(w).parabolic(u0)
+    err = decks.rel_l2(got, ref)
+    _say(f"[gpu]   rel L2 vs oracle = {err:.3e}")
+    assert err <= 1e-13
+
+
 @pytest.mark.parametrize("dims", [(100, 6, 7), (7, 70, 6), (6, 7, 67), (5, 5, 5), (130, 9, 5)])
 @pytest.mark.parametrize("interp,upwind,Re", [("characteristic", "roe", 1600.0), ("components", "rusanov", -1.0),
                                               ("components", "roe", -1.0), ("characteristic", "rusanov", 100.0)])
