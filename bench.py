@@ -186,8 +186,23 @@ def ensemble_throughput(nsteps=10):
     e.set_dt(dts)
     shape, grid = e.tune(u, reps=2)
     stream = torch.cuda.current_stream()
-    e.rk4(u, 1)  # capture + warm
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    # both launch structures on a scratch copy: one launch per direction pass
+    # (shapes 0, 1 at the tuned grid) and one per RK stage (shapes 2, 3)
+    per_shape = {}
+    for sh in (0, 1, 2, 3):
+        e.set_tiles(sh, grid if sh < 2 and sh == shape else (0, 0, 0))
+        w = u.clone()
+        e.rk4(w, 1)
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        e.rk4(w, 3)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        per_shape[sh] = ev0.elapsed_time(ev1) / 3
+        del w
+    e.set_tiles(shape, grid)
+    e.rk4(u, 1)  # capture + warm
     torch.cuda.synchronize()
     ev0.record(stream)
     e.rk4(u, nsteps)
@@ -217,7 +232,11 @@ def ensemble_throughput(nsteps=10):
                         "WENO5-JS components + Rusanov, RK4",
             "members": len(ws), "cells": ncell, "steps": nsteps, "ms_per_step": ms / nsteps,
             "value": ncell * nsteps / (ms / 1e3), "unit": "cell-updates/s",
-            "tiles": {"shape": shape, "grid": list(grid)}, "finite": finite,
+            "tiles": {"shape": shape, "grid": list(grid),
+                      "launches_per_stage": 1 if shape >= 2 else 3},
+            "ms_per_step_by_shape": {"per_pass_4wf": per_shape[0], "per_pass_2wf": per_shape[1],
+                                     "per_stage_4wf": per_shape[2], "per_stage_2wf": per_shape[3]},
+            "finite": finite,
             "serial_members_ms_per_step": serial / nsteps,
             "speedup_vs_serial_members": serial / ms}
 

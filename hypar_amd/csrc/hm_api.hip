@@ -884,6 +884,9 @@ struct hm_ensemble {
   unsigned nptiles = 0, ntiles = 0;
   unsigned *item0[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   unsigned nitems[2][3] = {{0, 0, 0}, {0, 0, 0}};
+  // x items of the one-launch-per-stage kernel (ENS_STAGE_XG line groups each)
+  unsigned *xitem0[2] = {nullptr, nullptr};
+  unsigned nxitems[2] = {0, 0};
   // adjoint: first line of every member along each direction, longest
   // lines, and the tape (checkpoints, stage states of one segment, scratch)
   unsigned *lines0[3] = {nullptr, nullptr, nullptr};
@@ -896,11 +899,17 @@ struct hm_ensemble {
   int shape = 0;
   unsigned grid[3] = {0, 0, 0};
   bool dt_set = false;
+  // one-launch-per-stage shapes: dependency counters (4 + 2*nmem, zero
+  // between launches), the wait bound, the CU count for the grid
+  unsigned *ctr = nullptr;
+  unsigned spin_limit = 1u << 21;
+  int cus = 256;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   const double *graph_u = nullptr;
   int graph_shape = -1;
   unsigned graph_grid[3] = {0, 0, 0};
+  unsigned graph_spin = 0;
 };
 
 namespace {
@@ -913,19 +922,44 @@ void ens_drop_graph(hm_ensemble *e) {
   e->graph_u = nullptr;
 }
 
-// the launches of one RK4 step of all members (stage s reads S[(s-1)%2],
-// writes S[s%2]; the last stage writes u): per stage the viscous fluxes of
-// the stage state (viscous members; the previous stage's z pass left its
-// primitive variables in Q) and the three direction passes
-void ens_step_launches(hm_ensemble *e, double *u, hipStream_t st) {
-  for (int stage = 0; stage < 4; stage++) {
-    const double *U = stage == 0 ? u : e->S[(stage - 1) % 2];
-    double *Sn = stage < 3 ? e->S[stage % 2] : nullptr;
-    if (e->sc.Re > 0) hm::launch_ens_visc(e->sc, e->mem_dev, e->nmem, e->ptiles0, e->nptiles, st);
-    for (int d = 0; d < 3; d++)
-      hm::launch_ens_pass(e->sc, d, e->shape, e->mem_dev, e->nmem, e->item0[e->shape][d], e->nitems[e->shape][d],
-                          e->grid[d], U, e->R, u, Sn, e->acc, stage, st);
+// the launches of RK stage `stage` of all members on stage state U: the
+// viscous terms of U (viscous members; the previous stage's z pass left its
+// primitive variables in Q), then the three direction passes, as three
+// launches (shapes 0, 1) or one (shapes 2, 3)
+void ens_stage_launches(hm_ensemble *e, int stage, const double *U, double *u, double *Sn, hipStream_t st) {
+  if (e->sc.Re > 0) hm::launch_ens_visc(e->sc, e->mem_dev, e->nmem, e->ptiles0, e->nptiles, st);
+  const int ts = e->shape & 1;  // 0: 4 wavefronts per block, 1: 2
+  if (e->shape >= 2) {
+    const unsigned *const item0[3] = {e->xitem0[ts], e->item0[ts][1], e->item0[ts][2]};
+    const unsigned nitems[3] = {e->nxitems[ts], e->nitems[ts][1], e->nitems[ts][2]};
+    hm::launch_ens_stage(e->sc, ts == 0 ? 4 : 2, e->mem_dev, e->nmem, item0, nitems, e->item0[ts][0], hm::ENS_STAGE_XG,
+                         e->grid[0], e->cus, e->ctr, e->spin_limit, U, e->R, u, Sn, e->acc, stage, st);
+    return;
   }
+  for (int d = 0; d < 3; d++)
+    hm::launch_ens_pass(e->sc, d, ts, e->mem_dev, e->nmem, e->item0[ts][d], e->nitems[ts][d], e->grid[d], U, e->R, u,
+                        Sn, e->acc, stage, st);
+}
+
+// the launches of one RK4 step of all members (stage s reads S[(s-1)%2],
+// writes S[s%2]; the last stage writes u)
+void ens_step_launches(hm_ensemble *e, double *u, hipStream_t st) {
+  for (int stage = 0; stage < 4; stage++)
+    ens_stage_launches(e, stage, stage == 0 ? u : e->S[(stage - 1) % 2], u, stage < 3 ? e->S[stage % 2] : nullptr, st);
+}
+
+// After launches of the one-launch-per-stage shapes: wait for the stream and
+// report (and clear) the flag of a dependency wait that gave up.
+hm_status ens_check_waits(hm_ensemble *e, const char *name) {
+  if (e->shape < 2) return HM_OK;
+  unsigned flag = 0;
+  HM_HIP(hipMemcpyAsync(&flag, e->ctr + 2, sizeof(unsigned), hipMemcpyDeviceToHost, e->stream));
+  HM_HIP(hipStreamSynchronize(e->stream));
+  if (!flag) return HM_OK;
+  const unsigned zero = 0;
+  HM_HIP(hipMemcpy(e->ctr + 2, &zero, sizeof(unsigned), hipMemcpyHostToDevice));
+  return fail(HM_ERR_TIMEOUT, std::string(name) + ": a dependency wait of the stage kernel gave up after " +
+                                  std::to_string(e->spin_limit) + " polls; the state is not valid");
 }
 
 hm_status ens_capture(hm_ensemble *e, double *u) {
@@ -937,6 +971,7 @@ hm_status ens_capture(hm_ensemble *e, double *u) {
   HM_HIP(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
   e->graph_u = u;
   e->graph_shape = e->shape;
+  e->graph_spin = e->spin_limit;
   for (int d = 0; d < 3; d++) e->graph_grid[d] = e->grid[d];
   return HM_OK;
 }
@@ -946,14 +981,8 @@ hm_status ens_capture(hm_ensemble *e, double *u) {
 hm_status ens_step_store(hm_ensemble *e, double *u, double *const U[3]) {
   hipStream_t st = e->stream;
   if (e->sc.Re > 0) hm::launch_ens_prims(e->sc, e->mem_dev, e->nmem, u, e->ncell, st);
-  for (int stage = 0; stage < 4; stage++) {
-    const double *Uin = stage == 0 ? u : U[stage - 1];
-    double *Sn = stage < 3 ? U[stage] : nullptr;
-    if (e->sc.Re > 0) hm::launch_ens_visc(e->sc, e->mem_dev, e->nmem, e->ptiles0, e->nptiles, st);
-    for (int d = 0; d < 3; d++)
-      hm::launch_ens_pass(e->sc, d, e->shape, e->mem_dev, e->nmem, e->item0[e->shape][d], e->nitems[e->shape][d],
-                          e->grid[d], Uin, e->R, u, Sn, e->acc, stage, st);
-  }
+  for (int stage = 0; stage < 4; stage++)
+    ens_stage_launches(e, stage, stage == 0 ? u : U[stage - 1], u, stage < 3 ? U[stage] : nullptr, st);
   return check_launch("hm_ensemble: step");
 }
 
@@ -975,12 +1004,15 @@ void hm_ensemble_destroy(hm_ensemble *e) {
   double *bufs[] = {e->dxg, e->R, e->S[0], e->S[1], e->acc, e->par, e->Q};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
+  if (e->ctr) (void)hipFree(e->ctr);
   if (e->mem_dev) (void)hipFree(e->mem_dev);
   if (e->tiles0) (void)hipFree(e->tiles0);
   if (e->ptiles0) (void)hipFree(e->ptiles0);
-  for (int sh = 0; sh < 2; sh++)
+  for (int sh = 0; sh < 2; sh++) {
     for (int d = 0; d < 3; d++)
       if (e->item0[sh][d]) (void)hipFree(e->item0[sh][d]);
+    if (e->xitem0[sh]) (void)hipFree(e->xitem0[sh]);
+  }
   for (int d = 0; d < 3; d++)
     if (e->lines0[d]) (void)hipFree(e->lines0[d]);
   for (auto *v : {&e->ckpt, &e->stages, &e->vscratch, &e->vwork})
@@ -1099,7 +1131,7 @@ hm_status hm_ensemble_create(const hm_config *members, int nmembers, hm_ensemble
     st = fail(HM_ERR_HIP, "hm_ensemble_create: copy of the member table failed");
     goto bad;
   }
-  for (int sh = 0; sh < 2; sh++)
+  for (int sh = 0; sh < 2; sh++) {
     for (int d = 0; d < 3; d++) {
       std::vector<unsigned> pre(nmembers + 1, 0);
       for (int m = 0; m < nmembers; m++) pre[m + 1] = pre[m] + hm::ens_items_per_member(d, sh, e->mem[m].n);
@@ -1111,6 +1143,18 @@ hm_status hm_ensemble_create(const hm_config *members, int nmembers, hm_ensemble
         goto bad;
       }
     }
+    std::vector<unsigned> pre(nmembers + 1, 0);
+    for (int m = 0; m < nmembers; m++) {
+      const unsigned groups = hm::ens_items_per_member(0, sh, e->mem[m].n);
+      pre[m + 1] = pre[m] + (groups + hm::ENS_STAGE_XG - 1) / hm::ENS_STAGE_XG;
+    }
+    e->nxitems[sh] = pre[nmembers];
+    if (hipMalloc((void **)&e->xitem0[sh], pre.size() * sizeof(unsigned)) != hipSuccess ||
+        hipMemcpy(e->xitem0[sh], pre.data(), pre.size() * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess) {
+      st = fail(HM_ERR_NOMEM, "hm_ensemble_create: work lists");
+      goto bad;
+    }
+  }
   for (int d = 0; d < 3; d++) {
     std::vector<unsigned> pre(nmembers + 1, 0);
     for (int m = 0; m < nmembers; m++) {
@@ -1124,6 +1168,13 @@ hm_status hm_ensemble_create(const hm_config *members, int nmembers, hm_ensemble
       goto bad;
     }
   }
+  if (hipMalloc((void **)&e->ctr, (4 + 2 * (size_t)nmembers) * sizeof(unsigned)) != hipSuccess ||
+      hipMemset(e->ctr, 0, (4 + 2 * (size_t)nmembers) * sizeof(unsigned)) != hipSuccess) {
+    st = fail(HM_ERR_NOMEM, "hm_ensemble_create: stage counters");
+    goto bad;
+  }
+  if (hipDeviceGetAttribute(&e->cus, hipDeviceAttributeMultiprocessorCount, e->device) != hipSuccess || e->cus < 1)
+    e->cus = 256;
   if (hipStreamCreate(&e->cap) != hipSuccess) {
     st = fail(HM_ERR_HIP, "hm_ensemble_create: stream creation failed");
     goto bad;
@@ -1165,7 +1216,7 @@ hm_status hm_ensemble_set_dt(hm_ensemble *e, const double *dt) {
 }
 
 hm_status hm_ensemble_set_tiles(hm_ensemble *e, int shape, const unsigned *grid) {
-  if (!e || (shape != 0 && shape != 1)) return fail(HM_ERR_ARG, "hm_ensemble_set_tiles: shape must be 0 or 1");
+  if (!e || shape < 0 || shape > 3) return fail(HM_ERR_ARG, "hm_ensemble_set_tiles: shape must be 0, 1, 2 or 3");
   e->shape = shape;
   for (int d = 0; d < 3; d++) e->grid[d] = grid ? grid[d] : 0;
   return HM_OK;
@@ -1185,13 +1236,19 @@ hm_status hm_ensemble_rk4(hm_ensemble *e, double *u, int nsteps) {
   if (nsteps < 0) return fail(HM_ERR_ARG, "hm_ensemble_rk4: nsteps < 0");
   if (!e->dt_set) return fail(HM_ERR_ARG, "hm_ensemble_rk4: set the members' dt first (hm_ensemble_set_dt)");
   if (nsteps == 0) return HM_OK;
-  bool stale = !e->exec || e->graph_u != u || e->graph_shape != e->shape;
+  bool stale = !e->exec || e->graph_u != u || e->graph_shape != e->shape || e->graph_spin != e->spin_limit;
   for (int d = 0; d < 3; d++) stale = stale || e->graph_grid[d] != e->grid[d];
   if (stale && (st = ens_capture(e, u)) != HM_OK) return st;
   // primitive variables of the current state for the first stage's viscous
   // fluxes; every later stage finds them written by the z pass before it
   if (e->sc.Re > 0) hm::launch_ens_prims(e->sc, e->mem_dev, e->nmem, u, e->ncell, e->stream);
   for (int it = 0; it < nsteps; it++) HM_HIP(hipGraphLaunch(e->exec, e->stream));
+  return ens_check_waits(e, "hm_ensemble_rk4");
+}
+
+hm_status hm_ensemble_set_wait_limit(hm_ensemble *e, unsigned polls) {
+  if (!e) return fail(HM_ERR_ARG, "hm_ensemble_set_wait_limit: ensemble is NULL");
+  e->spin_limit = polls;
   return HM_OK;
 }
 
@@ -1214,11 +1271,13 @@ hm_status hm_ensemble_tune(hm_ensemble *e, const double *u, int reps, int *shape
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
     st = fail(HM_ERR_HIP, "hm_ensemble_tune: event creation failed");
   }
-  for (int sh = 0; sh < 2 && st == HM_OK; sh++)
-    for (unsigned mi = 0; mi < 4 && st == HM_OK; mi++) {
-      const unsigned per_cu = sh == 0 ? 2 : 4;  // resident blocks per CU at 2 waves per SIMD
+  // shapes 0, 1: one launch per pass, four grid sizes each; shapes 2, 3:
+  // one launch per stage with as many blocks as the GPU holds at once
+  for (int sh = 0; sh < 4 && st == HM_OK; sh++)
+    for (unsigned mi = 0; mi < (sh < 2 ? 4u : 1u) && st == HM_OK; mi++) {
+      const unsigned per_cu = (sh & 1) == 0 ? 2 : 4;  // resident blocks per CU at 2 waves per SIMD
       e->shape = sh;
-      for (int d = 0; d < 3; d++) e->grid[d] = mults[mi] * per_cu * (unsigned)cus;
+      for (int d = 0; d < 3; d++) e->grid[d] = sh < 2 ? mults[mi] * per_cu * (unsigned)cus : 0u;
       // candidates time `reps` graph replays of one step on a scratch copy
       if (hipMemcpyAsync(w, u, n5 * sizeof(double), hipMemcpyDeviceToDevice, e->stream) != hipSuccess) {
         st = fail(HM_ERR_HIP, "hm_ensemble_tune: copy failed");
@@ -1307,7 +1366,7 @@ hm_status hm_ensemble_adjoint_forward(hm_ensemble *e, double *u, int nsteps, int
   for (int m = 0; m < e->nmem; m++) e->tape_dt[m] = e->mem[m].dt;
   e->tape_nsteps = nsteps;
   e->tape_every = every;
-  return HM_OK;
+  return ens_check_waits(e, "hm_ensemble_adjoint_forward");
 }
 
 static hm_status ens_reverse_sweep(hm_ensemble *e, double *lam) {
@@ -1369,7 +1428,7 @@ static hm_status ens_reverse_sweep(hm_ensemble *e, double *lam) {
     }
   }
   if (cur != lam) HM_HIP(hipMemcpyAsync(lam, cur, n5 * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
-  return HM_OK;
+  return ens_check_waits(e, "hm_ensemble_adjoint_reverse");
 }
 
 hm_status hm_ensemble_adjoint_reverse(hm_ensemble *e, double *lam) {

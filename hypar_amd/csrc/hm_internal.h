@@ -172,6 +172,21 @@ unsigned ens_items_per_member(int dir, int shape, const int n[3]);
 void launch_ens_pass(const Scheme &sc, int dir, int shape, const EnsMember *mem, int nmem, const unsigned *item0,
                      unsigned nitems, unsigned grid, const double *U, double *R, double *u, double *S, double *acc,
                      int stage, hipStream_t st);
+// all three passes of RK stage `stage` over all members in one persistent
+// launch (rows = 4 or 2 wavefronts per block, the tile shapes of the split
+// passes; grid 0 = the blocks the GPU holds at once, cus CUs).  An x item
+// is ENS_STAGE_XG groups of x-lines: item0[0] lists each member's first x
+// item, xgroups0 its first x-line group (the split x pass's items).  ctr
+// holds 4 + 2*nmem zeroed counters and is zero again when the launch ends;
+// a dependency wait that polls more than spin_limit times sets ctr[2].
+#ifndef HM_ENS_XG
+#define HM_ENS_XG 16
+#endif
+constexpr unsigned ENS_STAGE_XG = HM_ENS_XG;
+void launch_ens_stage(const Scheme &sc, int rows, const EnsMember *mem, int nmem, const unsigned *const item0[3],
+                      const unsigned nitems[3], const unsigned *xgroups0, unsigned xg, unsigned grid, int cus,
+                      unsigned *ctr, unsigned spin_limit, const double *U, double *R, double *u, double *S,
+                      double *acc, int stage, hipStream_t st);
 
 // viscous terms of all members (hm_kernels.hip): one launch over the
 // (member, par_tiles) items, ptiles0 = each member's first item (nmem + 1

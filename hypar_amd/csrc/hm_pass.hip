@@ -812,6 +812,151 @@ __global__ void __launch_bounds__(64 * MR) __attribute__((amdgpu_waves_per_eu(2)
   }
 }
 
+// ---------------------------------------------------------------------------
+// one persistent launch per RK stage: the x, y and z work items of all members
+// ---------------------------------------------------------------------------
+// Blocks take items in ticket order: every x item of every member (xg
+// groups of x-lines each), then the y items, then the z items.  A member's y items add to the rhs its x items
+// wrote, and its z items read what its y items left, so a y (z) item first
+// waits until all x (y) items of its member are done, as counted in ctr.
+// Every item a block waits on holds a smaller ticket, taken by a block that
+// is running and that waits only on smaller tickets in turn, so the waits
+// end whatever the number of resident blocks.  Each wait is bounded: after
+// spin_limit polls it sets the error flag ctr[2] and goes on, so the kernel
+// always exits, and the host reports the flag.  The last block to leave
+// zeroes the ticket and the counters for the next launch, so graph replays
+// need no memset.
+struct EnsStageArgs {
+  EnsArgs a;                 // item0 and nitems of a are not used
+  const unsigned *item0[3];  // first item of each member per direction
+  unsigned nitems[3];
+  const unsigned *xgroups0;  // first x-line group of each member
+  unsigned xg;               // x-line groups per x item
+  unsigned *ctr;             // [0] ticket, [1] blocks done, [2] error flag, [3] unused,
+                             // [4, 4+nmem) x items done, [4+nmem, 4+2nmem) y items done
+  unsigned spin_limit;
+};
+
+template <int SH>
+union StageShared {
+  XShared<SH> x;
+  MShared<SH> m;
+};
+
+// the member of item `item` of a direction whose first items are item0
+__device__ __forceinline__ int stage_member(const unsigned *item0, int nmem, unsigned item) {
+  int lo = 0, hi = nmem - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (item0[mid] <= item) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Wave 0 acts for the block: all 64 of its lanes take the same branches and
+// only lane 0 contributes to the counters (the others add 0).  A branch on
+// threadIdx.x == 0 between two barriers of the item loop would let the
+// compiler move lane 0's atomics out of the loop (a loop that lanes leave
+// one by one), so that the rest of the block passed the next barrier before
+// lane 0 had counted its item or fetched the next ticket.
+__device__ __forceinline__ bool lead_wave() { return __builtin_amdgcn_readfirstlane(threadIdx.x) < 64u; }
+
+__device__ __forceinline__ unsigned lane0_one() { return (threadIdx.x & 63u) == 0u ? 1u : 0u; }
+
+// The L2s of the XCDs are not coherent with each other: a release writes
+// back the L2 of the releasing XCD (buffer_wbl2), an acquire invalidates the
+// L1 of the CU and the L2 of the XCD (buffer_inv).  Both act for every wave
+// of the block (all of them run on one CU), so one wave issues them.
+//
+// Whole block: wait until *c >= need (bounded), then acquire.
+__device__ __forceinline__ void stage_wait(const unsigned *c, unsigned need, unsigned *err, unsigned limit) {
+  if (lead_wave()) {
+    for (unsigned polls = 0;; polls++) {
+      const unsigned v =
+          __builtin_amdgcn_readfirstlane(__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      if (v >= need) break;
+      if (polls >= limit) {
+        __hip_atomic_fetch_or(err, lane0_one(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
+// Whole block: every wave waits for its own stores to reach the L2, then
+// one release (L2 write-back) and the count of the item.
+__device__ __forceinline__ void stage_signal(unsigned *c) {
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (lead_wave()) __hip_atomic_fetch_add(c, lane0_one(), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int INTERP, int UPW, int SH, int W>
+__global__ void __launch_bounds__(64 * SH) __attribute__((amdgpu_waves_per_eu(2))) k_ens_stage(EnsStageArgs s) {
+  __shared__ StageShared<SH> sh;
+  __shared__ unsigned s_ticket, s_last;
+  const int nmem = s.a.nmem;
+  unsigned *const done_x = s.ctr + 4, *const done_y = s.ctr + 4 + nmem;
+  const unsigned nx_items = s.nitems[0], nxy_items = s.nitems[0] + s.nitems[1];
+  const unsigned total = nxy_items + s.nitems[2];
+  for (;;) {
+    if (lead_wave()) {
+      const unsigned old = __hip_atomic_fetch_add(&s.ctr[0], lane0_one(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_ticket = __builtin_amdgcn_readfirstlane(old);  // lane 0's: the value before its add
+    }
+    __syncthreads();
+    const unsigned t = __builtin_amdgcn_readfirstlane(s_ticket);
+    if (t >= total) break;
+    const int phase = t < nx_items ? 0 : (t < nxy_items ? 1 : 2);
+    const unsigned item = t - (phase == 0 ? 0u : (phase == 1 ? nx_items : nxy_items));
+    const unsigned *const item0 = s.item0[phase];
+    const int m = stage_member(item0, nmem, item);
+    if (phase > 0) {
+      const unsigned *before = s.item0[phase - 1];
+      stage_wait(phase == 1 ? &done_x[m] : &done_y[m], before[m + 1] - before[m], &s.ctr[2], s.spin_limit);
+    }
+    const unsigned local = item - item0[m];
+    if (phase == 0) {
+      PassConst k;
+      PassIO io;
+      ens_setup(s.a, m, 0, k, io);
+      const unsigned g1 = min((local + 1) * s.xg, s.xgroups0[m + 1] - s.xgroups0[m]);
+      for (unsigned g = local * s.xg; g < g1; g++) {
+        pass_x_body<INTERP, UPW, SH, W>(k, io, g, sh.x);
+        __syncthreads();
+      }
+      stage_signal(&done_x[m]);
+    } else if (phase == 1) {
+      PassConst k;
+      PassIO io;
+      ens_setup(s.a, m, 1, k, io);
+      pass_m_body<1, INTERP, UPW, SH, W>(k, io, local, sh.m);
+      stage_signal(&done_y[m]);
+    } else {
+      PassConst k;
+      PassIO io;
+      ens_setup(s.a, m, 2, k, io);
+      pass_m_body<2, INTERP, UPW, SH, W>(k, io, local, sh.m);
+      __syncthreads();
+    }
+  }
+  // the last block out resets the counters (every other block has stopped
+  // touching them)
+  if (lead_wave()) {
+    const unsigned old = __hip_atomic_fetch_add(&s.ctr[1], lane0_one(), __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = __builtin_amdgcn_readfirstlane(old) == gridDim.x - 1 ? 1u : 0u;
+  }
+  __syncthreads();
+  if (__builtin_amdgcn_readfirstlane(s_last)) {
+    for (unsigned i = threadIdx.x; i < 2u * (unsigned)nmem; i += blockDim.x) s.ctr[4 + i] = 0u;
+    if (threadIdx.x < 2u) s.ctr[threadIdx.x] = 0u;  // ticket, blocks done
+  }
+}
+
 // primitive variables (u, v, w, T) of a state
 __global__ void __launch_bounds__(256) k_prims(const double *__restrict__ u, double *__restrict__ Q, size_t ncell,
                                                double gamma) {
@@ -991,6 +1136,78 @@ void launch_ens_pass(const Scheme &sc, int dir, int shape, const EnsMember *mem,
   } else {
     if (sc.upwind == UPWIND_ROE) ens_launch_t<INTERP_COMP, UPWIND_ROE>(sc.weno, dir, shape, a, grid, st);
     else ens_launch_t<INTERP_COMP, UPWIND_RUSANOV>(sc.weno, dir, shape, a, grid, st);
+  }
+}
+
+namespace {
+// blocks of `kern` that one CU holds at once
+template <class K>
+unsigned resident_per_cu(K kern, int threads) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, threads, 0) != hipSuccess || nb < 1) nb = 1;
+  return (unsigned)nb;
+}
+
+template <int INTERP, int UPW, int W>
+void stage_launch_w(int rows, const EnsStageArgs &s, unsigned grid, int cus, hipStream_t st) {
+  const unsigned total = s.nitems[0] + s.nitems[1] + s.nitems[2];
+  if (rows == 4) {
+    static const unsigned per_cu = resident_per_cu(k_ens_stage<INTERP, UPW, 4, W>, 256);
+    if (grid == 0) grid = per_cu * (unsigned)cus;
+    hipLaunchKernelGGL((k_ens_stage<INTERP, UPW, 4, W>), dim3(std::min(grid, total)), dim3(256), 0, st, s);
+  } else {
+    static const unsigned per_cu = resident_per_cu(k_ens_stage<INTERP, UPW, 2, W>, 128);
+    if (grid == 0) grid = per_cu * (unsigned)cus;
+    hipLaunchKernelGGL((k_ens_stage<INTERP, UPW, 2, W>), dim3(std::min(grid, total)), dim3(128), 0, st, s);
+  }
+}
+
+template <int INTERP, int UPW>
+void stage_launch_t(int weno, int rows, const EnsStageArgs &s, unsigned grid, int cus, hipStream_t st) {
+  if (weno == WENO_MAPPED) stage_launch_w<INTERP, UPW, WENO_MAPPED>(rows, s, grid, cus, st);
+  else if (weno == WENO_Z) stage_launch_w<INTERP, UPW, WENO_Z>(rows, s, grid, cus, st);
+  else if (weno == WENO_YC) stage_launch_w<INTERP, UPW, WENO_YC>(rows, s, grid, cus, st);
+  else if (weno == WENO_OPTIMAL) stage_launch_w<INTERP, UPW, WENO_OPTIMAL>(rows, s, grid, cus, st);
+  else stage_launch_w<INTERP, UPW, WENO_JS>(rows, s, grid, cus, st);
+}
+}  // namespace
+
+void launch_ens_stage(const Scheme &sc, int rows, const EnsMember *mem, int nmem, const unsigned *const item0[3],
+                      const unsigned nitems[3], const unsigned *xgroups0, unsigned xg, unsigned grid, int cus,
+                      unsigned *ctr, unsigned spin_limit, const double *U, double *R, double *u, double *S,
+                      double *acc, int stage, hipStream_t st) {
+  EnsStageArgs s;
+  EnsArgs &a = s.a;
+  a.mem = mem;
+  a.nmem = nmem;
+  a.item0 = nullptr;
+  a.nitems = 0;
+  a.gamma = sc.gamma;
+  a.gm1 = sc.gamma - 1.0;
+  a.inv_gm1 = 1.0 / (sc.gamma - 1.0);
+  a.eps4 = 4.0 * sc.eps;
+  a.U = U;
+  a.R = R;
+  a.u = u;
+  a.S = S;
+  a.acc = acc;
+  a.mode = OUT_STAGE;
+  a.stage = stage;
+  a.visc = sc.Re > 0;
+  for (int d = 0; d < 3; d++) {
+    s.item0[d] = item0[d];
+    s.nitems[d] = nitems[d];
+  }
+  s.xgroups0 = xgroups0;
+  s.xg = xg;
+  s.ctr = ctr;
+  s.spin_limit = spin_limit;
+  if (sc.interp == INTERP_CHAR) {
+    if (sc.upwind == UPWIND_ROE) stage_launch_t<INTERP_CHAR, UPWIND_ROE>(sc.weno, rows, s, grid, cus, st);
+    else stage_launch_t<INTERP_CHAR, UPWIND_RUSANOV>(sc.weno, rows, s, grid, cus, st);
+  } else {
+    if (sc.upwind == UPWIND_ROE) stage_launch_t<INTERP_COMP, UPWIND_ROE>(sc.weno, rows, s, grid, cus, st);
+    else stage_launch_t<INTERP_COMP, UPWIND_RUSANOV>(sc.weno, rows, s, grid, cus, st);
   }
 }
 

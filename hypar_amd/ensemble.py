@@ -111,6 +111,11 @@ class Ensemble:
         g = (ctypes.c_uint * 3)(*grid)
         _lib.check(self.L.hm_ensemble_set_tiles(self.h, int(shape), g))
 
+    def set_wait_limit(self, polls: int):
+        """Polls after which a dependency wait of the one-launch-per-stage
+        kernel (shapes 2, 3) gives up and the call reports HM_ERR_TIMEOUT."""
+        _lib.check(self.L.hm_ensemble_set_wait_limit(self.h, int(polls)))
+
     def tiles(self):
         sh, g = ctypes.c_int(), (ctypes.c_uint * 3)()
         _lib.check(self.L.hm_ensemble_get_tiles(self.h, ctypes.byref(sh), g))

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libhypar_mi355x.so")
 
 HM_OK = 0
-HM_ERR_ARG, HM_ERR_HIP, HM_ERR_NONFINITE, HM_ERR_UNSUPPORTED, HM_ERR_NOMEM = 1, 2, 3, 4, 5
+HM_ERR_ARG, HM_ERR_HIP, HM_ERR_NONFINITE, HM_ERR_UNSUPPORTED, HM_ERR_NOMEM, HM_ERR_TIMEOUT = 1, 2, 3, 4, 5, 6
 INTERP = {"components": 0, "characteristic": 1}
 UPWIND = {"roe": 0, "rusanov": 1}
 WENO = {"js": 0, "mapped": 1, "borges": 2, "yc": 3, "optimal": 4}
@@ -90,6 +90,7 @@ SIGNATURES = [
     ("hm_ensemble_get_tiles", [_VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint)], ctypes.c_int),
     ("hm_ensemble_tune", [_VP, _DP, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint)],
      ctypes.c_int),
+    ("hm_ensemble_set_wait_limit", [_VP, ctypes.c_uint], ctypes.c_int),
     # include/hypar_mi355x_slots.h
     ("hm_bind", [_VP, _VP], ctypes.c_int),
     ("hm_bind_state", [_VP, _DP, ctypes.c_int], ctypes.c_int),

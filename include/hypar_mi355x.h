@@ -47,7 +47,9 @@ typedef enum {
   HM_ERR_HIP = 2,         /* a HIP runtime call failed                   */
   HM_ERR_NONFINITE = 3,   /* a reduction found NaN/Inf in the state      */
   HM_ERR_UNSUPPORTED = 4, /* scheme or option not implemented            */
-  HM_ERR_NOMEM = 5        /* device allocation failed                    */
+  HM_ERR_NOMEM = 5,       /* device allocation failed                    */
+  HM_ERR_TIMEOUT = 6      /* a bounded wait inside a kernel gave up; the
+                             results of that call are not valid          */
 } hm_status;
 
 /* hyp_interp_type of solver.inp (src/Simulation/ReadInputs.c) */
@@ -251,17 +253,29 @@ hm_status hm_ensemble_member(const hm_ensemble *e, int m, size_t *offset, size_t
 hm_status hm_ensemble_set_stream(hm_ensemble *e, void *stream);
 /* per-member time steps dt[0..nmembers-1] (synchronous) */
 hm_status hm_ensemble_set_dt(hm_ensemble *e, const double *dt);
-/* nsteps RK4 steps of every member, in place (graph replays on the stream) */
+/* nsteps RK4 steps of every member, in place (graph replays on the
+ * stream).  With the one-launch-per-stage shapes (2, 3) it synchronises the
+ * stream at the end and returns HM_ERR_TIMEOUT if a dependency wait inside
+ * a stage kernel gave up. */
 hm_status hm_ensemble_rk4(hm_ensemble *e, double *u, int nsteps);
-/* Tile shape (0: blocks of 4 wavefronts = 4 x-lines / 4 rows per march
- * step; 1: blocks of 2) and persistent grid sizes of the x, y, z passes
- * (0: one block per work item). */
+/* Tile shape and persistent grid sizes.  Shapes 0 and 1: one launch per
+ * direction pass, blocks of 4 (0) or 2 (1) wavefronts = x-lines / rows per
+ * march step, grid[d] blocks for pass d (0: one block per work item).
+ * Shapes 2 and 3: one launch per RK stage for the x, y and z work items of
+ * all members (blocks of 4 or 2 wavefronts), grid[0] blocks (0: as many
+ * as the GPU holds at once); viscous members add one launch per stage for
+ * their viscous terms. */
 hm_status hm_ensemble_set_tiles(hm_ensemble *e, int shape, const unsigned *grid);
 hm_status hm_ensemble_get_tiles(const hm_ensemble *e, int *shape, unsigned *grid);
-/* Autotuner: times `reps` graph-replayed steps of every candidate tile
- * shape and grid size on a scratch copy of u (u is not modified), keeps
- * the fastest and returns it. */
+/* Autotuner: times `reps` graph-replayed steps of every candidate (shapes
+ * 0 and 1 with four grid sizes each, shapes 2 and 3 with the co-resident
+ * grid) on a scratch copy of u (u is not modified), keeps the fastest and
+ * returns it. */
 hm_status hm_ensemble_tune(hm_ensemble *e, const double *u, int reps, int *shape, unsigned *grid);
+/* Polls after which a dependency wait of the one-launch-per-stage kernel
+ * gives up and flags HM_ERR_TIMEOUT (default 2^21, well above a stage's
+ * length).  For tests of that error path. */
+hm_status hm_ensemble_set_wait_limit(hm_ensemble *e, unsigned polls);
 /* Batched discrete adjoint (no counterpart in the reference): the adjoint
  * of hm_adjoint_forward / hm_adjoint_reverse for all members at once, each
  * with its own dt.  The forward sweep advances the batched state nsteps RK4

@@ -84,6 +84,61 @@ def test_reference_ensemble_fixture():
         assert err <= TOL
 
 
+@pytest.mark.parametrize("interp,upwind,Re", [("components", "rusanov", -1.0), ("characteristic", "roe", -1.0),
+                                              ("characteristic", "roe", 1000.0)])
+def test_one_launch_per_stage_matches_pass_launches(interp, upwind, Re):
+    """The stage kernel (shapes 2, 3: the x, y and z items of all members in
+    one launch, ordered by dependency counters) gives the same bits as the
+    launch-per-pass kernels: each cell's arithmetic is the same, so any
+    missed dependency would show as a difference."""
+    from hypar_amd.ensemble import Ensemble
+    ws, u0s, dts = _members(DIMS, interp, upwind, Re)
+    print(f"[gpu] one launch per stage vs per pass: {interp}/{upwind} Re={Re}", flush=True)
+    e = Ensemble(DIMS, interp, upwind, Re=Re, Minf=0.1 if Re > 0 else 1.0)
+    u0 = e.to_device(u0s)
+    e.set_dt(dts)
+    out = {}
+    for shape in (0, 2, 1, 3):
+        e.set_tiles(shape)
+        u = u0.clone()
+        e.rk4(u, 3)
+        out[shape] = u
+    assert torch.equal(out[2], out[0])
+    assert torch.equal(out[3], out[1])
+    assert torch.equal(out[0], out[1])
+    e.close()
+
+
+def test_stage_kernel_wait_limit_reports_timeout():
+    """A dependency wait that gives up sets a flag the host reports as
+    HM_ERR_TIMEOUT, and the kernel still exits.  Forced from the host with a
+    wait limit of zero polls: on a single 64^3 member every block holds an
+    x item at first, so the first y items find the x count short.  Then the
+    normal limit gives the launch-per-pass result again."""
+    from hypar_amd import lib
+    from hypar_amd.ensemble import Ensemble
+    dims = [(64, 64, 64)]
+    ws, u0s, dts = _members(dims)
+    e = Ensemble(dims)
+    u0 = e.to_device(u0s)
+    e.set_dt(dts)
+    e.set_tiles(0)
+    ref = u0.clone()
+    e.rk4(ref, 2)
+    e.set_tiles(2)
+    e.set_wait_limit(0)
+    u = u0.clone()
+    with pytest.raises(lib.HMError) as err:
+        e.rk4(u, 2)
+    assert err.value.code == lib.HM_ERR_TIMEOUT
+    print(f"[gpu] wait limit 0: {err.value}", flush=True)
+    e.set_wait_limit(1 << 21)
+    u = u0.clone()
+    e.rk4(u, 2)
+    assert torch.equal(u, ref)
+    e.close()
+
+
 def test_tile_shapes_and_autotuner_agree():
     from hypar_amd.ensemble import Ensemble
     ws, u0s, dts = _members(DIMS)
@@ -91,12 +146,16 @@ def test_tile_shapes_and_autotuner_agree():
     u0 = e.to_device(u0s)
     e.set_dt(dts)
     results = {}
-    for shape in (0, 1):
-        for grid in ((0, 0, 0), (256, 256, 256), (1024, 512, 2048)):
-            e.set_tiles(shape, grid)
-            u = u0.clone()
-            e.rk4(u, 2)
-            results[(shape, grid)] = u
+    # shapes 0, 1: a launch per pass; 2, 3: one launch per stage, with the
+    # co-resident grid, a grid far smaller (blocks take many items and wait
+    # on other members), and one larger than the work list
+    cases = [(sh, g) for sh in (0, 1) for g in ((0, 0, 0), (256, 256, 256), (1024, 512, 2048))]
+    cases += [(sh, g) for sh in (2, 3) for g in ((0, 0, 0), (7, 0, 0), (100000, 0, 0))]
+    for shape, grid in cases:
+        e.set_tiles(shape, grid)
+        u = u0.clone()
+        e.rk4(u, 2)
+        results[(shape, grid)] = u
     first = next(iter(results.values()))
     for key, u in results.items():
         assert torch.equal(u, first), key
@@ -110,9 +169,11 @@ def test_tile_shapes_and_autotuner_agree():
     assert torch.equal(u, first)
 
 
-def test_graph_replays_between_eager_work():
+@pytest.mark.parametrize("shape", [0, 2])
+def test_graph_replays_between_eager_work(shape):
     """Replay the captured step several times with eager kernels and host
-    synchronisation in between; every replay matches eager single-grid steps."""
+    synchronisation in between; every replay matches eager single-grid steps
+    (a launch per pass, and one launch per stage)."""
     from hypar_amd import decks
     from hypar_amd.ensemble import Ensemble
     from hypar_amd.solver import NS3D
@@ -121,6 +182,7 @@ def test_graph_replays_between_eager_work():
     e = Ensemble(dims)
     u = e.to_device(u0s)
     e.set_dt(dts)
+    e.set_tiles(shape)
     singles = [NS3D.from_workload(w) for w in ws]
     refs = [s.to_device(a) for s, a in zip(singles, u0s)]
     for rep in range(4):
