@@ -180,7 +180,7 @@ void launch_ens_pass(const Scheme &sc, int dir, int shape, const EnsMember *mem,
 // holds 4 + 2*nmem zeroed counters and is zero again when the launch ends;
 // a dependency wait that polls more than spin_limit times sets ctr[2].
 #ifndef HM_ENS_XG
-#define HM_ENS_XG 16
+#define HM_ENS_XG 64
 #endif
 constexpr unsigned ENS_STAGE_XG = HM_ENS_XG;
 void launch_ens_stage(const Scheme &sc, int rows, const EnsMember *mem, int nmem, const unsigned *const item0[3],
