@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <algorithm>
+#include <cstdint>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -15,13 +16,62 @@
 #include "../../include/hypar_mi355x.h"
 #include "hm_internal.h"
 
-// Segments of the adjoint tape whose RK stage states are in HBM at once
-// (4K states each, K = checkpoint_every): the forward sweep keeps the last
-// TAPE_SETS segments, and the reverse sweep recomputes every earlier segment
-// from its checkpoint into the set that segment's later neighbour freed.  A
-// third set costs 4K more states (128^3, K = 10: 3.4 GB) and saves K
-// recomputed steps.
-constexpr int TAPE_SETS = 3;
+// The adjoint tape keeps a checkpoint every K steps (K = checkpoint_every)
+// and, for `sets` segments of K steps at a time, either the four RK stage
+// states of every step (stage level) or only the state each step starts
+// from (step level: the reverse sweep runs each step forward again into
+// three scratch states before taking it back).  The forward sweep keeps the
+// last `sets` segments; the reverse sweep recomputes every earlier segment
+// from its checkpoint into the set that segment's later neighbour freed.
+// The forward sweep takes the cheapest schedule whose states fit the tape
+// budget: stage level with 3, 2 or 1 sets (nck-3, nck-2 or nck-1 segments
+// recomputed, nck = number of checkpoints), then step level with 3, 2 or 1
+// sets (the same, plus one forward step per step).  A third stage-level
+// set costs 4K states (128^3, K = 10: 3.4 GB) and saves K recomputed steps.
+constexpr int TAPE_SETS = 3;  // most sets a schedule uses
+
+struct TapePlan {
+  int sets = TAPE_SETS;
+  bool step_level = false;
+};
+
+namespace {
+// states in the sets of a plan
+size_t tape_slots(const TapePlan &p, int nsteps, int every) {
+  return (size_t)(p.step_level ? 1 : 4) * (size_t)std::min(nsteps, p.sets * every);
+}
+// every state the tape and the reverse sweep hold: the checkpoints, the
+// sets, 4 scratch states, and 3 stage states at step level
+size_t tape_total_states(const TapePlan &p, int nsteps, int every) {
+  const size_t nck = (size_t)((nsteps + every - 1) / every);
+  return nck + tape_slots(p, nsteps, every) + 4 + (p.step_level ? 3 : 0);
+}
+// the cheapest plan whose states (state_bytes each, plus fixed_bytes of
+// work space) fit in budget bytes
+bool choose_tape(size_t budget, size_t state_bytes, size_t fixed_bytes, int nsteps, int every, TapePlan *out) {
+  for (int level = 0; level < 2; level++)
+    for (int sets = TAPE_SETS; sets >= 1; sets--) {
+      TapePlan p;
+      p.sets = sets;
+      p.step_level = level == 1;
+      if (fixed_bytes + tape_total_states(p, nsteps, every) * state_bytes <= budget) {
+        *out = p;
+        return true;
+      }
+    }
+  return false;
+}
+// The budget when the caller gave none: the device's free memory plus what
+// the tape already holds (it is reused or freed), less a reserve of 2 GiB or
+// 2% of the device, whichever is larger.
+size_t auto_tape_budget(size_t held_bytes) {
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return SIZE_MAX;
+  const size_t reserve = std::max<size_t>((size_t)2 << 30, total_b / 50);
+  const size_t avail = free_b + held_bytes;
+  return avail > reserve ? avail - reserve : 0;
+}
+}  // namespace
 
 struct hm_solver {
   hm_solver() : g(), sc(), device(0), stream(nullptr), dxg(nullptr), R(nullptr), par(nullptr), Q(nullptr),
@@ -50,6 +100,9 @@ struct hm_solver {
   std::vector<double *> vwork;  // viscous adjoint work space (one buffer)
   int tape_nsteps, tape_every;
   double tape_dt;
+  TapePlan tape_plan;            // schedule of the recorded trajectory
+  size_t tape_budget = 0;        // bytes the tape may hold (0: from the device's free memory)
+  std::vector<double *> sscratch;  // step level: the stage states of the step being taken back
   // segment whose RK stage states set i holds (-1: none).  A reverse sweep
   // recomputes every segment its set does not hold, so a later sweep over
   // the same forward (or one after hm_rk4_vjp reused set 0) stays exact.
@@ -206,7 +259,7 @@ void hm_destroy(hm_solver *s) {
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   if (s->part_host) (void)hipHostFree(s->part_host);
-  for (auto *v : {&s->ckpt, &s->stages, &s->vscratch, &s->vwork})
+  for (auto *v : {&s->ckpt, &s->stages, &s->vscratch, &s->vwork, &s->sscratch})
     for (double *b : *v) (void)hipFree(b);
   for (int i = 0; i < TAPE_SETS; i++)
     for (hipEvent_t e : {s->ev_ready[i], s->ev_done[i]})
@@ -577,6 +630,18 @@ static hm_status ensure_buffers(std::vector<double *> &v, size_t count, size_t n
   return HM_OK;
 }
 
+// exactly `count` buffers of n doubles: frees the ones beyond count, so the
+// tape holds what its schedule needs and no more
+static hm_status fit_buffers(std::vector<double *> &v, size_t count, size_t n) {
+  while (v.size() > count) {
+    (void)hipFree(v.back());
+    v.pop_back();
+  }
+  return ensure_buffers(v, count, n);
+}
+
+static size_t held_bytes(const std::vector<double *> &v, size_t n) { return v.size() * n * sizeof(double); }
+
 // one RK4 step of u in place keeping the stage states U1, U2, U3
 static hm_status rk4_step_store(hm_solver *s, double *u, double dt, double *const U[3]) {
   if (s->sc.Re > 0) hm::launch_prims(s->g, s->sc, u, s->Q, s->stream);
@@ -653,25 +718,38 @@ hm_status hm_rk4_vjp(hm_solver *s, const double *u_n, double dt, double *lam) {
   HM_HIP(hipMemcpyAsync(w, u_n, n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
   double *const Us[3] = {U[1], U[2], U[3]};
   if ((st = rk4_step_store(s, w, dt, Us)) != HM_OK) return st;
-  // U[0..3] are the first step of stage set 0: a recorded trajectory no
-  // longer finds its segment there and recomputes it
-  s->tape_holds[0] = -1;
+  // U[0..3] are the first tape states: a recorded trajectory no longer finds
+  // the segments there and recomputes them
+  for (int i = 0; i < TAPE_SETS; i++) s->tape_holds[i] = -1;
   double *L = s->vscratch[2];
   if ((st = rk4_step_vjp(s, U, dt, lam, L)) != HM_OK) return st;
   HM_HIP(hipMemcpyAsync(lam, L, n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
   return HM_OK;
 }
 
-// TAPE_SETS sets of segment stage states: segment seg uses set
-// seg % TAPE_SETS; step m of a segment keeps u_n, U1, U2, U3 at
-// stages[set*4K + 4m .. +3].  A trajectory of N steps needs
-// 4*min(N, TAPE_SETS*K) stage states.
-static double **segment_stages(hm_solver *s, int seg, int K) {
-  return &s->stages[(size_t)4 * K * (seg % TAPE_SETS)];
+// Segment seg's set is seg % sets; within it step m keeps u_n, U1, U2, U3
+// at slots[4m .. 4m+3] (stage level) or u_n at slots[m] (step level).
+static double **segment_slots(hm_solver *s, int seg, int K) {
+  const TapePlan &p = s->tape_plan;
+  return &s->stages[(size_t)(p.step_level ? 1 : 4) * K * (seg % p.sets)];
 }
 
-static size_t tape_states(int nsteps, int every) {
-  return (size_t)4 * (size_t)std::min(nsteps, TAPE_SETS * every);
+hm_status hm_set_tape_budget(hm_solver *s, size_t bytes) {
+  if (!s) return fail(HM_ERR_ARG, "hm_set_tape_budget: solver is NULL");
+  s->tape_budget = bytes;
+  return HM_OK;
+}
+
+hm_status hm_tape_info(const hm_solver *s, int *sets, int *step_level, size_t *bytes) {
+  if (!s) return fail(HM_ERR_ARG, "hm_tape_info: solver is NULL");
+  if (s->tape_nsteps < 1) return fail(HM_ERR_ARG, "hm_tape_info: no forward sweep recorded");
+  const size_t n5 = 5 * s->g.ncell;
+  if (sets) *sets = s->tape_plan.sets;
+  if (step_level) *step_level = s->tape_plan.step_level ? 1 : 0;
+  if (bytes)
+    *bytes = hm::vjp_work_size(s->g, s->sc) * sizeof(double) +
+             tape_total_states(s->tape_plan, s->tape_nsteps, s->tape_every) * n5 * sizeof(double);
+  return HM_OK;
 }
 
 hm_status hm_adjoint_forward(hm_solver *s, double *u, double dt, int nsteps, int every) {
@@ -682,26 +760,50 @@ hm_status hm_adjoint_forward(hm_solver *s, double *u, double dt, int nsteps, int
   // a failed or partial sweep leaves no tape behind
   s->tape_nsteps = 0;
   for (int i = 0; i < TAPE_SETS; i++) s->tape_holds[i] = -1;
-  const size_t n5 = 5 * s->g.ncell;
+  const size_t n5 = 5 * s->g.ncell, state_bytes = n5 * sizeof(double);
+  const size_t work_bytes = hm::vjp_work_size(s->g, s->sc) * sizeof(double);
+  const size_t held = held_bytes(s->ckpt, n5) + held_bytes(s->stages, n5) + held_bytes(s->vscratch, n5) +
+                      held_bytes(s->sscratch, n5) + (s->vwork.empty() ? 0 : work_bytes);
+  const size_t budget = s->tape_budget ? s->tape_budget : auto_tape_budget(held);
+  TapePlan plan;
+  if (!choose_tape(budget, state_bytes, work_bytes, nsteps, every, &plan)) {
+    TapePlan least;
+    least.sets = 1;
+    least.step_level = true;
+    return fail(HM_ERR_NOMEM, "hm_adjoint_forward: the smallest tape (one checkpoint every " + std::to_string(every) +
+                                  " steps, one segment of step states) needs " +
+                                  std::to_string(work_bytes + tape_total_states(least, nsteps, every) * state_bytes) +
+                                  " bytes; the budget is " + std::to_string(budget) +
+                                  " (hm_set_tape_budget, or a longer checkpoint interval)");
+  }
+  s->tape_plan = plan;
   const int nck = (nsteps + every - 1) / every;
-  if ((st = ensure_buffers(s->ckpt, (size_t)nck, n5)) != HM_OK) return st;
-  if ((st = ensure_buffers(s->stages, tape_states(nsteps, every), n5)) != HM_OK) return st;
-  // the last TAPE_SETS segments keep their stage states, one in each set:
-  // the reverse sweep starts on them without recomputation
-  const int nkeep = (nck >= TAPE_SETS ? nck - TAPE_SETS : 0) * every;
+  if ((st = fit_buffers(s->ckpt, (size_t)nck, n5)) != HM_OK) return st;
+  if ((st = fit_buffers(s->stages, tape_slots(plan, nsteps, every), n5)) != HM_OK) return st;
+  if ((st = fit_buffers(s->sscratch, plan.step_level ? 3 : 0, n5)) != HM_OK) return st;
+  if ((st = ensure_buffers(s->vscratch, 4, n5)) != HM_OK) return st;
+  // the last `sets` segments keep their states, one in each set: the
+  // reverse sweep starts on them without recomputation
+  const int nkeep = (nck >= plan.sets ? nck - plan.sets : 0) * every;
   for (int n = 0; n < nsteps; n++) {
     if (n % every == 0)
       HM_HIP(hipMemcpyAsync(s->ckpt[n / every], u, n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
     if (n < nkeep) {
       if ((st = hm_rk4(s, u, dt, 1)) != HM_OK) return st;
+      continue;
+    }
+    const int seg = n / every, m = n - seg * every;
+    double **slots = segment_slots(s, seg, every);
+    if (plan.step_level) {
+      HM_HIP(hipMemcpyAsync(slots[m], u, n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+      if ((st = hm_rk4(s, u, dt, 1)) != HM_OK) return st;
     } else {
-      const int seg = n / every;
-      double **U = segment_stages(s, seg, every) + (size_t)4 * (n - seg * every);
+      double **U = slots + (size_t)4 * m;
       HM_HIP(hipMemcpyAsync(U[0], u, n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
       double *const Us[3] = {U[1], U[2], U[3]};
       if ((st = rk4_step_store(s, u, dt, Us)) != HM_OK) return st;
-      s->tape_holds[seg % TAPE_SETS] = seg;
     }
+    s->tape_holds[seg % plan.sets] = seg;
   }
   s->tape_nsteps = nsteps;
   s->tape_every = every;
@@ -709,13 +811,22 @@ hm_status hm_adjoint_forward(hm_solver *s, double *u, double dt, int nsteps, int
   return HM_OK;
 }
 
-// recompute segment seg from its checkpoint, keeping every step's stage states
+// recompute segment seg from its checkpoint, keeping every step's stage
+// states (stage level) or starting states (step level)
 static hm_status recompute_segment(hm_solver *s, int seg, int n0, int n1, double dt) {
   const size_t n5 = 5 * s->g.ncell;
   const int K = s->tape_every;
-  double **S = segment_stages(s, seg, K);
-  hm_status st = ensure_buffers(s->vscratch, 4, n5);
-  if (st != HM_OK) return st;
+  double **S = segment_slots(s, seg, K);
+  hm_status st;
+  if (s->tape_plan.step_level) {
+    HM_HIP(hipMemcpyAsync(S[0], s->ckpt[seg], n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+    for (int m = 1; m < n1 - n0; m++) {
+      HM_HIP(hipMemcpyAsync(S[m], S[m - 1], n5 * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+      if ((st = hm_rk4(s, S[m], dt, 1)) != HM_OK) return st;
+    }
+    return HM_OK;
+  }
+  if ((st = ensure_buffers(s->vscratch, 4, n5)) != HM_OK) return st;
   for (int n = n0; n < n1; n++) {
     double **U = S + (size_t)4 * (n - n0);
     // U[0] of the first step is the checkpoint; later steps find u_n in
@@ -733,64 +844,83 @@ static hm_status adjoint_reverse_sweep(hm_solver *s, double *lam) {
   hm_status st;
   const int N = s->tape_nsteps, K = s->tape_every;
   const double dt = s->tape_dt;
+  const TapePlan plan = s->tape_plan;
+  const int sets = plan.sets;
   const size_t n5 = 5 * s->g.ncell;
-  if ((st = ensure_buffers(s->stages, tape_states(N, K), n5)) != HM_OK) return st;
+  if ((st = ensure_buffers(s->stages, tape_slots(plan, N, K), n5)) != HM_OK) return st;
+  if ((st = ensure_buffers(s->sscratch, plan.step_level ? 3 : 0, n5)) != HM_OK) return st;
   if ((st = ensure_buffers(s->vscratch, 4, n5)) != HM_OK) return st;
-  if (!s->aux) HM_HIP(hipStreamCreate(&s->aux));  // default priority
-  for (int i = 0; i < TAPE_SETS; i++)
-    for (hipEvent_t *e : {&s->ev_ready[i], &s->ev_done[i]})
-      if (!*e) HM_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
-  if (!s->ev_start) HM_HIP(hipEventCreateWithFlags(&s->ev_start, hipEventDisableTiming));
+  // With two or more stage-level sets, a second stream recomputes the
+  // segment before the current one while the solver's stream runs the
+  // current one's adjoint.  With one set, or at step level (whose adjoint
+  // runs forward steps on the solver's scratch), everything runs in order
+  // on the solver's stream.
+  const bool overlap = sets >= 2 && !plan.step_level;
+  if (overlap) {
+    if (!s->aux) HM_HIP(hipStreamCreate(&s->aux));  // default priority
+    for (int i = 0; i < TAPE_SETS; i++)
+      for (hipEvent_t *e : {&s->ev_ready[i], &s->ev_done[i]})
+        if (!*e) HM_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    if (!s->ev_start) HM_HIP(hipEventCreateWithFlags(&s->ev_start, hipEventDisableTiming));
+  }
   const int nck = (N + K - 1) / K;
   hipStream_t main = s->stream;
   double *cur = lam;
-  // the recomputation stream starts after everything queued so far
-  HM_HIP(hipEventRecord(s->ev_start, main));
-  HM_HIP(hipStreamWaitEvent(s->aux, s->ev_start, 0));
-  // After a forward sweep the sets hold segments nck-TAPE_SETS .. nck-1;
-  // every other segment, and any whose set was reused since (an earlier
-  // reverse sweep, hm_rk4_vjp), is recomputed from its checkpoint.  The last
-  // segment is recomputed on the solver's stream, every earlier one on the
-  // second stream while the adjoint of the segment after it runs.
-  bool on_aux[TAPE_SETS] = {};
-  {
-    const int seg = nck - 1, o = seg % TAPE_SETS;
-    if (s->tape_holds[o] != seg) {
-      s->tape_holds[o] = -1;
-      if ((st = recompute_segment(s, seg, seg * K, N, dt)) != HM_OK) return st;
-      s->tape_holds[o] = seg;
-    }
+  if (overlap) {
+    // the recomputation stream starts after everything queued so far
+    HM_HIP(hipEventRecord(s->ev_start, main));
+    HM_HIP(hipStreamWaitEvent(s->aux, s->ev_start, 0));
   }
+  // After a forward sweep the sets hold the last `sets` segments; every
+  // other segment, and any whose set was reused since (an earlier reverse
+  // sweep, hm_rk4_vjp), is recomputed from its checkpoint.
+  bool on_aux[TAPE_SETS] = {};
   for (int seg = nck - 1; seg >= 0; seg--) {
     const int n0 = seg * K, n1 = (n0 + K < N) ? n0 + K : N;
-    if (seg > 0 && s->tape_holds[(seg - 1) % TAPE_SETS] != seg - 1) {
-      // recompute the previous segment into its set on the second stream,
-      // once the adjoint that last read that set (segment seg-1+TAPE_SETS)
-      // is done
-      const int o = (seg - 1) % TAPE_SETS;
+    const int o = seg % sets;
+    if (s->tape_holds[o] != seg) {
+      // the last segment, or (without overlap) any segment: on the solver's
+      // stream, after the adjoint of the segment that used the set before
       s->tape_holds[o] = -1;
-      if (seg - 1 + TAPE_SETS < nck) HM_HIP(hipStreamWaitEvent(s->aux, s->ev_done[o], 0));
+      if ((st = recompute_segment(s, seg, n0, n1, dt)) != HM_OK) return st;
+      s->tape_holds[o] = seg;
+    }
+    if (overlap && seg > 0 && s->tape_holds[(seg - 1) % sets] != seg - 1) {
+      // recompute the previous segment into its set on the second stream,
+      // once the adjoint that last read that set (segment seg-1+sets) is done
+      const int op = (seg - 1) % sets;
+      s->tape_holds[op] = -1;
+      if (seg - 1 + sets < nck) HM_HIP(hipStreamWaitEvent(s->aux, s->ev_done[op], 0));
       s->stream = s->aux;
       st = recompute_segment(s, seg - 1, n0 - K, n0, dt);
       s->stream = main;
       if (st != HM_OK) return st;
-      HM_HIP(hipEventRecord(s->ev_ready[o], s->aux));
-      s->tape_holds[o] = seg - 1;
-      on_aux[o] = true;
+      HM_HIP(hipEventRecord(s->ev_ready[op], s->aux));
+      s->tape_holds[op] = seg - 1;
+      on_aux[op] = true;
     }
-    const int o = seg % TAPE_SETS;
     if (on_aux[o]) {
       HM_HIP(hipStreamWaitEvent(main, s->ev_ready[o], 0));
       on_aux[o] = false;
     }
-    double **S = segment_stages(s, seg, K);
+    double **S = segment_slots(s, seg, K);
     for (int n = n1 - 1; n >= n0; n--) {
       // the step adjoint alternates between the caller's array and scratch
       double *out = cur == lam ? s->vscratch[2] : lam;
-      if ((st = rk4_step_vjp(s, S + (size_t)4 * (n - n0), dt, cur, out)) != HM_OK) return st;
+      if (plan.step_level) {
+        // the step's stage states, from its starting state
+        double *w = s->vscratch[3];
+        HM_HIP(hipMemcpyAsync(w, S[n - n0], n5 * sizeof(double), hipMemcpyDeviceToDevice, main));
+        double *const Us[3] = {s->sscratch[0], s->sscratch[1], s->sscratch[2]};
+        if ((st = rk4_step_store(s, w, dt, Us)) != HM_OK) return st;
+        double *const U[4] = {S[n - n0], Us[0], Us[1], Us[2]};
+        if ((st = rk4_step_vjp(s, U, dt, cur, out)) != HM_OK) return st;
+      } else if ((st = rk4_step_vjp(s, S + (size_t)4 * (n - n0), dt, cur, out)) != HM_OK) {
+        return st;
+      }
       cur = out;
     }
-    HM_HIP(hipEventRecord(s->ev_done[o], main));
+    if (overlap) HM_HIP(hipEventRecord(s->ev_done[o], main));
   }
   if (cur != lam) HM_HIP(hipMemcpyAsync(lam, cur, n5 * sizeof(double), hipMemcpyDeviceToDevice, main));
   return HM_OK;
@@ -895,7 +1025,10 @@ struct hm_ensemble {
   std::vector<double *> ckpt, stages, vscratch, vwork;
   int tape_nsteps = 0, tape_every = 0;
   std::vector<double> tape_dt;        // the members' dt of the recorded trajectory
-  int tape_holds[TAPE_SETS] = {-1, -1, -1};  // segment held by each stage set (-1: none)
+  int tape_holds[TAPE_SETS] = {-1, -1, -1};  // segment held by each set (-1: none)
+  TapePlan tape_plan;                 // schedule of the recorded trajectory
+  size_t tape_budget = 0;             // bytes the tape may hold (0: from the device's free memory)
+  std::vector<double *> sscratch;     // step level: the stage states of the step being taken back
   int shape = 0;
   unsigned grid[3] = {0, 0, 0};
   bool dt_set = false;
@@ -1015,7 +1148,7 @@ void hm_ensemble_destroy(hm_ensemble *e) {
   }
   for (int d = 0; d < 3; d++)
     if (e->lines0[d]) (void)hipFree(e->lines0[d]);
-  for (auto *v : {&e->ckpt, &e->stages, &e->vscratch, &e->vwork})
+  for (auto *v : {&e->ckpt, &e->stages, &e->vscratch, &e->vwork, &e->sscratch})
     for (double *b : *v) (void)hipFree(b);
   delete e;
 }
@@ -1334,6 +1467,31 @@ static hm_status ens_buffers(std::vector<double *> &v, size_t count, size_t n) {
   return HM_OK;
 }
 
+// Segment seg's set is seg % sets; within it step m keeps u_n, U1, U2, U3
+// at slots[4m .. 4m+3] (stage level) or u_n at slots[m] (step level).
+static double **ens_segment_slots(hm_ensemble *e, int seg, int K) {
+  const TapePlan &p = e->tape_plan;
+  return &e->stages[(size_t)(p.step_level ? 1 : 4) * K * (seg % p.sets)];
+}
+
+hm_status hm_ensemble_set_tape_budget(hm_ensemble *e, size_t bytes) {
+  if (!e) return fail(HM_ERR_ARG, "hm_ensemble_set_tape_budget: ensemble is NULL");
+  e->tape_budget = bytes;
+  return HM_OK;
+}
+
+hm_status hm_ensemble_tape_info(const hm_ensemble *e, int *sets, int *step_level, size_t *bytes) {
+  if (!e) return fail(HM_ERR_ARG, "hm_ensemble_tape_info: ensemble is NULL");
+  if (e->tape_nsteps < 1) return fail(HM_ERR_ARG, "hm_ensemble_tape_info: no forward sweep recorded");
+  const size_t n5 = 5 * e->ncell;
+  if (sets) *sets = e->tape_plan.sets;
+  if (step_level) *step_level = e->tape_plan.step_level ? 1 : 0;
+  if (bytes)
+    *bytes = hm::ens_vjp_work_size(e->sc, e->ncell) * sizeof(double) +
+             tape_total_states(e->tape_plan, e->tape_nsteps, e->tape_every) * n5 * sizeof(double);
+  return HM_OK;
+}
+
 hm_status hm_ensemble_adjoint_forward(hm_ensemble *e, double *u, int nsteps, int every) {
   hm_status st = ens_check_vjp(e, u, "hm_ensemble_adjoint_forward");
   if (st != HM_OK) return st;
@@ -1341,26 +1499,49 @@ hm_status hm_ensemble_adjoint_forward(hm_ensemble *e, double *u, int nsteps, int
     return fail(HM_ERR_ARG, "hm_ensemble_adjoint_forward: nsteps and checkpoint_every must be >= 1");
   e->tape_nsteps = 0;
   for (int i = 0; i < TAPE_SETS; i++) e->tape_holds[i] = -1;
-  const size_t n5 = 5 * e->ncell;
+  const size_t n5 = 5 * e->ncell, state_bytes = n5 * sizeof(double);
+  const size_t work_bytes = hm::ens_vjp_work_size(e->sc, e->ncell) * sizeof(double);
+  const size_t held = held_bytes(e->ckpt, n5) + held_bytes(e->stages, n5) + held_bytes(e->vscratch, n5) +
+                      held_bytes(e->sscratch, n5) + (e->vwork.empty() ? 0 : work_bytes);
+  const size_t budget = e->tape_budget ? e->tape_budget : auto_tape_budget(held);
+  TapePlan plan;
+  if (!choose_tape(budget, state_bytes, work_bytes, nsteps, every, &plan)) {
+    TapePlan least;
+    least.sets = 1;
+    least.step_level = true;
+    return fail(HM_ERR_NOMEM, "hm_ensemble_adjoint_forward: the smallest tape (one checkpoint every " +
+                                  std::to_string(every) + " steps, one segment of step states) needs " +
+                                  std::to_string(work_bytes + tape_total_states(least, nsteps, every) * state_bytes) +
+                                  " bytes; the budget is " + std::to_string(budget) +
+                                  " (hm_ensemble_set_tape_budget, or a longer checkpoint interval)");
+  }
+  e->tape_plan = plan;
   const int nck = (nsteps + every - 1) / every;
-  if ((st = ens_buffers(e->ckpt, (size_t)nck, n5)) != HM_OK) return st;
-  if ((st = ens_buffers(e->stages, tape_states(nsteps, every), n5)) != HM_OK) return st;
-  // the last TAPE_SETS segments keep their stage states (segment seg in set
-  // seg % TAPE_SETS)
-  const int nkeep = (nck >= TAPE_SETS ? nck - TAPE_SETS : 0) * every;
+  if ((st = fit_buffers(e->ckpt, (size_t)nck, n5)) != HM_OK) return st;
+  if ((st = fit_buffers(e->stages, tape_slots(plan, nsteps, every), n5)) != HM_OK) return st;
+  if ((st = fit_buffers(e->sscratch, plan.step_level ? 3 : 0, n5)) != HM_OK) return st;
+  if ((st = ens_buffers(e->vscratch, 4, n5)) != HM_OK) return st;
+  // the last `sets` segments keep their states (segment seg in set seg % sets)
+  const int nkeep = (nck >= plan.sets ? nck - plan.sets : 0) * every;
   for (int n = 0; n < nsteps; n++) {
     if (n % every == 0)
       HM_HIP(hipMemcpyAsync(e->ckpt[n / every], u, n5 * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
     if (n < nkeep) {
       if ((st = hm_ensemble_rk4(e, u, 1)) != HM_OK) return st;
+      continue;
+    }
+    const int seg = n / every, m = n - seg * every;
+    double **slots = ens_segment_slots(e, seg, every);
+    if (plan.step_level) {
+      HM_HIP(hipMemcpyAsync(slots[m], u, n5 * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+      if ((st = hm_ensemble_rk4(e, u, 1)) != HM_OK) return st;
     } else {
-      const int seg = n / every;
-      double **U = &e->stages[(size_t)4 * every * (seg % TAPE_SETS) + (size_t)4 * (n - seg * every)];
+      double **U = slots + (size_t)4 * m;
       HM_HIP(hipMemcpyAsync(U[0], u, n5 * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
       double *const Us[3] = {U[1], U[2], U[3]};
       if ((st = ens_step_store(e, u, Us)) != HM_OK) return st;
-      e->tape_holds[seg % TAPE_SETS] = seg;
     }
+    e->tape_holds[seg % plan.sets] = seg;
   }
   e->tape_dt.resize(e->nmem);
   for (int m = 0; m < e->nmem; m++) e->tape_dt[m] = e->mem[m].dt;
@@ -1372,8 +1553,10 @@ hm_status hm_ensemble_adjoint_forward(hm_ensemble *e, double *u, int nsteps, int
 static hm_status ens_reverse_sweep(hm_ensemble *e, double *lam) {
   hm_status st;
   const int N = e->tape_nsteps, K = e->tape_every;
+  const TapePlan plan = e->tape_plan;
   const size_t n5 = 5 * e->ncell;
-  if ((st = ens_buffers(e->stages, tape_states(N, K), n5)) != HM_OK) return st;
+  if ((st = ens_buffers(e->stages, tape_slots(plan, N, K), n5)) != HM_OK) return st;
+  if ((st = ens_buffers(e->sscratch, plan.step_level ? 3 : 0, n5)) != HM_OK) return st;
   if ((st = ens_buffers(e->vscratch, 4, n5)) != HM_OK) return st;
   double *work = nullptr;
   const size_t nw = hm::ens_vjp_work_size(e->sc, e->ncell);
@@ -1381,45 +1564,66 @@ static hm_status ens_reverse_sweep(hm_ensemble *e, double *lam) {
     if ((st = ens_buffers(e->vwork, 1, nw)) != HM_OK) return st;
     work = e->vwork[0];
   }
-  hm::EnsVjpPlan plan;
-  plan.mem = e->mem_dev;
-  plan.nmem = e->nmem;
-  plan.ncell = e->ncell;
+  hm::EnsVjpPlan vp;
+  vp.mem = e->mem_dev;
+  vp.nmem = e->nmem;
+  vp.ncell = e->ncell;
   for (int d = 0; d < 3; d++) {
-    plan.lines0[d] = e->lines0[d];
-    plan.nlines[d] = e->nlines[d];
-    plan.nmax[d] = e->nmax[d];
+    vp.lines0[d] = e->lines0[d];
+    vp.nlines[d] = e->nlines[d];
+    vp.nmax[d] = e->nmax[d];
   }
-  plan.tiles0 = e->tiles0;
-  plan.ntiles = e->ntiles;
+  vp.tiles0 = e->tiles0;
+  vp.ntiles = e->ntiles;
   const int nck = (N + K - 1) / K;
   double *cur = lam;
   for (int seg = nck - 1; seg >= 0; seg--) {
     const int n0 = seg * K, n1 = (n0 + K < N) ? n0 + K : N;
-    const int o = seg % TAPE_SETS;
-    double **S = &e->stages[(size_t)4 * K * o];
+    const int o = seg % plan.sets;
+    double **S = ens_segment_slots(e, seg, K);
     if (e->tape_holds[o] != seg) {
-      // recompute the segment from its checkpoint, keeping the stage states
+      // recompute the segment from its checkpoint into its set
       e->tape_holds[o] = -1;
-      for (int n = n0; n < n1; n++) {
-        double **U = S + (size_t)4 * (n - n0);
-        if (n == n0) HM_HIP(hipMemcpyAsync(U[0], e->ckpt[seg], n5 * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
-        double *next = n + 1 < n1 ? S[(size_t)4 * (n - n0 + 1)] : e->vscratch[3];
-        HM_HIP(hipMemcpyAsync(next, U[0], n5 * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
-        double *const Us[3] = {U[1], U[2], U[3]};
-        if ((st = ens_step_store(e, next, Us)) != HM_OK) return st;
+      if (plan.step_level) {
+        HM_HIP(hipMemcpyAsync(S[0], e->ckpt[seg], n5 * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+        for (int m = 1; m < n1 - n0; m++) {
+          HM_HIP(hipMemcpyAsync(S[m], S[m - 1], n5 * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+          double *const Us[3] = {e->sscratch[0], e->sscratch[1], e->sscratch[2]};
+          if ((st = ens_step_store(e, S[m], Us)) != HM_OK) return st;
+        }
+      } else {
+        for (int n = n0; n < n1; n++) {
+          double **U = S + (size_t)4 * (n - n0);
+          if (n == n0)
+            HM_HIP(hipMemcpyAsync(U[0], e->ckpt[seg], n5 * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+          double *next = n + 1 < n1 ? S[(size_t)4 * (n - n0 + 1)] : e->vscratch[3];
+          HM_HIP(hipMemcpyAsync(next, U[0], n5 * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+          double *const Us[3] = {U[1], U[2], U[3]};
+          if ((st = ens_step_store(e, next, Us)) != HM_OK) return st;
+        }
       }
       e->tape_holds[o] = seg;
     }
     for (int n = n1 - 1; n >= n0; n--) {
-      double **U = S + (size_t)4 * (n - n0);
+      double *Ust[4];
+      if (plan.step_level) {
+        // the step's stage states, from its starting state
+        double *w = e->vscratch[3];
+        HM_HIP(hipMemcpyAsync(w, S[n - n0], n5 * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+        double *const Us[3] = {e->sscratch[0], e->sscratch[1], e->sscratch[2]};
+        if ((st = ens_step_store(e, w, Us)) != HM_OK) return st;
+        Ust[0] = S[n - n0];
+        for (int i = 0; i < 3; i++) Ust[i + 1] = Us[i];
+      } else {
+        for (int i = 0; i < 4; i++) Ust[i] = S[(size_t)4 * (n - n0) + i];
+      }
       double *out = cur == lam ? e->vscratch[2] : lam;
       // step adjoint: out = cur + sum_s nu_s (stage coefficients formed per member)
       const double *nu_in = nullptr;
       double *nu_out = e->vscratch[0];
       for (int stage = 3; stage >= 0; stage--) {
         const double A = stage < 3 ? RK_A_SUB[stage] : 0.0, B = RK_B[stage];
-        hm::launch_ens_rhs_vjp(e->sc, plan, U[stage], cur, nu_in, A, B, nu_out, out, stage == 3, work, e->stream);
+        hm::launch_ens_rhs_vjp(e->sc, vp, Ust[stage], cur, nu_in, A, B, nu_out, out, stage == 3, work, e->stream);
         if ((st = check_launch("hm_ensemble_adjoint_reverse")) != HM_OK) return st;
         nu_in = nu_out;
         nu_out = nu_out == e->vscratch[0] ? e->vscratch[1] : e->vscratch[0];

@@ -111,6 +111,16 @@ class Ensemble:
         g = (ctypes.c_uint * 3)(*grid)
         _lib.check(self.L.hm_ensemble_set_tiles(self.h, int(shape), g))
 
+    def set_tape_budget(self, nbytes: int):
+        """Bytes the batched adjoint tape may hold (0: from the device's free memory)."""
+        _lib.check(self.L.hm_ensemble_set_tape_budget(self.h, int(nbytes)))
+
+    def tape_info(self) -> dict:
+        """Schedule of the last adjoint_forward: sets, step level, bytes."""
+        sets, step, nb = ctypes.c_int(), ctypes.c_int(), ctypes.c_size_t()
+        _lib.check(self.L.hm_ensemble_tape_info(self.h, ctypes.byref(sets), ctypes.byref(step), ctypes.byref(nb)))
+        return {"sets": sets.value, "step_level": bool(step.value), "bytes": nb.value}
+
     def set_wait_limit(self, polls: int):
         """Polls after which a dependency wait of the one-launch-per-stage
         kernel (shapes 2, 3) gives up and the call reports HM_ERR_TIMEOUT."""

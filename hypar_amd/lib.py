@@ -91,6 +91,12 @@ SIGNATURES = [
     ("hm_ensemble_tune", [_VP, _DP, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint)],
      ctypes.c_int),
     ("hm_ensemble_set_wait_limit", [_VP, ctypes.c_uint], ctypes.c_int),
+    ("hm_set_tape_budget", [_VP, _SZ], ctypes.c_int),
+    ("hm_tape_info", [_VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_SZ)],
+     ctypes.c_int),
+    ("hm_ensemble_set_tape_budget", [_VP, _SZ], ctypes.c_int),
+    ("hm_ensemble_tape_info", [_VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_SZ)],
+     ctypes.c_int),
     # include/hypar_mi355x_slots.h
     ("hm_bind", [_VP, _VP], ctypes.c_int),
     ("hm_bind_state", [_VP, _DP, ctypes.c_int], ctypes.c_int),

@@ -198,6 +198,16 @@ class NS3D:
         _lib.check(self.L.hm_adjoint_reverse(self.h, ctypes.c_void_p(lam.data_ptr())))
         return lam
 
+    def set_tape_budget(self, nbytes: int):
+        """Bytes the adjoint tape may hold (0: from the device's free memory)."""
+        _lib.check(self.L.hm_set_tape_budget(self.h, int(nbytes)))
+
+    def tape_info(self) -> dict:
+        """Schedule of the last adjoint_forward: sets, step level, bytes."""
+        sets, step, nb = ctypes.c_int(), ctypes.c_int(), ctypes.c_size_t()
+        _lib.check(self.L.hm_tape_info(self.h, ctypes.byref(sets), ctypes.byref(step), ctypes.byref(nb)))
+        return {"sets": sets.value, "step_level": bool(step.value), "bytes": nb.value}
+
     def profile_kernels(self, u: torch.Tensor, reps: int = 5) -> dict:
         """Mean duration (ms) of each kernel of one RK stage, HIP events on
         the solver's stream."""

@@ -205,15 +205,28 @@ hm_status hm_rhs_vjp(hm_solver *s, double *out, const double *u, const double *l
 /* lam <- (d u_{n+1} / d u_n)^T lam for one RK4 step of size dt from u_n */
 hm_status hm_rk4_vjp(hm_solver *s, const double *u_n, double dt, double *lam);
 /* Forward sweep of nsteps RK4 steps in place on u, keeping a checkpoint of
- * the state every `checkpoint_every` steps in HBM (the solver owns them),
- * and the RK stage states of the last three checkpoint segments
- * (12 * checkpoint_every states). */
+ * the state every `checkpoint_every` (K) steps in HBM (the solver owns
+ * them) and, for the last `sets` segments of K steps, either the four RK
+ * stage states of every step (stage level) or the state every step starts
+ * from (step level).  The schedule is the cheapest that fits the tape
+ * budget, in this order: stage level with 3, 2, 1 sets (12K, 8K, 4K
+ * states), then step level with 3, 2, 1 sets (3K, 2K, K states, plus 3).
+ * HM_ERR_NOMEM names the bytes the smallest schedule needs when none fits. */
 hm_status hm_adjoint_forward(hm_solver *s, double *u, double dt, int nsteps, int checkpoint_every);
 /* Reverse sweep of the last hm_adjoint_forward: lam holds dJ/du_N on entry
- * and dJ/du_0 on return.  Every segment before the last three is recomputed
- * once from its checkpoint with its RK stage states kept in HBM, then
- * stepped back. */
+ * and dJ/du_0 on return.  Every segment the sets do not hold is recomputed
+ * once from its checkpoint; at step level every step also runs forward
+ * once more for its stage states before it is taken back.  The gradient is
+ * the same, bit for bit, whatever the schedule. */
 hm_status hm_adjoint_reverse(hm_solver *s, double *lam);
+/* Bytes the tape of the next forward sweep may hold: checkpoints, sets,
+ * scratch states and viscous work space.  0 (the default) takes the
+ * device's free memory plus what the tape already holds, less a reserve of
+ * 2 GiB or 2% of the device, whichever is larger. */
+hm_status hm_set_tape_budget(hm_solver *s, size_t bytes);
+/* The schedule of the recorded trajectory: sets (1..3), step_level (0/1)
+ * and the bytes it holds (any pointer may be NULL). */
+hm_status hm_tape_info(const hm_solver *s, int *sets, int *step_level, size_t *bytes);
 
 /* Kernel timing for roofline reports (no counterpart in the reference; the
  * reference's GPU_STAT build prints per-kernel cudaEvent times,
@@ -280,13 +293,17 @@ hm_status hm_ensemble_set_wait_limit(hm_ensemble *e, unsigned polls);
  * of hm_adjoint_forward / hm_adjoint_reverse for all members at once, each
  * with its own dt.  The forward sweep advances the batched state nsteps RK4
  * steps in place with a checkpoint every `checkpoint_every` steps and the
- * RK stage states of the last three segments; the reverse sweep recomputes
- * the earlier segments, takes lam = dJ/du_N of all members and returns
+ * schedule of hm_adjoint_forward; the reverse sweep recomputes what the
+ * schedule did not keep, takes lam = dJ/du_N of all members and returns
  * dJ/du_0.
  * Every VJP launch of a stage covers the (member, line) work items of all
  * members.  Built for the schemes hm_rhs_vjp supports. */
 hm_status hm_ensemble_adjoint_forward(hm_ensemble *e, double *u, int nsteps, int checkpoint_every);
 hm_status hm_ensemble_adjoint_reverse(hm_ensemble *e, double *lam);
+/* The tape budget and schedule of the batched adjoint, as hm_set_tape_budget
+ * and hm_tape_info for one grid. */
+hm_status hm_ensemble_set_tape_budget(hm_ensemble *e, size_t bytes);
+hm_status hm_ensemble_tape_info(const hm_ensemble *e, int *sets, int *step_level, size_t *bytes);
 
 #ifdef __cplusplus
 }

@@ -332,3 +332,55 @@ def test_reverse_sweep_repeats_and_survives_step_vjp():
     (a2,) = torch.autograd.grad(yN, xg, _hm(lam))
     assert torch.equal(a1, g1) and torch.equal(a2, g1)
     s.close()
+
+
+# (sets, step level) in the order the forward sweep prefers them, and the
+# states each holds for 13 steps with a checkpoint every 4 (4 checkpoints):
+# checkpoints + sets x (4K or K) + 4 scratch (+3 stage states at step level)
+TAPE_PLANS = [((3, False), 56), ((2, False), 40), ((1, False), 24), ((3, True), 23), ((2, True), 19),
+              ((1, True), 15)]
+
+
+@pytest.mark.gpu
+def test_tape_schedules_under_budgets_give_the_same_gradient():
+    """The forward sweep takes the cheapest tape schedule that fits the
+    budget.  13 steps with a checkpoint every 4 (segments 4, 4, 4, 1): with
+    the budget set to the bytes of each schedule in turn, that schedule is
+    chosen, and the gradient is bit for bit the one of the default tape and
+    matches torch autograd through the CPU trajectory (viscous
+    characteristic/Roe).  Below the smallest schedule the sweep reports
+    HM_ERR_NOMEM with the bytes it needs."""
+    from hypar_amd import lib
+    w, u0, dt, dx, sch = _scheme_case("characteristic", "roe", 1000.0)
+    x = T.to_soa(u0).requires_grad_(True)
+    y = x
+    for _ in range(13):
+        y = T.rk4_step(y, dt, dx, **sch)
+    lam = torch.tensor(np.random.default_rng(13).normal(size=x.shape))
+    (ref,) = torch.autograd.grad(y, x, lam)
+    print("[gpu] tape schedules, char/roe viscous, 13 steps, checkpoint every 4", flush=True)
+    s = _gpu_solver(w)
+    state_bytes = 5 * int(np.prod(w.dims)) * 8
+    s.adjoint_forward(_hm(x), dt, 13, 4)
+    info = s.tape_info()
+    assert (info["sets"], info["step_level"]) == (3, False)
+    work_bytes = info["bytes"] - 56 * state_bytes
+    base = s.adjoint_reverse(_hm(lam))
+    err = _rel(base.cpu().view_as(ref), ref)
+    print(f"[gpu]   default tape {info}: rel err {err:.2e}", flush=True)
+    assert err <= 1e-11
+    for (sets, step), nstates in TAPE_PLANS:
+        s.set_tape_budget(work_bytes + nstates * state_bytes)
+        u = _hm(x)
+        s.adjoint_forward(u, dt, 13, 4)
+        info = s.tape_info()
+        g = s.adjoint_reverse(_hm(lam))
+        print(f"[gpu]   budget of {nstates} states -> {info}, equal {torch.equal(g, base)}", flush=True)
+        assert (info["sets"], info["step_level"]) == (sets, step)
+        assert info["bytes"] == work_bytes + nstates * state_bytes
+        assert torch.equal(g, base)
+    s.set_tape_budget(work_bytes + 15 * state_bytes - 1)
+    with pytest.raises(lib.HMError) as err:
+        s.adjoint_forward(_hm(x), dt, 13, 4)
+    assert err.value.code == lib.HM_ERR_NOMEM and b"needs" in lib.load().hm_last_error()
+    s.close()

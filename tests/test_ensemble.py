@@ -291,3 +291,37 @@ def test_ensemble_tape_keeps_its_dt_and_repeats():
     print(f"[gpu]   equal: {torch.equal(g1, g2)}, {torch.equal(g1, g3)}", flush=True)
     assert torch.equal(g1, g2) and torch.equal(g1, g3)
     e.close()
+
+
+@pytest.mark.gpu
+def test_batched_tape_schedules_give_the_same_gradient():
+    """The batched adjoint under tape budgets of each schedule (three small
+    viscous characteristic/Roe members, 13 steps, a checkpoint every 4): the
+    schedule named by the budget is chosen and every gradient equals the
+    default tape's bit for bit."""
+    from hypar_amd.ensemble import Ensemble
+    dims = [(12, 10, 14), (10, 12, 8), (14, 8, 10)]
+    ws, u0s, dts = _members(dims, "characteristic", "roe", 1000.0)
+    e = Ensemble(dims, "characteristic", "roe", Re=1000.0, Minf=0.1)
+    u0 = e.to_device(u0s)
+    e.set_dt(dts)
+    lam = torch.randn(u0.shape, dtype=torch.float64, generator=torch.Generator().manual_seed(7)).cuda()
+    state_bytes = 5 * e.ncell * 8
+    u = u0.clone()
+    e.adjoint_forward(u, 13, 4)
+    info = e.tape_info()
+    assert (info["sets"], info["step_level"]) == (3, False)
+    work_bytes = info["bytes"] - 56 * state_bytes
+    base = e.adjoint_reverse(lam.clone())
+    plans = [((3, False), 56), ((2, False), 40), ((1, False), 24), ((3, True), 23), ((2, True), 19),
+             ((1, True), 15)]
+    for (sets, step), nstates in plans:
+        e.set_tape_budget(work_bytes + nstates * state_bytes)
+        u = u0.clone()
+        e.adjoint_forward(u, 13, 4)
+        info = e.tape_info()
+        g = e.adjoint_reverse(lam.clone())
+        print(f"[gpu] batched tape, budget of {nstates} states -> {info}, equal {torch.equal(g, base)}", flush=True)
+        assert (info["sets"], info["step_level"]) == (sets, step)
+        assert torch.equal(g, base)
+    e.close()
