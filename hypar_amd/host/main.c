@@ -110,7 +110,9 @@ int main(int argc, char **argv) {
   printf("Setting up time integration.\n");
   if (TimeInitialize(&TS, sims, mpi, nsims, !slots)) return 1;
   printf("Time integration: %s %s (%s%s)\n", sims[0].time_scheme, sims[0].time_scheme_type,
-         TS.fused ? (TS.ssp ? "fused SSPRK3 step" : "fused 3N RK4 step") : "slot composition",
+         TS.fused ? (TS.ssp ? "fused SSPRK3 step" : "fused 3N RK4 step")
+                  : (TS.RHSFunction == hm_slot_RHSFunction ? "stage loop around the fused right-hand side"
+                                                           : "slot composition"),
          TS.ens ? ", batched over all members" : "");
   printf("Solving in time (from %d to %d iterations)\n", sims[0].restart_iter, TS.n_iter);
   double total = 0.0;

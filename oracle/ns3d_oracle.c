@@ -803,6 +803,13 @@ int orc_rk4(const orc_params *p, double *u, double dt, int nsteps) {
   return orc_rk(p, u, dt, nsteps, 4, A, b);
 }
 
+/* any explicit tableau of TimeExplicitRKInitialize.c:26-79 (ns <= 4) */
+int orc_rk_tableau(const orc_params *p, double *u, double dt, int nsteps, int ns, const double *A,
+                   const double *b) {
+  if (ns < 1 || ns > 4) return 1;
+  return orc_rk(p, u, dt, nsteps, ns, A, b);
+}
+
 /* "rk ssprk3" / "tvdrk3" (TimeExplicitRKInitialize.c:69-79) */
 int orc_ssprk3(const orc_params *p, double *u, double dt, int nsteps) {
   const double A[9] = {0, 0, 0, 1.0, 0, 0, 0.25, 0.25, 0};

@@ -53,6 +53,10 @@ int orc_parabolic(const orc_params *p, double *par, const double *u);
 int orc_rk4(const orc_params *p, double *u, double dt, int nsteps);
 /* nsteps SSPRK3 steps ("rk ssprk3", TimeExplicitRKInitialize.c:69-79) */
 int orc_ssprk3(const orc_params *p, double *u, double dt, int nsteps);
+/* nsteps steps of the explicit tableau A (ns x ns, row-major) and b, the
+ * stage loop of TimeRK.c:126-195; ns <= 4 (returns 1 otherwise). */
+int orc_rk_tableau(const orc_params *p, double *u, double dt, int nsteps, int ns, const double *A,
+                   const double *b);
 
 /* Max CFL over the interior (NavierStokes3DComputeCFL). */
 double orc_cfl(const orc_params *p, const double *u, double dt);

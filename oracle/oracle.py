@@ -43,6 +43,8 @@ def lib():
         L.orc_parabolic.argtypes = [pp, dp, dp]; L.orc_parabolic.restype = ctypes.c_int
         L.orc_rk4.argtypes = [pp, dp, ctypes.c_double, ctypes.c_int]; L.orc_rk4.restype = ctypes.c_int
         L.orc_ssprk3.argtypes = [pp, dp, ctypes.c_double, ctypes.c_int]; L.orc_ssprk3.restype = ctypes.c_int
+        L.orc_rk_tableau.argtypes = [pp, dp, ctypes.c_double, ctypes.c_int, ctypes.c_int, dp, dp]
+        L.orc_rk_tableau.restype = ctypes.c_int
         L.orc_cfl.argtypes = [pp, dp, ctypes.c_double]; L.orc_cfl.restype = ctypes.c_double
         L.orc_volume_integral.argtypes = [pp, dp, dp]; L.orc_volume_integral.restype = None
         L.orc_periodic_bc.argtypes = [pp, dp]; L.orc_periodic_bc.restype = None
@@ -106,12 +108,24 @@ class Oracle:
     def rk4(self, u, dt, nsteps):
         return self.step(u, dt, nsteps, "44")
 
+    # the other explicit tableaux of TimeExplicitRKInitialize.c:26-57,
+    # (A row-major, b)
+    TABLEAUX = {"1fe": ([0.0], [1.0]),
+                "22": ([0.0, 0.0, 1.0, 0.0], [0.5, 0.5]),
+                "33": ([0.0, 0.0, 0.0, 2.0 / 3.0, 0.0, 0.0, 2.0 / 3.0 - 1.0 / 4.0, 1.0 / 4.0, 0.0],
+                       [1.0 / 4.0, -1.0 / 4.0, 1.0])}
+
     def step(self, u, dt, nsteps, scheme="44"):
         """nsteps steps of the rk-class scheme named as in solver.inp's
-        time_scheme_type ("44", or "ssprk3"/"tvdrk3")."""
-        fn = {"44": lib().orc_rk4, "ssprk3": lib().orc_ssprk3, "tvdrk3": lib().orc_ssprk3}[scheme]
+        time_scheme_type ("44", "ssprk3"/"tvdrk3", "1fe", "22", "33")."""
         G = self.ghosted(u)
-        if fn(ctypes.byref(self.p), _dp(G), float(dt), int(nsteps)):
+        if scheme in self.TABLEAUX:
+            A, b = (np.ascontiguousarray(x, dtype=np.float64) for x in self.TABLEAUX[scheme])
+            err = lib().orc_rk_tableau(ctypes.byref(self.p), _dp(G), float(dt), int(nsteps), len(b), _dp(A), _dp(b))
+        else:
+            fn = {"44": lib().orc_rk4, "ssprk3": lib().orc_ssprk3, "tvdrk3": lib().orc_ssprk3}[scheme]
+            err = fn(ctypes.byref(self.p), _dp(G), float(dt), int(nsteps))
+        if err:
             raise RuntimeError(f"oracle {scheme} step failed")
         return self.interior(G)
 

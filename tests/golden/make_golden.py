@@ -56,6 +56,10 @@ def _workload(kind: str, dims, n_iter: int) -> decks.Workload:
     if kind == "c0_ssprk3":
         return dataclasses.replace(decks.config0(dims[0], n_iter), dims=tuple(dims), n_iter=n_iter,
                                    dt=None, time_scheme_type="ssprk3")
+    # "<c0|c1>_rk<type>": the other explicit tableaux of solver.inp (1fe, 22, 33)
+    for base in ("c0", "c1"):
+        if kind.startswith(base + "_rk"):
+            return dataclasses.replace(_workload(base, dims, n_iter), time_scheme_type=kind[len(base) + 3:])
     if kind == "dns":
         return dataclasses.replace(decks.config_dns(dims[0], n_iter), dims=tuple(dims), dt=None)
     if kind == "c0":
@@ -83,6 +87,11 @@ FIXTURES = {
     # component-wise WENO5 + Roe, SSPRK3), dt from CFL 0.5
     "dns_16_s4": ("dns", (16, 16, 16), 4, True),
     "dns_ragged_s3": ("dns", (12, 20, 16), 3, True),
+    # the tableaux without a fused step: HyPar's stage loop around the
+    # library's right-hand side
+    "c0_rk1fe_16_s2": ("c0_rk1fe", (16, 16, 16), 2, True),
+    "c0_rk22_16_s3": ("c0_rk22", (16, 16, 16), 3, True),
+    "c1_rk33_16_s3": ("c1_rk33", (16, 16, 16), 3, True),
     # mapped (Henrick et al.), Z (Borges et al.) and Yamaleev-Carpenter weights
     "c0_mapped_16_s4": ("c0+mapped", (16, 16, 16), 4, True),
     "c0_borges_16_s4": ("c0+borges", (16, 16, 16), 4, True),

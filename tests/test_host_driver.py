@@ -74,25 +74,30 @@ def test_missing_solver_inp(tmp_path):
                                        ("c0_ssprk3_16_s4", ["--slots"]), ("dns_16_s4", []), ("dns_ragged_s3", ["--slots"]),
                                        ("c1_mapped_16_s4", []), ("c0_yc_16_s4", ["--slots"]),
                                        ("c1_borges_16_s4", ["--slots"]), ("c1_optimal_16_s4", []),
-                                       ("c0_optimal_16_s4", ["--slots"])])
+                                       ("c0_optimal_16_s4", ["--slots"]), ("c0_rk1fe_16_s2", []),
+                                       ("c0_rk22_16_s3", []), ("c1_rk33_16_s3", []),
+                                       ("c1_rk33_16_s3", ["--slots"])])
 def test_driver_matches_reference(tmp_path, name, args):
     d, w, meta, z = _deck(tmp_path, name)
     print(f"[gpu] driver {name} {args}", flush=True)
     r = run_driver(d, *args)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    scheme = meta.get("time_scheme_type", "44")
     if args:
         assert "slot composition" in r.stdout
-    elif meta.get("time_scheme_type", "44") == "ssprk3":
+    elif scheme == "ssprk3":
         assert "fused SSPRK3 step" in r.stdout
-    else:
+    elif scheme == "44":
         assert "fused 3N RK4 step" in r.stdout
+    else:  # HyPar's own stage loop with the library's RHSFunction slot
+        assert "stage loop around the fused right-hand side" in r.stdout
     _, _, u = decks.read_op_bin(os.path.join(d, "op.bin"))
     assert decks.rel_l2(u, z["uN"]) <= 1e-12
     # function_counts.dat (SimulationWriteErrors.c:91-95): one hyperbolic,
     # parabolic (viscous only) and source evaluation per stage
     # (HyperbolicFunction.c:67, NavierStokes3DParabolicFunction.c:71-73,
     # SourceFunction.c:46-49)
-    nrhs = meta["steps"] * (3 if meta.get("time_scheme_type", "44") == "ssprk3" else 4)
+    nrhs = meta["steps"] * {"44": 4, "ssprk3": 3, "33": 3, "22": 2, "1fe": 1}[scheme]
     with open(os.path.join(d, "function_counts.dat")) as f:
         counts = [int(x) for x in f.read().split()]
     assert counts == [meta["steps"], nrhs, nrhs if meta["Re"] > 0 else 0, nrhs], counts

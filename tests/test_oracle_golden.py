@@ -12,7 +12,9 @@ from oracle.oracle import Oracle
 
 WEIGHTS = [f"{c}_{t}_16_s4" for c in ("c0", "c1") for t in ("mapped", "borges", "yc", "optimal")]
 SSP = ["c0_ssprk3_16_s4", "dns_16_s4", "dns_ragged_s3"]
-FULL = ["c0_16_s1", "c0_16_s10", "c1_16_s1", "c1_16_s10", "c0_ragged_s3", "c1_ragged_s3"] + WEIGHTS + SSP
+# forward Euler, "22" and "33" (TimeExplicitRKInitialize.c:26-57)
+TABLEAUX = ["c0_rk1fe_16_s2", "c0_rk22_16_s3", "c1_rk33_16_s3"]
+FULL = ["c0_16_s1", "c0_16_s10", "c1_16_s1", "c1_16_s10", "c0_ragged_s3", "c1_ragged_s3"] + WEIGHTS + SSP + TABLEAUX
 
 
 @pytest.mark.parametrize("name", FULL)
