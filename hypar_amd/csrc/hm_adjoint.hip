@@ -32,8 +32,36 @@
 namespace hm {
 
 constexpr int VJP_LINE_MAX = 512;  // longest grid line an adjoint workgroup holds
+constexpr int VJP_LDS = 23;        // doubles of LDS per cell of the component-wise line VJP
 
 __device__ __forceinline__ int wrapn(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+// Lines per workgroup of the line VJPs: a block of `threads` threads (a
+// multiple of 64, at least n) takes threads / n consecutive lines of length
+// n, so that lines shorter than the block do not leave wavefronts idle (two
+// 64-cell lines per 128-thread block).
+__host__ __device__ __forceinline__ int vjp_lines_per_block(int threads, int n) { return threads / n; }
+
+// the line and interface of this thread for work item `item` (a group of
+// vjp_lines_per_block lines) of a direction with nlines lines of length n;
+// sub is the thread's line within the group, whose share of the dynamic LDS
+// starts sub * (doubles per cell) * n in.  live is false for threads past the
+// group's last line.
+struct LineSlot {
+  size_t line;
+  int t, sub;
+  bool live;
+};
+__device__ __forceinline__ LineSlot line_slot(size_t item, int n, size_t nlines) {
+  const int lpb = vjp_lines_per_block((int)blockDim.x, n), sub = (int)threadIdx.x / n;
+  LineSlot s;
+  s.line = item * lpb + sub;
+  s.live = sub < lpb && s.line < nlines;
+  s.t = (int)threadIdx.x - sub * n;
+  s.sub = s.live ? sub : 0;
+  if (!s.live) s.line = 0;
+  return s;
+}
 
 // 1/d, d > 0 and normal: hardware estimate and two Newton-Raphson steps
 __device__ __forceinline__ double vjp_rcp(double d) {
@@ -456,7 +484,7 @@ __device__ __forceinline__ void rhs_vjp_line(const double *__restrict__ u, const
                                              const double *__restrict__ nu_in, double ca, double cb,
                                              double *__restrict__ nu, double *__restrict__ L, int mode, int nx,
                                              int ny, int nz, const double *__restrict__ dxg, double gamma,
-                                             double eps, size_t line, double *smem) {
+                                             double eps, size_t line, int t, bool active, double *smem) {
   const size_t ncell = (size_t)nx * ny * nz;
   const int n = DIR == 0 ? nx : (DIR == 1 ? ny : nz);
   size_t base, stride;
@@ -469,8 +497,6 @@ __device__ __forceinline__ void rhs_vjp_line(const double *__restrict__ u, const
   double *sC4 = sFl + 5 * n;    // [n]
   double *cF = sC4 + n;         // [6][n]
   double *cU = cF + 6 * n;      // [6][n]
-  const int t = threadIdx.x;
-  const bool active = t < n;
   const double inv_gm1 = 1.0 / (gamma - 1.0);
   const int tm = active ? wrapn(t - 1, n) : 0;
   double g[NV] = {0, 0, 0, 0, 0};
@@ -713,7 +739,7 @@ template <int DIR, int UPW, int W>
 __device__ __forceinline__ void rhs_vjp_char_line(
     const double *__restrict__ u, const double *__restrict__ lam, const double *__restrict__ nu_in, double ca,
     double cb, double *__restrict__ nu, double *__restrict__ L, int mode, int nx, int ny, int nz,
-    const double *__restrict__ dxg, double gamma, double eps, size_t line, double *smem) {
+    const double *__restrict__ dxg, double gamma, double eps, size_t line, int t, bool active, double *smem) {
   const size_t ncell = (size_t)nx * ny * nz;
   const int n = DIR == 0 ? nx : (DIR == 1 ? ny : nz);
   size_t base, stride;
@@ -724,8 +750,6 @@ __device__ __forceinline__ void rhs_vjp_char_line(
   constexpr int CS = VJP_CHAR_CS, ES = VJP_CHAR_ES;
   double *sC = smem;        // [n][CS]: state [5], flux [5]
   double *E = sC + CS * n;  // [n][ES]: six stencil adjoints [6], row of L [5]
-  const int t = threadIdx.x;
-  const bool active = t < n;
   const double gm1 = gamma - 1.0;
   double lF[NV] = {0, 0, 0, 0, 0};  // adjoint of the interface flux
   if (active) {
@@ -1273,7 +1297,10 @@ __global__ void __launch_bounds__(VJP_LINE_MAX) k_rhs_vjp(const double *__restri
                                                       int nx, int ny, int nz, const double *__restrict__ dxg,
                                                       double gamma, double eps) {
   extern __shared__ double smem[];
-  rhs_vjp_line<DIR, UPW, W>(u, lam, nu_in, ca, cb, nu, L, mode, nx, ny, nz, dxg, gamma, eps, xcd_block(), smem);
+  const int n = DIR == 0 ? nx : (DIR == 1 ? ny : nz);
+  const LineSlot s = line_slot(xcd_block(), n, (size_t)nx * ny * nz / n);
+  rhs_vjp_line<DIR, UPW, W>(u, lam, nu_in, ca, cb, nu, L, mode, nx, ny, nz, dxg, gamma, eps, s.line, s.t, s.live,
+                            smem + (size_t)s.sub * VJP_LDS * n);
 }
 
 template <int DIR, int UPW, int W>
@@ -1282,7 +1309,10 @@ __global__ void __launch_bounds__(VJP_CHAR_LINE_MAX) __attribute__((amdgpu_waves
     double cb, double *__restrict__ nu, double *__restrict__ L, int mode, int nx, int ny, int nz,
     const double *__restrict__ dxg, double gamma, double eps) {
   extern __shared__ double smem[];
-  rhs_vjp_char_line<DIR, UPW, W>(u, lam, nu_in, ca, cb, nu, L, mode, nx, ny, nz, dxg, gamma, eps, xcd_block(), smem);
+  const int n = DIR == 0 ? nx : (DIR == 1 ? ny : nz);
+  const LineSlot s = line_slot(xcd_block(), n, (size_t)nx * ny * nz / n);
+  rhs_vjp_char_line<DIR, UPW, W>(u, lam, nu_in, ca, cb, nu, L, mode, nx, ny, nz, dxg, gamma, eps, s.line, s.t,
+                                 s.live, smem + (size_t)s.sub * VJP_CHAR_LDS * n);
 }
 
 __global__ void __launch_bounds__(VT) k_visc_vjp_tile(const double *__restrict__ Q, const double *__restrict__ lam,
@@ -1357,7 +1387,8 @@ __global__ void __launch_bounds__(VJP_LINE_MAX) k_ens_rhs_vjp(EnsVjp a, const un
   for (unsigned item = r.first; item < r.end; item += r.step) {
     const int m = ens_find(first, a.nmem, item);
     const EnsMember M = a.mem[m];
-    rhs_vjp_line<DIR, UPW, W>(ENS_MEMBER_ARGS(a, M), item - first[m], smem);
+    const LineSlot s = line_slot(item - first[m], M.n[DIR], M.ncell / M.n[DIR]);
+    rhs_vjp_line<DIR, UPW, W>(ENS_MEMBER_ARGS(a, M), s.line, s.t, s.live, smem + (size_t)s.sub * VJP_LDS * M.n[DIR]);
     __syncthreads();
   }
 }
@@ -1370,7 +1401,9 @@ __global__ void __launch_bounds__(VJP_CHAR_LINE_MAX) __attribute__((amdgpu_waves
   for (unsigned item = r.first; item < r.end; item += r.step) {
     const int m = ens_find(first, a.nmem, item);
     const EnsMember M = a.mem[m];
-    rhs_vjp_char_line<DIR, UPW, W>(ENS_MEMBER_ARGS(a, M), item - first[m], smem);
+    const LineSlot s = line_slot(item - first[m], M.n[DIR], M.ncell / M.n[DIR]);
+    rhs_vjp_char_line<DIR, UPW, W>(ENS_MEMBER_ARGS(a, M), s.line, s.t, s.live,
+                                   smem + (size_t)s.sub * VJP_CHAR_LDS * M.n[DIR]);
     __syncthreads();
   }
 }
@@ -1472,17 +1505,18 @@ static void with_scheme(const Scheme &sc, F &&f) {
 template <int DIR>
 static void launch_vjp_dir(const Grid &g, const Scheme &sc, const double *u, const double *lam, const double *nu_in,
                            double ca, double cb, double *nu, double *L, int mode, const double *dxg, hipStream_t st) {
-  const unsigned lines = (unsigned)(g.ncell / g.n[DIR]);
   const unsigned threads = (unsigned)((g.n[DIR] + 63) / 64 * 64);
+  const unsigned lpb = (unsigned)vjp_lines_per_block((int)threads, g.n[DIR]);
+  const unsigned lines = (unsigned)((g.ncell / g.n[DIR] + lpb - 1) / lpb);  // work items
   with_scheme(sc, [&](auto tag) {
     constexpr int U = decltype(tag)::upw, W = decltype(tag)::weno;
     if (sc.interp == INTERP_COMP) {
-      const size_t lds = (size_t)23 * g.n[DIR] * sizeof(double);
+      const size_t lds = (size_t)VJP_LDS * lpb * g.n[DIR] * sizeof(double);
       allow_lds((const void *)k_rhs_vjp<DIR, U, W>, lds);
       hipLaunchKernelGGL((k_rhs_vjp<DIR, U, W>), dim3(lines), dim3(threads), lds, st, u, lam, nu_in, ca, cb, nu, L,
                          mode, g.n[0], g.n[1], g.n[2], dxg, sc.gamma, sc.eps);
     } else {
-      const size_t lds = (size_t)VJP_CHAR_LDS * g.n[DIR] * sizeof(double);
+      const size_t lds = (size_t)VJP_CHAR_LDS * lpb * g.n[DIR] * sizeof(double);
       allow_lds((const void *)k_rhs_vjp_char<DIR, U, W>, lds);
       hipLaunchKernelGGL((k_rhs_vjp_char<DIR, U, W>), dim3(lines), dim3(threads), lds, st, u, lam, nu_in, ca, cb,
                          nu, L, mode, g.n[0], g.n[1], g.n[2], dxg, sc.gamma, sc.eps);
@@ -1530,21 +1564,25 @@ bool ens_vjp_supported(const Scheme &sc, const int nmax[3]) {
 
 size_t ens_vjp_work_size(const Scheme &sc, size_t ncell) { return sc.Re > 0 ? 20 * ncell : 0; }
 
+int ens_vjp_threads(int nmax) { return (nmax + 63) / 64 * 64; }
+int ens_vjp_lines_per_item(int nmax, int n) { return vjp_lines_per_block(ens_vjp_threads(nmax), n); }
+
 template <int DIR>
 static void launch_ens_vjp_dir(const Scheme &sc, const EnsVjpPlan &p, const EnsVjp &a, hipStream_t st) {
-  const unsigned items = p.nlines[DIR];
-  const unsigned grid = items < 4096 ? items : 4096;  // persistent blocks stride over the lines
-  const unsigned threads = (unsigned)((p.nmax[DIR] + 63) / 64 * 64);
+  // work items: groups of vjp_lines_per_block(threads, n) lines of a member
+  const unsigned items = p.nitems[DIR];
+  const unsigned grid = items < 4096 ? items : 4096;  // persistent blocks stride over the items
+  const unsigned threads = (unsigned)ens_vjp_threads(p.nmax[DIR]);
   with_scheme(sc, [&](auto tag) {
     constexpr int U = decltype(tag)::upw, W = decltype(tag)::weno;
     if (sc.interp == INTERP_COMP) {
-      const size_t lds = (size_t)23 * p.nmax[DIR] * sizeof(double);
+      const size_t lds = (size_t)VJP_LDS * threads * sizeof(double);
       allow_lds((const void *)k_ens_rhs_vjp<DIR, U, W>, lds);
-      hipLaunchKernelGGL((k_ens_rhs_vjp<DIR, U, W>), dim3(grid), dim3(threads), lds, st, a, p.lines0[DIR], items);
+      hipLaunchKernelGGL((k_ens_rhs_vjp<DIR, U, W>), dim3(grid), dim3(threads), lds, st, a, p.items0[DIR], items);
     } else {
-      const size_t lds = (size_t)VJP_CHAR_LDS * p.nmax[DIR] * sizeof(double);
+      const size_t lds = (size_t)VJP_CHAR_LDS * threads * sizeof(double);
       allow_lds((const void *)k_ens_rhs_vjp_char<DIR, U, W>, lds);
-      hipLaunchKernelGGL((k_ens_rhs_vjp_char<DIR, U, W>), dim3(grid), dim3(threads), lds, st, a, p.lines0[DIR],
+      hipLaunchKernelGGL((k_ens_rhs_vjp_char<DIR, U, W>), dim3(grid), dim3(threads), lds, st, a, p.items0[DIR],
                          items);
     }
   });

@@ -1021,6 +1021,9 @@ struct hm_ensemble {
   // lines, and the tape (checkpoints, stage states of one segment, scratch)
   unsigned *lines0[3] = {nullptr, nullptr, nullptr};
   unsigned nlines[3] = {0, 0, 0};
+  // work items of the line VJPs (groups of lines, hm::ens_vjp_lines_per_item)
+  unsigned *vitems0[3] = {nullptr, nullptr, nullptr};
+  unsigned nvitems[3] = {0, 0, 0};
   int nmax[3] = {0, 0, 0};
   std::vector<double *> ckpt, stages, vscratch, vwork;
   int tape_nsteps = 0, tape_every = 0;
@@ -1147,7 +1150,8 @@ void hm_ensemble_destroy(hm_ensemble *e) {
     if (e->xitem0[sh]) (void)hipFree(e->xitem0[sh]);
   }
   for (int d = 0; d < 3; d++)
-    if (e->lines0[d]) (void)hipFree(e->lines0[d]);
+    for (unsigned *p : {e->lines0[d], e->vitems0[d]})
+      if (p) (void)hipFree(p);
   for (auto *v : {&e->ckpt, &e->stages, &e->vscratch, &e->vwork, &e->sscratch})
     for (double *b : *v) (void)hipFree(b);
   delete e;
@@ -1295,8 +1299,16 @@ hm_status hm_ensemble_create(const hm_config *members, int nmembers, hm_ensemble
       e->nmax[d] = std::max(e->nmax[d], e->mem[m].n[d]);
     }
     e->nlines[d] = pre[nmembers];
+    std::vector<unsigned> vpre(nmembers + 1, 0);
+    for (int m = 0; m < nmembers; m++) {
+      const unsigned lpi = (unsigned)hm::ens_vjp_lines_per_item(e->nmax[d], e->mem[m].n[d]);
+      vpre[m + 1] = vpre[m] + (pre[m + 1] - pre[m] + lpi - 1) / lpi;
+    }
+    e->nvitems[d] = vpre[nmembers];
     if (hipMalloc((void **)&e->lines0[d], pre.size() * sizeof(unsigned)) != hipSuccess ||
-        hipMemcpy(e->lines0[d], pre.data(), pre.size() * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess) {
+        hipMemcpy(e->lines0[d], pre.data(), pre.size() * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMalloc((void **)&e->vitems0[d], vpre.size() * sizeof(unsigned)) != hipSuccess ||
+        hipMemcpy(e->vitems0[d], vpre.data(), vpre.size() * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess) {
       st = fail(HM_ERR_NOMEM, "hm_ensemble_create: line lists");
       goto bad;
     }
@@ -1571,6 +1583,8 @@ static hm_status ens_reverse_sweep(hm_ensemble *e, double *lam) {
   for (int d = 0; d < 3; d++) {
     vp.lines0[d] = e->lines0[d];
     vp.nlines[d] = e->nlines[d];
+    vp.items0[d] = e->vitems0[d];
+    vp.nitems[d] = e->nvitems[d];
     vp.nmax[d] = e->nmax[d];
   }
   vp.tiles0 = e->tiles0;

@@ -210,18 +210,24 @@ void launch_axpby(const double *x, double a, double *y, double b, size_t n, hipS
 
 // the same for all members of an ensemble at once: member m's stage
 // coefficients are dt_m*A and dt_m*B; lines0[d] lists each member's first
-// line along d (nmem + 1 entries), nmax[d] the longest line along d
+// line along d (nmem + 1 entries), nmax[d] the longest line along d, and
+// items0[d] each member's first work item of the line VJPs along d, an item
+// being ens_vjp_lines_per_item(nmax[d], n[d]) consecutive lines of the member
 struct EnsVjpPlan {
   const EnsMember *mem;  // device member table
   int nmem;
   size_t ncell;
   const unsigned *lines0[3];
   unsigned nlines[3];
+  const unsigned *items0[3];
+  unsigned nitems[3];
   int nmax[3];
   const unsigned *tiles0;  // viscous members: each member's first viscous tile (nmem + 1 entries)
   unsigned ntiles;
 };
 bool ens_vjp_supported(const Scheme &sc, const int nmax[3]);
+int ens_vjp_threads(int nmax);                 // block size of the line VJPs along a direction
+int ens_vjp_lines_per_item(int nmax, int n);  // lines of length n per work item
 size_t ens_vjp_work_size(const Scheme &sc, size_t ncell);
 void launch_ens_rhs_vjp(const Scheme &sc, const EnsVjpPlan &p, const double *u, const double *lam,
                         const double *nu_in, double A, double B, double *nu, double *L, bool set_L, double *work,
