@@ -986,7 +986,8 @@ __device__ __forceinline__ void rhs_vjp_char_line(
 //  k_visc_vjp_ghost  the same for the ghost-layer fluxes of each line, whose
 //                    gradient adjoints are spread over their stencils into QB;
 //  k_visc_vjp_gather QB plus the transposed gradient stencils over GB, then
-//                    back through the primitive variables onto u.
+//                    back through the primitive variables, added onto the
+//                    nu that the y and z line VJPs left.
 struct ViscC {
   double gamma, inv_Re, inv_gm1, inv_Pr;
 };
@@ -1242,7 +1243,7 @@ __device__ __forceinline__ void visc_vjp_ghost_line(const double *__restrict__ Q
     for (int c = 0; c < 4; c++) QB[(size_t)c * ncell + cw[j]] += acc[j][c];
 }
 
-// nu = (d par/d u)^T mu: QB plus the transposed gradient stencils over GB,
+// nu += (d par/d u)^T mu: QB plus the transposed gradient stencils over GB,
 // then back through (u, v, w, T)(u) (NavierStokes3DParabolicFunction.c:85-108)
 __device__ __forceinline__ void visc_vjp_gather(const double *__restrict__ U, const double *__restrict__ QB,
                                                 const double *__restrict__ GB, double *__restrict__ nu, int nx,
@@ -1282,8 +1283,11 @@ __device__ __forceinline__ void visc_vjp_gather(const double *__restrict__ U, co
   ub[2] = qb[1] * ir - k * vy * ir;
   ub[3] = qb[2] * ir - k * vz * ir;
   ub[4] = k * ir;
+  double r[NV];
 #pragma unroll
-  for (int v = 0; v < NV; v++) nu[(size_t)v * ncell + idx] = ub[v];
+  for (int v = 0; v < NV; v++) r[v] = nu[(size_t)v * ncell + idx];
+#pragma unroll
+  for (int v = 0; v < NV; v++) nu[(size_t)v * ncell + idx] = r[v] + ub[v];
 }
 
 // ---------------------------------------------------------------------------
@@ -1528,6 +1532,12 @@ void launch_rhs_vjp(const Grid &g, const Scheme &sc, const double *u, const doub
                     double ca, double cb, double *nu, double *L, bool set_L, const double *dxg, double *work,
                     hipStream_t st) {
   const bool visc = sc.Re > 0;
+  // z first and x last: the strided z lines take the plain store; the y
+  // lines, the viscous gather and the contiguous x lines add into nu (x also
+  // into L)
+  const int last = L ? (set_L ? VJP_ADD_LSET : VJP_ADD_L) : VJP_ADD;
+  launch_vjp_dir<2>(g, sc, u, lam, nu_in, ca, cb, nu, L, VJP_WRITE, dxg, st);
+  launch_vjp_dir<1>(g, sc, u, lam, nu_in, ca, cb, nu, L, VJP_ADD, dxg, st);
   if (visc) {
     // work: Q [4], QB [4], GB [12] (times ncell)
     double *Q = work, *QB = work + 4 * g.ncell, *GB = work + 8 * g.ncell;
@@ -1547,11 +1557,6 @@ void launch_rhs_vjp(const Grid &g, const Scheme &sc, const double *u, const doub
     hipLaunchKernelGGL(k_visc_vjp_gather, dim3(nb), dim3(256), 0, st, u, QB, GB, nu, g.n[0], g.n[1], g.n[2],
                        sc.gamma);
   }
-  // z first and x last: the strided z lines take the plain store, the
-  // contiguous x lines the read-modify-write of nu and L
-  const int last = L ? (set_L ? VJP_ADD_LSET : VJP_ADD_L) : VJP_ADD;
-  launch_vjp_dir<2>(g, sc, u, lam, nu_in, ca, cb, nu, L, visc ? VJP_ADD : VJP_WRITE, dxg, st);
-  launch_vjp_dir<1>(g, sc, u, lam, nu_in, ca, cb, nu, L, VJP_ADD, dxg, st);
   launch_vjp_dir<0>(g, sc, u, lam, nu_in, ca, cb, nu, L, last, dxg, st);
 }
 
@@ -1606,6 +1611,11 @@ void launch_ens_rhs_vjp(const Scheme &sc, const EnsVjpPlan &p, const double *u, 
   a.eps = sc.eps;
   a.vc = {sc.gamma, sc.Re > 0 ? 1.0 / sc.Re : 0.0, 1.0 / (sc.gamma - 1.0), 1.0 / sc.Pr};
   const bool visc = sc.Re > 0;
+  // the order of the single grid: z (plain store), y, the viscous terms, x
+  a.mode = VJP_WRITE;
+  launch_ens_vjp_dir<2>(sc, p, a, st);
+  a.mode = VJP_ADD;
+  launch_ens_vjp_dir<1>(sc, p, a, st);
   if (visc) {
     const unsigned nb = (unsigned)((p.ncell + 255) / 256);
     hipLaunchKernelGGL(k_ens_vjp_prims, dim3(nb), dim3(256), 0, st, a, p.ncell);
@@ -1618,10 +1628,6 @@ void launch_ens_rhs_vjp(const Scheme &sc, const EnsVjpPlan &p, const double *u, 
                        p.nlines[2]);
     hipLaunchKernelGGL(k_ens_visc_vjp_gather, dim3(nb), dim3(256), 0, st, a, p.ncell);
   }
-  a.mode = visc ? VJP_ADD : VJP_WRITE;  // z first, x last (as for one grid)
-  launch_ens_vjp_dir<2>(sc, p, a, st);
-  a.mode = VJP_ADD;
-  launch_ens_vjp_dir<1>(sc, p, a, st);
   a.mode = L ? (set_L ? VJP_ADD_LSET : VJP_ADD_L) : VJP_ADD;
   launch_ens_vjp_dir<0>(sc, p, a, st);
 }
