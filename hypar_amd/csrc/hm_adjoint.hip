@@ -1435,14 +1435,14 @@ __global__ void __launch_bounds__(256) k_ens_vjp_prims(EnsVjp a, size_t ncell) {
 __global__ void __launch_bounds__(VT) k_ens_visc_vjp_tile(EnsVjp a, const unsigned *__restrict__ tiles0,
                                                          unsigned ntiles) {
   __shared__ double sQ[4][VHY][VHX], sM[4][VHY][VHX];
-  for (unsigned item = blockIdx.x; item < ntiles; item += gridDim.x) {
-    const int m = ens_find(tiles0, a.nmem, item);
-    const EnsMember M = a.mem[m];
-    const double *Q = a.work + 20 * M.off;
-    double *QB = a.work + 20 * M.off + 4 * M.ncell, *GB = a.work + 20 * M.off + 8 * M.ncell;
-    visc_vjp_tile(Q, a.lam + 5 * M.off, a.nu_in ? a.nu_in + 5 * M.off : nullptr, M.dt * a.A, M.dt * a.B, QB, GB,
-                  M.n[0], M.n[1], M.n[2], M.dxg, a.vc, (int)(item - tiles0[m]), sQ, sM);
-  }
+  const unsigned item = blockIdx.x;  // one block per tile (a loop over tiles spills 12 B per lane)
+  if (item >= ntiles) return;
+  const int m = ens_find(tiles0, a.nmem, item);
+  const EnsMember M = a.mem[m];
+  const double *Q = a.work + 20 * M.off;
+  double *QB = a.work + 20 * M.off + 4 * M.ncell, *GB = a.work + 20 * M.off + 8 * M.ncell;
+  visc_vjp_tile(Q, a.lam + 5 * M.off, a.nu_in ? a.nu_in + 5 * M.off : nullptr, M.dt * a.A, M.dt * a.B, QB, GB,
+                M.n[0], M.n[1], M.n[2], M.dxg, a.vc, (int)(item - tiles0[m]), sQ, sM);
 }
 
 template <int DIR>

@@ -402,6 +402,23 @@ __device__ __forceinline__ void iface(const LD &ld, const PassConst &k, double f
   }
 }
 
+// The hyperbolic-term accumulator R, read and written plainly by the
+// per-pass kernels.  In the one-launch-per-stage kernel (COH) blocks hand R
+// to each other within the launch, so there R goes through agent-scope
+// relaxed atomics: coherent loads and write-through stores (sc1) that
+// bypass the stale lines of the eight non-coherent XCD L2s.  The stage
+// kernel's dependency counters then need no L2 write-back or invalidate.
+template <bool COH>
+__device__ __forceinline__ double load_r(const double *p) {
+  if (COH) return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return *p;
+}
+template <bool COH>
+__device__ __forceinline__ void store_r(double *p, double v) {
+  if (COH) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else *p = v;
+}
+
 // Inputs of cell idx's output that come from global memory: its dxinv, the
 // hyperbolic term so far (y, z), the viscous term and the RK4 operands (z).
 // The marching passes issue these loads in one block with the next ring
@@ -410,7 +427,7 @@ struct EmitIn {
   double dxi, par[NV], rin[NV], a[NV], b[NV];
 };
 
-template <int DIR, int RK = RK_CLASSIC>
+template <int DIR, int RK = RK_CLASSIC, bool COH = false>
 __device__ __forceinline__ EmitIn emit_load(const PassConst &k, const PassIO &io, size_t idx, int pos,
                                             const SspIO &sp = SspIO{}) {
   EmitIn in;
@@ -419,7 +436,7 @@ __device__ __forceinline__ EmitIn emit_load(const PassConst &k, const PassIO &io
   for (int v = 0; v < NV; v++) in.par[v] = in.rin[v] = in.a[v] = in.b[v] = 0.0;
   if (DIR > 0)
 #pragma unroll
-    for (int v = 0; v < NV; v++) in.rin[v] = io.R[(size_t)v * k.ncell + idx];
+    for (int v = 0; v < NV; v++) in.rin[v] = load_r<COH>(io.R + (size_t)v * k.ncell + idx);
   if (DIR == 2 && io.visc && io.mode != OUT_HYP)
 #pragma unroll
     for (int c = 0; c < 4; c++) in.par[c + 1] = io.par[(size_t)c * k.ncell + idx];
@@ -451,7 +468,7 @@ __device__ __forceinline__ EmitIn emit_load(const PassConst &k, const PassIO &io
 }
 
 // Finish cell idx of the pass: flux difference and the pass's output.
-template <int DIR, int RK = RK_CLASSIC>
+template <int DIR, int RK = RK_CLASSIC, bool COH = false>
 __device__ __forceinline__ void emit_store(const PassConst &k, const PassIO &io, size_t idx, const EmitIn &in,
                                            const double (&fm)[NV], const double (&fp)[NV],
                                            const SspIO &sp = SspIO{}) {
@@ -464,10 +481,14 @@ __device__ __forceinline__ void emit_store(const PassConst &k, const PassIO &io,
     else if (DIR == 1 || io.mode == OUT_HYP) r[v] = in.rin[v] + h;
     else r[v] = (-(in.rin[v] + h)) + in.par[v];
   }
-  if (DIR < 2 || io.mode != OUT_STAGE) {
-    double *dst = DIR < 2 ? io.R : io.out;
+  if (DIR < 2) {
 #pragma unroll
-    for (int v = 0; v < NV; v++) dst[(size_t)v * k.ncell + idx] = r[v];
+    for (int v = 0; v < NV; v++) store_r<COH>(io.R + (size_t)v * k.ncell + idx, r[v]);
+    return;
+  }
+  if (io.mode != OUT_STAGE) {
+#pragma unroll
+    for (int v = 0; v < NV; v++) io.out[(size_t)v * k.ncell + idx] = r[v];
     return;
   }
   double nu[NV];
@@ -510,11 +531,11 @@ __device__ __forceinline__ void emit_store(const PassConst &k, const PassIO &io,
   }
 }
 
-template <int DIR, int RK = RK_CLASSIC>
+template <int DIR, int RK = RK_CLASSIC, bool COH = false>
 __device__ __forceinline__ void emit(const PassConst &k, const PassIO &io, size_t idx, int pos,
                                      const double (&fm)[NV], const double (&fp)[NV],
                                      const SspIO &sp = SspIO{}) {
-  emit_store<DIR, RK>(k, io, idx, emit_load<DIR, RK>(k, io, idx, pos, sp), fm, fp, sp);
+  emit_store<DIR, RK, COH>(k, io, idx, emit_load<DIR, RK, COH>(k, io, idx, pos, sp), fm, fp, sp);
 }
 
 // ---------------------------------------------------------------------------
@@ -532,7 +553,7 @@ struct XShared {
 };
 
 // lines LPB*group .. LPB*group + LPB-1 of the grid (line = j + ny*k)
-template <int INTERP, int UPW, int LPB, int W>
+template <int INTERP, int UPW, int LPB, int W, bool COH = false>
 __device__ __forceinline__ void pass_x_body(const PassConst &k, const PassIO &io, size_t group,
                                             XShared<LPB> &sh) {
   const int lane = threadIdx.x & 63;
@@ -578,7 +599,7 @@ __device__ __forceinline__ void pass_x_body(const PassConst &k, const PassIO &io
         fm[v] = sh.sF[w][v][lane];
         fp[v] = sh.sF[w][v][lane + 1];
       }
-      emit<0>(k, io, base + cell, cell, fm, fp);
+      emit<0, RK_CLASSIC, COH>(k, io, base + cell, cell, fm, fp);
     }
     __syncthreads();
   }
@@ -591,7 +612,7 @@ __device__ __forceinline__ void pass_x_body(const PassConst &k, const PassIO &io
       fm[v] = sh.sF[w][v][nx - x0];
       fp[v] = sh.sF0[w][v];
     }
-    emit<0>(k, io, base + nx - 1, nx - 1, fm, fp);
+    emit<0, RK_CLASSIC, COH>(k, io, base + nx - 1, nx - 1, fm, fp);
   }
 }
 
@@ -610,7 +631,7 @@ struct MShared {
 };
 
 // columns MW*group .. MW*group + MW-1 of the plane across DIR
-template <int DIR, int INTERP, int UPW, int MR, int W, int RK = RK_CLASSIC>
+template <int DIR, int INTERP, int UPW, int MR, int W, int RK = RK_CLASSIC, bool COH = false>
 __device__ __forceinline__ void pass_m_body(const PassConst &k, const PassIO &io, size_t group,
                                             MShared<MR> &sh, const SspIO &sp = SspIO{}) {
   constexpr int NRING = MShared<MR>::NRING, NFR = MShared<MR>::NFR;
@@ -687,7 +708,7 @@ __device__ __forceinline__ void pass_m_body(const PassConst &k, const PassIO &io
     double un[NV];
     fetch_row(rn, un);
     const int ec = e < 0 ? 0 : (e > n - 2 ? n - 2 : e);
-    const EmitIn in = emit_load<DIR, RK>(k, io, cbase + (size_t)ec * stride, ec, sp);
+    const EmitIn in = emit_load<DIR, RK, COH>(k, io, cbase + (size_t)ec * stride, ec, sp);
     if (more) stage_row(rn, un);
     if (emits) {
       double fm[NV], fp[NV];
@@ -697,7 +718,7 @@ __device__ __forceinline__ void pass_m_body(const PassConst &k, const PassIO &io
         fm[v] = sh.sF[v][a][lane];
         fp[v] = sh.sF[v][b][lane];
       }
-      emit_store<DIR, RK>(k, io, cbase + (size_t)e * stride, in, fm, fp, sp);
+      emit_store<DIR, RK, COH>(k, io, cbase + (size_t)e * stride, in, fm, fp, sp);
     }
   }
   // last row: fI[n-1] and fI[n] = fI[0]
@@ -709,7 +730,7 @@ __device__ __forceinline__ void pass_m_body(const PassConst &k, const PassIO &io
       fm[v] = sh.sF[v][a][lane];
       fp[v] = sh.sF0[v][lane];
     }
-    emit<DIR, RK>(k, io, cbase + (size_t)(n - 1) * stride, n - 1, fm, fp, sp);
+    emit<DIR, RK, COH>(k, io, cbase + (size_t)(n - 1) * stride, n - 1, fm, fp, sp);
   }
 }
 
@@ -864,12 +885,17 @@ __device__ __forceinline__ bool lead_wave() { return __builtin_amdgcn_readfirstl
 
 __device__ __forceinline__ unsigned lane0_one() { return (threadIdx.x & 63u) == 0u ? 1u : 0u; }
 
-// The L2s of the XCDs are not coherent with each other: a release writes
-// back the L2 of the releasing XCD (buffer_wbl2), an acquire invalidates the
-// L1 of the CU and the L2 of the XCD (buffer_inv).  Both act for every wave
-// of the block (all of them run on one CU), so one wave issues them.
+// The only data blocks of the launch hand to each other is R, and every
+// access to it here is an agent-scope atomic (load_r / store_r with COH):
+// the stores write through the XCD's L2, the loads bypass stale lines of
+// the L1 and the L2.  A count of finished items therefore needs only each
+// wave's stores acknowledged before it, and a wait only the count seen
+// before the block's first load of R; neither writes back nor invalidates
+// an L2 (the fences a release or acquire at agent scope would add, once per
+// work item, for every XCD).
 //
-// Whole block: wait until *c >= need (bounded), then acquire.
+// Whole block: wait until *c >= need (bounded).  The barrier keeps the
+// block's loads after the poll that saw the count.
 __device__ __forceinline__ void stage_wait(const unsigned *c, unsigned need, unsigned *err, unsigned limit) {
   if (lead_wave()) {
     for (unsigned polls = 0;; polls++) {
@@ -882,17 +908,16 @@ __device__ __forceinline__ void stage_wait(const unsigned *c, unsigned need, uns
       }
       __builtin_amdgcn_s_sleep(2);
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
 }
 
-// Whole block: every wave waits for its own stores to reach the L2, then
-// one release (L2 write-back) and the count of the item.
+// Whole block: every wave waits until its write-through stores of R are
+// acknowledged, then one wave counts the item.
 __device__ __forceinline__ void stage_signal(unsigned *c) {
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
-  if (lead_wave()) __hip_atomic_fetch_add(c, lane0_one(), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  if (lead_wave()) __hip_atomic_fetch_add(c, lane0_one(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <int INTERP, int UPW, int SH, int W>
@@ -926,7 +951,7 @@ __global__ void __launch_bounds__(64 * SH) __attribute__((amdgpu_waves_per_eu(2)
       ens_setup(s.a, m, 0, k, io);
       const unsigned g1 = min((local + 1) * s.xg, s.xgroups0[m + 1] - s.xgroups0[m]);
       for (unsigned g = local * s.xg; g < g1; g++) {
-        pass_x_body<INTERP, UPW, SH, W>(k, io, g, sh.x);
+        pass_x_body<INTERP, UPW, SH, W, true>(k, io, g, sh.x);
         __syncthreads();
       }
       stage_signal(&done_x[m]);
@@ -934,13 +959,13 @@ __global__ void __launch_bounds__(64 * SH) __attribute__((amdgpu_waves_per_eu(2)
       PassConst k;
       PassIO io;
       ens_setup(s.a, m, 1, k, io);
-      pass_m_body<1, INTERP, UPW, SH, W>(k, io, local, sh.m);
+      pass_m_body<1, INTERP, UPW, SH, W, RK_CLASSIC, true>(k, io, local, sh.m);
       stage_signal(&done_y[m]);
     } else {
       PassConst k;
       PassIO io;
       ens_setup(s.a, m, 2, k, io);
-      pass_m_body<2, INTERP, UPW, SH, W>(k, io, local, sh.m);
+      pass_m_body<2, INTERP, UPW, SH, W, RK_CLASSIC, true>(k, io, local, sh.m);
       __syncthreads();
     }
   }
