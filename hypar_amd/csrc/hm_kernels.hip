@@ -407,8 +407,8 @@ __global__ void __launch_bounds__(PTX * PTY) k_visc_par(const double *__restrict
   visc_par_tile<PTX, PTY>(Q, par, nx, ny, nz, dxg, vp, blockIdx.x, pks, sh);
 }
 
-// Viscous terms of all members of an ensemble: one persistent launch over
-// the (member, tile) work items; ptiles0 holds each member's first item.
+// Viscous terms of all members of an ensemble: one launch with a block per
+// (member, tile) work item; ptiles0 holds each member's first item.
 struct EnsViscArgs {
   const EnsMember *mem;
   int nmem;
@@ -419,18 +419,16 @@ struct EnsViscArgs {
 
 __global__ void __launch_bounds__(PTX * PTY) k_ens_visc(EnsViscArgs a) {
   __shared__ ViscParShared<PTX, PTY> sh;
-  for (unsigned item = blockIdx.x; item < a.nitems; item += gridDim.x) {
-    int lo = 0, hi = a.nmem - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (a.ptiles0[mid] <= item) lo = mid;
-      else hi = mid - 1;
-    }
-    const EnsMember &M = a.mem[lo];
-    visc_par_tile<PTX, PTY>(M.q, M.par, M.n[0], M.n[1], M.n[2], M.dxg, a.vp, (int)(item - a.ptiles0[lo]), PKS,
-                            sh);
-    __syncthreads();
+  const unsigned item = blockIdx.x;  // one block per item (a loop over items spills 8 B per lane)
+  if (item >= a.nitems) return;
+  int lo = 0, hi = a.nmem - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (a.ptiles0[mid] <= item) lo = mid;
+    else hi = mid - 1;
   }
+  const EnsMember &M = a.mem[lo];
+  visc_par_tile<PTX, PTY>(M.q, M.par, M.n[0], M.n[1], M.n[2], M.dxg, a.vp, (int)(item - a.ptiles0[lo]), PKS, sh);
 }
 
 // primitive variables of every member's state
