@@ -324,4 +324,13 @@ def test_batched_tape_schedules_give_the_same_gradient():
         print(f"[gpu] batched tape, budget of {nstates} states -> {info}, equal {torch.equal(g, base)}", flush=True)
         assert (info["sets"], info["step_level"]) == (sets, step)
         assert torch.equal(g, base)
+    # the one-launch-per-stage kernel for the taped and the recomputed steps
+    # (one set: three of the four segments are recomputed)
+    e.set_tiles(2)
+    e.set_tape_budget(work_bytes + 24 * state_bytes)
+    u = u0.clone()
+    e.adjoint_forward(u, 13, 4)
+    g = e.adjoint_reverse(lam.clone())
+    print(f"[gpu] batched tape with the stage kernel: {e.tape_info()}, equal {torch.equal(g, base)}", flush=True)
+    assert torch.equal(g, base)
     e.close()
