@@ -248,7 +248,9 @@ def ensemble_adjoint(nsteps=30, every=10):
     characteristic WENO5-JS + Roe, RK4.  Per-member gradient of the final
     kinetic energy with respect to the member's initial velocity field,
     through nsteps forward + adjoint steps with a checkpoint every `every`
-    steps, all members advanced by the batched launches.  Ratio = (batched
+    steps, all members advanced by the batched launches with the tiles the
+    autotuner picks (reported under "tiles": the stage kernel is one launch
+    per RK stage, after the viscous-term launch).  Ratio = (batched
     forward with checkpoints + reverse sweep + the chain rule onto the
     velocity) / (batched plain forward of the same steps)."""
     import torch
@@ -280,6 +282,7 @@ def ensemble_adjoint(nsteps=30, every=10):
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
     with torch.no_grad():
         u0 = state(vels).contiguous()
+    shape, grid = e.tune(u0, reps=2)  # the forward, taped and recomputed steps all use the tuned tiles
     u = u0.clone()
     e.rk4(u, 1)  # capture + warm
     u = u0.clone()
@@ -299,7 +302,9 @@ def ensemble_adjoint(nsteps=30, every=10):
     out = {"workload": f"{len(ws)} viscous NS3D members (Re 1000, Mach 0.1), sides from {{64,96,128}}, own dt each "
                        f"(CFL 0.5), characteristic WENO5-JS + Roe, RK4, {nsteps} steps, checkpoint every {every}, "
                        "per-member dJ/d(initial velocity) of the final kinetic energy",
-           "members": len(ws), "cells": e.ncell, "forward_ms": fwd, "adjoint_ms": adj, "ratio": adj / fwd,
+           "members": len(ws), "cells": e.ncell,
+           "tiles": {"shape": shape, "grid": list(grid), "launches_per_stage": (1 if shape >= 2 else 3) + 1},
+           "forward_ms": fwd, "adjoint_ms": adj, "ratio": adj / fwd,
            "forward_states_agree": bool(torch.equal(u, uN.detach())),
            "gradients_finite": bool(all(torch.isfinite(g).all() for g in grads))}
     e.close()
