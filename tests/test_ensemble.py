@@ -84,17 +84,22 @@ def test_reference_ensemble_fixture():
         assert err <= TOL
 
 
-@pytest.mark.parametrize("interp,upwind,Re", [("components", "rusanov", -1.0), ("characteristic", "roe", -1.0),
-                                              ("characteristic", "roe", 1000.0)])
-def test_one_launch_per_stage_matches_pass_launches(interp, upwind, Re):
+@pytest.mark.parametrize("interp,upwind,Re,weno", [("components", "rusanov", -1.0, "js"),
+                                                   ("characteristic", "roe", -1.0, "js"),
+                                                   ("characteristic", "roe", 1000.0, "js"),
+                                                   ("characteristic", "roe", 1000.0, "mapped"),
+                                                   ("components", "rusanov", -1.0, "borges"),
+                                                   ("characteristic", "rusanov", -1.0, "yc"),
+                                                   ("components", "roe", -1.0, "optimal")])
+def test_one_launch_per_stage_matches_pass_launches(interp, upwind, Re, weno):
     """The stage kernel (shapes 2, 3: the x, y and z items of all members in
     one launch, ordered by dependency counters) gives the same bits as the
     launch-per-pass kernels: each cell's arithmetic is the same, so any
     missed dependency would show as a difference."""
     from hypar_amd.ensemble import Ensemble
-    ws, u0s, dts = _members(DIMS, interp, upwind, Re)
-    print(f"[gpu] one launch per stage vs per pass: {interp}/{upwind} Re={Re}", flush=True)
-    e = Ensemble(DIMS, interp, upwind, Re=Re, Minf=0.1 if Re > 0 else 1.0)
+    ws, u0s, dts = _members(DIMS, interp, upwind, Re, weno)
+    print(f"[gpu] one launch per stage vs per pass: {interp}/{upwind} Re={Re} weights={weno}", flush=True)
+    e = Ensemble(DIMS, interp, upwind, Re=Re, Minf=0.1 if Re > 0 else 1.0, weno_type=weno)
     u0 = e.to_device(u0s)
     e.set_dt(dts)
     out = {}
