@@ -235,8 +235,9 @@ hm_status hm_tape_info(const hm_solver *s, int *sets, int *step_level, size_t *b
  * duration in milliseconds, measured with HIP events on that stream:
  * ms[0..2] the fused x, y, z passes of an RK stage (the z pass in its
  * stage-1 form, which reads u and the accumulator and writes the next stage
- * state); ms[3] viscous fluxes and ms[4] primitive variables (0 if
- * inviscid).  u is not modified. */
+ * state); ms[3] the viscous-term kernel and ms[4] the primitive-variable
+ * kernel that hm_rk4 runs once per call (0 if inviscid).  u is not
+ * modified. */
 hm_status hm_profile_kernels(hm_solver *s, const double *u, int reps, double *ms);
 
 /* ------------------------------------------------------------------------
@@ -248,9 +249,10 @@ hm_status hm_profile_kernels(hm_solver *s, const double *u, int reps, double *ms
  * one after another with a shared dt.  Here all members, of uneven grid
  * sizes and each with its own dt, advance together: every direction pass
  * of an RK stage is one persistent launch over the (member, tile) work
- * items of all members, and one RK4 step is a captured HIP graph replayed
- * per step; viscous members add one launch per stage over the (member,
- * tile) items of the viscous fluxes.  Members share the scheme
+ * items of all members (or, with tile shapes 2 and 3, the three passes of
+ * a stage are one launch), and one RK4 step is a captured HIP graph
+ * replayed per step; viscous members add one launch per stage over the
+ * (member, tile) items of the viscous term.  Members share the scheme
  * (interpolation, upwinding, WENO weights, gamma, eps, Re, Minf, Pr).
  * The batched state holds member m's hm-layout state at
  * u[5*offset_m .. 5*(offset_m + ncell_m)).
@@ -296,8 +298,9 @@ hm_status hm_ensemble_set_wait_limit(hm_ensemble *e, unsigned polls);
  * schedule of hm_adjoint_forward; the reverse sweep recomputes what the
  * schedule did not keep, takes lam = dJ/du_N of all members and returns
  * dJ/du_0.
- * Every VJP launch of a stage covers the (member, line) work items of all
- * members.  Built for the schemes hm_rhs_vjp supports. */
+ * Every VJP launch of a stage covers the work items of all members (groups
+ * of whole lines of one member, one tile or one cell).  Built for the
+ * schemes hm_rhs_vjp supports. */
 hm_status hm_ensemble_adjoint_forward(hm_ensemble *e, double *u, int nsteps, int checkpoint_every);
 hm_status hm_ensemble_adjoint_reverse(hm_ensemble *e, double *lam);
 /* The tape budget and schedule of the batched adjoint, as hm_set_tape_budget
