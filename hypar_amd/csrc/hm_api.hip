@@ -221,12 +221,15 @@ hm_status ssprk3_step(hm_solver *s, double *u, double dt) {
   const double c1[3] = {dt * SSP_B[0], dt * SSP_B[1], dt * SSP_B[2]};
   const double c2[3] = {dt * SSP_A10, dt * SSP_A21, dt * SSP_B[2]};
   const double c3[3] = {dt * SSP_A20, dt * SSP_B[1], dt * SSP_B[2]};
+  // distinct outputs per stage: stage 1 stores acc twice (O3 = O1), stage 2
+  // stores u three times (O1 = O2 = O3, with B = A)
+  const int nout[3] = {3, 2, 1};
   for (int stage = 0; stage < 3; stage++) {
     const hm::PassBuffers b = stage_buffers(s, u, Uin[stage], nullptr);
     if (visc) hm::launch_visc_par(s->g, s->sc, s->Q, s->par, s->dxg, s->stream);
     for (int d = 0; d < 2; d++) hm::launch_pass(s->g, s->sc, d, b, hm::OUT_STAGE, stage, 0.0, 0.0, s->dxg, s->stream);
     hm::launch_pass_z_affine(s->g, s->sc, b, A[stage], B[stage], O1[stage], O2[stage], O3[stage], c1[stage],
-                             c2[stage], c3[stage], s->dxg, s->stream);
+                             c2[stage], c3[stage], nout[stage], s->dxg, s->stream);
     hm_status st = check_launch("SSPRK3 stage passes");
     if (st != HM_OK) return st;
   }

@@ -105,11 +105,12 @@ void launch_pass(const Grid &g, const Scheme &sc, int dir, const PassBuffers &b,
 // the z pass with a three-output stage update: with r the right-hand side
 // and a, b the operands A, B at each cell, O1 = a + c1 r, O2 = b + c2 r (the
 // next stage state; its primitive variables go to b.Q when viscous),
-// O3 = a + c3 r, all stored (outputs may coincide when their values do, and
-// O2 may be B).  The x and y passes of the stage are launch_pass's.
+// O3 = a + c3 r.  The first nout (1-3) outputs are stored; an output left
+// out must repeat O1 (same array and value).  O2 may be B.  The x and y
+// passes of the stage are launch_pass's.
 void launch_pass_z_affine(const Grid &g, const Scheme &sc, const PassBuffers &b, const double *A, const double *B,
-                          double *O1, double *O2, double *O3, double c1, double c2, double c3, const double *dxg,
-                          hipStream_t st);
+                          double *O1, double *O2, double *O3, double c1, double c2, double c3, int nout,
+                          const double *dxg, hipStream_t st);
 // primitive variables (u, v, w, T) of u into Q
 void launch_prims(const Grid &g, const Scheme &sc, const double *u, double *Q, hipStream_t st);
 

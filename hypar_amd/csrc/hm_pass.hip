@@ -87,17 +87,19 @@ struct PassIO {
 // Stage update of the z pass in the separate kernel instantiations RK =
 // RK_AFFINE: with r the stage's right-hand side and a, b the operands A, B
 // at the cell,
-//   O1 = a + c1 r,  O2 = b + c2 r (the next stage state),  O3 = a + c3 r,
-// all three stored unconditionally; the caller picks operands, outputs and
-// coefficients per stage (outputs may coincide when their values do).  The
-// update has no per-stage branches: the compiler merged the stores of a
-// three-way stage branch into one store whose address register one path
-// never set.
+//   O1 = a + c1 r,  O2 = b + c2 r (the next stage state),  O3 = a + c3 r;
+// the caller picks operands, outputs and coefficients per stage.  The first
+// nout outputs are stored; the caller leaves out only an output that
+// repeats O1 (same array, same value), and every pointer is valid whether
+// stored or not.  The update has no per-stage branches that choose among
+// outputs: the compiler merged the stores of a three-way stage branch into
+// one store whose address register one path never set.
 enum { RK_CLASSIC = 0, RK_AFFINE = 1 };
 struct SspIO {
   const double *A, *B;
   double *O1, *O2, *O3;
   double c1, c2, c3;
+  int nout;  // outputs stored: O1, then O2 (nout >= 2), then O3 (nout = 3)
 };
 
 // per-cell data of the pass direction (NavierStokes3DFlux.c:23-57 and
@@ -498,8 +500,8 @@ __device__ __forceinline__ void emit_store(const PassConst &k, const PassIO &io,
       const size_t o = (size_t)v * k.ncell + idx;
       nu[v] = in.b[v] + sp.c2 * r[v];
       sp.O1[o] = in.a[v] + sp.c1 * r[v];
-      sp.O2[o] = nu[v];
-      sp.O3[o] = in.a[v] + sp.c3 * r[v];
+      if (sp.nout > 1) sp.O2[o] = nu[v];
+      if (sp.nout > 2) sp.O3[o] = in.a[v] + sp.c3 * r[v];
     }
   } else {
   // RK4 stage update in 3N form (TimeRK.c:126-195)
@@ -1084,12 +1086,12 @@ void launch_pass(const Grid &g, const Scheme &sc, int dir, const PassBuffers &b,
 }
 
 void launch_pass_z_affine(const Grid &g, const Scheme &sc, const PassBuffers &b, const double *A, const double *B,
-                          double *O1, double *O2, double *O3, double c1, double c2, double c3, const double *dxg,
-                          hipStream_t st) {
+                          double *O1, double *O2, double *O3, double c1, double c2, double c3, int nout,
+                          const double *dxg, hipStream_t st) {
   PassConst k;
   PassIO io;
   pass_args(g, sc, b, 2, OUT_STAGE, 0, 0.0, 0.0, dxg, k, io);
-  const SspIO sp{A, B, O1, O2, O3, c1, c2, c3};
+  const SspIO sp{A, B, O1, O2, O3, c1, c2, c3, nout};
   if (sc.interp == INTERP_CHAR) {
     if (sc.upwind == UPWIND_ROE) launch_ssp_w<INTERP_CHAR, UPWIND_ROE>(g, sc, k, io, sp, st);
     else launch_ssp_w<INTERP_CHAR, UPWIND_RUSANOV>(g, sc, k, io, sp, st);
