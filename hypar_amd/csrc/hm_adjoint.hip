@@ -1379,6 +1379,18 @@ __device__ __forceinline__ int ens_find_cell(const EnsMember *mem, int nmem, siz
   return lo;
 }
 
+// Calls f(member index) for cell idx of a launch with one thread per cell.
+// A block of consecutive cells nearly always lies in one member: the member
+// is then found once for the block, the same for every lane (scalar loads
+// of its table entry); otherwise every lane searches for its own.
+template <class F>
+__device__ __forceinline__ void with_cell_member(const EnsMember *mem, int nmem, size_t idx, const F &f) {
+  const size_t b0 = (size_t)blockIdx.x * blockDim.x;
+  const int m0 = ens_find_cell(mem, nmem, b0);
+  if (b0 + blockDim.x <= mem[m0].off + mem[m0].ncell) f(m0);
+  else f(ens_find_cell(mem, nmem, idx));
+}
+
 #define ENS_MEMBER_ARGS(a, M)                                                                              \
   a.u + 5 * M.off, a.lam + 5 * M.off, a.nu_in ? a.nu_in + 5 * M.off : nullptr, M.dt * a.A, M.dt * a.B,     \
       a.nu + 5 * M.off, a.L ? a.L + 5 * M.off : nullptr, a.mode, M.n[0], M.n[1], M.n[2], M.dxg, a.gamma, a.eps
@@ -1418,17 +1430,19 @@ __global__ void __launch_bounds__(VJP_CHAR_LINE_MAX) __attribute__((amdgpu_waves
 __global__ void __launch_bounds__(256) k_ens_vjp_prims(EnsVjp a, size_t ncell) {
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= ncell) return;
-  const EnsMember M = a.mem[ens_find_cell(a.mem, a.nmem, idx)];
-  const size_t c = idx - M.off;
-  double s[NV];
+  with_cell_member(a.mem, a.nmem, idx, [&](int m) {
+    const EnsMember &M = a.mem[m];
+    const size_t c = idx - M.off;
+    double s[NV];
 #pragma unroll
-  for (int v = 0; v < NV; v++) s[v] = a.u[5 * M.off + (size_t)v * M.ncell + c];
-  const Prim q = flowvar(s, a.gamma);
-  double *Q = a.work + 20 * M.off;
-  Q[c] = q.vx;
-  Q[M.ncell + c] = q.vy;
-  Q[2 * M.ncell + c] = q.vz;
-  Q[3 * M.ncell + c] = a.gamma * q.P / q.rho;
+    for (int v = 0; v < NV; v++) s[v] = a.u[5 * M.off + (size_t)v * M.ncell + c];
+    const Prim q = flowvar(s, a.gamma);
+    double *Q = a.work + 20 * M.off;
+    Q[c] = q.vx;
+    Q[M.ncell + c] = q.vy;
+    Q[2 * M.ncell + c] = q.vz;
+    Q[3 * M.ncell + c] = a.gamma * q.P / q.rho;
+  });
 }
 
 // tiles of all members; tiles0 holds each member's first tile
@@ -1461,9 +1475,11 @@ __global__ void __launch_bounds__(256) k_ens_visc_vjp_ghost(EnsVjp a, const unsi
 __global__ void __launch_bounds__(256) k_ens_visc_vjp_gather(EnsVjp a, size_t ncell) {
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= ncell) return;
-  const EnsMember M = a.mem[ens_find_cell(a.mem, a.nmem, idx)];
-  const double *QB = a.work + 20 * M.off + 4 * M.ncell, *GB = a.work + 20 * M.off + 8 * M.ncell;
-  visc_vjp_gather(a.u + 5 * M.off, QB, GB, a.nu + 5 * M.off, M.n[0], M.n[1], M.n[2], a.gamma, idx - M.off);
+  with_cell_member(a.mem, a.nmem, idx, [&](int m) {
+    const EnsMember &M = a.mem[m];
+    const double *QB = a.work + 20 * M.off + 4 * M.ncell, *GB = a.work + 20 * M.off + 8 * M.ncell;
+    visc_vjp_gather(a.u + 5 * M.off, QB, GB, a.nu + 5 * M.off, M.n[0], M.n[1], M.n[2], a.gamma, idx - M.off);
+  });
 }
 
 // y = a*x + b*y
