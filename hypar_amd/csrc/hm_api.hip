@@ -887,6 +887,13 @@ static hm_status adjoint_reverse_sweep(hm_solver *s, double *lam) {
       s->tape_holds[o] = -1;
       if ((st = recompute_segment(s, seg, n0, n1, dt)) != HM_OK) return st;
       s->tape_holds[o] = seg;
+      if (overlap) {
+        // a recomputation writes the solver's step buffers (primitive
+        // variables, viscous terms, right-hand side, accumulator, the last
+        // step's scratch): the second stream's next one starts after this one
+        HM_HIP(hipEventRecord(s->ev_start, main));
+        HM_HIP(hipStreamWaitEvent(s->aux, s->ev_start, 0));
+      }
     }
     if (overlap && seg > 0 && s->tape_holds[(seg - 1) % sets] != seg - 1) {
       // recompute the previous segment into its set on the second stream,

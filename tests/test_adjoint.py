@@ -298,10 +298,14 @@ def test_viscous_characteristic_gradient_matches_finite_differences():
 
 
 @pytest.mark.gpu
-def test_reverse_sweep_repeats_and_survives_step_vjp():
-    """7 steps with a checkpoint every 2: four segments for three stage
-    sets, so the first reverse sweep recomputes segment 0 into the set that
-    held segment 3.  A second sweep over the same forward (a retain_graph
+@pytest.mark.parametrize("nsteps", [7, 9])
+def test_reverse_sweep_repeats_and_survives_step_vjp(nsteps):
+    """A checkpoint every 2 steps and three stage sets.  7 steps: four
+    segments, so the first reverse sweep recomputes segment 0 into the set
+    that held segment 3.  9 steps: five segments, so a second sweep
+    recomputes segment 4 on the solver's stream while the second stream
+    recomputes segment 3 (the two share the step buffers and must run one
+    after the other).  A second sweep over the same forward (a retain_graph
     backward, or a second objective), and one after hm_rk4_vjp reused stage
     set 0, must recompute what their sets no longer hold: all three give the
     same gradient, equal to torch autograd through the CPU trajectory."""
@@ -309,14 +313,15 @@ def test_reverse_sweep_repeats_and_survives_step_vjp():
     w, u0, dt, dx, sch = _scheme_case("characteristic", "roe", 1000.0)
     x = T.to_soa(u0).requires_grad_(True)
     y = x
-    for _ in range(7):
+    for _ in range(nsteps):
         y = T.rk4_step(y, dt, dx, **sch)
     lam = torch.tensor(np.random.default_rng(11).normal(size=x.shape))
     (ref,) = torch.autograd.grad(y, x, lam)
-    print("[gpu] repeated reverse sweeps, char/roe viscous, 7 steps, checkpoint every 2", flush=True)
+    print(f"[gpu] repeated reverse sweeps, char/roe viscous, {nsteps} steps, checkpoint every 2", flush=True)
     s = _gpu_solver(w)
     u = _hm(x)
-    s.adjoint_forward(u, dt, 7, 2)
+    s.adjoint_forward(u, dt, nsteps, 2)
+    assert s.tape_info()["sets"] == 3
     g1 = s.adjoint_reverse(_hm(lam))
     g2 = s.adjoint_reverse(_hm(lam))
     s.rk4_vjp(_hm(x), dt, _hm(lam))
@@ -327,7 +332,7 @@ def test_reverse_sweep_repeats_and_survives_step_vjp():
     assert torch.equal(g1, g2) and torch.equal(g1, g3)
     # the same through autograd: two backward passes over one graph
     xg = _hm(x).requires_grad_(True)
-    yN = trajectory(xg, s, dt, 7, 2)
+    yN = trajectory(xg, s, dt, nsteps, 2)
     (a1,) = torch.autograd.grad(yN, xg, _hm(lam), retain_graph=True)
     (a2,) = torch.autograd.grad(yN, xg, _hm(lam))
     assert torch.equal(a1, g1) and torch.equal(a2, g1)
