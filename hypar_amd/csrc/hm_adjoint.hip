@@ -717,7 +717,10 @@ __device__ __forceinline__ void rhs_vjp_line(const double *__restrict__ u, const
 // and finally taken back through the Roe average onto cells I-1 and I.  The
 // interpolated values use the modified solution, which equals u exactly in
 // exact arithmetic, so both adjoints land on u.
-constexpr int VJP_CHAR_LINE_MAX = 256;
+// longest grid line a characteristic adjoint workgroup holds: at 512 cells
+// the block is 8 wavefronts, two per SIMD (the register budget the
+// two-wave hint sets for every block size) and 88 KiB of LDS
+constexpr int VJP_CHAR_LINE_MAX = 512;
 constexpr int VJP_CHAR_CS = 11;   // doubles per cell record: state [5], flux [5], pad
 constexpr int VJP_CHAR_ES = 11;   // doubles per exchange record
 constexpr int VJP_CHAR_LDS = VJP_CHAR_CS + VJP_CHAR_ES;  // doubles of LDS per cell of the line

@@ -687,9 +687,7 @@ static hm_status rk4_step_vjp(hm_solver *s, double *const U[4], double dt, const
 static hm_status check_vjp(hm_solver *s, const char *name) {
   if (!s) return fail(HM_ERR_ARG, std::string(name) + ": solver is NULL");
   if (!hm::vjp_supported(s->g, s->sc))
-    return fail(HM_ERR_UNSUPPORTED, std::string(name) +
-                                        ": the adjoint is built for WENO5-JS weights, component-wise (grid "
-                                        "sides <= 512) or characteristic-wise (grid sides <= 256)");
+    return fail(HM_ERR_UNSUPPORTED, std::string(name) + ": the adjoint holds grid lines of at most 512 cells");
   return HM_OK;
 }
 
@@ -1472,9 +1470,7 @@ static hm_status ens_check_vjp(hm_ensemble *e, const void *u, const char *name) 
   hm_status st = ens_check(e, u, name);
   if (st != HM_OK) return st;
   if (!hm::ens_vjp_supported(e->sc, e->nmax))
-    return fail(HM_ERR_UNSUPPORTED, std::string(name) +
-                                        ": the adjoint is built for WENO5-JS weights, component-wise (sides "
-                                        "<= 512) or characteristic-wise (sides <= 256)");
+    return fail(HM_ERR_UNSUPPORTED, std::string(name) + ": the adjoint holds member lines of at most 512 cells");
   if (!e->dt_set) return fail(HM_ERR_ARG, std::string(name) + ": set the members' dt first (hm_ensemble_set_dt)");
   return HM_OK;
 }

@@ -197,8 +197,8 @@ hm_status hm_fill(hm_solver *s, double *x, double value, size_t n);
  * No counterpart in the reference (it has no adjoint).  Reverse-mode
  * derivatives of this library's own discretisation, inviscid or viscous;
  * every WENO5 weight formulation (JS, mapped, Z, YC) with Roe or Rusanov
- * upwinding, component-wise (grid sides <= 512) or characteristic-wise
- * (grid sides <= 256) (HM_ERR_UNSUPPORTED otherwise).  lam, out: hm-layout
+ * upwinding, component-wise or characteristic-wise, grid sides <= 512
+ * (HM_ERR_UNSUPPORTED otherwise).  lam, out: hm-layout
  * device arrays. */
 /* out = (d rhs / d u)^T lam at state u */
 hm_status hm_rhs_vjp(hm_solver *s, double *out, const double *u, const double *lam);

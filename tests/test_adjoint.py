@@ -120,10 +120,11 @@ def test_rhs_vjp_matches_autograd(interp, upwind, Re):
 @pytest.mark.parametrize("interp,upwind,Re", [("characteristic", "roe", 1000.0), ("characteristic", "rusanov", -1.0)])
 def test_rhs_vjp_on_ragged_lines(interp, upwind, Re):
     """Sides of 5 and 7 (stencils wrap more than once around a line), sides
-    above one and two wavefronts, and for the viscous tiles (64 x 8 columns,
-    16 planes per segment) partial tiles in x, three tiles in y and three
-    z segments."""
-    for dims in [(5, 7, 6), (70, 5, 9), (130, 6, 5), (9, 19, 37)]:
+    above one and two wavefronts, 300- and 512-cell lines (one to a block of
+    320 and 512 threads), and for the viscous tiles (64 x 8 columns, 16
+    planes per segment) partial tiles in x, three tiles in y and three z
+    segments."""
+    for dims in [(5, 7, 6), (70, 5, 9), (130, 6, 5), (9, 19, 37), (300, 5, 6), (6, 5, 512)]:
         w, u0, dt, dx, sch = _scheme_case(interp, upwind, Re, dims)
         x = T.to_soa(u0).requires_grad_(True)
         lam = torch.tensor(np.random.default_rng(7).normal(size=x.shape))
@@ -137,11 +138,11 @@ def test_rhs_vjp_on_ragged_lines(interp, upwind, Re):
 
 @pytest.mark.gpu
 def test_vjp_rejects_lines_longer_than_a_workgroup():
-    """One workgroup holds one grid line: 512 cells component-wise, 256
-    characteristic-wise."""
+    """One workgroup holds one grid line of at most 512 cells, component-wise
+    and characteristic-wise."""
     from hypar_amd import lib
     from hypar_amd.solver import NS3D
-    for dims, kw in [((513, 5, 5), {}), ((5, 257, 5), dict(interp="characteristic", upwind="roe"))]:
+    for dims, kw in [((513, 5, 5), {}), ((5, 513, 5), dict(interp="characteristic", upwind="roe"))]:
         s = NS3D(dims, **kw)
         u = torch.ones(5 * s.ncell, dtype=torch.float64, device="cuda")
         with pytest.raises(lib.HMError) as ex:

@@ -259,6 +259,37 @@ def test_ensemble_adjoint_matches_single_grid(interp, upwind, Re, weno):
         s.close()
 
 
+def test_ensemble_adjoint_long_lines_match_autograd():
+    """Characteristic/Roe viscous members with 300- and 512-cell lines beside a
+    short one: the batched line VJPs take 512-thread blocks, each holding one
+    long line or up to 102 five-cell lines.  Two steps, checkpoint every step;
+    each member's gradient against torch autograd through the CPU trajectory."""
+    from hypar_amd import decks
+    from hypar_amd.adjoint import ensemble_trajectory, kinetic_energy
+    from hypar_amd.ensemble import Ensemble
+    from oracle import torch_ref as T
+    dims = [(300, 5, 6), (6, 5, 512), (5, 7, 6)]
+    ws, u0s, dts = _members(dims, "characteristic", "roe", 1000.0)
+    print(f"[gpu] ensemble adjoint, long lines: char/roe Re=1000 dims={dims}", flush=True)
+    e = Ensemble(dims, "characteristic", "roe", Re=1000.0, Minf=0.1)
+    e.set_dt(dts)
+    u0 = e.to_device(u0s).requires_grad_(True)
+    uN = ensemble_trajectory(u0, e, 2, 1)
+    (g,) = torch.autograd.grad(sum(kinetic_energy(e.member(uN, m), dims[m]) for m in range(len(dims))), u0)
+    for m, w in enumerate(ws):
+        dx = [torch.tensor(decks.dxinv_1d(decks.grid_1d(n))[3:-3]) for n in w.dims]
+        sch = dict(interp="characteristic", upwind="roe", Re=w.Re / w.Minf, Pr=w.Pr, weights="js")
+        x = T.to_soa(u0s[m]).requires_grad_(True)
+        y = x
+        for _ in range(2):
+            y = T.rk4_step(y, dts[m], dx, **sch)
+        (ref,) = torch.autograd.grad(T.kinetic_energy(y, dx), x)
+        err = float((e.member(g, m).cpu().view_as(ref) - ref).norm() / ref.norm())
+        print(f"[gpu]   member {m} {w.dims}: rel err vs torch autograd {err:.2e}", flush=True)
+        assert err <= 1e-11
+    e.close()
+
+
 def test_rejections():
     from hypar_amd import lib
     from hypar_amd.ensemble import Ensemble
