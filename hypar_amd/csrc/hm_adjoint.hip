@@ -26,6 +26,8 @@
 // of the branch the forward evaluation selects.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "hm_device.h"
 #include "hm_internal.h"
 
@@ -448,15 +450,63 @@ __device__ __forceinline__ void vjp_load_cell(const double *__restrict__ u, cons
     for (int v = 0; v < NV; v++) ni[v] = nu_in[(size_t)v * ncell + c];
 }
 
-// the finished adjoint acc of cell c into nu (and L) by mode; every load
-// issues before the first store, so that the stores (nu, L and lam may
-// alias as far as the compiler knows) do not serialise the loads
+// weight of F(r+s) in D at reader r, s = -2, -1, +1, +2
+__device__ __forceinline__ double d4w(int s) {
+  return s == -2 ? 1.0 / 12.0 : (s == -1 ? -8.0 / 12.0 : (s == 1 ? 8.0 / 12.0 : -1.0 / 12.0));
+}
+
+// The viscous term's share of the adjoint of cell idx, (d par/d u)^T mu: QB
+// plus the transposed gradient stencils over GB, then back through
+// (u, v, w, T)(u) at the cell's state us (NavierStokes3DParabolicFunction.c:
+// 85-108).  QB and GB are complete once the tile and ghost-layer kernels of
+// the viscous adjoint have run; the x line VJP, which runs after them, adds
+// this share into nu together with its own.
+__device__ __forceinline__ void visc_gather_cell(const double (&us)[NV], const double *__restrict__ QB,
+                                                 const double *__restrict__ GB, int nx, int ny, int nz,
+                                                 double gamma, size_t idx, double (&ub)[NV]) {
+  const size_t ncell = (size_t)nx * ny * nz;
+  const int n[3] = {nx, ny, nz};
+  const int pos[3] = {(int)(idx % nx), (int)((idx / nx) % ny), (int)(idx / ((size_t)nx * ny))};
+  const size_t st[3] = {1, (size_t)nx, (size_t)nx * ny};
+  double qb[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) qb[c] = QB[(size_t)c * ncell + idx];
+  const int offs[4] = {-2, -1, 1, 2};
+#pragma unroll
+  for (int e = 0; e < 3; e++) {
+    const size_t line0 = idx - (size_t)pos[e] * st[e];
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      // reader j = m - s reads this cell with weight d4w(s) (periodic)
+      const size_t j = line0 + (size_t)wrapn(pos[e] - offs[s], n[e]) * st[e];
+      const double w = d4w(offs[s]);
+#pragma unroll
+      for (int c = 0; c < 4; c++) qb[c] += w * GB[(size_t)(c * 3 + e) * ncell + j];
+    }
+  }
+  const double gm1 = gamma - 1.0;
+  const double ir = 1.0 / us[0];
+  const double vx = us[1] * ir, vy = us[2] * ir, vz = us[3] * ir;
+  const double q2 = vx * vx + vy * vy + vz * vz;
+  const double k = gamma * gm1 * qb[3];  // T = gamma (gamma-1) (e/rho - |m|^2/(2 rho^2))
+  ub[0] = -(qb[0] * vx + qb[1] * vy + qb[2] * vz) * ir + k * (-us[4] * ir * ir + q2 * ir);
+  ub[1] = qb[0] * ir - k * vx * ir;
+  ub[2] = qb[1] * ir - k * vy * ir;
+  ub[3] = qb[2] * ir - k * vz * ir;
+  ub[4] = k * ir;
+}
+
+// the finished adjoint acc of cell c into nu (and L) by mode, after the
+// viscous share vb when has_vb: nu = (nu + vb) + acc, the order of a
+// separate viscous pass that adds vb before this one.  Every load issues
+// before the first store, so that the stores (nu, L and lam may alias as
+// far as the compiler knows) do not serialise the loads
 __device__ __forceinline__ void vjp_store(double *__restrict__ nu, double *__restrict__ L,
                                           const double *__restrict__ lam, int mode, size_t ncell, size_t c,
-                                          const double (&acc)[NV]) {
+                                          const double (&acc)[NV], bool has_vb, const double (&vb)[NV]) {
   if (mode == VJP_WRITE) {
 #pragma unroll
-    for (int v = 0; v < NV; v++) nu[(size_t)v * ncell + c] = acc[v];
+    for (int v = 0; v < NV; v++) nu[(size_t)v * ncell + c] = has_vb ? vb[v] + acc[v] : acc[v];
     return;
   }
   double r[NV], l[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -469,6 +519,9 @@ __device__ __forceinline__ void vjp_store(double *__restrict__ nu, double *__res
 #pragma unroll
     for (int v = 0; v < NV; v++) l[v] = lam[(size_t)v * ncell + c];
   }
+  if (has_vb)
+#pragma unroll
+    for (int v = 0; v < NV; v++) r[v] = r[v] + vb[v];
 #pragma unroll
   for (int v = 0; v < NV; v++) {
     r[v] = r[v] + acc[v];
@@ -479,12 +532,14 @@ __device__ __forceinline__ void vjp_store(double *__restrict__ nu, double *__res
     for (int v = 0; v < NV; v++) L[(size_t)v * ncell + c] = l[v] + r[v];
 }
 
-template <int DIR, int UPW, int W>
+template <int DIR, int UPW, int W, bool VG = false>
 __device__ __forceinline__ void rhs_vjp_line(const double *__restrict__ u, const double *__restrict__ lam,
                                              const double *__restrict__ nu_in, double ca, double cb,
                                              double *__restrict__ nu, double *__restrict__ L, int mode, int nx,
                                              int ny, int nz, const double *__restrict__ dxg, double gamma,
-                                             double eps, size_t line, int t, bool active, double *smem) {
+                                             double eps, const double *__restrict__ QB,
+                                             const double *__restrict__ GB, size_t line, int t, bool active,
+                                             double *smem) {
   const size_t ncell = (size_t)nx * ny * nz;
   const int n = DIR == 0 ? nx : (DIR == 1 ? ny : nz);
   size_t base, stride;
@@ -697,7 +752,9 @@ __device__ __forceinline__ void rhs_vjp_line(const double *__restrict__ u, const
     }
     flux_jac_T<DIR>(uc, gamma, accF, accU);
     const size_t c = base + (size_t)t * stride;
-    vjp_store(nu, L, lam, mode, ncell, c, accU);
+    double vb[NV] = {0, 0, 0, 0, 0};
+    if (VG) visc_gather_cell(uc, QB, GB, nx, ny, nz, gamma, c, vb);
+    vjp_store(nu, L, lam, mode, ncell, c, accU, VG, vb);
   }
 }
 
@@ -738,11 +795,12 @@ constexpr int VJP_CHAR_LDS = VJP_CHAR_CS + VJP_CHAR_ES;  // doubles of LDS per c
 // that touch it back through those rows, in a fixed order, so the sums are
 // deterministic.  The Roe-state derivatives of R, L and |lambda_k| are summed
 // field by field in registers.
-template <int DIR, int UPW, int W>
+template <int DIR, int UPW, int W, bool VG = false>
 __device__ __forceinline__ void rhs_vjp_char_line(
     const double *__restrict__ u, const double *__restrict__ lam, const double *__restrict__ nu_in, double ca,
     double cb, double *__restrict__ nu, double *__restrict__ L, int mode, int nx, int ny, int nz,
-    const double *__restrict__ dxg, double gamma, double eps, size_t line, int t, bool active, double *smem) {
+    const double *__restrict__ dxg, double gamma, double eps, const double *__restrict__ QB,
+    const double *__restrict__ GB, size_t line, int t, bool active, double *smem) {
   const size_t ncell = (size_t)nx * ny * nz;
   const int n = DIR == 0 ? nx : (DIR == 1 ? ny : nz);
   size_t base, stride;
@@ -965,7 +1023,14 @@ __device__ __forceinline__ void rhs_vjp_char_line(
 #pragma unroll
     for (int v = 0; v < NV; v++) acc[v] += E[tp * ES + v] + E[t * ES + NV + v];
     const size_t c = base + (size_t)t * stride;
-    vjp_store(nu, L, lam, mode, ncell, c, acc);
+    double vb[NV] = {0, 0, 0, 0, 0};
+    if (VG) {
+      double uc[NV];
+#pragma unroll
+      for (int v = 0; v < NV; v++) uc[v] = sC[t * CS + v];
+      visc_gather_cell(uc, QB, GB, nx, ny, nz, gamma, c, vb);
+    }
+    vjp_store(nu, L, lam, mode, ncell, c, acc, VG, vb);
   }
 }
 
@@ -988,17 +1053,13 @@ __device__ __forceinline__ void rhs_vjp_char_line(
 //                    registers, so each is read about once per tile;
 //  k_visc_vjp_ghost  the same for the ghost-layer fluxes of each line, whose
 //                    gradient adjoints are spread over their stencils into QB;
-//  k_visc_vjp_gather QB plus the transposed gradient stencils over GB, then
-//                    back through the primitive variables, added onto the
-//                    nu that the y and z line VJPs left.
+//  visc_gather_cell  QB plus the transposed gradient stencils over GB, then
+//                    back through the primitive variables: the x line VJP,
+//                    which runs last, adds this onto the nu that the z and y
+//                    line VJPs left, before its own share.
 struct ViscC {
   double gamma, inv_Re, inv_gm1, inv_Pr;
 };
-
-// weight of F(r+s) in D at reader r, s = -2, -1, +1, +2
-__device__ __forceinline__ double d4w(int s) {
-  return s == -2 ? 1.0 / 12.0 : (s == -1 ? -8.0 / 12.0 : (s == 1 ? 8.0 / 12.0 : -1.0 / 12.0));
-}
 
 // adjoint of the viscous flux along DIR at a point: fb = adjoint of
 // (tau_d1, tau_d2, tau_d3, energy flux); adds to qb (u, v, w, T values) and
@@ -1246,80 +1307,35 @@ __device__ __forceinline__ void visc_vjp_ghost_line(const double *__restrict__ Q
     for (int c = 0; c < 4; c++) QB[(size_t)c * ncell + cw[j]] += acc[j][c];
 }
 
-// nu += (d par/d u)^T mu: QB plus the transposed gradient stencils over GB,
-// then back through (u, v, w, T)(u) (NavierStokes3DParabolicFunction.c:85-108)
-__device__ __forceinline__ void visc_vjp_gather(const double *__restrict__ U, const double *__restrict__ QB,
-                                                const double *__restrict__ GB, double *__restrict__ nu, int nx,
-                                                int ny, int nz, double gamma, size_t idx) {
-  const size_t ncell = (size_t)nx * ny * nz;
-  if (idx >= ncell) return;
-  const int n[3] = {nx, ny, nz};
-  const int pos[3] = {(int)(idx % nx), (int)((idx / nx) % ny), (int)(idx / ((size_t)nx * ny))};
-  const size_t st[3] = {1, (size_t)nx, (size_t)nx * ny};
-  double qb[4];
-#pragma unroll
-  for (int c = 0; c < 4; c++) qb[c] = QB[(size_t)c * ncell + idx];
-  const int offs[4] = {-2, -1, 1, 2};
-#pragma unroll
-  for (int e = 0; e < 3; e++) {
-    const size_t line0 = idx - (size_t)pos[e] * st[e];
-#pragma unroll
-    for (int s = 0; s < 4; s++) {
-      // reader j = m - s reads this cell with weight d4w(s) (periodic)
-      const size_t j = line0 + (size_t)wrapn(pos[e] - offs[s], n[e]) * st[e];
-      const double w = d4w(offs[s]);
-#pragma unroll
-      for (int c = 0; c < 4; c++) qb[c] += w * GB[(size_t)(c * 3 + e) * ncell + j];
-    }
-  }
-  double u[NV];
-#pragma unroll
-  for (int v = 0; v < NV; v++) u[v] = U[(size_t)v * ncell + idx];
-  const double gm1 = gamma - 1.0;
-  const double ir = 1.0 / u[0];
-  const double vx = u[1] * ir, vy = u[2] * ir, vz = u[3] * ir;
-  const double q2 = vx * vx + vy * vy + vz * vz;
-  const double k = gamma * gm1 * qb[3];  // T = gamma (gamma-1) (e/rho - |m|^2/(2 rho^2))
-  double ub[NV];
-  ub[0] = -(qb[0] * vx + qb[1] * vy + qb[2] * vz) * ir + k * (-u[4] * ir * ir + q2 * ir);
-  ub[1] = qb[0] * ir - k * vx * ir;
-  ub[2] = qb[1] * ir - k * vy * ir;
-  ub[3] = qb[2] * ir - k * vz * ir;
-  ub[4] = k * ir;
-  double r[NV];
-#pragma unroll
-  for (int v = 0; v < NV; v++) r[v] = nu[(size_t)v * ncell + idx];
-#pragma unroll
-  for (int v = 0; v < NV; v++) nu[(size_t)v * ncell + idx] = r[v] + ub[v];
-}
-
 // ---------------------------------------------------------------------------
 // kernels: one grid
 // ---------------------------------------------------------------------------
 // neighbouring lines share cache lines along x: keep them on one XCD
-template <int DIR, int UPW, int W>
+template <int DIR, int UPW, int W, bool VG>
 __global__ void __launch_bounds__(VJP_LINE_MAX) k_rhs_vjp(const double *__restrict__ u, const double *__restrict__ lam,
                                                       const double *__restrict__ nu_in, double ca, double cb,
                                                       double *__restrict__ nu, double *__restrict__ L, int mode,
                                                       int nx, int ny, int nz, const double *__restrict__ dxg,
-                                                      double gamma, double eps) {
+                                                      double gamma, double eps, const double *__restrict__ QB,
+                                                      const double *__restrict__ GB) {
   extern __shared__ double smem[];
   const int n = DIR == 0 ? nx : (DIR == 1 ? ny : nz);
   const LineSlot s = line_slot(xcd_block(), n, (size_t)nx * ny * nz / n);
-  rhs_vjp_line<DIR, UPW, W>(u, lam, nu_in, ca, cb, nu, L, mode, nx, ny, nz, dxg, gamma, eps, s.line, s.t, s.live,
-                            smem + (size_t)s.sub * VJP_LDS * n);
+  rhs_vjp_line<DIR, UPW, W, VG>(u, lam, nu_in, ca, cb, nu, L, mode, nx, ny, nz, dxg, gamma, eps, QB, GB, s.line,
+                                s.t, s.live, smem + (size_t)s.sub * VJP_LDS * n);
 }
 
-template <int DIR, int UPW, int W>
+template <int DIR, int UPW, int W, bool VG>
 __global__ void __launch_bounds__(VJP_CHAR_LINE_MAX) __attribute__((amdgpu_waves_per_eu(2))) k_rhs_vjp_char(
     const double *__restrict__ u, const double *__restrict__ lam, const double *__restrict__ nu_in, double ca,
     double cb, double *__restrict__ nu, double *__restrict__ L, int mode, int nx, int ny, int nz,
-    const double *__restrict__ dxg, double gamma, double eps) {
+    const double *__restrict__ dxg, double gamma, double eps, const double *__restrict__ QB,
+    const double *__restrict__ GB) {
   extern __shared__ double smem[];
   const int n = DIR == 0 ? nx : (DIR == 1 ? ny : nz);
   const LineSlot s = line_slot(xcd_block(), n, (size_t)nx * ny * nz / n);
-  rhs_vjp_char_line<DIR, UPW, W>(u, lam, nu_in, ca, cb, nu, L, mode, nx, ny, nz, dxg, gamma, eps, s.line, s.t,
-                                 s.live, smem + (size_t)s.sub * VJP_CHAR_LDS * n);
+  rhs_vjp_char_line<DIR, UPW, W, VG>(u, lam, nu_in, ca, cb, nu, L, mode, nx, ny, nz, dxg, gamma, eps, QB, GB,
+                                     s.line, s.t, s.live, smem + (size_t)s.sub * VJP_CHAR_LDS * n);
 }
 
 __global__ void __launch_bounds__(VT) k_visc_vjp_tile(const double *__restrict__ Q, const double *__restrict__ lam,
@@ -1337,12 +1353,6 @@ __global__ void __launch_bounds__(256) k_visc_vjp_ghost(const double *__restrict
                                                         const double *__restrict__ dxg, ViscC vc) {
   visc_vjp_ghost_line<DIR>(Q, lam, nu_in, ca, cb, QB, nx, ny, nz, dxg, vc,
                            (size_t)blockIdx.x * blockDim.x + threadIdx.x);
-}
-
-__global__ void __launch_bounds__(256) k_visc_vjp_gather(const double *__restrict__ U, const double *__restrict__ QB,
-                                                         const double *__restrict__ GB, double *__restrict__ nu,
-                                                         int nx, int ny, int nz, double gamma) {
-  visc_vjp_gather(U, QB, GB, nu, nx, ny, nz, gamma, (size_t)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1394,11 +1404,14 @@ __device__ __forceinline__ void with_cell_member(const EnsMember *mem, int nmem,
   else f(ens_find_cell(mem, nmem, idx));
 }
 
+// with VG (the x lines of a viscous ensemble) also QB and GB of the member's
+// work space, whose viscous share the lines add
 #define ENS_MEMBER_ARGS(a, M)                                                                              \
   a.u + 5 * M.off, a.lam + 5 * M.off, a.nu_in ? a.nu_in + 5 * M.off : nullptr, M.dt * a.A, M.dt * a.B,     \
-      a.nu + 5 * M.off, a.L ? a.L + 5 * M.off : nullptr, a.mode, M.n[0], M.n[1], M.n[2], M.dxg, a.gamma, a.eps
+      a.nu + 5 * M.off, a.L ? a.L + 5 * M.off : nullptr, a.mode, M.n[0], M.n[1], M.n[2], M.dxg, a.gamma, a.eps, \
+      VG ? a.work + 20 * M.off + 4 * M.ncell : nullptr, VG ? a.work + 20 * M.off + 8 * M.ncell : nullptr
 
-template <int DIR, int UPW, int W>
+template <int DIR, int UPW, int W, bool VG>
 __global__ void __launch_bounds__(VJP_LINE_MAX) k_ens_rhs_vjp(EnsVjp a, const unsigned *__restrict__ first,
                                                           unsigned nitems) {
   extern __shared__ double smem[];
@@ -1407,12 +1420,13 @@ __global__ void __launch_bounds__(VJP_LINE_MAX) k_ens_rhs_vjp(EnsVjp a, const un
     const int m = ens_find(first, a.nmem, item);
     const EnsMember M = a.mem[m];
     const LineSlot s = line_slot(item - first[m], M.n[DIR], M.ncell / M.n[DIR]);
-    rhs_vjp_line<DIR, UPW, W>(ENS_MEMBER_ARGS(a, M), s.line, s.t, s.live, smem + (size_t)s.sub * VJP_LDS * M.n[DIR]);
+    rhs_vjp_line<DIR, UPW, W, VG>(ENS_MEMBER_ARGS(a, M), s.line, s.t, s.live,
+                                  smem + (size_t)s.sub * VJP_LDS * M.n[DIR]);
     __syncthreads();
   }
 }
 
-template <int DIR, int UPW, int W>
+template <int DIR, int UPW, int W, bool VG>
 __global__ void __launch_bounds__(VJP_CHAR_LINE_MAX) __attribute__((amdgpu_waves_per_eu(2))) k_ens_rhs_vjp_char(EnsVjp a, const unsigned *__restrict__ first,
                                                                     unsigned nitems) {
   extern __shared__ double smem[];
@@ -1421,8 +1435,8 @@ __global__ void __launch_bounds__(VJP_CHAR_LINE_MAX) __attribute__((amdgpu_waves
     const int m = ens_find(first, a.nmem, item);
     const EnsMember M = a.mem[m];
     const LineSlot s = line_slot(item - first[m], M.n[DIR], M.ncell / M.n[DIR]);
-    rhs_vjp_char_line<DIR, UPW, W>(ENS_MEMBER_ARGS(a, M), s.line, s.t, s.live,
-                                   smem + (size_t)s.sub * VJP_CHAR_LDS * M.n[DIR]);
+    rhs_vjp_char_line<DIR, UPW, W, VG>(ENS_MEMBER_ARGS(a, M), s.line, s.t, s.live,
+                                       smem + (size_t)s.sub * VJP_CHAR_LDS * M.n[DIR]);
     __syncthreads();
   }
 }
@@ -1475,16 +1489,6 @@ __global__ void __launch_bounds__(256) k_ens_visc_vjp_ghost(EnsVjp a, const unsi
                            M.dt * a.B, QB, M.n[0], M.n[1], M.n[2], M.dxg, a.vc, item - first[m]);
 }
 
-__global__ void __launch_bounds__(256) k_ens_visc_vjp_gather(EnsVjp a, size_t ncell) {
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= ncell) return;
-  with_cell_member(a.mem, a.nmem, idx, [&](int m) {
-    const EnsMember &M = a.mem[m];
-    const double *QB = a.work + 20 * M.off + 4 * M.ncell, *GB = a.work + 20 * M.off + 8 * M.ncell;
-    visc_vjp_gather(a.u + 5 * M.off, QB, GB, a.nu + 5 * M.off, M.n[0], M.n[1], M.n[2], a.gamma, idx - M.off);
-  });
-}
-
 // y = a*x + b*y
 __global__ void k_axpby(const double *__restrict__ x, double a, double *__restrict__ y, double b, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -1527,23 +1531,30 @@ static void with_scheme(const Scheme &sc, F &&f) {
 
 template <int DIR>
 static void launch_vjp_dir(const Grid &g, const Scheme &sc, const double *u, const double *lam, const double *nu_in,
-                           double ca, double cb, double *nu, double *L, int mode, const double *dxg, hipStream_t st) {
+                           double ca, double cb, double *nu, double *L, int mode, const double *dxg, hipStream_t st,
+                           const double *QB = nullptr, const double *GB = nullptr) {
   const unsigned threads = (unsigned)((g.n[DIR] + 63) / 64 * 64);
   const unsigned lpb = (unsigned)vjp_lines_per_block((int)threads, g.n[DIR]);
   const unsigned lines = (unsigned)((g.ncell / g.n[DIR] + lpb - 1) / lpb);  // work items
   with_scheme(sc, [&](auto tag) {
     constexpr int U = decltype(tag)::upw, W = decltype(tag)::weno;
-    if (sc.interp == INTERP_COMP) {
-      const size_t lds = (size_t)VJP_LDS * lpb * g.n[DIR] * sizeof(double);
-      allow_lds((const void *)k_rhs_vjp<DIR, U, W>, lds);
-      hipLaunchKernelGGL((k_rhs_vjp<DIR, U, W>), dim3(lines), dim3(threads), lds, st, u, lam, nu_in, ca, cb, nu, L,
-                         mode, g.n[0], g.n[1], g.n[2], dxg, sc.gamma, sc.eps);
-    } else {
-      const size_t lds = (size_t)VJP_CHAR_LDS * lpb * g.n[DIR] * sizeof(double);
-      allow_lds((const void *)k_rhs_vjp_char<DIR, U, W>, lds);
-      hipLaunchKernelGGL((k_rhs_vjp_char<DIR, U, W>), dim3(lines), dim3(threads), lds, st, u, lam, nu_in, ca, cb,
-                         nu, L, mode, g.n[0], g.n[1], g.n[2], dxg, sc.gamma, sc.eps);
-    }
+    auto go = [&](auto vg) {
+      constexpr bool VG = decltype(vg)::value;
+      if (sc.interp == INTERP_COMP) {
+        const size_t lds = (size_t)VJP_LDS * lpb * g.n[DIR] * sizeof(double);
+        allow_lds((const void *)k_rhs_vjp<DIR, U, W, VG>, lds);
+        hipLaunchKernelGGL((k_rhs_vjp<DIR, U, W, VG>), dim3(lines), dim3(threads), lds, st, u, lam, nu_in, ca, cb,
+                           nu, L, mode, g.n[0], g.n[1], g.n[2], dxg, sc.gamma, sc.eps, QB, GB);
+      } else {
+        const size_t lds = (size_t)VJP_CHAR_LDS * lpb * g.n[DIR] * sizeof(double);
+        allow_lds((const void *)k_rhs_vjp_char<DIR, U, W, VG>, lds);
+        hipLaunchKernelGGL((k_rhs_vjp_char<DIR, U, W, VG>), dim3(lines), dim3(threads), lds, st, u, lam, nu_in, ca,
+                           cb, nu, L, mode, g.n[0], g.n[1], g.n[2], dxg, sc.gamma, sc.eps, QB, GB);
+      }
+    };
+    // only the x lines of a viscous run add the viscous share
+    if (DIR == 0 && QB) go(std::integral_constant<bool, DIR == 0>{});
+    else go(std::false_type{});
   });
 }
 
@@ -1552,16 +1563,17 @@ void launch_rhs_vjp(const Grid &g, const Scheme &sc, const double *u, const doub
                     hipStream_t st) {
   const bool visc = sc.Re > 0;
   // z first and x last: the strided z lines take the plain store; the y
-  // lines, the viscous gather and the contiguous x lines add into nu (x also
-  // into L)
+  // lines add into nu; the viscous kernels leave the flux and gradient
+  // adjoints in QB and GB, and the contiguous x lines add the viscous share
+  // and their own into nu (x also into L)
   const int last = L ? (set_L ? VJP_ADD_LSET : VJP_ADD_L) : VJP_ADD;
   launch_vjp_dir<2>(g, sc, u, lam, nu_in, ca, cb, nu, L, VJP_WRITE, dxg, st);
   launch_vjp_dir<1>(g, sc, u, lam, nu_in, ca, cb, nu, L, VJP_ADD, dxg, st);
+  // work: Q [4], QB [4], GB [12] (times ncell)
+  double *QB = visc ? work + 4 * g.ncell : nullptr, *GB = visc ? work + 8 * g.ncell : nullptr;
   if (visc) {
-    // work: Q [4], QB [4], GB [12] (times ncell)
-    double *Q = work, *QB = work + 4 * g.ncell, *GB = work + 8 * g.ncell;
+    double *Q = work;
     const ViscC vc = {sc.gamma, 1.0 / sc.Re, 1.0 / (sc.gamma - 1.0), 1.0 / sc.Pr};
-    const unsigned nb = (unsigned)((g.ncell + 255) / 256);
     launch_prims(g, sc, u, Q, st);
     hipLaunchKernelGGL(k_visc_vjp_tile, dim3(visc_tiles(g.n[0], g.n[1], g.n[2])), dim3(VT), 0, st, Q, lam, nu_in, ca,
                        cb, QB, GB, g.n[0], g.n[1], g.n[2], dxg, vc);
@@ -1573,10 +1585,8 @@ void launch_rhs_vjp(const Grid &g, const Scheme &sc, const double *u, const doub
                        g.n[2], dxg, vc);
     hipLaunchKernelGGL(k_visc_vjp_ghost<2>, dim3(gz), dim3(256), 0, st, Q, lam, nu_in, ca, cb, QB, g.n[0], g.n[1],
                        g.n[2], dxg, vc);
-    hipLaunchKernelGGL(k_visc_vjp_gather, dim3(nb), dim3(256), 0, st, u, QB, GB, nu, g.n[0], g.n[1], g.n[2],
-                       sc.gamma);
   }
-  launch_vjp_dir<0>(g, sc, u, lam, nu_in, ca, cb, nu, L, last, dxg, st);
+  launch_vjp_dir<0>(g, sc, u, lam, nu_in, ca, cb, nu, L, last, dxg, st, QB, GB);
 }
 
 bool ens_vjp_supported(const Scheme &sc, const int nmax[3]) {
@@ -1599,16 +1609,23 @@ static void launch_ens_vjp_dir(const Scheme &sc, const EnsVjpPlan &p, const EnsV
   const unsigned threads = (unsigned)ens_vjp_threads(p.nmax[DIR]);
   with_scheme(sc, [&](auto tag) {
     constexpr int U = decltype(tag)::upw, W = decltype(tag)::weno;
-    if (sc.interp == INTERP_COMP) {
-      const size_t lds = (size_t)VJP_LDS * threads * sizeof(double);
-      allow_lds((const void *)k_ens_rhs_vjp<DIR, U, W>, lds);
-      hipLaunchKernelGGL((k_ens_rhs_vjp<DIR, U, W>), dim3(grid), dim3(threads), lds, st, a, p.items0[DIR], items);
-    } else {
-      const size_t lds = (size_t)VJP_CHAR_LDS * threads * sizeof(double);
-      allow_lds((const void *)k_ens_rhs_vjp_char<DIR, U, W>, lds);
-      hipLaunchKernelGGL((k_ens_rhs_vjp_char<DIR, U, W>), dim3(grid), dim3(threads), lds, st, a, p.items0[DIR],
-                         items);
-    }
+    auto go = [&](auto vg) {
+      constexpr bool VG = decltype(vg)::value;
+      if (sc.interp == INTERP_COMP) {
+        const size_t lds = (size_t)VJP_LDS * threads * sizeof(double);
+        allow_lds((const void *)k_ens_rhs_vjp<DIR, U, W, VG>, lds);
+        hipLaunchKernelGGL((k_ens_rhs_vjp<DIR, U, W, VG>), dim3(grid), dim3(threads), lds, st, a, p.items0[DIR],
+                           items);
+      } else {
+        const size_t lds = (size_t)VJP_CHAR_LDS * threads * sizeof(double);
+        allow_lds((const void *)k_ens_rhs_vjp_char<DIR, U, W, VG>, lds);
+        hipLaunchKernelGGL((k_ens_rhs_vjp_char<DIR, U, W, VG>), dim3(grid), dim3(threads), lds, st, a,
+                           p.items0[DIR], items);
+      }
+    };
+    // only the x lines of a viscous ensemble add the viscous share (a.work)
+    if (DIR == 0 && a.work) go(std::integral_constant<bool, DIR == 0>{});
+    else go(std::false_type{});
   });
 }
 
@@ -1630,7 +1647,7 @@ void launch_ens_rhs_vjp(const Scheme &sc, const EnsVjpPlan &p, const double *u, 
   a.eps = sc.eps;
   a.vc = {sc.gamma, sc.Re > 0 ? 1.0 / sc.Re : 0.0, 1.0 / (sc.gamma - 1.0), 1.0 / sc.Pr};
   const bool visc = sc.Re > 0;
-  // the order of the single grid: z (plain store), y, the viscous terms, x
+  // the order of the single grid: z (plain store), y, the viscous kernels, x
   a.mode = VJP_WRITE;
   launch_ens_vjp_dir<2>(sc, p, a, st);
   a.mode = VJP_ADD;
@@ -1645,8 +1662,8 @@ void launch_ens_rhs_vjp(const Scheme &sc, const EnsVjpPlan &p, const double *u, 
                        p.nlines[1]);
     hipLaunchKernelGGL(k_ens_visc_vjp_ghost<2>, dim3((p.nlines[2] + 255) / 256), dim3(256), 0, st, a, p.lines0[2],
                        p.nlines[2]);
-    hipLaunchKernelGGL(k_ens_visc_vjp_gather, dim3(nb), dim3(256), 0, st, a, p.ncell);
   }
+  // the x lines add the viscous share (QB, GB of a.work) before their own
   a.mode = L ? (set_L ? VJP_ADD_LSET : VJP_ADD_L) : VJP_ADD;
   launch_ens_vjp_dir<0>(sc, p, a, st);
 }
