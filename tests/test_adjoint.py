@@ -84,7 +84,7 @@ def test_torch_reference_gradient_matches_finite_differences(interp, upwind, Re)
 
 
 # ------------------------------------------------------------------ GPU
-VJP_SCHEMES = SCHEMES + [("components", "roe", 1000.0)]
+VJP_SCHEMES = SCHEMES + [("components", "roe", 1000.0), ("characteristic", "rusanov", 1000.0)]
 
 
 def _gpu_solver(w):

@@ -229,7 +229,8 @@ def test_viscous_reference_fixtures():
                                                    ("components", "roe", -1.0, "js"),
                                                    ("characteristic", "roe", 1000.0, "yc"),
                                                    ("components", "rusanov", 1000.0, "js"),
-                                                   ("components", "roe", 1000.0, "mapped")])
+                                                   ("components", "roe", 1000.0, "mapped"),
+                                                   ("characteristic", "rusanov", 1000.0, "borges")])
 def test_ensemble_adjoint_matches_single_grid(interp, upwind, Re, weno):
     """Batched trajectory gradient (5 steps, checkpoint every step: the last
     three segments kept by the forward sweep, the first two recomputed; each
